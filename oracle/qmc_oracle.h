@@ -1,0 +1,93 @@
+/* CPU FP64 oracle for the batched walker-move path of real-space VMC.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker: only tests/, the
+ * smoke() entry and bench.py's cpu_baseline leg may load it. The product
+ * library (qmcpack_amd/) never links, loads or calls anything here.
+ *
+ * Every routine is a from-scratch restatement of the reference algorithm
+ * (QMCPACK, see the file:line citations in qmc_oracle.cpp), written with the
+ * reference's operation order where that is cheap, so that GPU results can be
+ * compared at 1e-10 relative.
+ */
+#ifndef QMC_ORACLE_H
+#define QMC_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Synthetic system: orthorhombic periodic cell (L[d] <= 0 means open along d),
+ * ions, two spin groups, one real B-spline orbital table, J1 and J2 B-spline
+ * functors.  Mirrors the fields of qmi_system_desc in include/qmcmi.h. */
+typedef struct
+{
+  double L[3];
+  int n_up, n_dn;
+  int n_ion;
+  const double* ions; /* [n_ion][3] */
+  int M[3];           /* periodic grid points per dimension */
+  int norb;           /* orbitals in the table (>= max(n_up, n_dn)); 0 = no determinant */
+  const double* coefs; /* unique coefficients [Mx][My][Mz][norb] */
+  int j1_np;           /* 0 = no J1 */
+  const double* j1_params;
+  double j1_rcut, j1_cusp;
+  int j2_np; /* 0 = no J2 */
+  const double* j2_uu;
+  const double* j2_ud;
+  double j2_rcut, j2_cusp_uu, j2_cusp_ud;
+  int delay_rank; /* k: 1 = Sherman-Morrison, >1 delayed Woodbury */
+} orc_system_desc;
+
+/* RNG: mt19937 + boost-style uniform + Box-Muller */
+int orc_rng_uniform(uint32_t seed, int n, double* out);
+int orc_rng_gauss(uint32_t seed, int n, double* out);
+
+/* spline pieces */
+void orc_spline_prefactors(double tx, double* a, double* da, double* d2a);
+double orc_symtrace(const double* h6, const double* gg6);
+/* raw 3D VGH in unit-cube coordinates: out[10][norb] = v, gx,gy,gz, hxx,hxy,hxz,hyy,hyz,hzz */
+int orc_spline_vgh_raw(const int* M, int norb, const double* unique_coefs, const double* ru, double* out);
+/* SplineR2R::evaluateVGL at Cartesian r for an orthorhombic cell: out[5][n] = v, gx,gy,gz, lapl */
+int orc_spo_vgl(const double* L, const int* M, int norb, const double* unique_coefs, const double* r, int n,
+                double* out);
+
+/* radial functor value, du/dr, d2u/dr2 */
+int orc_functor_vgl(int np, const double* params, double rcut, double cusp, double r, double* out3);
+
+/* minimum-image distances from pos to each src point: r[n], dr[n][3] = src - pos */
+int orc_min_image(const double* L, const double* src, int nsrc, const double* pos, double* r, double* dr);
+
+/* DiracMatrix::invert_transpose: ainv = inverse of transpose(a), logdet = log det(a) (complex) */
+int orc_invert_transpose(int n, const double* a, double* ainv, double* logdet2);
+
+/* batched determinant + delayed-update engine, nw walkers in lock-step */
+void* orc_det_create(int nw, int n, int k);
+void orc_det_destroy(void* h);
+int orc_det_recompute(void* h, int iw, const double* psiM, const double* dpsiM, const double* d2psiM);
+int orc_det_eval_grad(void* h, int row, double* grads);
+int orc_det_ratio_grad(void* h, int row, const double* phi_vgl, double* ratios, double* grads);
+int orc_det_accept_reject(void* h, int row, const int* accepted, int safe_to_delay);
+int orc_det_complete(void* h);
+int orc_det_get(void* h, int iw, double* ainv, double* logdet2);
+
+/* wavefunction + VMC driver over a walker population split in crowds */
+void* orc_vmc_create(const orc_system_desc* d, int nw, const double* R, int walkers_per_crowd, uint32_t seed);
+void orc_vmc_destroy(void* h);
+/* initial log evaluation (recompute everything); logpsi[nw], G[nw][N][3], L[nw][N] (nullable) */
+int orc_vmc_evaluate_log(void* h, double* logpsi, double* G, double* L);
+/* single-walker protocol used by the known-answer tests */
+int orc_wf_ratio_grad(void* h, int iw, int iat, const double* newpos, double* ratio, double* grad);
+int orc_wf_accept_reject(void* h, int iw, int iat, int accepted);
+int orc_wf_eval_grad(void* h, int iw, int iat, double* grad);
+int orc_wf_evaluate_gl(void* h, int iw, int fromscratch, double* logpsi, double* G, double* L);
+/* VMC steps.  energies[steps][nw] (kinetic local energy), trace (nullable) per
+ * sub-step, electron and walker: {ratio, accepted} */
+int orc_vmc_run(void* h, int steps, int sub_steps, double tau, int recompute_period, int nthreads,
+                double* energies, double* trace);
+/* R[nw][N][3], logpsi[nw], G, L, stats[2] = {accepted, rejected} (nullable each) */
+int orc_vmc_get(void* h, double* R, double* logpsi, double* G, double* L, double* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

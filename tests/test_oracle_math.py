@@ -1,0 +1,184 @@
+"""Pin the CPU oracle against what the mathematics fixes: brute-force basis sums, finite
+differences of log|psi|, ratios against full recomputation, determinant identities and
+delayed-update invariance.  These catch dropped terms, sign errors and transposed operands
+that the reference's known answers leave open (for example non-cubic cells and n_up != n_dn).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from qmcpack_amd.system import System, jastrow_params
+
+
+def bspline_basis(t):
+    """Closed-form uniform cubic B-spline weights for the 4 points left of t in [0, 1)."""
+    return np.array([(1 - t) ** 3 / 6, (3 * t ** 3 - 6 * t ** 2 + 4) / 6,
+                     (-3 * t ** 3 + 3 * t ** 2 + 3 * t + 1) / 6, t ** 3 / 6])
+
+
+def test_spline_brute_force():
+    rng = np.random.default_rng(1)
+    M = (6, 7, 5)
+    coefs = rng.uniform(-1, 1, size=M + (3,))
+    for _ in range(20):
+        ru = rng.uniform(0, 1, size=3)
+        out = orc.spline_vgh_raw(coefs, ru)
+        ref = np.zeros(3)
+        idx, w = [], []
+        for d in range(3):
+            x = ru[d] * M[d]
+            i0 = int(np.floor(x))
+            idx.append([(i0 + a - 1) % M[d] for a in range(4)])
+            w.append(bspline_basis(x - i0))
+        for a in range(4):
+            for b in range(4):
+                for c in range(4):
+                    ref += w[0][a] * w[1][b] * w[2][c] * coefs[idx[0][a], idx[1][b], idx[2][c]]
+        assert np.allclose(out[0], ref, rtol=1e-13, atol=1e-13)
+
+
+def test_spo_vgl_finite_difference():
+    rng = np.random.default_rng(2)
+    L = np.array([7.0, 8.0, 9.5])
+    coefs = rng.uniform(-1, 1, size=(10, 12, 11, 4))
+    h = 1e-4
+    for _ in range(5):
+        r = rng.uniform(-3, 12, size=3)  # outside the cell too: periodic images
+        out = orc.spo_vgl(L, coefs, r)
+        lap = np.zeros(4)
+        for d in range(3):
+            e = np.zeros(3)
+            e[d] = h
+            vp, vm = orc.spo_vgl(L, coefs, r + e)[0], orc.spo_vgl(L, coefs, r - e)[0]
+            assert np.allclose((vp - vm) / (2 * h), out[1 + d], rtol=1e-6, atol=1e-6)
+            lap += (vp - 2 * out[0] + vm) / h ** 2
+        assert np.allclose(lap, out[4], rtol=1e-4, atol=1e-3)
+        # translation by a lattice vector leaves the orbitals unchanged
+        assert np.allclose(orc.spo_vgl(L, coefs, r + L * np.array([1, -2, 3])), out, rtol=1e-12, atol=1e-11)
+
+
+def functor_closed_form(params, rcut, cusp, r):
+    n = len(params)
+    dr = rcut / (n + 1)
+    c = np.zeros(n + 4)
+    c[1], c[2] = params[0], params[1]
+    c[0] = params[1] - 2 * dr * cusp
+    c[3:n + 1] = params[2:]
+    if r >= rcut:
+        return 0.0
+    x = r / dr
+    i = int(np.floor(x))
+    return float(np.dot(c[i:i + 4], bspline_basis(x - i)))
+
+
+def test_functor_closed_form_and_fd():
+    p = jastrow_params(8)
+    for r in np.linspace(0.01, 5.2, 37):
+        v = orc.functor_vgl(p, 5.0, -0.25, r)
+        assert abs(v[0] - functor_closed_form(p, 5.0, -0.25, r)) < 1e-13
+        if r < 4.99:
+            h = 1e-5
+            fp, fm = orc.functor_vgl(p, 5.0, -0.25, r + h)[0], orc.functor_vgl(p, 5.0, -0.25, r - h)[0]
+            assert abs((fp - fm) / (2 * h) - v[1]) < 1e-7
+            assert abs((fp - 2 * v[0] + fm) / h ** 2 - v[2]) < 1e-4
+    assert abs(orc.functor_vgl(p, 5.0, -0.25, 0.0)[1] - (-0.25)) < 1e-14  # cusp
+
+
+def small_system(n_up=4, n_dn=3, k=1, seed=3):
+    rng = np.random.default_rng(seed)
+    L = np.array([7.0, 8.0, 9.5])
+    M = (10, 12, 11)
+    norb = 5
+    p = jastrow_params(8)
+    return System(L=L, ions=rng.uniform(0, 1, size=(6, 3)) * L, n_up=n_up, n_dn=n_dn, M=M, norb=norb,
+                  coefs=rng.uniform(-1, 1, size=M + (norb,)) + 0.3,
+                  j1_params=p * 0.7, j1_rcut=3.4, j1_cusp=0.0, j2_uu=p, j2_ud=p * 1.3, j2_rcut=3.4,
+                  j2_cusp_uu=-0.25, j2_cusp_ud=-0.5, delay_rank=k)
+
+
+def logpsi_of(system, R):
+    v = orc.VMC(system, R[None], 1, 1)
+    lp, G, L = v.evaluate_log()
+    return lp[0], G[0], L[0]
+
+
+def test_twf_gradient_laplacian_fd():
+    s = small_system()
+    rng = np.random.default_rng(4)
+    R = rng.uniform(0, 1, size=(s.N, 3)) * s.L
+    lp, G, L = logpsi_of(s, R)
+    h = 1e-4
+    for i in (0, 3, 4, 6):
+        lap = 0.0
+        for d in range(3):
+            Rp, Rm = R.copy(), R.copy()
+            Rp[i, d] += h
+            Rm[i, d] -= h
+            fp, fm = logpsi_of(s, Rp)[0], logpsi_of(s, Rm)[0]
+            assert abs((fp - fm) / (2 * h) - G[i, d]) < 1e-5 * max(1, abs(G[i, d]))
+            lap += (fp - 2 * lp + fm) / h ** 2
+        assert abs(lap - L[i]) < 2e-3 * max(1, abs(L[i]))
+
+
+@pytest.mark.parametrize("k", [1, 3])
+def test_twf_ratio_grad_vs_recompute(k):
+    s = small_system(k=k)
+    rng = np.random.default_rng(5)
+    R = rng.uniform(0, 1, size=(s.N, 3)) * s.L
+    v = orc.VMC(s, R[None], 1, 1)
+    v.evaluate_log()
+    cur = R.copy()
+    lp_cur = logpsi_of(s, cur)[0]
+    for step, iat in enumerate([0, 2, 5, 1, 6, 4]):
+        newpos = cur[iat] + rng.normal(size=3) * 0.6
+        ratio, grad = v.ratio_grad(0, iat, newpos)
+        Rn = cur.copy()
+        Rn[iat] = newpos
+        lp_new, Gn, _ = logpsi_of(s, Rn)
+        assert abs(abs(ratio) - np.exp(lp_new - lp_cur)) < 1e-10 * abs(ratio)
+        assert np.allclose(grad, Gn[iat], rtol=1e-9, atol=1e-9)
+        accept = step % 3 != 1
+        v.accept_reject(0, iat, accept)
+        if accept:
+            cur, lp_cur = Rn, lp_new
+        g_now = v.eval_grad(0, (iat + 1) % s.N)
+        assert np.allclose(g_now, logpsi_of(s, cur)[1][(iat + 1) % s.N], rtol=1e-9, atol=1e-9)
+    lp, G, L = v.evaluate_gl(0)
+    lp_ref, G_ref, L_ref = logpsi_of(s, cur)
+    assert abs(lp - lp_ref) < 1e-9
+    assert np.allclose(G, G_ref, rtol=1e-9, atol=1e-9) and np.allclose(L, L_ref, rtol=1e-9, atol=1e-8)
+
+
+def test_vmc_delay_rank_invariance():
+    """Delayed (Woodbury) and Sherman-Morrison engines give the same Markov chain."""
+    R = None
+    out = {}
+    for k in (1, 2, 5):
+        s = small_system(k=k, seed=6)
+        if R is None:
+            R = np.random.default_rng(7).uniform(0, 1, size=(3, s.N, 3)) * s.L
+        v = orc.VMC(s, R, 3, 11)
+        v.evaluate_log()
+        E, tr = v.run(steps=4, sub_steps=2, tau=0.3, recompute_period=3, trace=True)
+        out[k] = (E, tr, v.get())
+    for k in (2, 5):
+        assert np.array_equal(out[k][1][..., 1], out[1][1][..., 1])  # identical accept decisions
+        assert np.allclose(out[k][1][..., 0], out[1][1][..., 0], rtol=1e-10)
+        assert np.allclose(out[k][0], out[1][0], rtol=1e-9)
+        assert np.allclose(out[k][2]["R"], out[1][2]["R"], rtol=0, atol=1e-12)
+
+
+def test_vmc_energy_matches_recompute():
+    """The local energy from the incrementally updated state equals a from-scratch evaluation."""
+    s = small_system(k=3, seed=8)
+    R = np.random.default_rng(9).uniform(0, 1, size=(2, s.N, 3)) * s.L
+    v = orc.VMC(s, R, 2, 5)
+    v.evaluate_log()
+    E, _ = v.run(steps=3, sub_steps=1, tau=0.4)
+    st = v.get()
+    for iw in range(2):
+        lp, G, L = logpsi_of(s, st["R"][iw])
+        ke = -0.5 * (np.sum(G * G) + np.sum(L))
+        assert abs(ke - E[-1, iw]) < 1e-8 * max(1, abs(ke))
+        assert abs(lp - st["logpsi"][iw]) < 1e-9
+    assert 0 < st["accepted"] and 0 < st["rejected"]
