@@ -1,0 +1,161 @@
+/* qmcmi: MI355X-native batched walker-move path of real-space VMC.
+ *
+ * C ABI (plain pointers and sizes, no exceptions across the boundary).  Every
+ * entry point returns 0 on success and a nonzero status otherwise; the message
+ * is available from qmi_last_error(ctx).  All calls on one context must come
+ * from one host thread (as one crowd is driven by one thread in the
+ * reference); contexts are independent.
+ *
+ * Each entry point states the reference interface it replaces (QMCPACK source
+ * tree, file:line).  Walker data lives in HBM for the lifetime of the context;
+ * host arrays passed in or out are copied.
+ */
+#ifndef QMCMI_H
+#define QMCMI_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qmi_ctx qmi_ctx;
+
+enum
+{
+  QMI_OK            = 0,
+  QMI_ERR_ARG       = 1, /* invalid argument or call sequence */
+  QMI_ERR_DEVICE    = 2, /* HIP runtime / device error */
+  QMI_ERR_NUMERIC   = 3, /* NaN ratio/gradient, zero ratio on accept, singular matrix */
+  QMI_ERR_UNSUPPORTED = 4
+};
+
+/* precision of the orbital table and of the delayed-update engine */
+enum
+{
+  QMI_PREC_FP64 = 0
+};
+
+/* System description.  Orthorhombic cell (L[d] <= 0: open along d); ions;
+ * two spin groups (electrons [0, n_up) up, [n_up, N) down); one real periodic
+ * cubic B-spline orbital table given as its unique periodic coefficients
+ * u[Mx][My][Mz][norb] (orbital fastest); J1 (e-ion, one ion species) and J2
+ * (uu = dd, ud = du) B-spline functors with NumParams coefficients each.
+ * Replaces the objects built by the reference's XML builders:
+ * SplineR2R (QMCWaveFunctions/BsplineFactory/SplineR2R.h:35-233),
+ * BsplineFunctor (QMCWaveFunctions/Jastrow/BsplineFunctor.h:104-133),
+ * DiracDeterminantBatched with delay_rank (Fermion/SlaterDetBuilder.cpp:393-412). */
+typedef struct
+{
+  double L[3];
+  int n_up, n_dn;
+  int n_ion;
+  const double* ions; /* [n_ion][3] */
+  int M[3];
+  int norb;
+  const double* coefs; /* [Mx][My][Mz][norb] */
+  int j1_np;
+  const double* j1_params;
+  double j1_rcut, j1_cusp;
+  int j2_np;
+  const double* j2_uu;
+  const double* j2_ud;
+  double j2_rcut, j2_cusp_uu, j2_cusp_ud;
+  int delay_rank; /* 1: Sherman-Morrison (Fahy) update; k > 1: rank-k delayed Woodbury */
+} qmi_system_desc;
+
+/* ---- context and data ---------------------------------------------------------------------- */
+/* device: HIP device ordinal.  Creates one stream; all work of the context runs on it. */
+int qmi_ctx_create(qmi_ctx** ctx, int device, int precision);
+int qmi_ctx_destroy(qmi_ctx* ctx);
+const char* qmi_last_error(const qmi_ctx* ctx);
+
+/* Upload the system: spline table (wrapped to the padded [M+3]^3 layout on the device, as
+ * einspline/bspline_create.c:1027-1034), ions, functors.  Replaces SplineR2R construction and
+ * SplineSetReader, J1/J2 RadialJastrowBuilder functors. */
+int qmi_system_create(qmi_ctx* ctx, const qmi_system_desc* desc);
+
+/* Load nw walkers' electron positions R[nw][N][3] and allocate per-walker state.
+ * walkers_per_crowd: walkers sharing one RNG stream and one delay counter
+ * (QMCDrivers/Crowd.h:37-135; DelayedUpdateBatched.h:183). */
+int qmi_walkers_load(qmi_ctx* ctx, int nw, const double* R, int walkers_per_crowd);
+/* Copy positions back: R[nw][N][3]. */
+int qmi_walkers_get(qmi_ctx* ctx, double* R);
+
+/* ---- multi-walker wave-function API (one call = all walkers of the context) ----------------- */
+/* TrialWaveFunction::mw_evaluateLog (TrialWaveFunction.cpp:161-213) including
+ * DiracDeterminantBatched::mw_recompute / mw_invertPsiM (DiracDeterminantBatched.cpp:117-146,
+ * 1121-1198), J1/J2 recompute.  Outputs (nullable): logpsi[nw], G[nw][N][3], L[nw][N]. */
+int qmi_mw_evaluate_log(qmi_ctx* ctx, double* logpsi, double* G, double* L);
+
+/* TrialWaveFunction::mw_evalGrad (TrialWaveFunction.cpp:568-593) ->
+ * DiracDeterminantBatched::mw_evalGrad (DiracDeterminantBatched.cpp:183-211),
+ * DelayedUpdateBatched::mw_evalGrad (DelayedUpdateBatched.h:353-400).  grads[nw][3]. */
+int qmi_mw_eval_grad(qmi_ctx* ctx, int iat, double* grads);
+
+/* ParticleSet::mw_makeMove + mw_computeNewPosDistTables (ParticleSet.cpp:397-495),
+ * DistanceTable::mw_move (DistanceTable.h:153-161).  displ[nw][3]; proposed position
+ * R[iat] + displ (not wrapped into the cell). */
+int qmi_mw_make_move(qmi_ctx* ctx, int iat, const double* displ);
+
+/* TrialWaveFunction::mw_calcRatioGrad (TrialWaveFunction.cpp:685-741) ->
+ * DiracDeterminantBatched::mw_ratioGrad (DiracDeterminantBatched.cpp:307-354) with
+ * SPOSet::mw_evaluateVGLandDetRatioGrads (SPOSet.h:346-352), J1/J2 ratioGrad.
+ * ratios[nw]; grads[nw][3] (total gradient of log psi at the proposed position). */
+int qmi_mw_ratio_grad(qmi_ctx* ctx, int iat, double* ratios, double* grads);
+
+/* TrialWaveFunction::mw_accept_rejectMove (TrialWaveFunction.cpp:790-821) ->
+ * DelayedUpdateBatched::mw_accept_rejectRow (DelayedUpdateBatched.h:542-670) and
+ * ParticleSet::mw_accept_rejectMove forward mode (ParticleSet.cpp:717-758).
+ * accepted[nw] (0/1).  safe_to_delay = 0 flushes the delayed updates immediately. */
+int qmi_mw_accept_reject(qmi_ctx* ctx, int iat, const int* accepted, int safe_to_delay);
+
+/* TrialWaveFunction::mw_completeUpdates -> DiracDeterminantBatched::mw_completeUpdates
+ * (DiracDeterminantBatched.cpp:544-591): flush pending delayed rows of both determinants. */
+int qmi_mw_complete_updates(qmi_ctx* ctx);
+
+/* TrialWaveFunction::mw_evaluateGL (TrialWaveFunction.cpp:869-918); fromscratch = 1 recomputes
+ * the determinants (re-inversion in FP64).  Outputs (nullable): logpsi[nw], G, L, and the kinetic
+ * local energy -1/2 (sum L + sum G.G) per walker (BareKineticEnergy.cpp:100-118) in ekin[nw]. */
+int qmi_mw_evaluate_gl(qmi_ctx* ctx, int fromscratch, double* logpsi, double* G, double* L, double* ekin);
+
+/* SPOSet::mw_evaluateVGL (SPOSet.h:313-318): orbital value, gradient, Laplacian of the first n
+ * orbitals at one position per walker.  pos[nw][3]; out[nw][5][n] (v, gx, gy, gz, lapl). */
+int qmi_mw_spo_vgl(qmi_ctx* ctx, const double* pos, int n, double* out);
+
+/* SPOSet::mw_evaluateDetRatios (SPOSet.h:257-261): value-only orbitals at nvp virtual positions
+ * per walker dotted with a given inverse row.  pos[nw][nvp][3], inv_row[nw][n],
+ * ratios[nw][nvp]. */
+int qmi_mw_det_ratios(qmi_ctx* ctx, int nvp, const double* pos, int n, const double* inv_row, double* ratios);
+
+/* DistanceTable::mw_move (DistanceTable.h:153-161) with SoaDistanceTableAA/AB rows
+ * (SoaDistanceTableAA.h:58-75, SoaDistanceTableAB.h:59-78): distances and displacements
+ * (source - target, minimum image) from pos[nw][3] to all electrons and all ions.
+ * ee_r[nw][N], ee_dr[nw][N][3], ei_r[nw][n_ion], ei_dr[nw][n_ion][3] (nullable). */
+int qmi_mw_distance_rows(qmi_ctx* ctx, const double* pos, double* ee_r, double* ee_dr, double* ei_r,
+                         double* ei_dr);
+
+/* ---- device-resident driver ---------------------------------------------------------------- */
+/* VMCBatched::advanceWalkers<POS> (QMCDrivers/VMC/VMCBatched.cpp:57-228) for every crowd of the
+ * context, `steps` times: each step runs sub_steps sweeps with drift-diffusion moves
+ * (DriftModifierUNR, a = 1), Metropolis-Hastings on the device with the reference's RNG order
+ * (one std::mt19937 per crowd seeded seed + crowd index, boost-style uniforms, Box-Muller
+ * Gaussians, uniforms drawn only for |ratio|^2 >= eps), then evaluateGL (fromscratch on steps
+ * where (step_count + 1) % recompute_period == 0) and the kinetic local energy.
+ * energies[steps][nw] (nullable); trace[steps][sub_steps][N][nw][2] = {ratio, accepted}
+ * (nullable: parity only, slows the sweep). */
+int qmi_vmc_run(qmi_ctx* ctx, int steps, int sub_steps, double tau, int recompute_period, uint32_t seed,
+                double* energies, double* trace);
+/* accepted / rejected move counts since the context was created: stats[2] */
+int qmi_vmc_stats(qmi_ctx* ctx, double* stats);
+
+/* ---- measurement ---------------------------------------------------------------------------- */
+/* Per-kernel device time accumulated with HIP events on the context stream (enabled by
+ * qmi_timing_enable(ctx, 1)).  names: '\n'-separated kernel names written to buf (size buflen);
+ * ms[i], calls[i] for up to maxk kernels; returns the number of kernels in *nk. */
+int qmi_timing_enable(qmi_ctx* ctx, int on);
+int qmi_timing_get(qmi_ctx* ctx, char* buf, int buflen, double* ms, double* calls, int maxk, int* nk);
+int qmi_timing_reset(qmi_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

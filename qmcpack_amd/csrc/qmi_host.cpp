@@ -1,0 +1,1044 @@
+// Host side of the qmcmi C ABI: context, device memory, the multi-walker API and the
+// device-resident VMC driver.  All device work of a context runs on one HIP stream.
+#include "../../include/qmcmi.h"
+#include "qmi_kernels.hpp"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+using namespace qmi;
+
+namespace
+{
+struct QmiError : std::runtime_error
+{
+  int code;
+  QmiError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHECK(expr)                                                                                      \
+  do                                                                                                        \
+  {                                                                                                         \
+    hipError_t e_ = (expr);                                                                                 \
+    if (e_ != hipSuccess)                                                                                   \
+      throw QmiError(QMI_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));                    \
+  } while (0)
+
+template<typename T>
+struct DevBuf
+{
+  T* p     = nullptr;
+  size_t n = 0;
+  void alloc(size_t count)
+  {
+    free();
+    if (count == 0)
+      return;
+    HIPCHECK(hipMalloc(&p, count * sizeof(T)));
+    n = count;
+  }
+  void free()
+  {
+    if (p)
+      (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { free(); }
+};
+
+template<typename T>
+struct PinnedBuf
+{
+  T* p     = nullptr;
+  size_t n = 0;
+  void alloc(size_t count)
+  {
+    if (count <= n)
+      return;
+    free();
+    HIPCHECK(hipHostMalloc(&p, count * sizeof(T), hipHostMallocDefault));
+    n = count;
+  }
+  void free()
+  {
+    if (p)
+      (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~PinnedBuf() { free(); }
+};
+
+// std::mt19937 + boost-style uniform (Utilities/StdRandom.h:26-85)
+inline double uniform01(std::mt19937& e)
+{
+  return static_cast<double>(e() - e.min()) / (static_cast<double>(e.max() - e.min()) + 1) * (1.0 - 0.0) + 0.0;
+}
+
+// Box-Muller (Particle/ParticleBase/RandomSeqGenerator.h:31-50); fp-contract off so the host
+// sequence is bit-identical across builds of this expression.
+void gauss_fill(double* a, unsigned n, std::mt19937& e)
+{
+#pragma clang fp contract(off)
+  const double slightly_less_than_one = 1.0 - std::numeric_limits<double>::epsilon();
+  const int nm1                       = int(n) - 1;
+  double temp1, temp2;
+  for (int i = 0; i < nm1; i += 2)
+  {
+    temp1    = std::sqrt(-2.0 * std::log(1.0 - slightly_less_than_one * uniform01(e)));
+    temp2    = 2.0 * M_PI * uniform01(e);
+    a[i]     = temp1 * std::cos(temp2);
+    a[i + 1] = temp1 * std::sin(temp2);
+  }
+  if (n % 2 == 1)
+  {
+    temp1  = std::sqrt(-2.0 * std::log(1.0 - slightly_less_than_one * uniform01(e)));
+    temp2  = 2.0 * M_PI * uniform01(e);
+    a[nm1] = temp1 * std::cos(temp2);
+  }
+}
+
+// pool of timing events
+struct Timing
+{
+  bool on = false;
+  struct Rec
+  {
+    int id;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> free_ev;
+  std::vector<std::string> names;
+  std::map<std::string, int> ids;
+  std::vector<double> ms, calls;
+
+  hipEvent_t get()
+  {
+    if (!free_ev.empty())
+    {
+      hipEvent_t e = free_ev.back();
+      free_ev.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIPCHECK(hipEventCreate(&e));
+    return e;
+  }
+  int id(const char* name)
+  {
+    auto it = ids.find(name);
+    if (it != ids.end())
+      return it->second;
+    const int i = int(names.size());
+    names.push_back(name);
+    ids[name] = i;
+    ms.push_back(0);
+    calls.push_back(0);
+    return i;
+  }
+  void resolve()
+  {
+    for (auto& r : pending)
+    {
+      float t = 0;
+      HIPCHECK(hipEventSynchronize(r.b));
+      HIPCHECK(hipEventElapsedTime(&t, r.a, r.b));
+      ms[r.id] += t;
+      calls[r.id] += 1;
+      free_ev.push_back(r.a);
+      free_ev.push_back(r.b);
+    }
+    pending.clear();
+  }
+  ~Timing()
+  {
+    for (auto e : free_ev)
+      (void)hipEventDestroy(e);
+    for (auto& r : pending)
+    {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+  }
+};
+
+} // namespace
+
+struct qmi_ctx
+{
+  int device     = 0;
+  int precision  = QMI_PREC_FP64;
+  hipStream_t stream = nullptr;
+  rocblas_handle blas = nullptr;
+  std::string err;
+  bool have_system = false;
+  SysC sys{};
+  DevBuf<double> coefs, ions;
+  // walkers
+  int nw = 0, wpc = 1, ncrowd = 0;
+  DevBuf<double> R, jas, logdet, stats, invRow, phi, mv, flushT, out, gl3, Gbuf, Lbuf, hostin;
+  DevBuf<double> Ainv[2], dpsi[2], U[2], V[2], Binv[2];
+  DevBuf<int> dlist[2], ipiv, info, ubase, acc;
+  DevBuf<double> gauss, unif, trace;
+  State st{};
+  int dcount[2] = {0, 0};
+  int invrow_id = -1;
+  long long step_count = 0;
+  // RNG: one stream per crowd
+  std::vector<std::mt19937> rng;
+  uint32_t rng_seed   = 0;
+  bool rng_seeded     = false;
+  PinnedBuf<double> h_gauss, h_unif;
+  Timing timing;
+
+  // -------------------------------------------------------------------------------------------
+  struct Scoped
+  {
+    qmi_ctx* c;
+    int id;
+    hipEvent_t a;
+    Scoped(qmi_ctx* ctx, const char* name) : c(ctx), id(-1), a(nullptr)
+    {
+      if (c->timing.on)
+      {
+        id = c->timing.id(name);
+        a  = c->timing.get();
+        HIPCHECK(hipEventRecord(a, c->stream));
+      }
+    }
+    ~Scoped()
+    {
+      if (c->timing.on && a)
+      {
+        hipEvent_t b = c->timing.get();
+        (void)hipEventRecord(b, c->stream);
+        c->timing.pending.push_back({id, a, b});
+      }
+    }
+  };
+
+  void sync()
+  {
+    HIPCHECK(hipStreamSynchronize(stream));
+    timing.resolve();
+  }
+
+  int group(int iat) const { return iat < sys.nup ? 0 : 1; }
+
+  void require_walkers() const
+  {
+    if (!have_system || nw == 0)
+      throw QmiError(QMI_ERR_ARG, "system and walkers must be loaded first");
+  }
+
+  // ---------------------------------------------------------------------------------------------
+  void system_create(const qmi_system_desc* d)
+  {
+    if (!d)
+      throw QmiError(QMI_ERR_ARG, "null system description");
+    SysC s{};
+    s.nup  = d->n_up;
+    s.ndn  = d->n_dn;
+    s.N    = s.nup + s.ndn;
+    s.nion = d->n_ion;
+    if (s.nup < 0 || s.ndn < 0 || s.N <= 0 || s.N > 1024)
+      throw QmiError(QMI_ERR_ARG, "electron counts must satisfy 0 < N <= 1024");
+    s.first[0] = 0;
+    s.first[1] = s.nup;
+    s.n[0]     = s.nup;
+    s.n[1]     = s.ndn;
+    s.Npad     = (s.N + 31) / 32 * 32;
+    const int nmax = std::max(s.nup, s.ndn);
+    if (nmax > MAX_N)
+      throw QmiError(QMI_ERR_UNSUPPORTED, "more than 512 electrons per spin");
+    s.ld = std::max(8, (nmax + 7) / 8 * 8);
+    s.k  = std::max(1, d->delay_rank);
+    if (s.k > MAX_K || (s.k > 1 && s.k > nmax))
+      throw QmiError(QMI_ERR_ARG, "delay rank must be in [1, 64] and not exceed the determinant size");
+    for (int i = 0; i < 3; i++)
+    {
+      s.periodic[i] = d->L[i] > 0;
+      s.L[i]        = s.periodic[i] ? d->L[i] : 1.0;
+      s.Linv[i]     = 1.0 / s.L[i];
+    }
+    std::fill(s.G, s.G + 9, 0.0);
+    double GGt[9] = {0};
+    for (int i = 0; i < 3; i++)
+    {
+      s.G[i * 4]  = s.Linv[i];
+      GGt[i * 4] = s.Linv[i] * s.Linv[i];
+    }
+    const double symGG[6] = {GGt[0], GGt[1] + GGt[3], GGt[2] + GGt[6], GGt[4], GGt[5] + GGt[7], GGt[8]};
+    std::copy(symGG, symGG + 6, s.symGG);
+    s.has_det = d->norb > 0 && d->coefs != nullptr;
+    if (s.has_det)
+    {
+      if (d->norb < nmax)
+        throw QmiError(QMI_ERR_ARG, "norb smaller than the electrons per spin");
+      for (int i = 0; i < 3; i++)
+      {
+        s.M[i] = d->M[i];
+        if (s.M[i] < 4)
+          throw QmiError(QMI_ERR_ARG, "spline grid needs at least 4 points per dimension");
+        const double delta = (1.0 - 0.0) / (double)(s.M[i]);
+        s.dinv[i]          = 1.0 / delta;
+      }
+      s.npad = (d->norb + 7) / 8 * 8;
+      s.zs   = s.npad;
+      s.ys   = (long long)(s.M[2] + 3) * s.npad;
+      s.xs   = (long long)(s.M[1] + 3) * s.ys;
+    }
+    auto functor = [](FunctorC& f, int np, const double* P, double rcut, double cusp) {
+      if (np < 2 || np + 4 > MAX_JCOEF)
+        throw QmiError(QMI_ERR_ARG, "functor needs 2 <= NumParams <= 64");
+      std::memset(&f, 0, sizeof(f));
+      const int numCoefs = np + 4, numKnots = numCoefs - 2;
+      const double DeltaR = rcut / (double)(numKnots - 1);
+      f.DeltaRInv         = 1.0 / DeltaR;
+      f.rcut              = rcut;
+      f.np                = np;
+      f.max_index         = numCoefs - 4;
+      f.c[1]              = P[0];
+      f.c[2]              = P[1];
+      f.c[0]              = P[1] - 2.0 * DeltaR * cusp;
+      for (int i = 2; i < np; i++)
+        f.c[i + 1] = P[i];
+    };
+    s.has_j1 = d->j1_np > 0 && d->n_ion > 0 && d->j1_params;
+    if (s.has_j1)
+      functor(s.j1, d->j1_np, d->j1_params, d->j1_rcut, d->j1_cusp);
+    s.has_j2 = d->j2_np > 0 && d->j2_uu && d->j2_ud;
+    if (s.has_j2)
+    {
+      functor(s.j2[0], d->j2_np, d->j2_uu, d->j2_rcut, d->j2_cusp_uu);
+      functor(s.j2[1], d->j2_np, d->j2_ud, d->j2_rcut, d->j2_cusp_ud);
+    }
+    // device copies
+    ions.alloc(std::max(3 * s.nion, 3));
+    if (s.nion > 0)
+    {
+      std::vector<double> t(3 * s.nion);
+      for (int j = 0; j < s.nion; j++)
+        for (int q = 0; q < 3; q++)
+          t[q * s.nion + j] = d->ions[3 * j + q];
+      HIPCHECK(hipMemcpy(ions.p, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (s.has_det)
+    {
+      const size_t nuniq = (size_t)s.M[0] * s.M[1] * s.M[2] * d->norb;
+      const size_t ntab  = (size_t)(s.M[0] + 3) * (s.M[1] + 3) * (s.M[2] + 3) * s.npad;
+      DevBuf<double> tmp;
+      tmp.alloc(nuniq);
+      HIPCHECK(hipMemcpy(tmp.p, d->coefs, nuniq * sizeof(double), hipMemcpyHostToDevice));
+      coefs.alloc(ntab);
+      HIPCHECK(launch_wrap_coefs(s, tmp.p, d->norb, coefs.p, stream));
+      HIPCHECK(hipStreamSynchronize(stream));
+    }
+    sys         = s;
+    have_system = true;
+    nw          = 0;
+  }
+
+  // ---------------------------------------------------------------------------------------------
+  void walkers_load(int nw_in, const double* Rin, int wpc_in)
+  {
+    if (!have_system)
+      throw QmiError(QMI_ERR_ARG, "system not created");
+    if (nw_in <= 0 || !Rin)
+      throw QmiError(QMI_ERR_ARG, "need at least one walker");
+    const SysC& s = sys;
+    nw            = nw_in;
+    wpc           = std::max(1, std::min(wpc_in, nw));
+    ncrowd        = (nw + wpc - 1) / wpc;
+    const int Np = s.Npad, ld = s.ld, k = s.k;
+    R.alloc((size_t)nw * 3 * Np);
+    {
+      std::vector<double> t((size_t)nw * 3 * Np, 0.0);
+      for (int w = 0; w < nw; w++)
+        for (int i = 0; i < s.N; i++)
+          for (int q = 0; q < 3; q++)
+            t[((size_t)w * 3 + q) * Np + i] = Rin[((size_t)w * s.N + i) * 3 + q];
+      HIPCHECK(hipMemcpy(R.p, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    jas.alloc((size_t)nw * J_ROWS * Np);
+    HIPCHECK(hipMemset(jas.p, 0, jas.n * sizeof(double)));
+    logdet.alloc((size_t)nw * 4);
+    HIPCHECK(hipMemset(logdet.p, 0, logdet.n * sizeof(double)));
+    stats.alloc((size_t)nw * 2);
+    HIPCHECK(hipMemset(stats.p, 0, stats.n * sizeof(double)));
+    mv.alloc((size_t)nw * MV_SIZE);
+    HIPCHECK(hipMemset(mv.p, 0, mv.n * sizeof(double)));
+    out.alloc((size_t)nw * 4 + 16);
+    gl3.alloc((size_t)nw * 3);
+    hostin.alloc((size_t)nw * 3 + 16);
+    acc.alloc(nw);
+    ubase.alloc((size_t)ncrowd * (s.N + 1));
+    if (s.has_det)
+    {
+      invRow.alloc((size_t)nw * ld);
+      phi.alloc((size_t)nw * 5 * ld);
+      for (int g = 0; g < 2; g++)
+      {
+        const int n = s.n[g];
+        Ainv[g].alloc((size_t)nw * std::max(n, 1) * ld);
+        dpsi[g].alloc((size_t)nw * std::max(n, 1) * 4 * ld);
+        U[g].alloc((size_t)nw * k * ld);
+        V[g].alloc((size_t)nw * k * ld);
+        Binv[g].alloc((size_t)nw * k * k);
+        dlist[g].alloc((size_t)nw * k);
+        HIPCHECK(hipMemset(U[g].p, 0, U[g].n * sizeof(double)));
+        HIPCHECK(hipMemset(V[g].p, 0, V[g].n * sizeof(double)));
+        HIPCHECK(hipMemset(Binv[g].p, 0, Binv[g].n * sizeof(double)));
+      }
+      ipiv.alloc((size_t)nw * std::max(s.nup, s.ndn));
+      info.alloc(nw);
+      if (k > 1)
+        flushT.alloc((size_t)nw * std::max(s.nup, s.ndn) * k);
+    }
+    st        = State{};
+    st.R      = R.p;
+    st.jas    = jas.p;
+    st.logdet = logdet.p;
+    st.stats  = stats.p;
+    st.invRow = invRow.p;
+    st.phi    = phi.p;
+    st.mv     = mv.p;
+    st.ions   = ions.p;
+    for (int g = 0; g < 2; g++)
+    {
+      st.Ainv[g]  = Ainv[g].p;
+      st.dpsi[g]  = dpsi[g].p;
+      st.U[g]     = U[g].p;
+      st.V[g]     = V[g].p;
+      st.Binv[g]  = Binv[g].p;
+      st.dlist[g] = dlist[g].p;
+    }
+    dcount[0] = dcount[1] = 0;
+    invrow_id             = -1;
+    step_count            = 0;
+    rng_seeded            = false;
+  }
+
+  // ---------------------------------------------------------------------------------------------
+  // determinant recompute: orbital rows, LU (rocSOLVER), log det, inverse
+  void recompute_dets()
+  {
+    const SysC& s = sys;
+    if (!s.has_det)
+      return;
+    if (dcount[0] || dcount[1])
+      throw QmiError(QMI_ERR_ARG, "recompute with pending delayed updates");
+    for (int g = 0; g < 2; g++)
+    {
+      const int n = s.n[g];
+      if (n == 0)
+        continue;
+      {
+        Scoped t(this, "spo_rows");
+        HIPCHECK(launch_spo_rows(s, st, coefs.p, g, nw, stream));
+      }
+      {
+        Scoped t(this, "lu_getrf");
+        if (rocsolver_dgetrf_strided_batched(blas, n, n, Ainv[g].p, s.ld, (rocblas_stride)n * s.ld, ipiv.p, n, info.p,
+                                             nw) != rocblas_status_success)
+          throw QmiError(QMI_ERR_DEVICE, "rocsolver_dgetrf_strided_batched failed");
+      }
+      check_info("LU factorisation: singular Slater matrix");
+      {
+        Scoped t(this, "logdet");
+        HIPCHECK(launch_logdet(s, st, g, nw, ipiv.p, stream));
+      }
+      {
+        Scoped t(this, "lu_getri");
+        if (rocsolver_dgetri_strided_batched(blas, n, Ainv[g].p, s.ld, (rocblas_stride)n * s.ld, ipiv.p, n, info.p,
+                                             nw) != rocblas_status_success)
+          throw QmiError(QMI_ERR_DEVICE, "rocsolver_dgetri_strided_batched failed");
+      }
+      check_info("matrix inverse: singular Slater matrix");
+    }
+    invrow_id = -1;
+  }
+
+  void check_info(const char* what)
+  {
+    std::vector<int> h(nw);
+    HIPCHECK(hipMemcpyAsync(h.data(), info.p, nw * sizeof(int), hipMemcpyDeviceToHost, stream));
+    sync();
+    for (int w = 0; w < nw; w++)
+      if (h[w] != 0)
+        throw QmiError(QMI_ERR_NUMERIC, std::string(what) + " (walker " + std::to_string(w) + ")");
+  }
+
+  void eval_gl(double* logpsi, double* G, double* L, double* ekin)
+  {
+    const SysC& s = sys;
+    double* dG    = nullptr;
+    double* dL    = nullptr;
+    if (G)
+    {
+      Gbuf.alloc((size_t)nw * s.N * 3);
+      dG = Gbuf.p;
+    }
+    if (L)
+    {
+      Lbuf.alloc((size_t)nw * s.N);
+      dL = Lbuf.p;
+    }
+    {
+      Scoped t(this, "eval_gl");
+      HIPCHECK(launch_eval_gl(s, st, nw, gl3.p, dG, dL, stream));
+    }
+    std::vector<double> h((size_t)nw * 3);
+    HIPCHECK(hipMemcpyAsync(h.data(), gl3.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    if (G)
+      HIPCHECK(hipMemcpyAsync(G, dG, (size_t)nw * s.N * 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    if (L)
+      HIPCHECK(hipMemcpyAsync(L, dL, (size_t)nw * s.N * sizeof(double), hipMemcpyDeviceToHost, stream));
+    sync();
+    for (int w = 0; w < nw; w++)
+    {
+      if (ekin)
+        ekin[w] = h[w * 3 + 0];
+      if (logpsi)
+        logpsi[w] = h[w * 3 + 1];
+      if (!std::isfinite(h[w * 3 + 0]) || !std::isfinite(h[w * 3 + 1]))
+        throw QmiError(QMI_ERR_NUMERIC, "non-finite log psi or local energy (walker " + std::to_string(w) + ")");
+    }
+  }
+
+  void flush(int g)
+  {
+    if (sys.k == 1 || dcount[g] == 0)
+      return;
+    {
+      Scoped t(this, "flush");
+      HIPCHECK(launch_flush(sys, st, g, dcount[g], nw, flushT.p, stream));
+    }
+    dcount[g] = 0;
+    invrow_id = -1;
+  }
+
+  MoveArgs move_args(int iat)
+  {
+    MoveArgs a{};
+    a.iat0 = iat;
+    a.iat1 = iat + 1;
+    a.g    = group(iat);
+    a.d0   = sys.k > 1 ? dcount[a.g] : 0;
+    a.nw   = nw;
+    a.wpc  = wpc;
+    return a;
+  }
+
+  // ---------------------------------------------------------------------------------------------
+  // RNG for one sub-step: Gaussians [iat][iw][3] per crowd (the reference layout of
+  // walker_deltas, VMCBatched.cpp:109) scattered into the global [iat][w][3] array, then the crowd's
+  // possible uniforms (N * crowd size), drawn speculatively and rewound after the sub-step.
+  std::vector<std::mt19937> rng_saved;
+  void rng_generate()
+  {
+    const int N = sys.N;
+    h_gauss.alloc((size_t)N * nw * 3);
+    h_unif.alloc((size_t)ncrowd * N * wpc);
+    rng_saved.resize(ncrowd);
+    auto work = [&](int c0, int c1) {
+      std::vector<double> buf;
+      for (int c = c0; c < c1; c++)
+      {
+        const int w0 = c * wpc, cw = std::min(nw, w0 + wpc) - w0;
+        buf.resize((size_t)N * cw * 3);
+        gauss_fill(buf.data(), unsigned(buf.size()), rng[c]);
+        for (int iat = 0; iat < N; iat++)
+          std::memcpy(h_gauss.p + ((size_t)iat * nw + w0) * 3, buf.data() + (size_t)iat * cw * 3,
+                      (size_t)cw * 3 * sizeof(double));
+        rng_saved[c] = rng[c];
+        double* u    = h_unif.p + (size_t)c * N * wpc;
+        for (int i = 0; i < N * cw; i++)
+          u[i] = uniform01(rng[c]);
+      }
+    };
+    const int nth = std::max(1, std::min<int>({ncrowd, 16, (int)std::thread::hardware_concurrency()}));
+    if (nth == 1)
+      work(0, ncrowd);
+    else
+    {
+      std::vector<std::thread> th;
+      const int per = (ncrowd + nth - 1) / nth;
+      for (int i = 0; i < nth; i++)
+        th.emplace_back(work, i * per, std::min(ncrowd, (i + 1) * per));
+      for (auto& x : th)
+        x.join();
+    }
+  }
+
+  void rng_rewind(const std::vector<int>& consumed)
+  {
+    for (int c = 0; c < ncrowd; c++)
+    {
+      rng[c] = rng_saved[c];
+      rng[c].discard((unsigned long long)consumed[c]);
+    }
+  }
+
+  void vmc_run(int steps, int sub_steps, double tau, int recompute_period, uint32_t seed, double* energies,
+               double* htrace)
+  {
+    require_walkers();
+    const SysC& s = sys;
+    const int N   = s.N;
+    if (steps < 0 || sub_steps < 1 || !(tau > 0))
+      throw QmiError(QMI_ERR_ARG, "invalid steps / sub_steps / tau");
+    if (!rng_seeded || seed != rng_seed)
+    {
+      rng.clear();
+      for (int c = 0; c < ncrowd; c++)
+        rng.emplace_back(seed + uint32_t(c));
+      rng_seed   = seed;
+      rng_seeded = true;
+    }
+    gauss.alloc((size_t)N * nw * 3);
+    unif.alloc((size_t)ncrowd * N * wpc);
+    if (htrace)
+      trace.alloc((size_t)N * nw * 2);
+    std::vector<int> hub((size_t)ncrowd * (N + 1));
+    std::vector<double> ek(nw);
+    for (int step = 0; step < steps; step++)
+    {
+      for (int sub = 0; sub < sub_steps; sub++)
+      {
+        rng_generate();
+        HIPCHECK(hipMemcpyAsync(gauss.p, h_gauss.p, (size_t)N * nw * 3 * sizeof(double), hipMemcpyHostToDevice,
+                                stream));
+        HIPCHECK(hipMemcpyAsync(unif.p, h_unif.p, (size_t)ncrowd * N * wpc * sizeof(double), hipMemcpyHostToDevice,
+                                stream));
+        HIPCHECK(hipMemsetAsync(ubase.p, 0, ubase.n * sizeof(int), stream));
+        MoveArgs a{};
+        a.nw    = nw;
+        a.wpc   = wpc;
+        a.tau   = tau;
+        a.gauss = gauss.p;
+        a.unif  = unif.p;
+        a.ubase = ubase.p;
+        a.trace = htrace ? trace.p : nullptr;
+        for (int g = 0; g < 2; g++)
+        {
+          int iat = s.first[g];
+          const int last = s.first[g] + s.n[g];
+          while (iat < last)
+          {
+            a.g  = g;
+            a.d0 = s.k > 1 ? dcount[g] : 0;
+            if (wpc == 1)
+            {
+              const int seg = s.k > 1 ? std::min(s.k - dcount[g], last - iat) : last - iat;
+              a.iat0        = iat;
+              a.iat1        = iat + seg;
+              {
+                Scoped t(this, "sweep_segment");
+                HIPCHECK(launch_move(MK_SEGMENT, s, st, coefs.p, a, stream));
+              }
+              iat += seg;
+              if (s.k > 1)
+                dcount[g] += seg;
+            }
+            else
+            {
+              a.iat0 = iat;
+              a.iat1 = iat + 1;
+              {
+                Scoped t(this, "propose");
+                HIPCHECK(launch_move(MK_PROPOSE, s, st, coefs.p, a, stream));
+              }
+              {
+                Scoped t(this, "accept_crowd");
+                HIPCHECK(launch_move(MK_ACCEPT_CROWD, s, st, coefs.p, a, stream));
+              }
+              iat += 1;
+              if (s.k > 1)
+                dcount[g] += 1;
+            }
+            if (s.k > 1 && dcount[g] == s.k)
+              flush(g);
+          }
+        }
+        flush(0);
+        flush(1);
+        HIPCHECK(hipMemcpyAsync(hub.data(), ubase.p, hub.size() * sizeof(int), hipMemcpyDeviceToHost, stream));
+        if (htrace)
+          HIPCHECK(hipMemcpyAsync(htrace + ((size_t)step * sub_steps + sub) * N * nw * 2, trace.p,
+                                  (size_t)N * nw * 2 * sizeof(double), hipMemcpyDeviceToHost, stream));
+        sync();
+        std::vector<int> consumed(ncrowd);
+        for (int c = 0; c < ncrowd; c++)
+          consumed[c] = hub[(size_t)c * (N + 1) + N];
+        rng_rewind(consumed);
+      }
+      const bool recompute = recompute_period > 0 && ((step_count + 1) % recompute_period == 0);
+      if (recompute)
+        recompute_dets();
+      eval_gl(nullptr, nullptr, nullptr, ek.data());
+      if (energies)
+        std::copy(ek.begin(), ek.end(), energies + (size_t)step * nw);
+      step_count++;
+    }
+  }
+};
+
+//--------------------------------------------------------------------------------------------------
+namespace
+{
+template<typename F>
+int guarded(qmi_ctx* c, F&& f)
+{
+  if (!c)
+    return QMI_ERR_ARG;
+  try
+  {
+    HIPCHECK(hipSetDevice(c->device));
+    f();
+    c->err.clear();
+    return QMI_OK;
+  }
+  catch (const QmiError& e)
+  {
+    c->err = e.what();
+    return e.code;
+  }
+  catch (const std::exception& e)
+  {
+    c->err = e.what();
+    return QMI_ERR_DEVICE;
+  }
+}
+} // namespace
+
+extern "C"
+{
+  int qmi_ctx_create(qmi_ctx** out, int device, int precision)
+  {
+    if (!out)
+      return QMI_ERR_ARG;
+    *out = nullptr;
+    if (precision != QMI_PREC_FP64)
+      return QMI_ERR_UNSUPPORTED;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+      return QMI_ERR_DEVICE;
+    auto* c      = new qmi_ctx;
+    c->device    = device;
+    c->precision = precision;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess)
+    {
+      delete c;
+      return QMI_ERR_DEVICE;
+    }
+    if (rocblas_create_handle(&c->blas) != rocblas_status_success ||
+        rocblas_set_stream(c->blas, c->stream) != rocblas_status_success)
+    {
+      (void)hipStreamDestroy(c->stream);
+      delete c;
+      return QMI_ERR_DEVICE;
+    }
+    *out = c;
+    return QMI_OK;
+  }
+
+  int qmi_ctx_destroy(qmi_ctx* c)
+  {
+    if (!c)
+      return QMI_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    if (c->blas)
+      rocblas_destroy_handle(c->blas);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return QMI_OK;
+  }
+
+  const char* qmi_last_error(const qmi_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+  int qmi_system_create(qmi_ctx* c, const qmi_system_desc* d)
+  {
+    return guarded(c, [&] { c->system_create(d); });
+  }
+
+  int qmi_walkers_load(qmi_ctx* c, int nw, const double* R, int wpc)
+  {
+    return guarded(c, [&] { c->walkers_load(nw, R, wpc); });
+  }
+
+  int qmi_walkers_get(qmi_ctx* c, double* Rout)
+  {
+    return guarded(c, [&] {
+      c->require_walkers();
+      const int N = c->sys.N, Np = c->sys.Npad;
+      std::vector<double> t((size_t)c->nw * 3 * Np);
+      HIPCHECK(hipMemcpyAsync(t.data(), c->R.p, t.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      c->sync();
+      for (int w = 0; w < c->nw; w++)
+        for (int i = 0; i < N; i++)
+          for (int q = 0; q < 3; q++)
+            Rout[((size_t)w * N + i) * 3 + q] = t[((size_t)w * 3 + q) * Np + i];
+    });
+  }
+
+  int qmi_mw_evaluate_log(qmi_ctx* c, double* logpsi, double* G, double* L)
+  {
+    return guarded(c, [&] {
+      c->require_walkers();
+      {
+        qmi_ctx::Scoped t(c, "jastrow_recompute");
+        HIPCHECK(launch_jastrow_recompute(c->sys, c->st, c->nw, c->stream));
+      }
+      c->dcount[0] = c->dcount[1] = 0;
+      c->recompute_dets();
+      c->eval_gl(logpsi, G, L, nullptr);
+    });
+  }
+
+  int qmi_mw_eval_grad(qmi_ctx* c, int iat, double* grads)
+  {
+    return guarded(c, [&] {
+      c->require_walkers();
+      if (iat < 0 || iat >= c->sys.N || !grads)
+        throw QmiError(QMI_ERR_ARG, "bad electron index");
+      MoveArgs a = c->move_args(iat);
+      a.out      = c->out.p;
+      {
+        qmi_ctx::Scoped t(c, "api_eval_grad");
+        HIPCHECK(launch_move(MK_API_GRAD, c->sys, c->st, c->coefs.p, a, c->stream));
+      }
+      HIPCHECK(hipMemcpyAsync(grads, c->out.p, (size_t)c->nw * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      c->sync();
+      if (c->sys.k > 1)
+        c->invrow_id = iat;
+      for (int i = 0; i < 3 * c->nw; i++)
+        if (!std::isfinite(grads[i]))
+          throw QmiError(QMI_ERR_NUMERIC, "NaN gradient in mw_evalGrad");
+    });
+  }
+
+  int qmi_mw_make_move(qmi_ctx* c, int iat, const double* displ)
+  {
+    return guarded(c, [&] {
+      c->require_walkers();
+      if (iat < 0 || iat >= c->sys.N || !displ)
+        throw QmiError(QMI_ERR_ARG, "bad electron index");
+      HIPCHECK(hipMemcpyAsync(c->hostin.p, displ, (size_t)c->nw * 3 * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+      HIPCHECK(launch_make_move(c->sys, c->st, iat, c->nw, c->hostin.p, c->stream));
+    });
+  }
+
+  int qmi_mw_ratio_grad(qmi_ctx* c, int iat, double* ratios, double* grads)
+  {
+    return guarded(c, [&] {
+      c->require_walkers();
+      if (iat < 0 || iat >= c->sys.N)
+        throw QmiError(QMI_ERR_ARG, "bad electron index");
+      MoveArgs a = c->move_args(iat);
+      a.out      = c->out.p;
+      a.prepare  = (c->sys.k == 1 || c->invrow_id != iat) ? 1 : 0;
+      {
+        qmi_ctx::Scoped t(c, "api_ratio_grad");
+        HIPCHECK(launch_move(MK_API_RATIO, c->sys, c->st, c->coefs.p, a, c->stream));
+      }
+      std::vector<double> h((size_t)c->nw * 4);
+      HIPCHECK(hipMemcpyAsync(h.data(), c->out.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      c->sync();
+      if (c->sys.k > 1)
+        c->invrow_id = iat;
+      for (int w = 0; w < c->nw; w++)
+      {
+        if (ratios)
+          ratios[w] = h[w];
+        if (grads)
+          for (int q = 0; q < 3; q++)
+            grads[w * 3 + q] = h[c->nw + w * 3 + q];
+        if (std::isnan(h[w]))
+          throw QmiError(QMI_ERR_NUMERIC, "NaN ratio in mw_calcRatioGrad");
+      }
+    });
+  }
+
+  int qmi_mw_accept_reject(qmi_ctx* c, int iat, const int* accepted, int safe_to_delay)
+  {
+    return guarded(c, [&] {
+      c->require_walkers();
+      if (iat < 0 || iat >= c->sys.N || !accepted)
+        throw QmiError(QMI_ERR_ARG, "bad electron index");
+      HIPCHECK(hipMemcpyAsync(c->acc.p, accepted, (size_t)c->nw * sizeof(int), hipMemcpyHostToDevice, c->stream));
+      MoveArgs a  = c->move_args(iat);
+      a.host_acc = c->acc.p;
+      {
+        qmi_ctx::Scoped t(c, "api_accept");
+        HIPCHECK(launch_move(MK_API_ACCEPT, c->sys, c->st, c->coefs.p, a, c->stream));
+      }
+      c->invrow_id = -1;
+      const int g  = c->group(iat);
+      if (c->sys.k > 1)
+      {
+        c->dcount[g]++;
+        if (c->dcount[g] == c->sys.k || !safe_to_delay)
+          c->flush(g);
+      }
+    });
+  }
+
+  int qmi_mw_complete_updates(qmi_ctx* c)
+  {
+    return guarded(c, [&] {
+      c->require_walkers();
+      c->flush(0);
+      c->flush(1);
+    });
+  }
+
+  int qmi_mw_evaluate_gl(qmi_ctx* c, int fromscratch, double* logpsi, double* G, double* L, double* ekin)
+  {
+    return guarded(c, [&] {
+      c->require_walkers();
+      if (fromscratch)
+        c->recompute_dets();
+      c->eval_gl(logpsi, G, L, ekin);
+    });
+  }
+
+  int qmi_mw_spo_vgl(qmi_ctx* c, const double* pos, int n, double* outv)
+  {
+    return guarded(c, [&] {
+      c->require_walkers();
+      if (!c->sys.has_det || n <= 0 || n > c->sys.npad)
+        throw QmiError(QMI_ERR_ARG, "no orbital table or bad orbital count");
+      DevBuf<double> dpos, dout;
+      dpos.alloc((size_t)c->nw * 3);
+      dout.alloc((size_t)c->nw * 5 * n);
+      HIPCHECK(hipMemcpyAsync(dpos.p, pos, dpos.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      {
+        qmi_ctx::Scoped t(c, "api_spo_vgl");
+        HIPCHECK(launch_api_spo_vgl(c->sys, c->coefs.p, c->nw, n, dpos.p, dout.p, c->stream));
+      }
+      HIPCHECK(hipMemcpyAsync(outv, dout.p, dout.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      c->sync();
+    });
+  }
+
+  int qmi_mw_det_ratios(qmi_ctx* c, int nvp, const double* pos, int n, const double* inv_row, double* ratios)
+  {
+    return guarded(c, [&] {
+      c->require_walkers();
+      if (!c->sys.has_det || n <= 0 || n > c->sys.npad || nvp <= 0)
+        throw QmiError(QMI_ERR_ARG, "no orbital table or bad sizes");
+      DevBuf<double> dpos, dinv, dout;
+      dpos.alloc((size_t)c->nw * nvp * 3);
+      dinv.alloc((size_t)c->nw * n);
+      dout.alloc((size_t)c->nw * nvp);
+      HIPCHECK(hipMemcpyAsync(dpos.p, pos, dpos.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      HIPCHECK(hipMemcpyAsync(dinv.p, inv_row, dinv.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      HIPCHECK(launch_api_det_ratios(c->sys, c->coefs.p, c->nw, nvp, n, dpos.p, dinv.p, dout.p, c->stream));
+      HIPCHECK(hipMemcpyAsync(ratios, dout.p, dout.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      c->sync();
+    });
+  }
+
+  int qmi_mw_distance_rows(qmi_ctx* c, const double* pos, double* ee_r, double* ee_dr, double* ei_r, double* ei_dr)
+  {
+    return guarded(c, [&] {
+      c->require_walkers();
+      const int nw = c->nw, N = c->sys.N, ni = c->sys.nion;
+      DevBuf<double> dpos, a, b, cc, dd;
+      dpos.alloc((size_t)nw * 3);
+      a.alloc((size_t)nw * N);
+      b.alloc((size_t)nw * N * 3);
+      cc.alloc((size_t)nw * std::max(ni, 1));
+      dd.alloc((size_t)nw * std::max(ni, 1) * 3);
+      HIPCHECK(hipMemcpyAsync(dpos.p, pos, dpos.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      HIPCHECK(launch_api_distances(c->sys, c->st, nw, dpos.p, a.p, b.p, cc.p, dd.p, c->stream));
+      if (ee_r)
+        HIPCHECK(hipMemcpyAsync(ee_r, a.p, (size_t)nw * N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      if (ee_dr)
+        HIPCHECK(hipMemcpyAsync(ee_dr, b.p, (size_t)nw * N * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      if (ei_r && ni)
+        HIPCHECK(hipMemcpyAsync(ei_r, cc.p, (size_t)nw * ni * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      if (ei_dr && ni)
+        HIPCHECK(hipMemcpyAsync(ei_dr, dd.p, (size_t)nw * ni * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      c->sync();
+    });
+  }
+
+  int qmi_vmc_run(qmi_ctx* c, int steps, int sub_steps, double tau, int recompute_period, uint32_t seed,
+                  double* energies, double* trace)
+  {
+    return guarded(c, [&] { c->vmc_run(steps, sub_steps, tau, recompute_period, seed, energies, trace); });
+  }
+
+  int qmi_vmc_stats(qmi_ctx* c, double* stats2)
+  {
+    return guarded(c, [&] {
+      c->require_walkers();
+      std::vector<double> h((size_t)c->nw * 2);
+      HIPCHECK(hipMemcpyAsync(h.data(), c->stats.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      c->sync();
+      stats2[0] = stats2[1] = 0;
+      for (int w = 0; w < c->nw; w++)
+      {
+        stats2[0] += h[w * 2];
+        stats2[1] += h[w * 2 + 1];
+      }
+    });
+  }
+
+  int qmi_timing_enable(qmi_ctx* c, int on)
+  {
+    return guarded(c, [&] {
+      c->sync();
+      c->timing.on = on != 0;
+    });
+  }
+
+  int qmi_timing_reset(qmi_ctx* c)
+  {
+    return guarded(c, [&] {
+      c->sync();
+      std::fill(c->timing.ms.begin(), c->timing.ms.end(), 0.0);
+      std::fill(c->timing.calls.begin(), c->timing.calls.end(), 0.0);
+    });
+  }
+
+  int qmi_timing_get(qmi_ctx* c, char* buf, int buflen, double* ms, double* calls, int maxk, int* nk)
+  {
+    return guarded(c, [&] {
+      c->sync();
+      std::string names;
+      const int n = std::min<int>(maxk, (int)c->timing.names.size());
+      for (int i = 0; i < n; i++)
+      {
+        names += c->timing.names[i];
+        names += '\n';
+        ms[i]    = c->timing.ms[i];
+        calls[i] = c->timing.calls[i];
+      }
+      if (buf && buflen > 0)
+      {
+        std::strncpy(buf, names.c_str(), buflen - 1);
+        buf[buflen - 1] = 0;
+      }
+      *nk = n;
+    });
+  }
+}

@@ -1,0 +1,382 @@
+// Initialisation, recompute and gradient/Laplacian kernels.
+#include "qmi_kernels.hpp"
+
+namespace qmi
+{
+
+// Padded table from the unique periodic coefficients: table[i][j][k][o] = u[(i-1)%Mx][(j-1)%My][(k-1)%Mz][o]
+// (einspline/bspline_create.c:1027-1034 periodic wrap; MultiBsplineBase.hpp:75-121 layout).
+__global__ void k_wrap_coefs(SysC s, const double* __restrict__ unique, int norb, double* __restrict__ table)
+{
+  const long long Nx = s.M[0] + 3, Ny = s.M[1] + 3, Nz = s.M[2] + 3;
+  const long long pt = blockIdx.x;
+  if (pt >= Nx * Ny * Nz)
+    return;
+  const int i = int(pt / (Ny * Nz)), j = int((pt / Nz) % Ny), kk = int(pt % Nz);
+  const int ui = (i - 1 + s.M[0]) % s.M[0], uj = (j - 1 + s.M[1]) % s.M[1], uk = (kk - 1 + s.M[2]) % s.M[2];
+  const double* src = unique + (((long long)ui * s.M[1] + uj) * s.M[2] + uk) * norb;
+  double* dst       = table + pt * s.npad;
+  for (int o = threadIdx.x; o < s.npad; o += blockDim.x)
+    dst[o] = o < norb ? src[o] : 0.0;
+}
+
+hipError_t launch_wrap_coefs(const SysC& s, const double* unique, int norb, double* table, hipStream_t stream)
+{
+  const long long npts = (long long)(s.M[0] + 3) * (s.M[1] + 3) * (s.M[2] + 3);
+  k_wrap_coefs<<<dim3((unsigned)npts), 256, 0, stream>>>(s, unique, norb, table);
+  return hipGetLastError();
+}
+
+// Orbital VGL at every electron of determinant g (BsplineSet::mw_evaluate_notranspose,
+// BsplineSet.h:142-189).  Values are written transposed into Ainv (Ainv[w][j][i] = phi_j(r_i),
+// the column-major psiM handed to the LU), gradients/Laplacians into dpsi[w][i][0..3][j].
+__global__ __launch_bounds__(WG) void k_spo_rows(SysC s, State st, const double* __restrict__ coefs, int g)
+{
+  const int i = blockIdx.x, w = blockIdx.y;
+  const int n = s.n[g], ld = s.ld, Np = s.Npad;
+  const int iat   = s.first[g] + i;
+  const double* R = st.R + (size_t)w * 3 * Np;
+  const double pos[3] = {R[iat], R[Np + iat], R[2 * Np + iat]};
+  SplineLoc L;
+  spline_locate(s, pos, L);
+  double* A    = st.Ainv[g] + (size_t)w * n * ld;
+  double* drow = st.dpsi[g] + ((size_t)w * n + i) * 4 * ld;
+  for (int o = threadIdx.x; o < n; o += WG)
+  {
+    double out[5];
+    spline_vgl_orbital<double>(s, coefs, L, o, out);
+    A[(size_t)o * ld + i] = out[0];
+    drow[o]               = out[1];
+    drow[ld + o]          = out[2];
+    drow[2 * ld + o]      = out[3];
+    drow[3 * ld + o]      = out[4];
+  }
+}
+
+hipError_t launch_spo_rows(const SysC& s, const State& st, const double* coefs, int g, int nw, hipStream_t stream)
+{
+  k_spo_rows<<<dim3(s.n[g], nw), WG, 0, stream>>>(s, st, coefs, g);
+  return hipGetLastError();
+}
+
+// log det from the LU diagonal and pivots (DiracMatrix.h:101-106): sum log(+-U_ii), sign flipped
+// where ipiv[i] != i+1; the phase counts pi per negative factor.
+__global__ void k_logdet(SysC s, State st, int g, int nw, const int* __restrict__ ipiv)
+{
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nw)
+    return;
+  const int n = s.n[g], ld = s.ld;
+  const double* A = st.Ainv[g] + (size_t)w * n * ld;
+  const int* p    = ipiv + (size_t)w * n;
+  double re = 0, im = 0;
+  for (int i = 0; i < n; i++)
+  {
+    const double dg = A[(size_t)i * ld + i];
+    const double v  = (p[i] == i + 1) ? dg : -dg;
+    re += log(fabs(v));
+    if (v < 0)
+      im += M_PI;
+  }
+  st.logdet[(size_t)w * 4 + g * 2 + 0] = re;
+  st.logdet[(size_t)w * 4 + g * 2 + 1] = im;
+}
+
+hipError_t launch_logdet(const SysC& s, const State& st, int g, int nw, const int* ipiv, hipStream_t stream)
+{
+  k_logdet<<<(nw + 63) / 64, 64, 0, stream>>>(s, st, g, nw, ipiv);
+  return hipGetLastError();
+}
+
+// J1 and J2 accumulators from scratch (J1OrbitalSoA.h:240-249; TwoBodyJastrow.cpp:667-713, summed
+// over the full row instead of the lower triangle).  One thread per electron.
+__global__ __launch_bounds__(WG) void k_jastrow_recompute(SysC s, State st)
+{
+  const int w = blockIdx.y;
+  const int i = blockIdx.x * WG + threadIdx.x;
+  const int N = s.N, Np = s.Npad;
+  if (i >= N)
+    return;
+  const double* R = st.R + (size_t)w * 3 * Np;
+  double* jas     = st.jas + (size_t)w * J_ROWS * Np;
+  const double pos[3] = {R[i], R[Np + i], R[2 * Np + i]};
+  if (s.has_j1)
+  {
+    double v = 0, gx = 0, gy = 0, gz = 0, lap = 0;
+    for (int j = 0; j < s.nion; j++)
+    {
+      double dr[3];
+      const double r = min_image(s, st.ions[j], st.ions[s.nion + j], st.ions[2 * s.nion + j], pos, dr);
+      if (r < s.j1.rcut)
+      {
+        double u, du, d2u;
+        functor_vgl(s.j1, r, u, du, d2u);
+        v += u;
+        gx += du * dr[0];
+        gy += du * dr[1];
+        gz += du * dr[2];
+        lap += d2u + 2.0 * du;
+      }
+    }
+    jas[J_VAT * Np + i]       = v;
+    jas[(J_G1X + 0) * Np + i] = gx;
+    jas[(J_G1X + 1) * Np + i] = gy;
+    jas[(J_G1X + 2) * Np + i] = gz;
+    jas[J_LAP1 * Np + i]      = lap;
+  }
+  if (s.has_j2)
+  {
+    const int gi = i < s.nup ? 0 : 1;
+    double v = 0, gx = 0, gy = 0, gz = 0, lap = 0;
+    for (int j = 0; j < N; j++)
+    {
+      if (j == i)
+        continue;
+      double dr[3];
+      const double r    = min_image(s, R[j], R[Np + j], R[2 * Np + j], pos, dr);
+      const FunctorC& f = s.j2[((j < s.nup) ? 0 : 1) == gi ? 0 : 1];
+      if (r < f.rcut)
+      {
+        double u, du, d2u;
+        functor_vgl(f, r, u, du, d2u);
+        v += u;
+        gx += du * dr[0];
+        gy += du * dr[1];
+        gz += du * dr[2];
+        lap += d2u + 2.0 * du;
+      }
+    }
+    jas[J_UAT * Np + i]       = v;
+    jas[(J_DUX + 0) * Np + i] = gx;
+    jas[(J_DUX + 1) * Np + i] = gy;
+    jas[(J_DUX + 2) * Np + i] = gz;
+    jas[J_D2U * Np + i]       = -lap;
+  }
+}
+
+hipError_t launch_jastrow_recompute(const SysC& s, const State& st, int nw, hipStream_t stream)
+{
+  k_jastrow_recompute<<<dim3((s.N + WG - 1) / WG, nw), WG, 0, stream>>>(s, st);
+  return hipGetLastError();
+}
+
+// TrialWaveFunction::mw_evaluateGL (TrialWaveFunction.cpp:869-918): determinant rows
+// (DiracDeterminantBatched::computeGL, :593-604), J1 (J1OrbitalSoA.h:115-123), J2
+// (TwoBodyJastrow.cpp:769-778); kinetic energy -1/2 (sum G.G + sum L) (BareKineticEnergy.cpp:109-118).
+__global__ __launch_bounds__(WG) void k_eval_gl(SysC s, State st, double* __restrict__ out3, double* __restrict__ Gout,
+                                                double* __restrict__ Lout)
+{
+  __shared__ double sG[3 * 1024];
+  __shared__ double sL[1024];
+  __shared__ double red[NWAVE * 4];
+  __shared__ double bc[4];
+  const int w = blockIdx.x;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int N = s.N, Np = s.Npad, ld = s.ld;
+  const double* jas = st.jas + (size_t)w * J_ROWS * Np;
+  for (int i = t; i < N; i += WG)
+  {
+    sG[3 * i] = sG[3 * i + 1] = sG[3 * i + 2] = 0.0;
+    sL[i]                                     = 0.0;
+  }
+  __syncthreads();
+  if (s.has_det)
+    for (int g = 0; g < 2; g++)
+    {
+      const int n     = s.n[g];
+      const double* A = st.Ainv[g] + (size_t)w * n * ld;
+      const double* D = st.dpsi[g] + (size_t)w * n * 4 * ld;
+      for (int i = wave; i < n; i += NWAVE)
+      {
+        double gx = 0, gy = 0, gz = 0, lp = 0;
+        const double* a = A + (size_t)i * ld;
+        const double* dd = D + (size_t)i * 4 * ld;
+        for (int o = lane; o < n; o += 64)
+        {
+          const double av = a[o];
+          gx += av * dd[o];
+          gy += av * dd[ld + o];
+          gz += av * dd[2 * ld + o];
+          lp += av * dd[3 * ld + o];
+        }
+        gx = wave_sum(gx);
+        gy = wave_sum(gy);
+        gz = wave_sum(gz);
+        lp = wave_sum(lp);
+        if (lane == 0)
+        {
+          const int iat = s.first[g] + i;
+          sG[3 * iat]     = gx;
+          sG[3 * iat + 1] = gy;
+          sG[3 * iat + 2] = gz;
+          sL[iat]         = lp - (gx * gx + gy * gy + gz * gz);
+        }
+      }
+    }
+  __syncthreads();
+  double acc[4] = {0, 0, 0, 0}; // G.G, sum L, sum Vat, sum Uat
+  for (int i = t; i < N; i += WG)
+  {
+    double g0 = sG[3 * i], g1 = sG[3 * i + 1], g2 = sG[3 * i + 2], l = sL[i];
+    if (s.has_j1)
+    {
+      g0 += jas[(J_G1X + 0) * Np + i];
+      g1 += jas[(J_G1X + 1) * Np + i];
+      g2 += jas[(J_G1X + 2) * Np + i];
+      l -= jas[J_LAP1 * Np + i];
+      acc[2] += jas[J_VAT * Np + i];
+    }
+    if (s.has_j2)
+    {
+      g0 += jas[(J_DUX + 0) * Np + i];
+      g1 += jas[(J_DUX + 1) * Np + i];
+      g2 += jas[(J_DUX + 2) * Np + i];
+      l += jas[J_D2U * Np + i];
+      acc[3] += jas[J_UAT * Np + i];
+    }
+    acc[0] += g0 * g0 + g1 * g1 + g2 * g2;
+    acc[1] += l;
+    if (Gout)
+    {
+      double* G = Gout + ((size_t)w * N + i) * 3;
+      G[0]      = g0;
+      G[1]      = g1;
+      G[2]      = g2;
+    }
+    if (Lout)
+      Lout[(size_t)w * N + i] = l;
+  }
+  block_sum<4>(acc, red, bc);
+  if (t == 0)
+  {
+    const double* ld4 = st.logdet + (size_t)w * 4;
+    double logpsi = 0, phase = 0;
+    if (s.has_det)
+    {
+      logpsi += ld4[0] + ld4[2];
+      phase += ld4[1] + ld4[3];
+    }
+    if (s.has_j1)
+      logpsi += -bc[2];
+    if (s.has_j2)
+      logpsi += -0.5 * bc[3];
+    out3[(size_t)w * 3 + 0] = -0.5 * (bc[0] + bc[1]);
+    out3[(size_t)w * 3 + 1] = logpsi;
+    out3[(size_t)w * 3 + 2] = phase;
+  }
+}
+
+hipError_t launch_eval_gl(const SysC& s, const State& st, int nw, double* out3, double* G, double* L,
+                          hipStream_t stream)
+{
+  if (s.N > 1024)
+    return hipErrorInvalidValue;
+  k_eval_gl<<<nw, WG, 0, stream>>>(s, st, out3, G, L);
+  return hipGetLastError();
+}
+
+//------------------------------------------------------------------------------------------------
+// API helpers
+
+// SPOSet::mw_evaluateVGL: out[w][5][n] at pos[w]
+__global__ __launch_bounds__(WG) void k_api_spo_vgl(SysC s, const double* __restrict__ coefs, int n,
+                                                    const double* __restrict__ pos, double* __restrict__ out)
+{
+  const int w = blockIdx.x;
+  const double p[3] = {pos[w * 3], pos[w * 3 + 1], pos[w * 3 + 2]};
+  SplineLoc L;
+  spline_locate(s, p, L);
+  for (int o = threadIdx.x; o < n; o += WG)
+  {
+    double v[5];
+    spline_vgl_orbital<double>(s, coefs, L, o, v);
+    for (int q = 0; q < 5; q++)
+      out[((size_t)w * 5 + q) * n + o] = v[q];
+  }
+}
+
+hipError_t launch_api_spo_vgl(const SysC& s, const double* coefs, int nw, int n, const double* pos, double* out,
+                              hipStream_t stream)
+{
+  k_api_spo_vgl<<<nw, WG, 0, stream>>>(s, coefs, n, pos, out);
+  return hipGetLastError();
+}
+
+// SPOSet::mw_evaluateDetRatios (SplineR2R.cpp:204-335): value-only orbitals at virtual positions
+// dotted with an inverse row.  Grid (nvp, nw).
+__global__ __launch_bounds__(WG) void k_api_det_ratios(SysC s, const double* __restrict__ coefs, int nvp, int n,
+                                                       const double* __restrict__ pos, const double* __restrict__ inv,
+                                                       double* __restrict__ out)
+{
+  __shared__ double red[NWAVE];
+  __shared__ double bc[1];
+  const int v = blockIdx.x, w = blockIdx.y;
+  const double* pp  = pos + ((size_t)w * nvp + v) * 3;
+  const double p[3] = {pp[0], pp[1], pp[2]};
+  SplineLoc L;
+  spline_locate(s, p, L);
+  double acc[1] = {0};
+  for (int o = threadIdx.x; o < n; o += WG)
+  {
+    // value only: sum_ij a_i b_j sum_k c_k coef
+    double val = 0;
+    const double* base = coefs + L.off + o;
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++)
+      {
+        const double* q = base + i * s.xs + j * s.ys;
+        const double s0 = L.c[0] * q[0] + L.c[1] * q[s.zs] + L.c[2] * q[2 * s.zs] + L.c[3] * q[3 * s.zs];
+        val += (L.a[i] * L.b[j]) * s0;
+      }
+    acc[0] += val * inv[(size_t)w * n + o];
+  }
+  block_sum<1>(acc, red, bc);
+  if (threadIdx.x == 0)
+    out[(size_t)w * nvp + v] = bc[0];
+}
+
+hipError_t launch_api_det_ratios(const SysC& s, const double* coefs, int nw, int nvp, int n, const double* pos,
+                                 const double* inv, double* out, hipStream_t stream)
+{
+  k_api_det_ratios<<<dim3(nvp, nw), WG, 0, stream>>>(s, coefs, nvp, n, pos, inv, out);
+  return hipGetLastError();
+}
+
+// DistanceTable::mw_move rows: electrons and ions relative to pos[w] (source - target)
+__global__ void k_api_distances(SysC s, State st, const double* __restrict__ pos, double* __restrict__ ee_r,
+                                double* __restrict__ ee_dr, double* __restrict__ ei_r, double* __restrict__ ei_dr)
+{
+  const int w  = blockIdx.x;
+  const int Np = s.Npad;
+  const double* R   = st.R + (size_t)w * 3 * Np;
+  const double p[3] = {pos[w * 3], pos[w * 3 + 1], pos[w * 3 + 2]};
+  for (int j = threadIdx.x; j < s.N; j += blockDim.x)
+  {
+    double dr[3];
+    const double r = min_image(s, R[j], R[Np + j], R[2 * Np + j], p, dr);
+    if (ee_r)
+      ee_r[(size_t)w * s.N + j] = r;
+    if (ee_dr)
+      for (int q = 0; q < 3; q++)
+        ee_dr[((size_t)w * s.N + j) * 3 + q] = dr[q];
+  }
+  for (int j = threadIdx.x; j < s.nion; j += blockDim.x)
+  {
+    double dr[3];
+    const double r = min_image(s, st.ions[j], st.ions[s.nion + j], st.ions[2 * s.nion + j], p, dr);
+    if (ei_r)
+      ei_r[(size_t)w * s.nion + j] = r;
+    if (ei_dr)
+      for (int q = 0; q < 3; q++)
+        ei_dr[((size_t)w * s.nion + j) * 3 + q] = dr[q];
+  }
+}
+
+hipError_t launch_api_distances(const SysC& s, const State& st, int nw, const double* pos, double* ee_r,
+                                double* ee_dr, double* ei_r, double* ei_dr, hipStream_t stream)
+{
+  k_api_distances<<<nw, 256, 0, stream>>>(s, st, pos, ee_r, ee_dr, ei_r, ei_dr);
+  return hipGetLastError();
+}
+
+} // namespace qmi

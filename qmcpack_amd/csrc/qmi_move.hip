@@ -1,0 +1,915 @@
+// Per-walker move kernels: one 256-thread workgroup per walker.
+//
+// Phases of one proposed single-electron move (VMCBatched::advanceWalkers inner loop,
+// QMCDrivers/VMC/VMCBatched.cpp:119-172):
+//   A  inverse row of the moving electron brought up to date through the delayed rows
+//      (DelayedUpdateBatched::mw_prepareInvRow, DelayedUpdateBatched.h:174-235) and the
+//      determinant gradient (mw_evalGrad, :353-400)
+//   B  drift-diffusion proposal (DriftModifierUNR, VMCBatched.cpp:121-134)
+//   C  J1/J2 ratio+gradient from fresh distance rows, orbital VGL at the proposed position and
+//      determinant ratio/gradient (mw_calcRatioGrad, TrialWaveFunction.cpp:685-741)
+//   D  Green's functions and Metropolis-Hastings (VMCBatched.cpp:141-167)
+//   E  accept (rank-1 growth of the delayed-update block, DelayedUpdateBatched.h:542-670, or the
+//      Sherman-Morrison update for delay rank 1, :252-340; Jastrow accumulators,
+//      TwoBodyJastrow.cpp:581-628, J1OrbitalSoA.h:456-468) or pseudo-accept on reject.
+#include "qmi_kernels.hpp"
+
+namespace qmi
+{
+
+struct MoveLDS
+{
+  double row[MAX_N]; // Ainv row / SM-1 temp vector
+  double vec[MAX_N]; // proposed orbital values
+  double p[MAX_K];
+  double y[MAX_K]; // row d of Binv (from phase A)
+  double Y[MAX_K]; // new column d of Binv
+  double red[NWAVE * 16];
+  double bc[48];   // broadcast scalars
+};
+
+enum BcField
+{
+  BC_GRAD = 0,  // 3 determinant gradient at the current position
+  BC_POS  = 3,  // 3 proposed position
+  BC_MOVE = 6,  // 3 displacement
+  BC_LOGGF = 9,
+  BC_C    = 10, // 13 phase-C sums: J1 (u, g3, lap), J2 (u, g3), det (ratio, g3)
+  BC_ACC  = 23,
+  BC_RATIO = 24,
+  BC_D2U  = 25,
+  BC_OLD  = 26 // 3 old position
+};
+
+//------------------------------------------------------------------------------------------------
+template<int MAXM>
+__device__ inline void phaseA(const SysC& s, int n, int ld, int k, int d, int r, const double* __restrict__ Ainv,
+                              const double* __restrict__ U, const double* __restrict__ V, double* __restrict__ Binv,
+                              MoveLDS& sh, double (&ainv_r)[MAXM], double (&inv)[MAXM])
+{
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const double* arow = Ainv + (size_t)r * ld;
+#pragma unroll
+  for (int m = 0; m < MAXM; m++)
+  {
+    const int o = t + WG * m;
+    ainv_r[m]   = o < n ? arow[o] : 0.0;
+    inv[m]      = ainv_r[m];
+  }
+  if (k > 1 && d > 0)
+  {
+#pragma unroll
+    for (int m = 0; m < MAXM; m++)
+    {
+      const int o = t + WG * m;
+      if (o < n)
+        sh.row[o] = ainv_r[m];
+    }
+    __syncthreads();
+    // p = U invRow over the d delayed rows (one wave per row)
+    for (int c = wave; c < d; c += NWAVE)
+    {
+      double acc = 0;
+      for (int o = lane; o < n; o += 64)
+        acc += U[(size_t)c * ld + o] * sh.row[o];
+      acc = wave_sum(acc);
+      if (lane == 0)
+        sh.p[c] = acc;
+    }
+    __syncthreads();
+    // Binv[d][i] = -sum_j Binv[j][i] p[j]
+    if (t < d)
+    {
+      double acc = 0;
+      for (int j = 0; j < d; j++)
+        acc += Binv[j * k + t] * sh.p[j];
+      const double y = -1.0 * acc;
+      sh.y[t]        = y;
+      Binv[d * k + t] = y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < MAXM; m++)
+    {
+      const int o = t + WG * m;
+      if (o < n)
+      {
+        double acc = 0;
+        for (int c = 0; c < d; c++)
+          acc += V[(size_t)c * ld + o] * sh.y[c];
+        inv[m] = 1.0 * acc + 1.0 * ainv_r[m];
+      }
+    }
+  }
+}
+
+// determinant gradient at the current position: invRow . dpsiM[r]  -> sh.bc[BC_GRAD..+3)
+template<int MAXM>
+__device__ inline void det_grad(int n, int ld, int r, const double* __restrict__ dpsi, const double (&inv)[MAXM],
+                                MoveLDS& sh)
+{
+  const int t = threadIdx.x;
+  double gv[3] = {0, 0, 0};
+  const double* drow = dpsi + (size_t)r * 4 * ld;
+#pragma unroll
+  for (int m = 0; m < MAXM; m++)
+  {
+    const int o = t + WG * m;
+    if (o < n)
+    {
+      gv[0] += inv[m] * drow[o];
+      gv[1] += inv[m] * drow[ld + o];
+      gv[2] += inv[m] * drow[2 * ld + o];
+    }
+  }
+  block_sum<3>(gv, sh.red, sh.bc + BC_GRAD);
+}
+
+// Phase C: all threads; position in sh.bc[BC_POS]
+template<typename ST, int MAXM>
+__device__ inline void phaseC(const SysC& s, int g, int iat, int n, const double* __restrict__ R,
+                              const double* __restrict__ ions, const ST* __restrict__ coefs,
+                              const double (&inv)[MAXM], double (&phv)[MAXM][5], MoveLDS& sh)
+{
+  const int t = threadIdx.x;
+  const int N = s.N, Np = s.Npad;
+  const double pos[3] = {sh.bc[BC_POS], sh.bc[BC_POS + 1], sh.bc[BC_POS + 2]};
+  double acc[13];
+#pragma unroll
+  for (int i = 0; i < 13; i++)
+    acc[i] = 0;
+  if (s.has_j1)
+    for (int j = t; j < s.nion; j += WG)
+    {
+      double dr[3];
+      const double r = min_image(s, ions[j], ions[s.nion + j], ions[2 * s.nion + j], pos, dr);
+      if (r < s.j1.rcut)
+      {
+        double u, du, d2u;
+        functor_vgl(s.j1, r, u, du, d2u);
+        acc[0] += u;
+        acc[1] += du * dr[0];
+        acc[2] += du * dr[1];
+        acc[3] += du * dr[2];
+        acc[4] += d2u + 2.0 * du;
+      }
+    }
+  if (s.has_j2)
+    for (int j = t; j < N; j += WG)
+    {
+      if (j == iat)
+        continue;
+      double dr[3];
+      const double r = min_image(s, R[j], R[Np + j], R[2 * Np + j], pos, dr);
+      const FunctorC& f = s.j2[((j < s.nup) ? 0 : 1) == g ? 0 : 1];
+      if (r < f.rcut)
+      {
+        double u, du, d2u;
+        functor_vgl(f, r, u, du, d2u);
+        acc[5] += u;
+        acc[6] += du * dr[0];
+        acc[7] += du * dr[1];
+        acc[8] += du * dr[2];
+      }
+    }
+  if (s.has_det)
+  {
+    SplineLoc L;
+    spline_locate(s, pos, L);
+#pragma unroll
+    for (int m = 0; m < MAXM; m++)
+    {
+      const int o = t + WG * m;
+      if (o < n)
+      {
+        spline_vgl_orbital<ST>(s, coefs, L, o, phv[m]);
+        acc[9] += inv[m] * phv[m][0];
+        acc[10] += inv[m] * phv[m][1];
+        acc[11] += inv[m] * phv[m][2];
+        acc[12] += inv[m] * phv[m][3];
+      }
+      else
+      {
+#pragma unroll
+        for (int q = 0; q < 5; q++)
+          phv[m][q] = 0.0;
+      }
+    }
+  }
+  block_sum<13>(acc, sh.red, sh.bc + BC_C);
+}
+
+// Phase D helper (one thread): total ratio, gradient at the proposed position, log G_b.
+__device__ inline void combine_ratio(const SysC& s, int iat, const double* __restrict__ jas, const MoveLDS& sh,
+                                     double& ratio, double gnew[3])
+{
+  const int Np = s.Npad;
+  const double* C = sh.bc + BC_C;
+  ratio           = 1.0;
+  gnew[0] = gnew[1] = gnew[2] = 0.0;
+  if (s.has_det)
+  {
+    ratio *= C[9];
+    gnew[0] += C[10] / C[9];
+    gnew[1] += C[11] / C[9];
+    gnew[2] += C[12] / C[9];
+  }
+  if (s.has_j1)
+  {
+    gnew[0] += C[1];
+    gnew[1] += C[2];
+    gnew[2] += C[3];
+    ratio *= exp(jas[J_VAT * Np + iat] - C[0]);
+  }
+  if (s.has_j2)
+  {
+    gnew[0] += C[6];
+    gnew[1] += C[7];
+    gnew[2] += C[8];
+    ratio *= exp(jas[J_UAT * Np + iat] - C[5]);
+  }
+}
+
+// Phase E: accept (sh.bc[BC_ACC] != 0) or reject.  All threads.
+template<int MAXM>
+__device__ inline void phaseE(const SysC& s, int g, int iat, int n, int ld, int k, int d, int r, double* __restrict__ R,
+                              double* __restrict__ Ainv, double* __restrict__ dpsi, double* __restrict__ U,
+                              double* __restrict__ V, double* __restrict__ Binv, int* __restrict__ dl,
+                              double* __restrict__ jas, double* __restrict__ logdet, const double (&ainv_r)[MAXM],
+                              const double (&phv)[MAXM][5], MoveLDS& sh)
+{
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int N = s.N, Np = s.Npad;
+  const bool accepted = sh.bc[BC_ACC] != 0.0;
+  const double* C     = sh.bc + BC_C;
+  if (s.has_det)
+  {
+    const double ratio_det = C[9];
+    if (k > 1)
+    {
+#pragma unroll
+      for (int m = 0; m < MAXM; m++)
+      {
+        const int o = t + WG * m;
+        if (o < n)
+          V[(size_t)d * ld + o] = ainv_r[m];
+      }
+      if (accepted)
+      {
+        const double sigma = 1.0 / ratio_det;
+#pragma unroll
+        for (int m = 0; m < MAXM; m++)
+        {
+          const int o = t + WG * m;
+          if (o < n)
+            sh.vec[o] = phv[m][0];
+        }
+        __syncthreads();
+        for (int c = wave; c < d; c += NWAVE)
+        {
+          double acc = 0;
+          for (int o = lane; o < n; o += 64)
+            acc += V[(size_t)c * ld + o] * sh.vec[o];
+          acc = wave_sum(acc);
+          if (lane == 0)
+            sh.p[c] = -1.0 * acc;
+        }
+        __syncthreads();
+        if (t < d)
+        {
+          double acc = 0;
+          for (int j = 0; j < d; j++)
+            acc += Binv[t * k + j] * sh.p[j];
+          const double Yv = sigma * acc;
+          Binv[t * k + d] = Yv;
+          sh.Y[t]         = Yv;
+        }
+        __syncthreads();
+        for (int idx = t; idx < d * d; idx += WG)
+        {
+          const int j = idx / d, i = idx - j * d;
+          Binv[j * k + i] += 1.0 * sh.y[i] * sh.Y[j];
+        }
+        __syncthreads();
+        if (t < d)
+          Binv[d * k + t] = sh.y[t] * sigma;
+        if (t == 0)
+        {
+          Binv[d * k + d] = sigma;
+          dl[d]           = r;
+        }
+        double* drow = dpsi + (size_t)r * 4 * ld;
+#pragma unroll
+        for (int m = 0; m < MAXM; m++)
+        {
+          const int o = t + WG * m;
+          if (o < n)
+          {
+            U[(size_t)d * ld + o] = phv[m][0];
+            drow[o]               = phv[m][1];
+            drow[ld + o]          = phv[m][2];
+            drow[2 * ld + o]      = phv[m][3];
+            drow[3 * ld + o]      = phv[m][4];
+          }
+        }
+      }
+      else
+      {
+        if (t < d)
+        {
+          Binv[d * k + t] = 0.0;
+          Binv[t * k + d] = 0.0;
+        }
+        if (t == 0)
+        {
+          Binv[d * k + d] = 1.0;
+          dl[d]           = -1;
+        }
+#pragma unroll
+        for (int m = 0; m < MAXM; m++)
+        {
+          const int o = t + WG * m;
+          if (o < n)
+            U[(size_t)d * ld + o] = 0.0;
+        }
+      }
+    }
+    else if (accepted)
+    {
+      // Fahy's Sherman-Morrison update: temp = Ainv phi, temp[r] -= 1,
+      // Ainv[i][:] += (-1/ratio * temp[i]) * Ainv_old[r][:]
+#pragma unroll
+      for (int m = 0; m < MAXM; m++)
+      {
+        const int o = t + WG * m;
+        if (o < n)
+          sh.vec[o] = phv[m][0];
+      }
+      __syncthreads();
+      for (int i = wave; i < n; i += NWAVE)
+      {
+        double acc = 0;
+        for (int o = lane; o < n; o += 64)
+          acc += Ainv[(size_t)i * ld + o] * sh.vec[o];
+        acc = wave_sum(acc);
+        if (lane == 0)
+          sh.row[i] = acc;
+      }
+      __syncthreads();
+      if (t == 0)
+        sh.row[r] = sh.row[r] - 1.0;
+      __syncthreads();
+      const double c_ratio_inv = -1.0 / ratio_det;
+      for (int i = 0; i < n; i++)
+      {
+        const double ci = c_ratio_inv * sh.row[i];
+#pragma unroll
+        for (int m = 0; m < MAXM; m++)
+        {
+          const int o = t + WG * m;
+          if (o < n)
+            Ainv[(size_t)i * ld + o] += ci * ainv_r[m];
+        }
+      }
+      double* drow = dpsi + (size_t)r * 4 * ld;
+#pragma unroll
+      for (int m = 0; m < MAXM; m++)
+      {
+        const int o = t + WG * m;
+        if (o < n)
+        {
+          drow[o]          = phv[m][1];
+          drow[ld + o]     = phv[m][2];
+          drow[2 * ld + o] = phv[m][3];
+          drow[3 * ld + o] = phv[m][4];
+        }
+      }
+    }
+    if (accepted && t == 0)
+    {
+      logdet[g * 2 + 0] += log(fabs(ratio_det));
+      if (ratio_det < 0)
+        logdet[g * 2 + 1] += M_PI;
+    }
+  }
+  if (accepted)
+  {
+    if (s.has_j1 && t == 0)
+    {
+      jas[J_VAT * Np + iat]      = C[0];
+      jas[(J_G1X + 0) * Np + iat] = C[1];
+      jas[(J_G1X + 1) * Np + iat] = C[2];
+      jas[(J_G1X + 2) * Np + iat] = C[3];
+      jas[J_LAP1 * Np + iat]     = C[4];
+    }
+    if (s.has_j2)
+    {
+      const double pos[3]  = {sh.bc[BC_POS], sh.bc[BC_POS + 1], sh.bc[BC_POS + 2]};
+      const double rold[3] = {R[iat], R[Np + iat], R[2 * Np + iat]};
+      double cd2[1]        = {0.0};
+      for (int j = t; j < N; j += WG)
+      {
+        if (j == iat)
+          continue;
+        double drn[3], dro[3];
+        const double rn   = min_image(s, R[j], R[Np + j], R[2 * Np + j], pos, drn);
+        const double ro   = min_image(s, R[j], R[Np + j], R[2 * Np + j], rold, dro);
+        const FunctorC& f = s.j2[((j < s.nup) ? 0 : 1) == g ? 0 : 1];
+        double un = 0, dun = 0, d2un = 0, uo = 0, duo = 0, d2uo = 0;
+        if (rn < f.rcut)
+          functor_vgl(f, rn, un, dun, d2un);
+        if (ro < f.rcut)
+          functor_vgl(f, ro, uo, duo, d2uo);
+        const double newl = d2un + 2.0 * dun;
+        jas[J_UAT * Np + j] += un - uo;
+        jas[J_D2U * Np + j] += d2uo + 2.0 * duo - newl;
+        cd2[0] -= newl;
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+        {
+          const double newg = dun * drn[q];
+          jas[(J_DUX + q) * Np + j] -= newg - duo * dro[q];
+        }
+      }
+      block_sum<1>(cd2, sh.red, sh.bc + BC_D2U);
+      if (t == 0)
+      {
+        jas[J_UAT * Np + iat]       = C[5];
+        jas[(J_DUX + 0) * Np + iat] = C[6];
+        jas[(J_DUX + 1) * Np + iat] = C[7];
+        jas[(J_DUX + 2) * Np + iat] = C[8];
+        jas[J_D2U * Np + iat]       = sh.bc[BC_D2U];
+      }
+    }
+    __syncthreads();
+    if (t < 3)
+      R[t * Np + iat] = sh.bc[BC_POS + t];
+  }
+  __syncthreads();
+}
+
+//------------------------------------------------------------------------------------------------
+struct WalkerView
+{
+  double *R, *Ainv, *dpsi, *U, *V, *Binv, *jas, *logdet, *stats;
+  int* dl;
+};
+
+__device__ inline WalkerView walker_view(const SysC& s, const State& st, int g, int w)
+{
+  WalkerView v;
+  const int n = s.n[g], ld = s.ld, k = s.k;
+  v.R      = st.R + (size_t)w * 3 * s.Npad;
+  v.Ainv   = st.Ainv[g] + (size_t)w * n * ld;
+  v.dpsi   = st.dpsi[g] + (size_t)w * n * 4 * ld;
+  v.U      = st.U[g] + (size_t)w * k * ld;
+  v.V      = st.V[g] + (size_t)w * k * ld;
+  v.Binv   = st.Binv[g] + (size_t)w * k * k;
+  v.dl     = st.dlist[g] + (size_t)w * k;
+  v.jas    = st.jas + (size_t)w * J_ROWS * s.Npad;
+  v.logdet = st.logdet + (size_t)w * 4;
+  v.stats  = st.stats + (size_t)w * 2;
+  return v;
+}
+
+// Phase B (thread 0): proposal from the gradient at the current position.
+__device__ inline void propose(const SysC& s, int iat, int w, const MoveArgs& a, const double* __restrict__ R,
+                               const double* __restrict__ jas, MoveLDS& sh)
+{
+  const int Np = s.Npad;
+  const double tauovermass = a.tau;
+  const double sqrttau     = sqrt(tauovermass);
+  const double oneover2tau = 0.5 / tauovermass;
+  double gnow[3] = {0, 0, 0};
+  if (s.has_det)
+    for (int q = 0; q < 3; q++)
+      gnow[q] += sh.bc[BC_GRAD + q];
+  if (s.has_j1)
+    for (int q = 0; q < 3; q++)
+      gnow[q] += jas[(J_G1X + q) * Np + iat];
+  if (s.has_j2)
+    for (int q = 0; q < 3; q++)
+      gnow[q] += jas[(J_DUX + q) * Np + iat];
+  double delta[3], drift[3];
+  const double* gz = a.gauss + ((size_t)iat * a.nw + w) * 3;
+  for (int q = 0; q < 3; q++)
+    delta[q] = gz[q] * sqrttau;
+  unr_drift(tauovermass, gnow, drift);
+  for (int q = 0; q < 3; q++)
+  {
+    drift[q] += delta[q];
+    sh.bc[BC_MOVE + q] = drift[q];
+    sh.bc[BC_POS + q]  = R[q * Np + iat] + drift[q];
+  }
+  sh.bc[BC_LOGGF] = -oneover2tau * (delta[0] * delta[0] + delta[1] * delta[1] + delta[2] * delta[2]);
+}
+
+// Phase D core (thread 0): returns prob_eff; flag = |ratio|^2 >= eps.
+__device__ inline double mh_prob(const SysC& s, int iat, const MoveArgs& a, const double* __restrict__ jas,
+                                 MoveLDS& sh, double& ratio, bool& flag)
+{
+  const double tauovermass = a.tau;
+  const double oneover2tau = 0.5 / tauovermass;
+  double gnew[3];
+  combine_ratio(s, iat, jas, sh, ratio, gnew);
+  double drev[3];
+  unr_drift(tauovermass, gnew, drev);
+  for (int q = 0; q < 3; q++)
+    drev[q] += sh.bc[BC_MOVE + q];
+  const double log_gb = -oneover2tau * (drev[0] * drev[0] + drev[1] * drev[1] + drev[2] * drev[2]);
+  const double prob   = ratio * ratio;
+  flag                = prob >= DBL_EPS;
+  sh.bc[BC_RATIO]     = ratio;
+  return prob * exp(log_gb - sh.bc[BC_LOGGF]);
+}
+
+//------------------------------------------------------------------------------------------------
+// Fused sweep segment: moves [iat0, iat1) of spin group g for every walker (one walker per
+// workgroup, one RNG stream per walker), delay count d0 at iat0 with d0 + (iat1 - iat0) <= k.
+template<typename ST, int MAXM>
+__global__ __launch_bounds__(WG) void k_sweep_segment(SysC s, State st, const ST* __restrict__ coefs, MoveArgs a)
+{
+  __shared__ MoveLDS sh;
+  const int w = blockIdx.x;
+  if (w >= a.nw)
+    return;
+  const int g = a.g, n = s.n[g], ld = s.ld, k = s.k, N = s.N;
+  WalkerView v = walker_view(s, st, g, w);
+  int ucount   = a.ubase[(size_t)w * (N + 1) + a.iat0];
+  double nacc = 0, nrej = 0;
+  int d = a.d0;
+  for (int iat = a.iat0; iat < a.iat1; ++iat)
+  {
+    const int r = iat - s.first[g];
+    double ainv_r[MAXM], inv[MAXM], phv[MAXM][5];
+    if (s.has_det)
+    {
+      phaseA<MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, ainv_r, inv);
+      det_grad<MAXM>(n, ld, r, v.dpsi, inv, sh);
+    }
+    if (threadIdx.x == 0)
+      propose(s, iat, w, a, v.R, v.jas, sh);
+    __syncthreads();
+    phaseC<ST, MAXM>(s, g, iat, n, v.R, st.ions, coefs, inv, phv, sh);
+    if (threadIdx.x == 0)
+    {
+      double ratio;
+      bool flag;
+      const double pe = mh_prob(s, iat, a, v.jas, sh, ratio, flag);
+      bool acc        = false;
+      if (flag)
+      {
+        const double u = a.unif[(size_t)w * N + ucount];
+        ucount++;
+        acc = u < pe;
+      }
+      sh.bc[BC_ACC] = acc ? 1.0 : 0.0;
+      if (acc)
+        nacc += 1;
+      else
+        nrej += 1;
+      if (a.trace)
+      {
+        a.trace[((size_t)iat * a.nw + w) * 2 + 0] = ratio;
+        a.trace[((size_t)iat * a.nw + w) * 2 + 1] = acc ? 1.0 : 0.0;
+      }
+    }
+    __syncthreads();
+    phaseE<MAXM>(s, g, iat, n, ld, k, d, r, v.R, v.Ainv, v.dpsi, v.U, v.V, v.Binv, v.dl, v.jas, v.logdet, ainv_r,
+                 phv, sh);
+    if (k > 1)
+      d++;
+  }
+  if (threadIdx.x == 0)
+  {
+    a.ubase[(size_t)w * (N + 1) + a.iat1] = ucount;
+    v.stats[0] += nacc;
+    v.stats[1] += nrej;
+  }
+}
+
+// Crowd path, first kernel: phases A-C and the MH probability for one move; the proposed orbitals
+// go to st.phi and the scalars to st.mv.
+template<typename ST, int MAXM>
+__global__ __launch_bounds__(WG) void k_propose(SysC s, State st, const ST* __restrict__ coefs, MoveArgs a)
+{
+  __shared__ MoveLDS sh;
+  const int w = blockIdx.x;
+  if (w >= a.nw)
+    return;
+  const int g = a.g, n = s.n[g], ld = s.ld, k = s.k;
+  const int iat = a.iat0, r = iat - s.first[g], d = a.d0;
+  WalkerView v = walker_view(s, st, g, w);
+  double ainv_r[MAXM], inv[MAXM], phv[MAXM][5];
+  if (s.has_det)
+  {
+    phaseA<MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, ainv_r, inv);
+    det_grad<MAXM>(n, ld, r, v.dpsi, inv, sh);
+  }
+  if (threadIdx.x == 0)
+    propose(s, iat, w, a, v.R, v.jas, sh);
+  __syncthreads();
+  phaseC<ST, MAXM>(s, g, iat, n, v.R, st.ions, coefs, inv, phv, sh);
+  double* mv = st.mv + (size_t)w * MV_SIZE;
+  if (threadIdx.x == 0)
+  {
+    double ratio;
+    bool flag;
+    const double pe = mh_prob(s, iat, a, v.jas, sh, ratio, flag);
+    mv[MV_RATIO]    = ratio;
+    mv[MV_PROB]     = pe;
+    mv[MV_FLAG]     = flag ? 1.0 : 0.0;
+    for (int q = 0; q < 3; q++)
+      mv[MV_POS + q] = sh.bc[BC_POS + q];
+    for (int q = 0; q < 13; q++)
+      mv[MV_C + q] = sh.bc[BC_C + q];
+  }
+  double* phi = st.phi + (size_t)w * 5 * ld;
+#pragma unroll
+  for (int m = 0; m < MAXM; m++)
+  {
+    const int o = threadIdx.x + WG * m;
+    if (o < n)
+      for (int q = 0; q < 5; q++)
+        phi[q * ld + o] = phv[m][q];
+  }
+}
+
+// Crowd path, second kernel: Metropolis-Hastings with the crowd's shared RNG stream (uniform index =
+// crowd base + number of earlier walkers of the crowd that draw), then phase E.
+template<int MAXM>
+__global__ __launch_bounds__(WG) void k_accept_crowd(SysC s, State st, MoveArgs a)
+{
+  __shared__ MoveLDS sh;
+  const int w = blockIdx.x;
+  if (w >= a.nw)
+    return;
+  const int g = a.g, n = s.n[g], ld = s.ld, k = s.k, N = s.N;
+  const int iat = a.iat0, r = iat - s.first[g], d = a.d0;
+  const int crowd = w / a.wpc, w0 = crowd * a.wpc, w1 = min(a.nw, w0 + a.wpc);
+  WalkerView v = walker_view(s, st, g, w);
+  const double* mv = st.mv + (size_t)w * MV_SIZE;
+  // prefix count of drawing walkers before w in the crowd, and the crowd total
+  double cnt[2] = {0, 0};
+  for (int j = w0 + threadIdx.x; j < w1; j += WG)
+  {
+    const double f = st.mv[(size_t)j * MV_SIZE + MV_FLAG];
+    cnt[1] += f;
+    if (j < w)
+      cnt[0] += f;
+  }
+  block_sum<2>(cnt, sh.red, sh.bc + BC_OLD);
+  if (threadIdx.x == 0)
+  {
+    const int base = a.ubase[(size_t)crowd * (N + 1) + iat];
+    bool acc       = false;
+    if (mv[MV_FLAG] != 0.0)
+    {
+      const double u = a.unif[(size_t)crowd * N * a.wpc + base + (int)sh.bc[BC_OLD]];
+      acc            = u < mv[MV_PROB];
+    }
+    if (w == w1 - 1)
+      a.ubase[(size_t)crowd * (N + 1) + iat + 1] = base + (int)sh.bc[BC_OLD + 1];
+    sh.bc[BC_ACC] = acc ? 1.0 : 0.0;
+    v.stats[acc ? 0 : 1] += 1;
+    if (a.trace)
+    {
+      a.trace[((size_t)iat * a.nw + w) * 2 + 0] = mv[MV_RATIO];
+      a.trace[((size_t)iat * a.nw + w) * 2 + 1] = acc ? 1.0 : 0.0;
+    }
+    for (int q = 0; q < 3; q++)
+      sh.bc[BC_POS + q] = mv[MV_POS + q];
+    for (int q = 0; q < 13; q++)
+      sh.bc[BC_C + q] = mv[MV_C + q];
+  }
+  if (k > 1 && threadIdx.x < d)
+    sh.y[threadIdx.x] = v.Binv[d * k + threadIdx.x];
+  __syncthreads();
+  double ainv_r[MAXM], phv[MAXM][5];
+  const double* phi = st.phi + (size_t)w * 5 * ld;
+#pragma unroll
+  for (int m = 0; m < MAXM; m++)
+  {
+    const int o = threadIdx.x + WG * m;
+    ainv_r[m]   = o < n ? v.Ainv[(size_t)r * ld + o] : 0.0;
+    for (int q = 0; q < 5; q++)
+      phv[m][q] = o < n ? phi[q * ld + o] : 0.0;
+  }
+  __syncthreads();
+  phaseE<MAXM>(s, g, iat, n, ld, k, d, r, v.R, v.Ainv, v.dpsi, v.U, v.V, v.Binv, v.dl, v.jas, v.logdet, ainv_r, phv,
+               sh);
+}
+
+//------------------------------------------------------------------------------------------------
+// API kernels (reference call protocol, host-side Metropolis-Hastings)
+
+// mw_evalGrad: inverse row (stored to st.invRow) and total gradient at R[iat] -> out[nw][3]
+template<int MAXM>
+__global__ __launch_bounds__(WG) void k_api_eval_grad(SysC s, State st, MoveArgs a)
+{
+  __shared__ MoveLDS sh;
+  const int w = blockIdx.x;
+  if (w >= a.nw)
+    return;
+  const int g = a.g, n = s.n[g], ld = s.ld, k = s.k, Np = s.Npad;
+  const int iat = a.iat0, r = iat - s.first[g], d = a.d0;
+  WalkerView v = walker_view(s, st, g, w);
+  double ainv_r[MAXM], inv[MAXM];
+  if (s.has_det)
+  {
+    phaseA<MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, ainv_r, inv);
+    det_grad<MAXM>(n, ld, r, v.dpsi, inv, sh);
+    double* ir = st.invRow + (size_t)w * ld;
+#pragma unroll
+    for (int m = 0; m < MAXM; m++)
+    {
+      const int o = threadIdx.x + WG * m;
+      if (o < n)
+        ir[o] = inv[m];
+    }
+  }
+  if (threadIdx.x == 0)
+  {
+    double gnow[3] = {0, 0, 0};
+    for (int q = 0; q < 3; q++)
+    {
+      if (s.has_det)
+        gnow[q] += sh.bc[BC_GRAD + q];
+      if (s.has_j1)
+        gnow[q] += v.jas[(J_G1X + q) * Np + iat];
+      if (s.has_j2)
+        gnow[q] += v.jas[(J_DUX + q) * Np + iat];
+      a.out[(size_t)w * 3 + q] = gnow[q];
+    }
+  }
+}
+
+// mw_calcRatioGrad at the position stored by mw_makeMove (st.mv[MV_POS]); prepares the inverse
+// row first when `a.d0 >= 0` and the row is not prepared (a.flag_prepare).
+template<typename ST, int MAXM>
+__global__ __launch_bounds__(WG) void k_api_ratio_grad(SysC s, State st, const ST* __restrict__ coefs, MoveArgs a)
+{
+  __shared__ MoveLDS sh;
+  const int w = blockIdx.x;
+  if (w >= a.nw)
+    return;
+  const int g = a.g, n = s.n[g], ld = s.ld, k = s.k;
+  const int iat = a.iat0, r = iat - s.first[g], d = a.d0;
+  WalkerView v = walker_view(s, st, g, w);
+  double* mv = st.mv + (size_t)w * MV_SIZE;
+  double ainv_r[MAXM], inv[MAXM], phv[MAXM][5];
+  if (s.has_det)
+  {
+    if (a.prepare)
+    {
+      phaseA<MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, ainv_r, inv);
+      double* ir = st.invRow + (size_t)w * ld;
+#pragma unroll
+      for (int m = 0; m < MAXM; m++)
+      {
+        const int o = threadIdx.x + WG * m;
+        if (o < n)
+          ir[o] = inv[m];
+      }
+    }
+    else
+    {
+      const double* ir = st.invRow + (size_t)w * ld;
+#pragma unroll
+      for (int m = 0; m < MAXM; m++)
+      {
+        const int o = threadIdx.x + WG * m;
+        inv[m]      = o < n ? ir[o] : 0.0;
+      }
+    }
+  }
+  if (threadIdx.x < 3)
+    sh.bc[BC_POS + threadIdx.x] = mv[MV_POS + threadIdx.x];
+  __syncthreads();
+  phaseC<ST, MAXM>(s, g, iat, n, v.R, st.ions, coefs, inv, phv, sh);
+  if (threadIdx.x == 0)
+  {
+    double ratio, gnew[3];
+    combine_ratio(s, iat, v.jas, sh, ratio, gnew);
+    mv[MV_RATIO] = ratio;
+    for (int q = 0; q < 13; q++)
+      mv[MV_C + q] = sh.bc[BC_C + q];
+    a.out[w] = ratio;
+    for (int q = 0; q < 3; q++)
+      a.out[a.nw + (size_t)w * 3 + q] = gnew[q];
+  }
+  double* phi = st.phi + (size_t)w * 5 * ld;
+#pragma unroll
+  for (int m = 0; m < MAXM; m++)
+  {
+    const int o = threadIdx.x + WG * m;
+    if (o < n)
+      for (int q = 0; q < 5; q++)
+        phi[q * ld + o] = phv[m][q];
+  }
+}
+
+// mw_accept_rejectMove with host decisions a.host_acc[nw]
+template<int MAXM>
+__global__ __launch_bounds__(WG) void k_api_accept(SysC s, State st, MoveArgs a)
+{
+  __shared__ MoveLDS sh;
+  const int w = blockIdx.x;
+  if (w >= a.nw)
+    return;
+  const int g = a.g, n = s.n[g], ld = s.ld, k = s.k;
+  const int iat = a.iat0, r = iat - s.first[g], d = a.d0;
+  WalkerView v = walker_view(s, st, g, w);
+  const double* mv = st.mv + (size_t)w * MV_SIZE;
+  if (threadIdx.x == 0)
+  {
+    const bool acc = a.host_acc[w] != 0;
+    sh.bc[BC_ACC]  = acc ? 1.0 : 0.0;
+    v.stats[acc ? 0 : 1] += 1;
+    for (int q = 0; q < 3; q++)
+      sh.bc[BC_POS + q] = mv[MV_POS + q];
+    for (int q = 0; q < 13; q++)
+      sh.bc[BC_C + q] = mv[MV_C + q];
+  }
+  if (k > 1 && threadIdx.x < d)
+    sh.y[threadIdx.x] = v.Binv[d * k + threadIdx.x];
+  __syncthreads();
+  double ainv_r[MAXM], phv[MAXM][5];
+  const double* phi = st.phi + (size_t)w * 5 * ld;
+#pragma unroll
+  for (int m = 0; m < MAXM; m++)
+  {
+    const int o = threadIdx.x + WG * m;
+    ainv_r[m]   = o < n ? v.Ainv[(size_t)r * ld + o] : 0.0;
+    for (int q = 0; q < 5; q++)
+      phv[m][q] = o < n ? phi[q * ld + o] : 0.0;
+  }
+  __syncthreads();
+  phaseE<MAXM>(s, g, iat, n, ld, k, d, r, v.R, v.Ainv, v.dpsi, v.U, v.V, v.Binv, v.dl, v.jas, v.logdet, ainv_r, phv,
+               sh);
+}
+
+// mw_makeMove: proposed position R[iat] + displ
+__global__ void k_api_make_move(SysC s, State st, int iat, int nw, const double* __restrict__ displ)
+{
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nw)
+    return;
+  const double* R = st.R + (size_t)w * 3 * s.Npad;
+  double* mv      = st.mv + (size_t)w * MV_SIZE;
+  for (int q = 0; q < 3; q++)
+    mv[MV_POS + q] = R[q * s.Npad + iat] + displ[(size_t)w * 3 + q];
+}
+
+//------------------------------------------------------------------------------------------------
+// launchers
+template<typename ST>
+static hipError_t launch_move_t(int which, const SysC& s, const State& st, const ST* coefs, const MoveArgs& a,
+                                hipStream_t stream)
+{
+  const int maxn = s.n[a.g];
+  dim3 grid(a.nw), block(WG);
+  if (maxn <= WG)
+  {
+    switch (which)
+    {
+    case MK_SEGMENT: k_sweep_segment<ST, 1><<<grid, block, 0, stream>>>(s, st, coefs, a); break;
+    case MK_PROPOSE: k_propose<ST, 1><<<grid, block, 0, stream>>>(s, st, coefs, a); break;
+    case MK_ACCEPT_CROWD: k_accept_crowd<1><<<grid, block, 0, stream>>>(s, st, a); break;
+    case MK_API_GRAD: k_api_eval_grad<1><<<grid, block, 0, stream>>>(s, st, a); break;
+    case MK_API_RATIO: k_api_ratio_grad<ST, 1><<<grid, block, 0, stream>>>(s, st, coefs, a); break;
+    case MK_API_ACCEPT: k_api_accept<1><<<grid, block, 0, stream>>>(s, st, a); break;
+    default: return hipErrorInvalidValue;
+    }
+  }
+  else if (maxn <= 2 * WG)
+  {
+    switch (which)
+    {
+    case MK_SEGMENT: k_sweep_segment<ST, 2><<<grid, block, 0, stream>>>(s, st, coefs, a); break;
+    case MK_PROPOSE: k_propose<ST, 2><<<grid, block, 0, stream>>>(s, st, coefs, a); break;
+    case MK_ACCEPT_CROWD: k_accept_crowd<2><<<grid, block, 0, stream>>>(s, st, a); break;
+    case MK_API_GRAD: k_api_eval_grad<2><<<grid, block, 0, stream>>>(s, st, a); break;
+    case MK_API_RATIO: k_api_ratio_grad<ST, 2><<<grid, block, 0, stream>>>(s, st, coefs, a); break;
+    case MK_API_ACCEPT: k_api_accept<2><<<grid, block, 0, stream>>>(s, st, a); break;
+    default: return hipErrorInvalidValue;
+    }
+  }
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_move(int which, const SysC& s, const State& st, const double* coefs, const MoveArgs& a,
+                       hipStream_t stream)
+{
+  return launch_move_t<double>(which, s, st, coefs, a, stream);
+}
+
+hipError_t launch_make_move(const SysC& s, const State& st, int iat, int nw, const double* displ, hipStream_t stream)
+{
+  k_api_make_move<<<(nw + 255) / 256, 256, 0, stream>>>(s, st, iat, nw, displ);
+  return hipGetLastError();
+}
+
+} // namespace qmi

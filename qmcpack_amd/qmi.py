@@ -1,0 +1,230 @@
+"""ctypes binding of libqmcmi.so, the MI355X-native batched walker-move path.
+
+The method names mirror the reference's multi-walker API (TrialWaveFunction / ParticleSet
+mw_* functions, QMCPACK src/QMCWaveFunctions/TrialWaveFunction.cpp and
+src/Particle/ParticleSet.cpp) so tests read like the reference's own.  There is no CPU
+fallback: loading fails loudly when the HIP library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libqmcmi.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "qmcmi.h")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class QmiError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"qmcmi error {code}: {msg}")
+        self.code = code
+
+
+class SystemDesc(C.Structure):
+    """Mirror of qmi_system_desc (include/qmcmi.h)."""
+    _fields_ = [
+        ("L", C.c_double * 3),
+        ("n_up", C.c_int), ("n_dn", C.c_int),
+        ("n_ion", C.c_int), ("ions", _dp),
+        ("M", C.c_int * 3), ("norb", C.c_int), ("coefs", _dp),
+        ("j1_np", C.c_int), ("j1_params", _dp), ("j1_rcut", C.c_double), ("j1_cusp", C.c_double),
+        ("j2_np", C.c_int), ("j2_uu", _dp), ("j2_ud", _dp), ("j2_rcut", C.c_double),
+        ("j2_cusp_uu", C.c_double), ("j2_cusp_ud", C.c_double),
+        ("delay_rank", C.c_int),
+    ]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build the HIP library first (python -c "
+                          f"'import __graft_entry__ as g; g.build()')")
+    L = C.CDLL(path)
+    L.qmi_last_error.restype = C.c_char_p
+    L.qmi_last_error.argtypes = [C.c_void_p]
+    _lib = L
+    return L
+
+
+def exported_symbols_from_header(header: str = HEADER) -> list[str]:
+    import re
+    txt = open(header).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(qmi_\w+)\s*\(", txt, flags=re.M)))
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def make_desc(system, keep: list) -> SystemDesc:
+    def arr(x):
+        if x is None:
+            return None
+        a = np.ascontiguousarray(x, dtype=np.float64)
+        keep.append(a)
+        return _p(a)
+
+    d = SystemDesc()
+    for i in range(3):
+        d.L[i] = float(system.L[i])
+    d.n_up, d.n_dn = int(system.n_up), int(system.n_dn)
+    ions = np.asarray(system.ions, dtype=np.float64).reshape(-1, 3)
+    d.n_ion = ions.shape[0]
+    d.ions = arr(ions) if d.n_ion else None
+    has_det = system.coefs is not None and system.norb > 0
+    for i in range(3):
+        d.M[i] = int(system.M[i]) if has_det else 0
+    d.norb = int(system.norb) if has_det else 0
+    d.coefs = arr(system.coefs) if has_det else None
+    d.j1_np = len(system.j1_params) if system.j1_params is not None else 0
+    d.j1_params = arr(system.j1_params)
+    d.j1_rcut, d.j1_cusp = float(system.j1_rcut), float(system.j1_cusp)
+    d.j2_np = len(system.j2_uu) if system.j2_uu is not None else 0
+    d.j2_uu, d.j2_ud = arr(system.j2_uu), arr(system.j2_ud)
+    d.j2_rcut = float(system.j2_rcut)
+    d.j2_cusp_uu, d.j2_cusp_ud = float(system.j2_cusp_uu), float(system.j2_cusp_ud)
+    d.delay_rank = int(system.delay_rank)
+    return d
+
+
+class Context:
+    """One qmcmi context: one device stream, one system, one walker population."""
+
+    def __init__(self, device: int = 0):
+        self.L = load_library()
+        self.h = C.c_void_p()
+        st = self.L.qmi_ctx_create(C.byref(self.h), device, 0)
+        if st != 0:
+            raise QmiError(st, "qmi_ctx_create failed (no HIP device?)")
+        self.nw = 0
+        self.N = 0
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.L.qmi_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != 0:
+            raise QmiError(st, self.L.qmi_last_error(self.h).decode())
+
+    # ---------------------------------------------------------------- setup
+    def system_create(self, system):
+        keep = []
+        d = make_desc(system, keep)
+        self._check(self.L.qmi_system_create(self.h, C.byref(d)))
+        self.N = system.n_up + system.n_dn
+        self.n_up = system.n_up
+
+    def walkers_load(self, R, walkers_per_crowd: int = 1):
+        R = np.ascontiguousarray(R, dtype=np.float64)
+        self.nw = R.shape[0]
+        self._check(self.L.qmi_walkers_load(self.h, self.nw, _p(R), int(walkers_per_crowd)))
+
+    def walkers_get(self):
+        R = np.empty((self.nw, self.N, 3))
+        self._check(self.L.qmi_walkers_get(self.h, _p(R)))
+        return R
+
+    # ---------------------------------------------------------------- multi-walker API
+    def mw_evaluateLog(self):
+        lp, G, L = np.empty(self.nw), np.empty((self.nw, self.N, 3)), np.empty((self.nw, self.N))
+        self._check(self.L.qmi_mw_evaluate_log(self.h, _p(lp), _p(G), _p(L)))
+        return lp, G, L
+
+    def mw_evalGrad(self, iat):
+        g = np.empty((self.nw, 3))
+        self._check(self.L.qmi_mw_eval_grad(self.h, int(iat), _p(g)))
+        return g
+
+    def mw_makeMove(self, iat, displ):
+        displ = np.ascontiguousarray(displ, dtype=np.float64).reshape(self.nw, 3)
+        self._check(self.L.qmi_mw_make_move(self.h, int(iat), _p(displ)))
+
+    def mw_calcRatioGrad(self, iat):
+        r, g = np.empty(self.nw), np.empty((self.nw, 3))
+        self._check(self.L.qmi_mw_ratio_grad(self.h, int(iat), _p(r), _p(g)))
+        return r, g
+
+    def mw_accept_rejectMove(self, iat, accepted, safe_to_delay=True):
+        acc = np.ascontiguousarray(np.asarray(accepted, dtype=np.int32).reshape(self.nw))
+        self._check(self.L.qmi_mw_accept_reject(self.h, int(iat), acc.ctypes.data_as(_ip), int(bool(safe_to_delay))))
+
+    def mw_completeUpdates(self):
+        self._check(self.L.qmi_mw_complete_updates(self.h))
+
+    def mw_evaluateGL(self, fromscratch=False):
+        lp, G, L, ek = (np.empty(self.nw), np.empty((self.nw, self.N, 3)), np.empty((self.nw, self.N)),
+                        np.empty(self.nw))
+        self._check(self.L.qmi_mw_evaluate_gl(self.h, int(bool(fromscratch)), _p(lp), _p(G), _p(L), _p(ek)))
+        return lp, G, L, ek
+
+    def mw_evaluateVGL(self, pos, n):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.nw, 3)
+        out = np.empty((self.nw, 5, n))
+        self._check(self.L.qmi_mw_spo_vgl(self.h, _p(pos), int(n), _p(out)))
+        return out
+
+    def mw_evaluateDetRatios(self, pos, inv_row):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        nvp = pos.shape[1]
+        inv_row = np.ascontiguousarray(inv_row, dtype=np.float64)
+        n = inv_row.shape[1]
+        out = np.empty((self.nw, nvp))
+        self._check(self.L.qmi_mw_det_ratios(self.h, nvp, _p(pos), n, _p(inv_row), _p(out)))
+        return out
+
+    def mw_distanceRows(self, pos, n_ion):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.nw, 3)
+        ee_r, ee_dr = np.empty((self.nw, self.N)), np.empty((self.nw, self.N, 3))
+        ei_r, ei_dr = np.empty((self.nw, max(n_ion, 1))), np.empty((self.nw, max(n_ion, 1), 3))
+        self._check(self.L.qmi_mw_distance_rows(self.h, _p(pos), _p(ee_r), _p(ee_dr), _p(ei_r), _p(ei_dr)))
+        return ee_r, ee_dr, ei_r[:, :n_ion], ei_dr[:, :n_ion]
+
+    # ---------------------------------------------------------------- driver
+    def vmc_run(self, steps, sub_steps=1, tau=0.3, recompute_period=0, seed=1, trace=False, energies=True):
+        E = np.empty((steps, self.nw)) if energies else None
+        tr = np.empty((steps, sub_steps, self.N, self.nw, 2)) if trace else None
+        self._check(self.L.qmi_vmc_run(self.h, int(steps), int(sub_steps), C.c_double(tau), int(recompute_period),
+                                       C.c_uint32(seed), _p(E), _p(tr)))
+        return E, tr
+
+    def vmc_stats(self):
+        s = np.empty(2)
+        self._check(self.L.qmi_vmc_stats(self.h, _p(s)))
+        return dict(accepted=s[0], rejected=s[1])
+
+    # ---------------------------------------------------------------- timing
+    def timing_enable(self, on=True):
+        self._check(self.L.qmi_timing_enable(self.h, int(bool(on))))
+
+    def timing_reset(self):
+        self._check(self.L.qmi_timing_reset(self.h))
+
+    def timing(self):
+        buf = C.create_string_buffer(4096)
+        ms, calls = np.empty(64), np.empty(64)
+        nk = C.c_int()
+        self._check(self.L.qmi_timing_get(self.h, buf, 4096, _p(ms), _p(calls), 64, C.byref(nk)))
+        names = buf.value.decode().split("\n")[: nk.value]
+        return {n: dict(ms=float(ms[i]), calls=int(calls[i])) for i, n in enumerate(names)}

@@ -1,0 +1,140 @@
+"""HIP path vs the CPU FP64 oracle on identical seeded inputs (all FP64).
+
+Tolerance (from the target of this path): ratios, gradients, Laplacians and local energies
+within 1e-10 relative; accept/reject decisions identical.  Values whose relative error is
+dominated by cancellation (a ratio near a node of the wave function) are compared with an
+absolute floor scaled by the magnitude of the summed terms; the floor is stated per check.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from qmcpack_amd import qmi
+from qmcpack_amd.system import System, config_system, jastrow_params, make_walkers
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-10
+
+
+def relerr(a, b, floor=0.0):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.max(np.abs(a - b) / (np.abs(b) + floor))
+
+
+def small_periodic(n_up=5, n_dn=4, k=1, seed=3, norb=6, nion=7):
+    rng = np.random.default_rng(seed)
+    L = np.array([7.0, 8.0, 9.5])
+    M = (10, 12, 11)
+    p = jastrow_params(8)
+    return System(L=L, ions=rng.uniform(0, 1, size=(nion, 3)) * L, n_up=n_up, n_dn=n_dn, M=M, norb=norb,
+                  coefs=rng.uniform(-1, 1, size=M + (norb,)) + 0.3,
+                  j1_params=p * 0.7, j1_rcut=3.4, j1_cusp=0.0, j2_uu=p, j2_ud=p * 1.3, j2_rcut=3.4,
+                  j2_cusp_uu=-0.25, j2_cusp_ud=-0.5, delay_rank=k)
+
+
+def gpu_ctx(system, R, wpc):
+    c = qmi.Context(0)
+    c.system_create(system)
+    c.walkers_load(R, wpc)
+    return c
+
+
+def test_spo_vgl_and_distances():
+    s = small_periodic(norb=37)
+    rng = np.random.default_rng(11)
+    R = rng.uniform(0, 1, size=(4, s.N, 3)) * s.L
+    c = gpu_ctx(s, R, 1)
+    pos = rng.uniform(-10, 20, size=(4, 3))
+    out = c.mw_evaluateVGL(pos, 37)
+    for w in range(4):
+        ref = orc.spo_vgl(s.L, s.coefs, pos[w])
+        assert relerr(out[w], ref, floor=1e-3) < 1e-13
+    ee_r, ee_dr, ei_r, ei_dr = c.mw_distanceRows(pos, s.ions.shape[0])
+    for w in range(4):
+        r, dr = orc.min_image(s.L, R[w], pos[w])
+        assert np.allclose(ee_r[w], r, rtol=1e-14, atol=1e-13) and np.allclose(ee_dr[w], dr, rtol=1e-14, atol=1e-13)
+        r, dr = orc.min_image(s.L, s.ions, pos[w])
+        assert np.allclose(ei_r[w], r, rtol=1e-14, atol=1e-13) and np.allclose(ei_dr[w], dr, rtol=1e-14, atol=1e-13)
+
+
+@pytest.mark.parametrize("k", [1, 3])
+def test_evaluate_log_and_protocol(k):
+    """mw_evaluateLog, then the reference call protocol (evalGrad, makeMove, calcRatioGrad,
+    accept_rejectMove with host decisions, completeUpdates, evaluateGL) against the oracle."""
+    s = small_periodic(k=k)
+    nw = 3
+    rng = np.random.default_rng(21)
+    R = rng.uniform(0, 1, size=(nw, s.N, 3)) * s.L
+    c = gpu_ctx(s, R, 1)
+    lp, G, L = c.mw_evaluateLog()
+    o = orc.VMC(s, R, 1, 1)
+    lp_o, G_o, L_o = o.evaluate_log()
+    assert relerr(lp, lp_o) < REL
+    assert relerr(G, G_o, floor=1.0) < REL and relerr(L, L_o, floor=1.0) < REL
+    acc_pattern = [1, 0, 1, 1, 0, 1, 1, 1, 0]
+    for sweep in range(2):
+        for iat in range(s.N):
+            g = c.mw_evalGrad(iat)
+            for w in range(nw):
+                assert relerr(g[w], o.eval_grad(w, iat), floor=1.0) < REL
+            displ = rng.normal(size=(nw, 3)) * 0.5
+            c.mw_makeMove(iat, displ)
+            r, gn = c.mw_calcRatioGrad(iat)
+            acc = np.array([acc_pattern[(iat + w + sweep) % len(acc_pattern)] for w in range(nw)])
+            for w in range(nw):
+                Rw = o.get()["R"][w]
+                ro, go = o.ratio_grad(w, iat, Rw[iat] + displ[w])
+                assert relerr(r[w], ro, floor=1e-6) < REL
+                assert relerr(gn[w], go, floor=1.0) < REL
+                o.accept_reject(w, iat, bool(acc[w]))
+            c.mw_accept_rejectMove(iat, acc)
+        c.mw_completeUpdates()
+        lp, G, L, ek = c.mw_evaluateGL(fromscratch=False)
+        for w in range(nw):
+            lpo, Go, Lo = o.evaluate_gl(w)
+            assert abs(lp[w] - lpo) < REL * max(1.0, abs(lpo))
+            assert relerr(G[w], Go, floor=1.0) < REL and relerr(L[w], Lo, floor=1.0) < REL
+            assert abs(ek[w] - (-0.5 * (np.sum(Go * Go) + np.sum(Lo)))) < REL * max(1.0, abs(ek[w]))
+    assert np.allclose(c.walkers_get(), o.get()["R"], rtol=0, atol=1e-12)
+
+
+def vmc_parity(system, R, wpc, steps, sub_steps, tau, recompute_period, seed):
+    c = gpu_ctx(system, R, wpc)
+    c.mw_evaluateLog()
+    E, tr = c.vmc_run(steps, sub_steps, tau, recompute_period, seed, trace=True)
+    o = orc.VMC(system, R, wpc, seed)
+    o.evaluate_log()
+    Eo, tro = o.run(steps, sub_steps, tau, recompute_period, trace=True)
+    return c, o, E, tr, Eo, tro
+
+
+def check_vmc(c, o, E, tr, Eo, tro):
+    nmis = int(np.sum(tr[..., 1] != tro[..., 1]))
+    assert nmis == 0, f"{nmis} accept/reject decisions differ"
+    assert relerr(tr[..., 0], tro[..., 0], floor=1e-6) < REL
+    assert relerr(E, Eo, floor=1.0) < REL
+    assert np.allclose(c.walkers_get(), o.get()["R"], rtol=0, atol=1e-10)
+    st, sto = c.vmc_stats(), o.get()
+    assert st["accepted"] == sto["accepted"] and st["rejected"] == sto["rejected"]
+
+
+def test_vmc_config0_one_crowd():
+    """Workload: 16 walkers x 128 electrons, 64 orbitals per spin on a 48^3 grid, 16 ions, one crowd
+    (shared RNG stream, device Metropolis-Hastings with the reference's uniform order), delay rank 1,
+    tau 0.3, 10 sweeps."""
+    s, cfg = config_system(0)
+    R = make_walkers(s, cfg["nw"], cfg["seed"])
+    res = vmc_parity(s, R, cfg["walkers_per_crowd"], steps=cfg["sweeps"], sub_steps=1, tau=cfg["tau"],
+                     recompute_period=0, seed=cfg["seed"])
+    check_vmc(*res)
+
+
+@pytest.mark.parametrize("k,wpc", [(4, 1), (8, 3), (1, 1)])
+def test_vmc_delayed_fused(k, wpc):
+    """Delayed updates (rank k) through the fused sweep segment (one walker per crowd) and the
+    two-kernel crowd path, with sub-steps and a from-scratch recompute inside the run."""
+    s = small_periodic(n_up=12, n_dn=10, k=k, norb=14, seed=5)
+    R = np.random.default_rng(6).uniform(0, 1, size=(6, s.N, 3)) * s.L
+    res = vmc_parity(s, R, wpc, steps=3, sub_steps=2, tau=0.4, recompute_period=2, seed=77)
+    check_vmc(*res)
