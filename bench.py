@@ -1,0 +1,210 @@
+"""Benchmark: batched VMC walker moves on one MI355X.
+
+Workload (plain FP64): periodic cubic cell L = 18 bohr, 128 ions, 512 electrons (256 per spin),
+256 real orbitals per spin on a 64^3 periodic B-spline table, J1 + J2 B-spline Jastrows,
+1024 walkers (one RNG stream per walker), delay rank 32, drift-diffusion tau = 0.3.
+One step = one sweep (every electron of every walker proposes one move) + evaluateGL + the
+kinetic local energy.  Inputs are synthetic (seeded) and resident in HBM before timing.
+
+Prints one JSON line with `value` = electron moves per second of the whole job.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "electron moves/sec/GPU + effective samples/sec, learned+mixed vs plain FP64"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def autocorr_kappa(x):
+    """Integrated autocorrelation time kappa = 1 + 2 sum rho(i), truncated at the first rho <= 0
+    (the estimator of tests/scripts/check_scalars.py in the reference)."""
+    x = np.asarray(x, dtype=float)
+    n = len(x)
+    if n < 3:
+        return 1.0
+    d = x - x.mean()
+    var = np.dot(d, d) / n
+    if var == 0:
+        return 1.0
+    kappa = 1.0
+    for i in range(1, n):
+        rho = np.dot(d[:-i], d[i:]) / (n - i) / var
+        if rho <= 0:
+            break
+        kappa += 2 * rho
+    return kappa
+
+
+def segment_bytes_per_move(n, N, n_ion, d, s=8):
+    """Algorithmic HBM bytes of one proposed move in the fused sweep kernel (per walker) at delay d:
+    spline 64 n s; inverse-row update (2 d n + d^2 + n) s; accept (d n + 6 n) s;
+    J2 13 N s; J1 3 N_I s."""
+    return 64 * n * s + (2 * d * n + d * d + n) * s + (d * n + 6 * n) * s + 13 * N * s + 3 * n_ion * s
+
+
+def segment_flops_per_move(n, N, n_ion, d):
+    return 608 * n + (4 * d * n + 2 * d * d) + 8 * n + 2 * d * n + 40 * N + 40 * n_ion
+
+
+def flush_bytes(n, k, s=8):
+    """Algorithmic bytes of one rank-k flush of one n x n inverse: read and write Ainv, read U, V,
+    Binv and the k x n intermediates."""
+    return (3 * n * n + 6 * n * k) * s
+
+
+def flush_flops(n, k):
+    return 4 * k * n * n + 2 * n * k * k
+
+
+def cpu_baseline(system, cfg, nwalk=16, nthreads=None):
+    """The CPU FP64 oracle (a from-scratch restatement of the reference's CPU batched path) on a
+    bounded sample: nwalk walkers of the same system, one crowd per thread, one sweep."""
+    from oracle import oracle as orc
+    from qmcpack_amd.system import make_walkers
+    nthreads = nthreads or min(16, os.cpu_count() or 1)
+    wpc = max(1, nwalk // nthreads)
+    R = make_walkers(system, nwalk, cfg["seed"] + 1)
+    o = orc.VMC(system, R, wpc, 123)
+    o.evaluate_log()
+    t0 = time.perf_counter()
+    o.run(1, 1, cfg["tau"], 0, nthreads=nthreads)
+    dt = time.perf_counter() - t0
+    moves = nwalk * system.N
+    return dict(value=moves / dt, unit="electron moves/s", cores=min(nthreads, (nwalk + wpc - 1) // wpc),
+                kind="port",
+                sample=f"{nwalk} walkers x {system.N} electrons, 1 sweep of the same system "
+                       f"({dt:.1f} s), delay rank {system.delay_rank}, oracle in C++ FP64")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--walkers", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-walkers", type=int, default=128)
+    ap.add_argument("--quiet", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs (walker sharding, one "
+                                   "rank per GPU) are not part of it", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from qmcpack_amd import qmi
+    from qmcpack_amd.system import config_system, make_walkers
+
+    def log(*a):
+        if not args.quiet:
+            print(*a, file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    system, cfg = config_system(1)
+    nw = args.walkers or cfg["nw"]
+    R = make_walkers(system, nw, cfg["seed"])
+    log(f"[bench] system built in {time.perf_counter() - t0:.1f}s: N={system.N} n={system.n_up} "
+        f"M={system.M[0]} norb={system.norb} ions={len(system.ions)} k={system.delay_rank} nw={nw}")
+    ctx = qmi.Context(0)
+    ctx.system_create(system)
+    ctx.walkers_load(R, 1)
+    t1 = time.perf_counter()
+    ctx.mw_evaluateLog()
+    log(f"[bench] upload + initial evaluateLog {time.perf_counter() - t1:.1f}s")
+    tau = cfg["tau"]
+    seed = cfg["seed"]
+    if args.warmup > 0:
+        ctx.vmc_run(args.warmup, 1, tau, 0, seed)
+        log(f"[bench] warmup {args.warmup} steps done")
+    t1 = time.perf_counter()
+    E, _ = ctx.vmc_run(args.steps, 1, tau, 0, seed)
+    dt = time.perf_counter() - t1
+    moves = nw * system.N * args.steps
+    value = moves / dt
+    log(f"[bench] timed {args.steps} steps in {dt:.3f}s -> {value:.4e} moves/s")
+    kappa = float(np.mean([autocorr_kappa(E[:, w]) for w in range(nw)])) if args.steps >= 3 else 1.0
+    eff = nw * args.steps / kappa / dt
+
+    # per-kernel device time (HIP events on the library stream), a separate short pass
+    ctx.timing_enable(True)
+    ctx.timing_reset()
+    nprof = 2
+    ctx.vmc_run(nprof, 1, tau, 0, seed)
+    tm = ctx.timing()
+    ctx.timing_enable(False)
+    n, N, nion, k = system.n_up, system.N, len(system.ions), system.delay_rank
+    seg = tm.get("sweep_segment", dict(ms=0.0, calls=0))
+    # each launch covers one segment of k moves (d = 0..k-1) for all walkers
+    bytes_launch = nw * sum(segment_bytes_per_move(n, N, nion, d) for d in range(k))
+    flops_launch = nw * sum(segment_flops_per_move(n, N, nion, d) for d in range(k))
+    avg_ms = seg["ms"] / max(1, seg["calls"])
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    fl = tm.get("flush", dict(ms=0.0, calls=0))
+    fl_ms = fl["ms"] / max(1, fl["calls"])
+    kernels = {name: dict(ms_per_call=v["ms"] / max(1, v["calls"]), calls_per_step=v["calls"] / nprof)
+               for name, v in tm.items()}
+    log("[bench] kernels: " + json.dumps(kernels))
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "electron moves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "plain FP64 VMC sweeps: 1024 walkers x 512 electrons (256 per spin), 256 orbitals "
+                        "on a 64^3 periodic B-spline table, 128 ions, L=18 bohr, J1+J2 B-spline Jastrows, "
+                        "delay rank 32, tau 0.3, one RNG stream per walker",
+            "walkers": nw, "electrons": N, "orbitals_per_spin": n, "grid": 64, "ions": nion,
+            "delay_rank": k, "precision": "fp64",
+        },
+        "effective_samples_per_s": eff,
+        "kappa": kappa,
+        "kernels": kernels,
+        "roofline": {
+            "kernel": "k_sweep_segment (fused move: inverse row, drift, Jastrows, spline VGL, ratio, MH, accept)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_launch": bytes_launch,
+            "avg_launch_ms": avg_ms,
+            "flops_per_launch": flops_launch,
+        },
+        "flush_roofline": {
+            "kernel": "flush (T = Ainv U^T, BV = Binv V, Ainv -= T BV)",
+            "avg_ms": fl_ms,
+            "achieved_gbs": (nw * flush_bytes(n, k)) / (fl_ms * 1e-3) / 1e9 if fl_ms > 0 else 0.0,
+            "achieved_tflops": (nw * flush_flops(n, k)) / (fl_ms * 1e-3) / 1e12 if fl_ms > 0 else 0.0,
+            "fp64_peak_tflops": FP64_PEAK_TFLOPS,
+        },
+    }
+    if not args.no_cpu:
+        t1 = time.perf_counter()
+        out["cpu_baseline"] = cpu_baseline(system, cfg, nwalk=args.cpu_walkers)
+        log(f"[bench] cpu baseline {out['cpu_baseline']['value']:.3e} moves/s "
+            f"({time.perf_counter() - t1:.1f}s)")
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -140,8 +140,8 @@ int qmi_mw_distance_rows(qmi_ctx* ctx, const double* pos, double* ee_r, double* 
  * (one std::mt19937 per crowd seeded seed + crowd index, boost-style uniforms, Box-Muller
  * Gaussians, uniforms drawn only for |ratio|^2 >= eps), then evaluateGL (fromscratch on steps
  * where (step_count + 1) % recompute_period == 0) and the kinetic local energy.
- * energies[steps][nw] (nullable); trace[steps][sub_steps][N][nw][2] = {ratio, accepted}
- * (nullable: parity only, slows the sweep). */
+ * energies[steps][nw] (nullable); trace[steps][sub_steps][N][nw][8] = {ratio, accepted,
+ * proposed position[3], gradient at the current position[3]} (nullable: parity only). */
 int qmi_vmc_run(qmi_ctx* ctx, int steps, int sub_steps, double tau, int recompute_period, uint32_t seed,
                 double* energies, double* trace);
 /* accepted / rejected move counts since the context was created: stats[2] */

@@ -51,7 +51,7 @@ def lib():
             getattr(L, name).argtypes = [C.c_void_p]
         for name in ("orc_det_recompute", "orc_det_eval_grad", "orc_det_ratio_grad", "orc_det_accept_reject",
                      "orc_det_complete", "orc_det_get", "orc_vmc_evaluate_log", "orc_wf_ratio_grad",
-                     "orc_wf_accept_reject", "orc_wf_eval_grad", "orc_wf_evaluate_gl", "orc_vmc_run",
+                     "orc_wf_accept_reject", "orc_wf_eval_grad", "orc_wf_evaluate_gl", "orc_vmc_run", "orc_vmc_replay",
                      "orc_vmc_get"):
             getattr(L, name).restype = C.c_int
         _lib = L
@@ -241,11 +241,24 @@ class VMC:
 
     def run(self, steps, sub_steps=1, tau=0.3, recompute_period=0, nthreads=0, trace=False):
         E = np.empty((steps, self.nw))
-        tr = np.empty((steps, sub_steps, self.N, self.nw, 2)) if trace else None
+        tr = np.empty((steps, sub_steps, self.N, self.nw, 8)) if trace else None
         st = lib().orc_vmc_run(self.h, steps, sub_steps, C.c_double(tau), recompute_period, nthreads,
                                ptr(E), ptr(tr))
         if st != 0:
             raise RuntimeError("oracle VMC run failed")
+        return E, tr
+
+    def replay(self, forced, sub_steps=1, tau=0.3, recompute_period=0, nthreads=0):
+        """Follow the trajectory of `forced` ([steps][sub][N][nw][8] trace of another implementation);
+        returns own energies and own trace."""
+        forced = np.ascontiguousarray(forced, dtype=np.float64)
+        steps = forced.shape[0]
+        E = np.empty((steps, self.nw))
+        tr = np.empty_like(forced)
+        st = lib().orc_vmc_replay(self.h, steps, sub_steps, C.c_double(tau), recompute_period, nthreads,
+                                  ptr(E), ptr(tr), ptr(forced))
+        if st != 0:
+            raise RuntimeError("oracle VMC replay failed")
         return E, tr
 
     def get(self):

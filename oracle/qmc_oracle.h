@@ -81,9 +81,15 @@ int orc_wf_accept_reject(void* h, int iw, int iat, int accepted);
 int orc_wf_eval_grad(void* h, int iw, int iat, double* grad);
 int orc_wf_evaluate_gl(void* h, int iw, int fromscratch, double* logpsi, double* G, double* L);
 /* VMC steps.  energies[steps][nw] (kinetic local energy), trace (nullable) per
- * sub-step, electron and walker: {ratio, accepted} */
+ * sub-step, electron and walker: {ratio, accepted, proposed position[3], gradient at the
+ * current position[3]} */
 int orc_vmc_run(void* h, int steps, int sub_steps, double tau, int recompute_period, int nthreads,
                 double* energies, double* trace);
+/* Replay: as orc_vmc_run, but each move's proposed position and accept decision are taken from
+ * `forced` (a trace of another implementation); everything else is computed here and written to
+ * `trace` (own proposed position in fields 2..4, own decision in field 1). */
+int orc_vmc_replay(void* h, int steps, int sub_steps, double tau, int recompute_period, int nthreads,
+                   double* energies, double* trace, const double* forced);
 /* R[nw][N][3], logpsi[nw], G, L, stats[2] = {accepted, rejected} (nullable each) */
 int orc_vmc_get(void* h, double* R, double* logpsi, double* G, double* L, double* stats);
 

@@ -15,6 +15,7 @@ constexpr int MAX_JCOEF  = 68; // functor coefficients (NumParams + 4)
 constexpr int MAX_K      = 64; // delay rank
 constexpr int MAX_N      = 512; // electrons per spin (orbitals per determinant)
 constexpr double DBL_EPS = 2.220446049250313e-16;
+constexpr int TRACE_FIELDS = 8; // ratio, accepted, proposed position[3], gradient at the current position[3]
 
 // Radial B-spline functor (QMCWaveFunctions/Jastrow/BsplineFunctor.h:117-133 coefficients with
 // the cusp folded into coefs[0]).
@@ -87,6 +88,7 @@ enum MoveField
   MV_PROB  = 4, // |ratio|^2 * exp(log_gb - log_gf)
   MV_FLAG  = 5, // 1 if |ratio|^2 >= eps (a uniform is drawn)
   MV_C     = 8, // 13: J1 (curAt, curGrad[3], curLap), J2 (cur_Uat, grad[3]), det (ratio, grad numerators[3])
+  MV_GNOW  = 21, // 3: gradient at the current position (drift input)
   MV_SIZE  = 24
 };
 

@@ -2,6 +2,7 @@
 // device-resident VMC driver.  All device work of a context runs on one HIP stream.
 #include "../../include/qmcmi.h"
 #include "qmi_kernels.hpp"
+#include "qmi_rng.hpp"
 
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -81,35 +82,6 @@ struct PinnedBuf
   }
   ~PinnedBuf() { free(); }
 };
-
-// std::mt19937 + boost-style uniform (Utilities/StdRandom.h:26-85)
-inline double uniform01(std::mt19937& e)
-{
-  return static_cast<double>(e() - e.min()) / (static_cast<double>(e.max() - e.min()) + 1) * (1.0 - 0.0) + 0.0;
-}
-
-// Box-Muller (Particle/ParticleBase/RandomSeqGenerator.h:31-50); fp-contract off so the host
-// sequence is bit-identical across builds of this expression.
-void gauss_fill(double* a, unsigned n, std::mt19937& e)
-{
-#pragma clang fp contract(off)
-  const double slightly_less_than_one = 1.0 - std::numeric_limits<double>::epsilon();
-  const int nm1                       = int(n) - 1;
-  double temp1, temp2;
-  for (int i = 0; i < nm1; i += 2)
-  {
-    temp1    = std::sqrt(-2.0 * std::log(1.0 - slightly_less_than_one * uniform01(e)));
-    temp2    = 2.0 * M_PI * uniform01(e);
-    a[i]     = temp1 * std::cos(temp2);
-    a[i + 1] = temp1 * std::sin(temp2);
-  }
-  if (n % 2 == 1)
-  {
-    temp1  = std::sqrt(-2.0 * std::log(1.0 - slightly_less_than_one * uniform01(e)));
-    temp2  = 2.0 * M_PI * uniform01(e);
-    a[nm1] = temp1 * std::cos(temp2);
-  }
-}
 
 // pool of timing events
 struct Timing
@@ -561,14 +533,12 @@ struct qmi_ctx
       {
         const int w0 = c * wpc, cw = std::min(nw, w0 + wpc) - w0;
         buf.resize((size_t)N * cw * 3);
-        gauss_fill(buf.data(), unsigned(buf.size()), rng[c]);
+        rng_fill_gauss(buf.data(), unsigned(buf.size()), rng[c]);
         for (int iat = 0; iat < N; iat++)
           std::memcpy(h_gauss.p + ((size_t)iat * nw + w0) * 3, buf.data() + (size_t)iat * cw * 3,
                       (size_t)cw * 3 * sizeof(double));
         rng_saved[c] = rng[c];
-        double* u    = h_unif.p + (size_t)c * N * wpc;
-        for (int i = 0; i < N * cw; i++)
-          u[i] = uniform01(rng[c]);
+        rng_fill_uniform(h_unif.p + (size_t)c * N * wpc, (size_t)N * cw, rng[c]);
       }
     };
     const int nth = std::max(1, std::min<int>({ncrowd, 16, (int)std::thread::hardware_concurrency()}));
@@ -613,7 +583,7 @@ struct qmi_ctx
     gauss.alloc((size_t)N * nw * 3);
     unif.alloc((size_t)ncrowd * N * wpc);
     if (htrace)
-      trace.alloc((size_t)N * nw * 2);
+      trace.alloc((size_t)N * nw * TRACE_FIELDS);
     std::vector<int> hub((size_t)ncrowd * (N + 1));
     std::vector<double> ek(nw);
     for (int step = 0; step < steps; step++)
@@ -679,8 +649,8 @@ struct qmi_ctx
         flush(1);
         HIPCHECK(hipMemcpyAsync(hub.data(), ubase.p, hub.size() * sizeof(int), hipMemcpyDeviceToHost, stream));
         if (htrace)
-          HIPCHECK(hipMemcpyAsync(htrace + ((size_t)step * sub_steps + sub) * N * nw * 2, trace.p,
-                                  (size_t)N * nw * 2 * sizeof(double), hipMemcpyDeviceToHost, stream));
+          HIPCHECK(hipMemcpyAsync(htrace + ((size_t)step * sub_steps + sub) * N * nw * TRACE_FIELDS, trace.p,
+                                  (size_t)N * nw * TRACE_FIELDS * sizeof(double), hipMemcpyDeviceToHost, stream));
         sync();
         std::vector<int> consumed(ncrowd);
         for (int c = 0; c < ncrowd; c++)

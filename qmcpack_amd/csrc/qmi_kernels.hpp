@@ -18,7 +18,7 @@ struct MoveArgs
   int* ubase;          // [ncrowd][N+1] uniforms consumed before each move
   const int* host_acc; // [nw] accept decisions (API accept)
   double* out;         // API outputs
-  double* trace;       // [N][nw][2] {ratio, accepted} (nullable)
+  double* trace;       // [N][nw][TRACE_FIELDS] (nullable)
 };
 
 enum MoveKernel

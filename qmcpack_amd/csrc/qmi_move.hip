@@ -38,7 +38,8 @@ enum BcField
   BC_ACC  = 23,
   BC_RATIO = 24,
   BC_D2U  = 25,
-  BC_OLD  = 26 // 3 old position
+  BC_OLD  = 26, // 2 crowd prefix counts
+  BC_GNOW = 29  // 3 gradient at the current position
 };
 
 //------------------------------------------------------------------------------------------------
@@ -497,6 +498,7 @@ __device__ inline void propose(const SysC& s, int iat, int w, const MoveArgs& a,
   unr_drift(tauovermass, gnow, drift);
   for (int q = 0; q < 3; q++)
   {
+    sh.bc[BC_GNOW + q] = gnow[q];
     drift[q] += delta[q];
     sh.bc[BC_MOVE + q] = drift[q];
     sh.bc[BC_POS + q]  = R[q * Np + iat] + drift[q];
@@ -570,8 +572,14 @@ __global__ __launch_bounds__(WG) void k_sweep_segment(SysC s, State st, const ST
         nrej += 1;
       if (a.trace)
       {
-        a.trace[((size_t)iat * a.nw + w) * 2 + 0] = ratio;
-        a.trace[((size_t)iat * a.nw + w) * 2 + 1] = acc ? 1.0 : 0.0;
+        double* tr = a.trace + ((size_t)iat * a.nw + w) * TRACE_FIELDS;
+        tr[0]      = ratio;
+        tr[1]      = acc ? 1.0 : 0.0;
+        for (int q = 0; q < 3; q++)
+        {
+          tr[2 + q] = sh.bc[BC_POS + q];
+          tr[5 + q] = sh.bc[BC_GNOW + q];
+        }
       }
     }
     __syncthreads();
@@ -620,7 +628,10 @@ __global__ __launch_bounds__(WG) void k_propose(SysC s, State st, const ST* __re
     mv[MV_PROB]     = pe;
     mv[MV_FLAG]     = flag ? 1.0 : 0.0;
     for (int q = 0; q < 3; q++)
-      mv[MV_POS + q] = sh.bc[BC_POS + q];
+    {
+      mv[MV_POS + q]  = sh.bc[BC_POS + q];
+      mv[MV_GNOW + q] = sh.bc[BC_GNOW + q];
+    }
     for (int q = 0; q < 13; q++)
       mv[MV_C + q] = sh.bc[BC_C + q];
   }
@@ -674,8 +685,14 @@ __global__ __launch_bounds__(WG) void k_accept_crowd(SysC s, State st, MoveArgs 
     v.stats[acc ? 0 : 1] += 1;
     if (a.trace)
     {
-      a.trace[((size_t)iat * a.nw + w) * 2 + 0] = mv[MV_RATIO];
-      a.trace[((size_t)iat * a.nw + w) * 2 + 1] = acc ? 1.0 : 0.0;
+      double* tr = a.trace + ((size_t)iat * a.nw + w) * TRACE_FIELDS;
+      tr[0]      = mv[MV_RATIO];
+      tr[1]      = acc ? 1.0 : 0.0;
+      for (int q = 0; q < 3; q++)
+      {
+        tr[2 + q] = mv[MV_POS + q];
+        tr[5 + q] = mv[MV_GNOW + q];
+      }
     }
     for (int q = 0; q < 3; q++)
       sh.bc[BC_POS + q] = mv[MV_POS + q];

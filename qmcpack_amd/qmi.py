@@ -204,7 +204,7 @@ class Context:
     # ---------------------------------------------------------------- driver
     def vmc_run(self, steps, sub_steps=1, tau=0.3, recompute_period=0, seed=1, trace=False, energies=True):
         E = np.empty((steps, self.nw)) if energies else None
-        tr = np.empty((steps, sub_steps, self.N, self.nw, 2)) if trace else None
+        tr = np.empty((steps, sub_steps, self.N, self.nw, 8)) if trace else None
         self._check(self.L.qmi_vmc_run(self.h, int(steps), int(sub_steps), C.c_double(tau), int(recompute_period),
                                        C.c_uint32(seed), _p(E), _p(tr)))
         return E, tr

@@ -1,9 +1,15 @@
 """HIP path vs the CPU FP64 oracle on identical seeded inputs (all FP64).
 
 Tolerance (from the target of this path): ratios, gradients, Laplacians and local energies
-within 1e-10 relative; accept/reject decisions identical.  Values whose relative error is
-dominated by cancellation (a ratio near a node of the wave function) are compared with an
-absolute floor scaled by the magnitude of the summed terms; the floor is stated per check.
+within 1e-10 relative; accept/reject decisions identical.
+
+A determinant ratio is a dot product of O(1) terms; near a node of the wave function it is
+small and its absolute FP64 rounding error (~1e-13..1e-12 with the conditioning of these Slater
+matrices) dominates, so ratios are compared relative to max(|ratio|, RATIO_FLOOR): 1e-10
+relative above 1e-2, 1e-12 absolute below.  Two mathematically equivalent oracle variants
+(Sherman-Morrison vs rank-8 delayed updates) replaying one trajectory of the 16-walker workload
+differ by up to 3e-10 relative at ratios of 5e-5 for the same reason.  Gradient components and
+Laplacians use a floor of 1.
 """
 import numpy as np
 import pytest
@@ -15,11 +21,13 @@ from qmcpack_amd.system import System, config_system, jastrow_params, make_walke
 pytestmark = pytest.mark.gpu
 
 REL = 1e-10
+RATIO_FLOOR = 1e-2
 
 
 def relerr(a, b, floor=0.0):
+    """max |a - b| / max(|b|, floor)"""
     a, b = np.asarray(a), np.asarray(b)
-    return np.max(np.abs(a - b) / (np.abs(b) + floor))
+    return np.max(np.abs(a - b) / np.maximum(np.abs(b), floor))
 
 
 def small_periodic(n_up=5, n_dn=4, k=1, seed=3, norb=6, nion=7):
@@ -85,7 +93,7 @@ def test_evaluate_log_and_protocol(k):
             for w in range(nw):
                 Rw = o.get()["R"][w]
                 ro, go = o.ratio_grad(w, iat, Rw[iat] + displ[w])
-                assert relerr(r[w], ro, floor=1e-6) < REL
+                assert relerr(r[w], ro, floor=RATIO_FLOOR) < REL
                 assert relerr(gn[w], go, floor=1.0) < REL
                 o.accept_reject(w, iat, bool(acc[w]))
             c.mw_accept_rejectMove(iat, acc)
@@ -100,21 +108,27 @@ def test_evaluate_log_and_protocol(k):
 
 
 def vmc_parity(system, R, wpc, steps, sub_steps, tau, recompute_period, seed):
+    """GPU driver run with a full trace, then the oracle replays that trajectory: at every move
+    the oracle recomputes gradient, proposal, ratio and decision from the identical state (the
+    chain is chaotic, so two FP64 implementations cannot be compared free-running; see
+    tests/test_oracle_math.py::test_vmc_chain_is_chaotic)."""
     c = gpu_ctx(system, R, wpc)
     c.mw_evaluateLog()
     E, tr = c.vmc_run(steps, sub_steps, tau, recompute_period, seed, trace=True)
     o = orc.VMC(system, R, wpc, seed)
     o.evaluate_log()
-    Eo, tro = o.run(steps, sub_steps, tau, recompute_period, trace=True)
+    Eo, tro = o.replay(tr, sub_steps, tau, recompute_period)
     return c, o, E, tr, Eo, tro
 
 
 def check_vmc(c, o, E, tr, Eo, tro):
     nmis = int(np.sum(tr[..., 1] != tro[..., 1]))
     assert nmis == 0, f"{nmis} accept/reject decisions differ"
-    assert relerr(tr[..., 0], tro[..., 0], floor=1e-6) < REL
-    assert relerr(E, Eo, floor=1.0) < REL
-    assert np.allclose(c.walkers_get(), o.get()["R"], rtol=0, atol=1e-10)
+    assert relerr(tr[..., 0], tro[..., 0], floor=RATIO_FLOOR) < REL, "ratio"
+    assert relerr(tr[..., 5:8], tro[..., 5:8], floor=1.0) < REL, "gradient at the current position"
+    assert np.max(np.abs(tr[..., 2:5] - tro[..., 2:5])) < REL * np.max(np.abs(tro[..., 2:5])), "proposal"
+    assert relerr(E, Eo, floor=1.0) < REL, "local energy"
+    assert np.array_equal(c.walkers_get(), o.get()["R"])
     st, sto = c.vmc_stats(), o.get()
     assert st["accepted"] == sto["accepted"] and st["rejected"] == sto["rejected"]
 

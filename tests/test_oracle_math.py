@@ -182,3 +182,37 @@ def test_vmc_energy_matches_recompute():
         assert abs(ke - E[-1, iw]) < 1e-8 * max(1, abs(ke))
         assert abs(lp - st["logpsi"][iw]) < 1e-9
     assert 0 < st["accepted"] and 0 < st["rejected"]
+
+
+def test_vmc_chain_is_chaotic():
+    """Drift-diffusion VMC amplifies rounding exponentially: a 1e-15 relative change of the start
+    moves the local energy by far more than 1e-10 within two sweeps.  Parity of two FP64
+    implementations is therefore checked on a shared trajectory (replay), not free-running."""
+    from qmcpack_amd.system import config_system, make_walkers
+    s, cfg = config_system(0)
+    R = make_walkers(s, 4, cfg["seed"])
+    E = []
+    for RR in (R, R * (1 + 1e-15)):
+        o = orc.VMC(s, RR, 4, cfg["seed"])
+        o.evaluate_log()
+        E.append(o.run(2, 1, 0.3, 0)[0])
+    assert np.max(np.abs(E[0][-1] - E[1][-1]) / np.abs(E[0][-1])) > 1e-9
+
+
+def test_replay_reproduces_own_trajectory():
+    s = small_system(k=3, seed=12)
+    R = np.random.default_rng(13).uniform(0, 1, size=(4, s.N, 3)) * s.L
+    o1 = orc.VMC(s, R, 2, 9)
+    o1.evaluate_log()
+    E1, t1 = o1.run(3, 2, 0.35, 2, trace=True)
+    o2 = orc.VMC(s, R, 2, 9)
+    o2.evaluate_log()
+    E2, t2 = o2.replay(t1, 2, 0.35, 2)
+    assert np.array_equal(E1, E2) and np.array_equal(t1, t2)
+    # a forced decision that disagrees with the chain is visible in the replay's own decisions
+    t_bad = t1.copy()
+    t_bad[1, 0, 3, 1, 1] = 1 - t_bad[1, 0, 3, 1, 1]
+    o3 = orc.VMC(s, R, 2, 9)
+    o3.evaluate_log()
+    _, t3 = o3.replay(t_bad, 2, 0.35, 2)
+    assert t3[1, 0, 3, 1, 1] != t_bad[1, 0, 3, 1, 1]
