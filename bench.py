@@ -67,6 +67,20 @@ def flush_flops(n, k):
     return 4 * k * n * n + 2 * n * k * k
 
 
+def measured_traffic(kernel_prefix="qmi::k_sweep_segment"):
+    """HBM bytes per dispatch of the dominant kernel from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE runs, calibrated against a copy kernel of known size)."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_config1_fp64.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        d = json.load(f)
+    for name, v in d["per_dispatch"].items():
+        if name.startswith(kernel_prefix):
+            return v["read_bytes"] + v["write_bytes"], f"profiles/hbm_traffic_config1_fp64.md ({name})"
+    return None, None
+
+
 def cpu_baseline(system, cfg, nwalk=16, nthreads=None):
     """The CPU FP64 oracle (a from-scratch restatement of the reference's CPU batched path) on a
     bounded sample: nwalk walkers of the same system, one crowd per thread, one sweep."""
@@ -155,6 +169,7 @@ def main():
                for name, v in tm.items()}
     log("[bench] kernels: " + json.dumps(kernels))
 
+    traffic, traffic_src = measured_traffic() if nw == 1024 else (None, None)
     out = {
         "metric": METRIC,
         "value": value,
@@ -185,7 +200,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "bytes_per_launch": bytes_launch,
             "avg_launch_ms": avg_ms,
             "flops_per_launch": flops_launch,
