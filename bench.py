@@ -157,10 +157,16 @@ def main():
     tm = ctx.timing()
     ctx.timing_enable(False)
     n, N, nion, k = system.n_up, system.N, len(system.ions), system.delay_rank
-    seg = tm.get("sweep_segment", dict(ms=0.0, calls=0))
-    # each launch covers one segment of k moves (d = 0..k-1) for all walkers
-    bytes_launch = nw * sum(segment_bytes_per_move(n, N, nion, d) for d in range(k))
-    flops_launch = nw * sum(segment_flops_per_move(n, N, nion, d) for d in range(k))
+    # the fused move kernel runs either once per move (move_one) or once per delay window of k
+    # moves (sweep_segment); either way one sweep performs N moves per walker at delays
+    # d = 0..k-1 repeated N/k times.
+    kname = "move_one" if "move_one" in tm else "sweep_segment"
+    seg = tm.get(kname, dict(ms=0.0, calls=0))
+    sweep_bytes = nw * (N // k) * sum(segment_bytes_per_move(n, N, nion, d) for d in range(k))
+    sweep_flops = nw * (N // k) * sum(segment_flops_per_move(n, N, nion, d) for d in range(k))
+    launches_per_sweep = seg["calls"] / nprof
+    bytes_launch = sweep_bytes / max(1.0, launches_per_sweep)
+    flops_launch = sweep_flops / max(1.0, launches_per_sweep)
     avg_ms = seg["ms"] / max(1, seg["calls"])
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     fl = tm.get("flush", dict(ms=0.0, calls=0))
@@ -169,7 +175,8 @@ def main():
                for name, v in tm.items()}
     log("[bench] kernels: " + json.dumps(kernels))
 
-    traffic, traffic_src = measured_traffic() if nw == 1024 else (None, None)
+    traffic, traffic_src = (measured_traffic("qmi::k_move_one" if kname == "move_one" else "qmi::k_sweep_segment")
+                            if nw == 1024 else (None, None))
     out = {
         "metric": METRIC,
         "value": value,
@@ -194,7 +201,8 @@ def main():
         "kappa": kappa,
         "kernels": kernels,
         "roofline": {
-            "kernel": "k_sweep_segment (fused move: inverse row, drift, Jastrows, spline VGL, ratio, MH, accept)",
+            "kernel": ("k_move_one" if kname == "move_one" else "k_sweep_segment")
+                      + " (fused move: inverse row, drift, Jastrows, spline VGL, ratio, MH, accept)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

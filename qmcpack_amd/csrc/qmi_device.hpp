@@ -55,8 +55,8 @@ struct State
   double* R;        // [nw][3][Npad]
   double* Ainv[2];  // [nw][n][ld]        rows = electrons, Ainv = (psiM^T)^-1
   double* dpsi[2];  // [nw][n][4][ld]     gx, gy, gz, lapl of orbitals at each electron
-  double* U[2];     // [nw][k][ld]
-  double* V[2];     // [nw][k][ld]
+  double* U[2];     // [nw][ld][k]  delayed orbital rows, stored transposed (orbital-major)
+  double* V[2];     // [nw][ld][k]  delayed stale inverse rows, stored transposed
   double* Binv[2];  // [nw][k][k]
   int* dlist[2];    // [nw][k]
   double* jas;      // [nw][10][Npad]  Vat, J1 gx gy gz, J1 lap, Uat, dUat x y z, d2Uat
@@ -260,6 +260,105 @@ __device__ inline void spline_vgl_orbital(const SysC& s, const ST* __restrict__ 
       hzz * s.symGG[5];
 }
 
+// Uniform per-position spline factors staged in LDS: the 16 (i,j) products
+// {d2a_i b_j, da_i b_j, a_i b_j, da_i db_j, a_i db_j, a_i d2b_j} and the z factors c, dc, d2c.
+struct SplineLDS
+{
+  double pre[16][6];
+  double cz[3][4];
+  long long off;
+};
+
+// Fill SplineLDS from a located point (one thread; the caller synchronises before use).
+__device__ inline void spline_stage(const SplineLoc& L, SplineLDS& S)
+{
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+    {
+      double* pr = S.pre[i * 4 + j];
+      pr[0]      = L.d2a[i] * L.b[j];
+      pr[1]      = L.da[i] * L.b[j];
+      pr[2]      = L.a[i] * L.b[j];
+      pr[3]      = L.da[i] * L.db[j];
+      pr[4]      = L.a[i] * L.db[j];
+      pr[5]      = L.a[i] * L.d2b[j];
+    }
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+  {
+    S.cz[0][q] = L.c[q];
+    S.cz[1][q] = L.dc[q];
+    S.cz[2][q] = L.d2c[q];
+  }
+  S.off = L.off;
+}
+
+// Same stencil and assign_vgl as spline_vgl_orbital, with the uniform factors read from LDS.
+template<typename ST>
+__device__ inline void spline_vgl_orbital_lds(const SysC& s, const ST* __restrict__ coefs, const SplineLDS& S, int o,
+                                              double out[5])
+{
+  double v = 0, gx = 0, gy = 0, gz = 0, hxx = 0, hxy = 0, hxz = 0, hyy = 0, hyz = 0, hzz = 0;
+  double c[4], dc[4], d2c[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+  {
+    c[q]   = S.cz[0][q];
+    dc[q]  = S.cz[1][q];
+    d2c[q] = S.cz[2][q];
+  }
+  const ST* base = coefs + S.off + o;
+  const long long zs = s.zs;
+#pragma unroll 1
+  for (int i = 0; i < 4; i++)
+  {
+    const ST* pi = base + i * s.xs;
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+    {
+      const ST* p        = pi + j * s.ys;
+      const double c0    = p[0];
+      const double c1    = p[zs];
+      const double c2    = p[2 * zs];
+      const double c3    = p[3 * zs];
+      const double* pre  = S.pre[i * 4 + j];
+      const double pre20 = pre[0], pre10 = pre[1], pre00 = pre[2], pre11 = pre[3], pre01 = pre[4], pre02 = pre[5];
+      const double sum0  = c[0] * c0 + c[1] * c1 + c[2] * c2 + c[3] * c3;
+      const double sum1  = dc[0] * c0 + dc[1] * c1 + dc[2] * c2 + dc[3] * c3;
+      const double sum2  = d2c[0] * c0 + d2c[1] * c1 + d2c[2] * c2 + d2c[3] * c3;
+      hxx += pre20 * sum0;
+      hxy += pre11 * sum0;
+      hxz += pre10 * sum1;
+      hyy += pre02 * sum0;
+      hyz += pre01 * sum1;
+      hzz += pre00 * sum2;
+      gx += pre10 * sum0;
+      gy += pre01 * sum0;
+      gz += pre00 * sum1;
+      v += pre00 * sum0;
+    }
+  }
+  const double dxInv = s.dinv[0], dyInv = s.dinv[1], dzInv = s.dinv[2];
+  gx *= dxInv;
+  gy *= dyInv;
+  gz *= dzInv;
+  hxx *= dxInv * dxInv;
+  hyy *= dyInv * dyInv;
+  hzz *= dzInv * dzInv;
+  hxy *= dxInv * dyInv;
+  hxz *= dxInv * dzInv;
+  hyz *= dyInv * dzInv;
+  const double* G = s.G;
+  out[0]          = v;
+  out[1]          = G[0] * gx + G[1] * gy + G[2] * gz;
+  out[2]          = G[3] * gx + G[4] * gy + G[5] * gz;
+  out[3]          = G[6] * gx + G[7] * gy + G[8] * gz;
+  out[4] = hxx * s.symGG[0] + hxy * s.symGG[1] + hxz * s.symGG[2] + hyy * s.symGG[3] + hyz * s.symGG[4] +
+      hzz * s.symGG[5];
+}
+
 //------------------------------------------------------------------------------------------------
 // reductions over a 256-thread workgroup
 __device__ inline double wave_sum(double v)
@@ -270,26 +369,37 @@ __device__ inline double wave_sum(double v)
   return v;
 }
 
-// Sum K per-thread values over the workgroup; the result lands in out[0..K) (LDS) for all threads.
+// Sum K (<= 16) per-thread values over the workgroup through LDS (no cross-lane shuffles):
+// every thread stores its K values, K*16 threads add 16 each, K threads add the 16 partials.
+// scratch holds K*WG doubles; the result lands in out[0..K) (LDS) for all threads.
 template<int K>
-__device__ inline void block_sum(double (&v)[K], double* scratch /*[NWAVE*K]*/, double* out)
+__device__ inline void block_sum(double (&v)[K], double* scratch, double* out)
 {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  static_assert(K * 16 <= WG, "block_sum handles at most 16 values");
+  const int t = threadIdx.x;
 #pragma unroll
   for (int k = 0; k < K; k++)
-    v[k] = wave_sum(v[k]);
-  if (lane == 0)
-#pragma unroll
-    for (int k = 0; k < K; k++)
-      scratch[wave * K + k] = v[k];
+    scratch[k * WG + t] = v[k];
   __syncthreads();
-  if (threadIdx.x < K)
+  double s = 0;
+  if (t < K * 16)
   {
-    double s = 0;
+    const double* src = scratch + (t >> 4) * WG + (t & 15) * 16;
 #pragma unroll
-    for (int w = 0; w < NWAVE; w++)
-      s += scratch[w * K + threadIdx.x];
-    out[threadIdx.x] = s;
+    for (int i = 0; i < 16; i++)
+      s += src[i];
+  }
+  __syncthreads();
+  if (t < K * 16)
+    scratch[t] = s;
+  __syncthreads();
+  if (t < K)
+  {
+    double r = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+      r += scratch[t * 16 + i];
+    out[t] = r;
   }
   __syncthreads();
 }

@@ -1,9 +1,10 @@
 // Delayed-update flush (DelayedUpdateBatched::mw_updateInvMat, DelayedUpdateBatched.h:675-738):
-//   T   = Ainv U^T            (n x d)
-//   T[delay_list[c]][c] -= 1  (applyW, detail/OMPTarget/AccelMatrixUpdateOMPTarget.hpp:167-189)
-//   BV  = Binv V              (d x n, written over U)
-//   Ainv -= T BV              (n x n)
-// LDS-tiled FP64 register-blocked contractions; one workgroup per (64-row block, walker).
+//   T    = Ainv U^T             (n x d)
+//   T[delay_list[c]][c] -= 1    (applyW, detail/OMPTarget/AccelMatrixUpdateOMPTarget.hpp:167-189)
+//   BV   = Binv V               (d x n; stored orbital-major as BV_T[o][c], written over U_T)
+//   Ainv -= T BV                (n x n)
+// The delayed rows are stored orbital-major (U_T[o][c], V_T[o][c]).  LDS-tiled FP64 register
+// blocking; one workgroup per (64-row block, walker).
 #include "qmi_kernels.hpp"
 
 namespace qmi
@@ -11,17 +12,16 @@ namespace qmi
 constexpr int FT_ROWS = 64; // rows of Ainv per workgroup
 constexpr int FT_KC   = 32; // contraction chunk
 
-// T = Ainv U^T - W for rows [i0, i0+64) of walker w.  Thread (row = t/4, cq = t%4) owns columns
-// c = cq + 4 q.
+// T = Ainv U^T - W for rows [i0, i0+64).  Thread (row = t/4, cq = t%4) owns columns c = cq + 4 q.
 __global__ __launch_bounds__(WG) void k_flush_T(SysC s, State st, int g, int d, double* __restrict__ T)
 {
   __shared__ double As[FT_ROWS][FT_KC + 1];
-  __shared__ double Us[MAX_K][FT_KC + 1];
+  __shared__ double Us[FT_KC][MAX_K + 1];
   const int w = blockIdx.y, i0 = blockIdx.x * FT_ROWS;
   const int n = s.n[g], ld = s.ld, k = s.k;
   const int t = threadIdx.x, row = t >> 2, cq = t & 3;
   const double* A = st.Ainv[g] + (size_t)w * n * ld;
-  const double* U = st.U[g] + (size_t)w * k * ld;
+  const double* U = st.U[g] + (size_t)w * ld * k;
   double acc[MAX_K / 4];
 #pragma unroll
   for (int q = 0; q < MAX_K / 4; q++)
@@ -34,11 +34,11 @@ __global__ __launch_bounds__(WG) void k_flush_T(SysC s, State st, int g, int d, 
       const int i = i0 + rr, j = j0 + jj;
       As[rr][jj] = (i < n && j < n) ? A[(size_t)i * ld + j] : 0.0;
     }
-    for (int e = t; e < d * FT_KC; e += WG)
+    for (int e = t; e < FT_KC * d; e += WG)
     {
-      const int c = e / FT_KC, jj = e % FT_KC;
-      const int j = j0 + jj;
-      Us[c][jj] = j < n ? U[(size_t)c * ld + j] : 0.0;
+      const int jj = e / d, c = e % d;
+      const int j  = j0 + jj;
+      Us[jj][c]    = j < n ? U[(size_t)j * k + c] : 0.0;
     }
     __syncthreads();
     for (int jj = 0; jj < FT_KC; jj++)
@@ -49,7 +49,7 @@ __global__ __launch_bounds__(WG) void k_flush_T(SysC s, State st, int g, int d, 
       {
         const int c = cq + 4 * q;
         if (c < d)
-          acc[q] += a * Us[c][jj];
+          acc[q] += a * Us[jj][c];
       }
     }
     __syncthreads();
@@ -69,20 +69,21 @@ __global__ __launch_bounds__(WG) void k_flush_T(SysC s, State st, int g, int d, 
   }
 }
 
-// BV = Binv V over the first d rows, written into U.  One workgroup per walker; threads over columns.
+// BV_T[o][c] = sum_e Binv[c][e] V_T[o][e] over the first d delayed rows, written into U_T.
 __global__ __launch_bounds__(WG) void k_flush_BV(SysC s, State st, int g, int d)
 {
   __shared__ double Bs[MAX_K * MAX_K];
   const int w = blockIdx.x;
   const int n = s.n[g], ld = s.ld, k = s.k;
   const double* B = st.Binv[g] + (size_t)w * k * k;
-  const double* V = st.V[g] + (size_t)w * k * ld;
-  double* U       = st.U[g] + (size_t)w * k * ld;
+  const double* V = st.V[g] + (size_t)w * ld * k;
+  double* U       = st.U[g] + (size_t)w * ld * k;
   for (int e = threadIdx.x; e < d * d; e += WG)
     Bs[(e / d) * MAX_K + e % d] = B[(e / d) * k + e % d];
   __syncthreads();
   for (int o = threadIdx.x; o < n; o += WG)
   {
+    const double* vo = V + (size_t)o * k;
     for (int c0 = 0; c0 < d; c0 += 16)
     {
       double acc[16];
@@ -91,7 +92,7 @@ __global__ __launch_bounds__(WG) void k_flush_BV(SysC s, State st, int g, int d)
         acc[cc] = 0;
       for (int e = 0; e < d; e++)
       {
-        const double v = V[(size_t)e * ld + o];
+        const double v = vo[e];
 #pragma unroll
         for (int cc = 0; cc < 16; cc++)
           if (c0 + cc < d)
@@ -100,12 +101,12 @@ __global__ __launch_bounds__(WG) void k_flush_BV(SysC s, State st, int g, int d)
 #pragma unroll
       for (int cc = 0; cc < 16; cc++)
         if (c0 + cc < d)
-          U[(size_t)(c0 + cc) * ld + o] = acc[cc];
+          U[(size_t)o * k + c0 + cc] = acc[cc];
     }
   }
 }
 
-// Ainv[i][o] = -sum_c T[i][c] BV[c][o] + Ainv[i][o] for rows [i0, i0+64).  64x64 output tiles,
+// Ainv[i][o] = -sum_c T[i][c] BV_T[o][c] + Ainv[i][o] for rows [i0, i0+64); 64x64 output tiles,
 // 4x4 register block per thread.
 __global__ __launch_bounds__(WG) void k_flush_update(SysC s, State st, int g, int d, const double* __restrict__ T)
 {
@@ -115,7 +116,7 @@ __global__ __launch_bounds__(WG) void k_flush_update(SysC s, State st, int g, in
   const int n = s.n[g], ld = s.ld, k = s.k;
   const int t = threadIdx.x, tr = t >> 4, tc = t & 15;
   double* A        = st.Ainv[g] + (size_t)w * n * ld;
-  const double* BV = st.U[g] + (size_t)w * k * ld;
+  const double* BV = st.U[g] + (size_t)w * ld * k;
   const double* Tw = T + (size_t)w * n * k;
   for (int e = t; e < FT_ROWS * d; e += WG)
   {
@@ -125,10 +126,10 @@ __global__ __launch_bounds__(WG) void k_flush_update(SysC s, State st, int g, in
   for (int o0 = 0; o0 < n; o0 += 64)
   {
     __syncthreads();
-    for (int e = t; e < d * 64; e += WG)
+    for (int e = t; e < 64 * d; e += WG)
     {
-      const int c = e / 64, oo = e % 64;
-      Bv[c][oo]   = (o0 + oo < n) ? BV[(size_t)c * ld + o0 + oo] : 0.0;
+      const int oo = e / d, c = e % d;
+      Bv[c][oo]    = (o0 + oo < n) ? BV[(size_t)(o0 + oo) * k + c] : 0.0;
     }
     __syncthreads();
     double acc[4][4];

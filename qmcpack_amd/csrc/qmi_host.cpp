@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -169,6 +170,9 @@ struct qmi_ctx
   State st{};
   int dcount[2] = {0, 0};
   int invrow_id = -1;
+  // fused move kernel: 1 = one launch per move (k_move_one), 0 = one launch per delay window
+  // (k_sweep_segment).  QMI_MOVE_KERNEL=segment selects the latter.
+  int move_one = 1;
   long long step_count = 0;
   // RNG: one stream per crowd
   std::vector<std::mt19937> rng;
@@ -368,8 +372,8 @@ struct qmi_ctx
         const int n = s.n[g];
         Ainv[g].alloc((size_t)nw * std::max(n, 1) * ld);
         dpsi[g].alloc((size_t)nw * std::max(n, 1) * 4 * ld);
-        U[g].alloc((size_t)nw * k * ld);
-        V[g].alloc((size_t)nw * k * ld);
+        U[g].alloc((size_t)nw * ld * k);
+        V[g].alloc((size_t)nw * ld * k);
         Binv[g].alloc((size_t)nw * k * k);
         dlist[g].alloc((size_t)nw * k);
         HIPCHECK(hipMemset(U[g].p, 0, U[g].n * sizeof(double)));
@@ -612,7 +616,19 @@ struct qmi_ctx
           {
             a.g  = g;
             a.d0 = s.k > 1 ? dcount[g] : 0;
-            if (wpc == 1)
+            if (wpc == 1 && move_one)
+            {
+              a.iat0 = iat;
+              a.iat1 = iat + 1;
+              {
+                Scoped t(this, "move_one");
+                HIPCHECK(launch_move(MK_MOVE_ONE, s, st, coefs.p, a, stream));
+              }
+              iat += 1;
+              if (s.k > 1)
+                dcount[g] += 1;
+            }
+            else if (wpc == 1)
             {
               const int seg = s.k > 1 ? std::min(s.k - dcount[g], last - iat) : last - iat;
               a.iat0        = iat;
@@ -716,6 +732,8 @@ extern "C"
       delete c;
       return QMI_ERR_DEVICE;
     }
+    if (const char* mk = std::getenv("QMI_MOVE_KERNEL"))
+      c->move_one = std::string(mk) != "segment";
     if (rocblas_create_handle(&c->blas) != rocblas_status_success ||
         rocblas_set_stream(c->blas, c->stream) != rocblas_status_success)
     {

@@ -28,7 +28,8 @@ enum MoveKernel
   MK_ACCEPT_CROWD,
   MK_API_GRAD,
   MK_API_RATIO,
-  MK_API_ACCEPT
+  MK_API_ACCEPT,
+  MK_MOVE_ONE
 };
 
 hipError_t launch_move(int which, const SysC& s, const State& st, const double* coefs, const MoveArgs& a,

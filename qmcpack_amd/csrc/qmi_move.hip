@@ -24,8 +24,10 @@ struct MoveLDS
   double p[MAX_K];
   double y[MAX_K]; // row d of Binv (from phase A)
   double Y[MAX_K]; // new column d of Binv
-  double red[NWAVE * 16];
+  double part[NWAVE][64]; // per-wave partial sums of the delayed-row GEMVs
+  double red[9 * WG];     // block_sum scratch (up to 9 values)
   double bc[48];   // broadcast scalars
+  SplineLDS spl;   // uniform spline factors of the proposed position
 };
 
 enum BcField
@@ -67,16 +69,19 @@ __device__ inline void phaseA(const SysC& s, int n, int ld, int k, int d, int r,
         sh.row[o] = ainv_r[m];
     }
     __syncthreads();
-    // p = U invRow over the d delayed rows (one wave per row)
-    for (int c = wave; c < d; c += NWAVE)
+    // p[c] = U[c] . invRow over the d delayed rows: lane = c, each wave a quarter of the orbitals
+    // (U stored orbital-major, so a wave reads d contiguous values per orbital)
     {
+      const int S = (n + NWAVE - 1) / NWAVE, o0 = wave * S, o1 = min(n, o0 + S);
       double acc = 0;
-      for (int o = lane; o < n; o += 64)
-        acc += U[(size_t)c * ld + o] * sh.row[o];
-      acc = wave_sum(acc);
-      if (lane == 0)
-        sh.p[c] = acc;
+      if (lane < d)
+        for (int o = o0; o < o1; o++)
+          acc += U[(size_t)o * k + lane] * sh.row[o];
+      sh.part[wave][lane] = acc;
     }
+    __syncthreads();
+    if (t < d)
+      sh.p[t] = sh.part[0][t] + sh.part[1][t] + sh.part[2][t] + sh.part[3][t];
     __syncthreads();
     // Binv[d][i] = -sum_j Binv[j][i] p[j]
     if (t < d)
@@ -96,8 +101,9 @@ __device__ inline void phaseA(const SysC& s, int n, int ld, int k, int d, int r,
       if (o < n)
       {
         double acc = 0;
+        const double* vo = V + (size_t)o * k;
         for (int c = 0; c < d; c++)
-          acc += V[(size_t)c * ld + o] * sh.y[c];
+          acc += vo[c] * sh.y[c];
         inv[m] = 1.0 * acc + 1.0 * ainv_r[m];
       }
     }
@@ -126,7 +132,8 @@ __device__ inline void det_grad(int n, int ld, int r, const double* __restrict__
   block_sum<3>(gv, sh.red, sh.bc + BC_GRAD);
 }
 
-// Phase C: all threads; position in sh.bc[BC_POS]
+// Phase C: all threads; position in sh.bc[BC_POS].  The Jastrow sums are reduced before the
+// spline gather so that they are not live across it.
 template<typename ST, int MAXM>
 __device__ inline void phaseC(const SysC& s, int g, int iat, int n, const double* __restrict__ R,
                               const double* __restrict__ ions, const ST* __restrict__ coefs,
@@ -135,59 +142,67 @@ __device__ inline void phaseC(const SysC& s, int g, int iat, int n, const double
   const int t = threadIdx.x;
   const int N = s.N, Np = s.Npad;
   const double pos[3] = {sh.bc[BC_POS], sh.bc[BC_POS + 1], sh.bc[BC_POS + 2]};
-  double acc[13];
-#pragma unroll
-  for (int i = 0; i < 13; i++)
-    acc[i] = 0;
-  if (s.has_j1)
-    for (int j = t; j < s.nion; j += WG)
-    {
-      double dr[3];
-      const double r = min_image(s, ions[j], ions[s.nion + j], ions[2 * s.nion + j], pos, dr);
-      if (r < s.j1.rcut)
-      {
-        double u, du, d2u;
-        functor_vgl(s.j1, r, u, du, d2u);
-        acc[0] += u;
-        acc[1] += du * dr[0];
-        acc[2] += du * dr[1];
-        acc[3] += du * dr[2];
-        acc[4] += d2u + 2.0 * du;
-      }
-    }
-  if (s.has_j2)
-    for (int j = t; j < N; j += WG)
-    {
-      if (j == iat)
-        continue;
-      double dr[3];
-      const double r = min_image(s, R[j], R[Np + j], R[2 * Np + j], pos, dr);
-      const FunctorC& f = s.j2[((j < s.nup) ? 0 : 1) == g ? 0 : 1];
-      if (r < f.rcut)
-      {
-        double u, du, d2u;
-        functor_vgl(f, r, u, du, d2u);
-        acc[5] += u;
-        acc[6] += du * dr[0];
-        acc[7] += du * dr[1];
-        acc[8] += du * dr[2];
-      }
-    }
-  if (s.has_det)
+  if (s.has_det && t == 64) // locate once; the Jastrow reduction's barriers publish it
   {
     SplineLoc L;
     spline_locate(s, pos, L);
+    spline_stage(L, sh.spl);
+  }
+  {
+    double acc[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++)
+      acc[i] = 0;
+    if (s.has_j1)
+      for (int j = t; j < s.nion; j += WG)
+      {
+        double dr[3];
+        const double r = min_image(s, ions[j], ions[s.nion + j], ions[2 * s.nion + j], pos, dr);
+        if (r < s.j1.rcut)
+        {
+          double u, du, d2u;
+          functor_vgl(s.j1, r, u, du, d2u);
+          acc[0] += u;
+          acc[1] += du * dr[0];
+          acc[2] += du * dr[1];
+          acc[3] += du * dr[2];
+          acc[4] += d2u + 2.0 * du;
+        }
+      }
+    if (s.has_j2)
+      for (int j = t; j < N; j += WG)
+      {
+        if (j == iat)
+          continue;
+        double dr[3];
+        const double r = min_image(s, R[j], R[Np + j], R[2 * Np + j], pos, dr);
+        const FunctorC& f = s.j2[((j < s.nup) ? 0 : 1) == g ? 0 : 1];
+        if (r < f.rcut)
+        {
+          double u, du, d2u;
+          functor_vgl(f, r, u, du, d2u);
+          acc[5] += u;
+          acc[6] += du * dr[0];
+          acc[7] += du * dr[1];
+          acc[8] += du * dr[2];
+        }
+      }
+    block_sum<9>(acc, sh.red, sh.bc + BC_C);
+  }
+  double acc[4] = {0, 0, 0, 0};
+  if (s.has_det)
+  {
 #pragma unroll
     for (int m = 0; m < MAXM; m++)
     {
       const int o = t + WG * m;
       if (o < n)
       {
-        spline_vgl_orbital<ST>(s, coefs, L, o, phv[m]);
-        acc[9] += inv[m] * phv[m][0];
-        acc[10] += inv[m] * phv[m][1];
-        acc[11] += inv[m] * phv[m][2];
-        acc[12] += inv[m] * phv[m][3];
+        spline_vgl_orbital_lds<ST>(s, coefs, sh.spl, o, phv[m]);
+        acc[0] += inv[m] * phv[m][0];
+        acc[1] += inv[m] * phv[m][1];
+        acc[2] += inv[m] * phv[m][2];
+        acc[3] += inv[m] * phv[m][3];
       }
       else
       {
@@ -197,7 +212,7 @@ __device__ inline void phaseC(const SysC& s, int g, int iat, int n, const double
       }
     }
   }
-  block_sum<13>(acc, sh.red, sh.bc + BC_C);
+  block_sum<4>(acc, sh.red, sh.bc + BC_C + 9);
 }
 
 // Phase D helper (one thread): total ratio, gradient at the proposed position, log G_b.
@@ -253,7 +268,7 @@ __device__ inline void phaseE(const SysC& s, int g, int iat, int n, int ld, int 
       {
         const int o = t + WG * m;
         if (o < n)
-          V[(size_t)d * ld + o] = ainv_r[m];
+          V[(size_t)o * k + d] = ainv_r[m];
       }
       if (accepted)
       {
@@ -266,15 +281,17 @@ __device__ inline void phaseE(const SysC& s, int g, int iat, int n, int ld, int 
             sh.vec[o] = phv[m][0];
         }
         __syncthreads();
-        for (int c = wave; c < d; c += NWAVE)
         {
+          const int S = (n + NWAVE - 1) / NWAVE, o0 = wave * S, o1 = min(n, o0 + S);
           double acc = 0;
-          for (int o = lane; o < n; o += 64)
-            acc += V[(size_t)c * ld + o] * sh.vec[o];
-          acc = wave_sum(acc);
-          if (lane == 0)
-            sh.p[c] = -1.0 * acc;
+          if (lane < d)
+            for (int o = o0; o < o1; o++)
+              acc += V[(size_t)o * k + lane] * sh.vec[o];
+          sh.part[wave][lane] = acc;
         }
+        __syncthreads();
+        if (t < d)
+          sh.p[t] = -1.0 * (sh.part[0][t] + sh.part[1][t] + sh.part[2][t] + sh.part[3][t]);
         __syncthreads();
         if (t < d)
         {
@@ -306,7 +323,7 @@ __device__ inline void phaseE(const SysC& s, int g, int iat, int n, int ld, int 
           const int o = t + WG * m;
           if (o < n)
           {
-            U[(size_t)d * ld + o] = phv[m][0];
+            U[(size_t)o * k + d]  = phv[m][0];
             drow[o]               = phv[m][1];
             drow[ld + o]          = phv[m][2];
             drow[2 * ld + o]      = phv[m][3];
@@ -331,7 +348,7 @@ __device__ inline void phaseE(const SysC& s, int g, int iat, int n, int ld, int 
         {
           const int o = t + WG * m;
           if (o < n)
-            U[(size_t)d * ld + o] = 0.0;
+            U[(size_t)o * k + d] = 0.0;
         }
       }
     }
@@ -347,14 +364,18 @@ __device__ inline void phaseE(const SysC& s, int g, int iat, int n, int ld, int 
           sh.vec[o] = phv[m][0];
       }
       __syncthreads();
-      for (int i = wave; i < n; i += NWAVE)
+#pragma unroll
+      for (int m = 0; m < MAXM; m++)
       {
-        double acc = 0;
-        for (int o = lane; o < n; o += 64)
-          acc += Ainv[(size_t)i * ld + o] * sh.vec[o];
-        acc = wave_sum(acc);
-        if (lane == 0)
+        const int i = t + WG * m;
+        if (i < n)
+        {
+          double acc      = 0;
+          const double* ai = Ainv + (size_t)i * ld;
+          for (int o = 0; o < n; o++)
+            acc += ai[o] * sh.vec[o];
           sh.row[i] = acc;
+        }
       }
       __syncthreads();
       if (t == 0)
@@ -463,8 +484,8 @@ __device__ inline WalkerView walker_view(const SysC& s, const State& st, int g, 
   v.R      = st.R + (size_t)w * 3 * s.Npad;
   v.Ainv   = st.Ainv[g] + (size_t)w * n * ld;
   v.dpsi   = st.dpsi[g] + (size_t)w * n * 4 * ld;
-  v.U      = st.U[g] + (size_t)w * k * ld;
-  v.V      = st.V[g] + (size_t)w * k * ld;
+  v.U      = st.U[g] + (size_t)w * ld * k;
+  v.V      = st.V[g] + (size_t)w * ld * k;
   v.Binv   = st.Binv[g] + (size_t)w * k * k;
   v.dl     = st.dlist[g] + (size_t)w * k;
   v.jas    = st.jas + (size_t)w * J_ROWS * s.Npad;
@@ -526,6 +547,64 @@ __device__ inline double mh_prob(const SysC& s, int iat, const MoveArgs& a, cons
 }
 
 //------------------------------------------------------------------------------------------------
+// One fused move (phases A-E) of electron iat for walker w at delay d; the walker's uniform
+// counter is carried in ucount.
+template<typename ST, int MAXM>
+__device__ inline void fused_move(const SysC& s, const State& st, const ST* __restrict__ coefs, const MoveArgs& a,
+                                  int w, int iat, int d, MoveLDS& sh, int& ucount, double& nacc, double& nrej)
+{
+  const int g = a.g, n = s.n[g], ld = s.ld, k = s.k, N = s.N;
+  WalkerView v = walker_view(s, st, g, w);
+  const int r = iat - s.first[g];
+  double ainv_r[MAXM], inv[MAXM], phv[MAXM][5];
+#pragma unroll
+  for (int m = 0; m < MAXM; m++)
+  {
+    ainv_r[m] = inv[m] = 0.0;
+  }
+  if (s.has_det)
+  {
+    phaseA<MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, ainv_r, inv);
+    det_grad<MAXM>(n, ld, r, v.dpsi, inv, sh);
+  }
+  if (threadIdx.x == 0)
+    propose(s, iat, w, a, v.R, v.jas, sh);
+  __syncthreads();
+  phaseC<ST, MAXM>(s, g, iat, n, v.R, st.ions, coefs, inv, phv, sh);
+  if (threadIdx.x == 0)
+  {
+    double ratio;
+    bool flag;
+    const double pe = mh_prob(s, iat, a, v.jas, sh, ratio, flag);
+    bool acc        = false;
+    if (flag)
+    {
+      const double u = a.unif[(size_t)w * N + ucount];
+      ucount++;
+      acc = u < pe;
+    }
+    sh.bc[BC_ACC] = acc ? 1.0 : 0.0;
+    if (acc)
+      nacc += 1;
+    else
+      nrej += 1;
+    if (a.trace)
+    {
+      double* tr = a.trace + ((size_t)iat * a.nw + w) * TRACE_FIELDS;
+      tr[0]      = ratio;
+      tr[1]      = acc ? 1.0 : 0.0;
+      for (int q = 0; q < 3; q++)
+      {
+        tr[2 + q] = sh.bc[BC_POS + q];
+        tr[5 + q] = sh.bc[BC_GNOW + q];
+      }
+    }
+  }
+  __syncthreads();
+  phaseE<MAXM>(s, g, iat, n, ld, k, d, r, v.R, v.Ainv, v.dpsi, v.U, v.V, v.Binv, v.dl, v.jas, v.logdet, ainv_r, phv,
+               sh);
+}
+
 // Fused sweep segment: moves [iat0, iat1) of spin group g for every walker (one walker per
 // workgroup, one RNG stream per walker), delay count d0 at iat0 with d0 + (iat1 - iat0) <= k.
 template<typename ST, int MAXM>
@@ -535,64 +614,40 @@ __global__ __launch_bounds__(WG) void k_sweep_segment(SysC s, State st, const ST
   const int w = blockIdx.x;
   if (w >= a.nw)
     return;
-  const int g = a.g, n = s.n[g], ld = s.ld, k = s.k, N = s.N;
-  WalkerView v = walker_view(s, st, g, w);
-  int ucount   = a.ubase[(size_t)w * (N + 1) + a.iat0];
+  const int N = s.N;
+  int ucount  = a.ubase[(size_t)w * (N + 1) + a.iat0];
   double nacc = 0, nrej = 0;
-  int d = a.d0;
   for (int iat = a.iat0; iat < a.iat1; ++iat)
   {
-    const int r = iat - s.first[g];
-    double ainv_r[MAXM], inv[MAXM], phv[MAXM][5];
-    if (s.has_det)
-    {
-      phaseA<MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, ainv_r, inv);
-      det_grad<MAXM>(n, ld, r, v.dpsi, inv, sh);
-    }
-    if (threadIdx.x == 0)
-      propose(s, iat, w, a, v.R, v.jas, sh);
-    __syncthreads();
-    phaseC<ST, MAXM>(s, g, iat, n, v.R, st.ions, coefs, inv, phv, sh);
-    if (threadIdx.x == 0)
-    {
-      double ratio;
-      bool flag;
-      const double pe = mh_prob(s, iat, a, v.jas, sh, ratio, flag);
-      bool acc        = false;
-      if (flag)
-      {
-        const double u = a.unif[(size_t)w * N + ucount];
-        ucount++;
-        acc = u < pe;
-      }
-      sh.bc[BC_ACC] = acc ? 1.0 : 0.0;
-      if (acc)
-        nacc += 1;
-      else
-        nrej += 1;
-      if (a.trace)
-      {
-        double* tr = a.trace + ((size_t)iat * a.nw + w) * TRACE_FIELDS;
-        tr[0]      = ratio;
-        tr[1]      = acc ? 1.0 : 0.0;
-        for (int q = 0; q < 3; q++)
-        {
-          tr[2 + q] = sh.bc[BC_POS + q];
-          tr[5 + q] = sh.bc[BC_GNOW + q];
-        }
-      }
-    }
-    __syncthreads();
-    phaseE<MAXM>(s, g, iat, n, ld, k, d, r, v.R, v.Ainv, v.dpsi, v.U, v.V, v.Binv, v.dl, v.jas, v.logdet, ainv_r,
-                 phv, sh);
-    if (k > 1)
-      d++;
+    const int d = s.k > 1 ? a.d0 + (iat - a.iat0) : 0;
+    fused_move<ST, MAXM>(s, st, coefs, a, w, iat, d, sh, ucount, nacc, nrej);
   }
   if (threadIdx.x == 0)
   {
     a.ubase[(size_t)w * (N + 1) + a.iat1] = ucount;
-    v.stats[0] += nacc;
-    v.stats[1] += nrej;
+    st.stats[(size_t)w * 2 + 0] += nacc;
+    st.stats[(size_t)w * 2 + 1] += nrej;
+  }
+}
+
+// The same fused move for a single electron a.iat0 (no move loop in the kernel: the register
+// allocation of the loop version is about twice that of one move, which halves occupancy).
+template<typename ST, int MAXM>
+__global__ __launch_bounds__(WG) void k_move_one(SysC s, State st, const ST* __restrict__ coefs, MoveArgs a)
+{
+  __shared__ MoveLDS sh;
+  const int w = blockIdx.x;
+  if (w >= a.nw)
+    return;
+  const int N = s.N;
+  int ucount  = a.ubase[(size_t)w * (N + 1) + a.iat0];
+  double nacc = 0, nrej = 0;
+  fused_move<ST, MAXM>(s, st, coefs, a, w, a.iat0, a.d0, sh, ucount, nacc, nrej);
+  if (threadIdx.x == 0)
+  {
+    a.ubase[(size_t)w * (N + 1) + a.iat0 + 1] = ucount;
+    st.stats[(size_t)w * 2 + 0] += nacc;
+    st.stats[(size_t)w * 2 + 1] += nrej;
   }
 }
 
@@ -891,6 +946,7 @@ static hipError_t launch_move_t(int which, const SysC& s, const State& st, const
     switch (which)
     {
     case MK_SEGMENT: k_sweep_segment<ST, 1><<<grid, block, 0, stream>>>(s, st, coefs, a); break;
+    case MK_MOVE_ONE: k_move_one<ST, 1><<<grid, block, 0, stream>>>(s, st, coefs, a); break;
     case MK_PROPOSE: k_propose<ST, 1><<<grid, block, 0, stream>>>(s, st, coefs, a); break;
     case MK_ACCEPT_CROWD: k_accept_crowd<1><<<grid, block, 0, stream>>>(s, st, a); break;
     case MK_API_GRAD: k_api_eval_grad<1><<<grid, block, 0, stream>>>(s, st, a); break;
@@ -904,6 +960,7 @@ static hipError_t launch_move_t(int which, const SysC& s, const State& st, const
     switch (which)
     {
     case MK_SEGMENT: k_sweep_segment<ST, 2><<<grid, block, 0, stream>>>(s, st, coefs, a); break;
+    case MK_MOVE_ONE: k_move_one<ST, 2><<<grid, block, 0, stream>>>(s, st, coefs, a); break;
     case MK_PROPOSE: k_propose<ST, 2><<<grid, block, 0, stream>>>(s, st, coefs, a); break;
     case MK_ACCEPT_CROWD: k_accept_crowd<2><<<grid, block, 0, stream>>>(s, st, a); break;
     case MK_API_GRAD: k_api_eval_grad<2><<<grid, block, 0, stream>>>(s, st, a); break;

@@ -152,3 +152,13 @@ def test_vmc_delayed_fused(k, wpc):
     R = np.random.default_rng(6).uniform(0, 1, size=(6, s.N, 3)) * s.L
     res = vmc_parity(s, R, wpc, steps=3, sub_steps=2, tau=0.4, recompute_period=2, seed=77)
     check_vmc(*res)
+
+
+def test_vmc_segment_kernel(monkeypatch):
+    """The move-loop kernel (one launch per delay window) gives the same trajectory checks as the
+    default one-launch-per-move kernel."""
+    monkeypatch.setenv("QMI_MOVE_KERNEL", "segment")
+    s = small_periodic(n_up=12, n_dn=10, k=4, norb=14, seed=5)
+    R = np.random.default_rng(6).uniform(0, 1, size=(6, s.N, 3)) * s.L
+    res = vmc_parity(s, R, 1, steps=2, sub_steps=2, tau=0.4, recompute_period=2, seed=78)
+    check_vmc(*res)
