@@ -168,7 +168,7 @@ __global__ __launch_bounds__(WG) void k_eval_gl(SysC s, State st, double* __rest
 {
   __shared__ double sG[3 * 1024];
   __shared__ double sL[1024];
-  __shared__ double red[NWAVE * 4];
+  __shared__ double red[4 * WG];
   __shared__ double bc[4];
   const int w = blockIdx.x;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -308,7 +308,7 @@ __global__ __launch_bounds__(WG) void k_api_det_ratios(SysC s, const double* __r
                                                        const double* __restrict__ pos, const double* __restrict__ inv,
                                                        double* __restrict__ out)
 {
-  __shared__ double red[NWAVE];
+  __shared__ double red[WG];
   __shared__ double bc[1];
   const int v = blockIdx.x, w = blockIdx.y;
   const double* pp  = pos + ((size_t)w * nvp + v) * 3;
