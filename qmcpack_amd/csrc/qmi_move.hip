@@ -22,13 +22,79 @@ struct MoveLDS
   double row[MAX_N]; // Ainv row / SM-1 temp vector
   double vec[MAX_N]; // proposed orbital values
   double p[MAX_K];
-  double y[MAX_K]; // row d of Binv (from phase A)
+  double y[MAX_K]; // row d of Binv (from phase A), zero beyond d
   double Y[MAX_K]; // new column d of Binv
-  double part[NWAVE][64]; // per-wave partial sums of the delayed-row GEMVs
-  double red[9 * WG];     // block_sum scratch (up to 9 values)
-  double bc[48];   // broadcast scalars
-  SplineLDS spl;   // uniform spline factors of the proposed position
+  double red[9 * WG]; // block_sum / delayed_gemv scratch (up to 9 values per thread)
+  double bc[48];      // broadcast scalars
+  SplineLDS spl;      // uniform spline factors of the proposed position
 };
+
+// The leading d x d block of Binv is staged in dynamically sized LDS with row stride k + 1
+// (column reads by consecutive threads then fall in different banks).
+__host__ __device__ constexpr size_t binv_lds_bytes(int k) { return (size_t)k * (k + 1) * sizeof(double); }
+
+__device__ inline void stage_binv(const double* __restrict__ Binv, int k, int d, double* sB)
+{
+  for (int e = threadIdx.x; e < d * d; e += WG)
+  {
+    const int j = e / d, i = e - j * d;
+    sB[j * (k + 1) + i] = Binv[j * k + i];
+  }
+}
+
+// out[c] = scale * sum_o X[o][c] vec[o] for c < d over an orbital-major delayed block X (row
+// stride k).  Thread (c = t % k, q = t / k) sums orbitals o = q, q + G, ... (G = WG / k groups),
+// so consecutive threads read consecutive doubles of a row and every load of the unrolled loop is
+// independent; the G partials per column are added through LDS.  Ends with a barrier.
+__device__ inline void delayed_gemv(const double* __restrict__ X, const double* vec, int n, int k, int d,
+                                    double* red, double* out, double scale)
+{
+  const int t = threadIdx.x, G = WG / k, c = t % k, q = t / k;
+  double a0 = 0;
+  if (q < G && c < d)
+  {
+    const double* xp = X + c;
+#pragma unroll 8
+    for (int o = q; o < n; o += G)
+      a0 += xp[(size_t)o * k] * vec[o];
+  }
+  red[t] = a0;
+  __syncthreads();
+  if (t < d)
+  {
+    double s = 0;
+    for (int j = 0; j < G; j++)
+      s += red[j * k + t];
+    out[t] = scale * s;
+  }
+  __syncthreads();
+}
+
+// sum_{c<d} v[c] y[c] over one orbital-major row; y is zero-padded to k so that an even-k row can
+// be read in 16-byte pairs (the column past an odd d is a finite stored value times zero).
+__device__ inline double row_dot(const double* __restrict__ v, const double* y, int d, int k)
+{
+  double a0 = 0, a1 = 0;
+  if ((k & 1) == 0)
+  {
+    const double2* v2 = reinterpret_cast<const double2*>(v);
+    const int h       = (d + 1) >> 1;
+#pragma unroll 4
+    for (int c = 0; c < h; c++)
+    {
+      const double2 x = v2[c];
+      a0 += x.x * y[2 * c];
+      a1 += x.y * y[2 * c + 1];
+    }
+  }
+  else
+  {
+#pragma unroll 4
+    for (int c = 0; c < d; c++)
+      a0 += v[c] * y[c];
+  }
+  return a0 + a1;
+}
 
 enum BcField
 {
@@ -48,9 +114,9 @@ enum BcField
 template<int MAXM>
 __device__ inline void phaseA(const SysC& s, int n, int ld, int k, int d, int r, const double* __restrict__ Ainv,
                               const double* __restrict__ U, const double* __restrict__ V, double* __restrict__ Binv,
-                              MoveLDS& sh, double (&ainv_r)[MAXM], double (&inv)[MAXM])
+                              MoveLDS& sh, double* sB, double (&ainv_r)[MAXM], double (&inv)[MAXM])
 {
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int t = threadIdx.x;
   const double* arow = Ainv + (size_t)r * ld;
 #pragma unroll
   for (int m = 0; m < MAXM; m++)
@@ -68,44 +134,30 @@ __device__ inline void phaseA(const SysC& s, int n, int ld, int k, int d, int r,
       if (o < n)
         sh.row[o] = ainv_r[m];
     }
+    stage_binv(Binv, k, d, sB);
     __syncthreads();
-    // p[c] = U[c] . invRow over the d delayed rows: lane = c, each wave a quarter of the orbitals
-    // (U stored orbital-major, so a wave reads d contiguous values per orbital)
-    {
-      const int S = (n + NWAVE - 1) / NWAVE, o0 = wave * S, o1 = min(n, o0 + S);
-      double acc = 0;
-      if (lane < d)
-        for (int o = o0; o < o1; o++)
-          acc += U[(size_t)o * k + lane] * sh.row[o];
-      sh.part[wave][lane] = acc;
-    }
-    __syncthreads();
-    if (t < d)
-      sh.p[t] = sh.part[0][t] + sh.part[1][t] + sh.part[2][t] + sh.part[3][t];
-    __syncthreads();
+    // p = U invRow over the d delayed rows
+    delayed_gemv(U, sh.row, n, k, d, sh.red, sh.p, 1.0);
     // Binv[d][i] = -sum_j Binv[j][i] p[j]
-    if (t < d)
+    if (t < k)
     {
       double acc = 0;
-      for (int j = 0; j < d; j++)
-        acc += Binv[j * k + t] * sh.p[j];
+      if (t < d)
+        for (int j = 0; j < d; j++)
+          acc += sB[j * (k + 1) + t] * sh.p[j];
       const double y = -1.0 * acc;
-      sh.y[t]        = y;
-      Binv[d * k + t] = y;
+      sh.y[t]        = t < d ? y : 0.0;
+      if (t < d)
+        Binv[d * k + t] = y;
     }
     __syncthreads();
+    // invRow = Ainv[r] + V^T y
 #pragma unroll
     for (int m = 0; m < MAXM; m++)
     {
       const int o = t + WG * m;
       if (o < n)
-      {
-        double acc = 0;
-        const double* vo = V + (size_t)o * k;
-        for (int c = 0; c < d; c++)
-          acc += vo[c] * sh.y[c];
-        inv[m] = 1.0 * acc + 1.0 * ainv_r[m];
-      }
+        inv[m] = 1.0 * row_dot(V + (size_t)o * k, sh.y, d, k) + 1.0 * ainv_r[m];
     }
   }
 }
@@ -252,9 +304,9 @@ __device__ inline void phaseE(const SysC& s, int g, int iat, int n, int ld, int 
                               double* __restrict__ Ainv, double* __restrict__ dpsi, double* __restrict__ U,
                               double* __restrict__ V, double* __restrict__ Binv, int* __restrict__ dl,
                               double* __restrict__ jas, double* __restrict__ logdet, const double (&ainv_r)[MAXM],
-                              const double (&phv)[MAXM][5], MoveLDS& sh)
+                              const double (&phv)[MAXM][5], MoveLDS& sh, double* sB, bool stage)
 {
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int t = threadIdx.x;
   const int N = s.N, Np = s.Npad;
   const bool accepted = sh.bc[BC_ACC] != 0.0;
   const double* C     = sh.bc + BC_C;
@@ -280,24 +332,16 @@ __device__ inline void phaseE(const SysC& s, int g, int iat, int n, int ld, int 
           if (o < n)
             sh.vec[o] = phv[m][0];
         }
+        if (stage)
+          stage_binv(Binv, k, d, sB);
         __syncthreads();
-        {
-          const int S = (n + NWAVE - 1) / NWAVE, o0 = wave * S, o1 = min(n, o0 + S);
-          double acc = 0;
-          if (lane < d)
-            for (int o = o0; o < o1; o++)
-              acc += V[(size_t)o * k + lane] * sh.vec[o];
-          sh.part[wave][lane] = acc;
-        }
-        __syncthreads();
-        if (t < d)
-          sh.p[t] = -1.0 * (sh.part[0][t] + sh.part[1][t] + sh.part[2][t] + sh.part[3][t]);
-        __syncthreads();
+        // p = -V phi over the d delayed rows (column d of V, written above, is not included)
+        delayed_gemv(V, sh.vec, n, k, d, sh.red, sh.p, -1.0);
         if (t < d)
         {
           double acc = 0;
           for (int j = 0; j < d; j++)
-            acc += Binv[t * k + j] * sh.p[j];
+            acc += sB[t * (k + 1) + j] * sh.p[j];
           const double Yv = sigma * acc;
           Binv[t * k + d] = Yv;
           sh.Y[t]         = Yv;
@@ -306,9 +350,8 @@ __device__ inline void phaseE(const SysC& s, int g, int iat, int n, int ld, int 
         for (int idx = t; idx < d * d; idx += WG)
         {
           const int j = idx / d, i = idx - j * d;
-          Binv[j * k + i] += 1.0 * sh.y[i] * sh.Y[j];
+          Binv[j * k + i] = sB[j * (k + 1) + i] + 1.0 * sh.y[i] * sh.Y[j];
         }
-        __syncthreads();
         if (t < d)
           Binv[d * k + t] = sh.y[t] * sigma;
         if (t == 0)
@@ -551,7 +594,8 @@ __device__ inline double mh_prob(const SysC& s, int iat, const MoveArgs& a, cons
 // counter is carried in ucount.
 template<typename ST, int MAXM>
 __device__ inline void fused_move(const SysC& s, const State& st, const ST* __restrict__ coefs, const MoveArgs& a,
-                                  int w, int iat, int d, MoveLDS& sh, int& ucount, double& nacc, double& nrej)
+                                  int w, int iat, int d, MoveLDS& sh, double* sB, int& ucount, double& nacc,
+                                  double& nrej)
 {
   const int g = a.g, n = s.n[g], ld = s.ld, k = s.k, N = s.N;
   WalkerView v = walker_view(s, st, g, w);
@@ -564,7 +608,7 @@ __device__ inline void fused_move(const SysC& s, const State& st, const ST* __re
   }
   if (s.has_det)
   {
-    phaseA<MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, ainv_r, inv);
+    phaseA<MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, sB, ainv_r, inv);
     det_grad<MAXM>(n, ld, r, v.dpsi, inv, sh);
   }
   if (threadIdx.x == 0)
@@ -602,7 +646,7 @@ __device__ inline void fused_move(const SysC& s, const State& st, const ST* __re
   }
   __syncthreads();
   phaseE<MAXM>(s, g, iat, n, ld, k, d, r, v.R, v.Ainv, v.dpsi, v.U, v.V, v.Binv, v.dl, v.jas, v.logdet, ainv_r, phv,
-               sh);
+               sh, sB, false);
 }
 
 // Fused sweep segment: moves [iat0, iat1) of spin group g for every walker (one walker per
@@ -611,6 +655,7 @@ template<typename ST, int MAXM>
 __global__ __launch_bounds__(WG) void k_sweep_segment(SysC s, State st, const ST* __restrict__ coefs, MoveArgs a)
 {
   __shared__ MoveLDS sh;
+  extern __shared__ double sB[];
   const int w = blockIdx.x;
   if (w >= a.nw)
     return;
@@ -620,7 +665,7 @@ __global__ __launch_bounds__(WG) void k_sweep_segment(SysC s, State st, const ST
   for (int iat = a.iat0; iat < a.iat1; ++iat)
   {
     const int d = s.k > 1 ? a.d0 + (iat - a.iat0) : 0;
-    fused_move<ST, MAXM>(s, st, coefs, a, w, iat, d, sh, ucount, nacc, nrej);
+    fused_move<ST, MAXM>(s, st, coefs, a, w, iat, d, sh, sB, ucount, nacc, nrej);
   }
   if (threadIdx.x == 0)
   {
@@ -636,13 +681,14 @@ template<typename ST, int MAXM>
 __global__ __launch_bounds__(WG) void k_move_one(SysC s, State st, const ST* __restrict__ coefs, MoveArgs a)
 {
   __shared__ MoveLDS sh;
+  extern __shared__ double sB[];
   const int w = blockIdx.x;
   if (w >= a.nw)
     return;
   const int N = s.N;
   int ucount  = a.ubase[(size_t)w * (N + 1) + a.iat0];
   double nacc = 0, nrej = 0;
-  fused_move<ST, MAXM>(s, st, coefs, a, w, a.iat0, a.d0, sh, ucount, nacc, nrej);
+  fused_move<ST, MAXM>(s, st, coefs, a, w, a.iat0, a.d0, sh, sB, ucount, nacc, nrej);
   if (threadIdx.x == 0)
   {
     a.ubase[(size_t)w * (N + 1) + a.iat0 + 1] = ucount;
@@ -657,6 +703,7 @@ template<typename ST, int MAXM>
 __global__ __launch_bounds__(WG) void k_propose(SysC s, State st, const ST* __restrict__ coefs, MoveArgs a)
 {
   __shared__ MoveLDS sh;
+  extern __shared__ double sB[];
   const int w = blockIdx.x;
   if (w >= a.nw)
     return;
@@ -666,7 +713,7 @@ __global__ __launch_bounds__(WG) void k_propose(SysC s, State st, const ST* __re
   double ainv_r[MAXM], inv[MAXM], phv[MAXM][5];
   if (s.has_det)
   {
-    phaseA<MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, ainv_r, inv);
+    phaseA<MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, sB, ainv_r, inv);
     det_grad<MAXM>(n, ld, r, v.dpsi, inv, sh);
   }
   if (threadIdx.x == 0)
@@ -707,6 +754,7 @@ template<int MAXM>
 __global__ __launch_bounds__(WG) void k_accept_crowd(SysC s, State st, MoveArgs a)
 {
   __shared__ MoveLDS sh;
+  extern __shared__ double sB[];
   const int w = blockIdx.x;
   if (w >= a.nw)
     return;
@@ -769,7 +817,7 @@ __global__ __launch_bounds__(WG) void k_accept_crowd(SysC s, State st, MoveArgs 
   }
   __syncthreads();
   phaseE<MAXM>(s, g, iat, n, ld, k, d, r, v.R, v.Ainv, v.dpsi, v.U, v.V, v.Binv, v.dl, v.jas, v.logdet, ainv_r, phv,
-               sh);
+               sh, sB, true);
 }
 
 //------------------------------------------------------------------------------------------------
@@ -780,6 +828,7 @@ template<int MAXM>
 __global__ __launch_bounds__(WG) void k_api_eval_grad(SysC s, State st, MoveArgs a)
 {
   __shared__ MoveLDS sh;
+  extern __shared__ double sB[];
   const int w = blockIdx.x;
   if (w >= a.nw)
     return;
@@ -789,7 +838,7 @@ __global__ __launch_bounds__(WG) void k_api_eval_grad(SysC s, State st, MoveArgs
   double ainv_r[MAXM], inv[MAXM];
   if (s.has_det)
   {
-    phaseA<MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, ainv_r, inv);
+    phaseA<MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, sB, ainv_r, inv);
     det_grad<MAXM>(n, ld, r, v.dpsi, inv, sh);
     double* ir = st.invRow + (size_t)w * ld;
 #pragma unroll
@@ -822,6 +871,7 @@ template<typename ST, int MAXM>
 __global__ __launch_bounds__(WG) void k_api_ratio_grad(SysC s, State st, const ST* __restrict__ coefs, MoveArgs a)
 {
   __shared__ MoveLDS sh;
+  extern __shared__ double sB[];
   const int w = blockIdx.x;
   if (w >= a.nw)
     return;
@@ -834,7 +884,7 @@ __global__ __launch_bounds__(WG) void k_api_ratio_grad(SysC s, State st, const S
   {
     if (a.prepare)
     {
-      phaseA<MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, ainv_r, inv);
+      phaseA<MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, sB, ainv_r, inv);
       double* ir = st.invRow + (size_t)w * ld;
 #pragma unroll
       for (int m = 0; m < MAXM; m++)
@@ -886,6 +936,7 @@ template<int MAXM>
 __global__ __launch_bounds__(WG) void k_api_accept(SysC s, State st, MoveArgs a)
 {
   __shared__ MoveLDS sh;
+  extern __shared__ double sB[];
   const int w = blockIdx.x;
   if (w >= a.nw)
     return;
@@ -918,7 +969,7 @@ __global__ __launch_bounds__(WG) void k_api_accept(SysC s, State st, MoveArgs a)
   }
   __syncthreads();
   phaseE<MAXM>(s, g, iat, n, ld, k, d, r, v.R, v.Ainv, v.dpsi, v.U, v.V, v.Binv, v.dl, v.jas, v.logdet, ainv_r, phv,
-               sh);
+               sh, sB, true);
 }
 
 // mw_makeMove: proposed position R[iat] + displ
@@ -941,17 +992,18 @@ static hipError_t launch_move_t(int which, const SysC& s, const State& st, const
 {
   const int maxn = s.n[a.g];
   dim3 grid(a.nw), block(WG);
+  const size_t dyn = binv_lds_bytes(s.k);
   if (maxn <= WG)
   {
     switch (which)
     {
-    case MK_SEGMENT: k_sweep_segment<ST, 1><<<grid, block, 0, stream>>>(s, st, coefs, a); break;
-    case MK_MOVE_ONE: k_move_one<ST, 1><<<grid, block, 0, stream>>>(s, st, coefs, a); break;
-    case MK_PROPOSE: k_propose<ST, 1><<<grid, block, 0, stream>>>(s, st, coefs, a); break;
-    case MK_ACCEPT_CROWD: k_accept_crowd<1><<<grid, block, 0, stream>>>(s, st, a); break;
-    case MK_API_GRAD: k_api_eval_grad<1><<<grid, block, 0, stream>>>(s, st, a); break;
-    case MK_API_RATIO: k_api_ratio_grad<ST, 1><<<grid, block, 0, stream>>>(s, st, coefs, a); break;
-    case MK_API_ACCEPT: k_api_accept<1><<<grid, block, 0, stream>>>(s, st, a); break;
+    case MK_SEGMENT: k_sweep_segment<ST, 1><<<grid, block, dyn, stream>>>(s, st, coefs, a); break;
+    case MK_MOVE_ONE: k_move_one<ST, 1><<<grid, block, dyn, stream>>>(s, st, coefs, a); break;
+    case MK_PROPOSE: k_propose<ST, 1><<<grid, block, dyn, stream>>>(s, st, coefs, a); break;
+    case MK_ACCEPT_CROWD: k_accept_crowd<1><<<grid, block, dyn, stream>>>(s, st, a); break;
+    case MK_API_GRAD: k_api_eval_grad<1><<<grid, block, dyn, stream>>>(s, st, a); break;
+    case MK_API_RATIO: k_api_ratio_grad<ST, 1><<<grid, block, dyn, stream>>>(s, st, coefs, a); break;
+    case MK_API_ACCEPT: k_api_accept<1><<<grid, block, dyn, stream>>>(s, st, a); break;
     default: return hipErrorInvalidValue;
     }
   }
@@ -959,13 +1011,13 @@ static hipError_t launch_move_t(int which, const SysC& s, const State& st, const
   {
     switch (which)
     {
-    case MK_SEGMENT: k_sweep_segment<ST, 2><<<grid, block, 0, stream>>>(s, st, coefs, a); break;
-    case MK_MOVE_ONE: k_move_one<ST, 2><<<grid, block, 0, stream>>>(s, st, coefs, a); break;
-    case MK_PROPOSE: k_propose<ST, 2><<<grid, block, 0, stream>>>(s, st, coefs, a); break;
-    case MK_ACCEPT_CROWD: k_accept_crowd<2><<<grid, block, 0, stream>>>(s, st, a); break;
-    case MK_API_GRAD: k_api_eval_grad<2><<<grid, block, 0, stream>>>(s, st, a); break;
-    case MK_API_RATIO: k_api_ratio_grad<ST, 2><<<grid, block, 0, stream>>>(s, st, coefs, a); break;
-    case MK_API_ACCEPT: k_api_accept<2><<<grid, block, 0, stream>>>(s, st, a); break;
+    case MK_SEGMENT: k_sweep_segment<ST, 2><<<grid, block, dyn, stream>>>(s, st, coefs, a); break;
+    case MK_MOVE_ONE: k_move_one<ST, 2><<<grid, block, dyn, stream>>>(s, st, coefs, a); break;
+    case MK_PROPOSE: k_propose<ST, 2><<<grid, block, dyn, stream>>>(s, st, coefs, a); break;
+    case MK_ACCEPT_CROWD: k_accept_crowd<2><<<grid, block, dyn, stream>>>(s, st, a); break;
+    case MK_API_GRAD: k_api_eval_grad<2><<<grid, block, dyn, stream>>>(s, st, a); break;
+    case MK_API_RATIO: k_api_ratio_grad<ST, 2><<<grid, block, dyn, stream>>>(s, st, coefs, a); break;
+    case MK_API_ACCEPT: k_api_accept<2><<<grid, block, dyn, stream>>>(s, st, a); break;
     default: return hipErrorInvalidValue;
     }
   }
