@@ -132,25 +132,32 @@ __device__ inline double min_image(const SysC& s, double sx, double sy, double s
   return sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
 }
 
-// BsplineFunctor::evaluateVGL element (BsplineFunctor.h:771-838): u, du/dr / r, d2u/dr2.
-__device__ inline void functor_vgl(const FunctorC& f, double r, double& u, double& du, double& d2u)
+// BsplineFunctor::evaluateVGL element (BsplineFunctor.h:771-838): u, du/dr / r, d2u/dr2, with the
+// coefficient table at c (kernel argument or an LDS copy).
+__device__ inline void functor_vgl_c(const double* c, double DeltaRInv, int max_index, double r, double& u,
+                                     double& du, double& d2u)
 {
-  const double dSquareDeltaRinv = f.DeltaRInv * f.DeltaRInv;
+  const double dSquareDeltaRinv = DeltaRInv * DeltaRInv;
   const double rinv             = 1.0 / r;
-  r *= f.DeltaRInv;
+  r *= DeltaRInv;
   int i;
   double t;
-  spline_bound(r, f.max_index, i, t);
-  const double s0 = f.c[i], s1 = f.c[i + 1], s2 = f.c[i + 2], s3 = f.c[i + 3];
+  spline_bound(r, max_index, i, t);
+  const double s0 = c[i], s1 = c[i + 1], s2 = c[i + 2], s3 = c[i + 3];
   d2u = dSquareDeltaRinv *
       (s0 * (-1.0 * t + 1.0) + s1 * (3.0 * t + -2.0) + s2 * (-3.0 * t + 1.0) + s3 * (1.0 * t + 0.0));
-  du = f.DeltaRInv * rinv *
+  du = DeltaRInv * rinv *
       (s0 * ((-0.5 * t + 1.0) * t + -0.5) + s1 * ((1.5 * t + -2.0) * t + 0.0) + s2 * ((-1.5 * t + 1.0) * t + 0.5) +
        s3 * ((0.5 * t + 0.0) * t + 0.0));
   u = (s0 * (((-1.0 / 6.0 * t + 3.0 / 6.0) * t + -3.0 / 6.0) * t + 1.0 / 6.0) +
        s1 * (((3.0 / 6.0 * t + -6.0 / 6.0) * t + 0.0 / 6.0) * t + 4.0 / 6.0) +
        s2 * (((-3.0 / 6.0 * t + 3.0 / 6.0) * t + 3.0 / 6.0) * t + 1.0 / 6.0) +
        s3 * (((1.0 / 6.0 * t + 0.0 / 6.0) * t + 0.0 / 6.0) * t + 0.0 / 6.0));
+}
+
+__device__ inline void functor_vgl(const FunctorC& f, double r, double& u, double& du, double& d2u)
+{
+  functor_vgl_c(f.c, f.DeltaRInv, f.max_index, r, u, du, d2u);
 }
 
 // spline2/MultiBsplineData.hpp:50-76

@@ -69,15 +69,16 @@ __global__ __launch_bounds__(WG) void k_flush_T(SysC s, State st, int g, int d, 
   }
 }
 
-// BV_T[o][c] = sum_e Binv[c][e] V_T[o][e] over the first d delayed rows, written into U_T.
-__global__ __launch_bounds__(WG) void k_flush_BV(SysC s, State st, int g, int d)
+// BV_T[o][c] = sum_e Binv[c][e] V_T[o][e] over the first d delayed rows, written to BVout
+// (orbital-major, row stride k, walker stride ld * k).
+__global__ __launch_bounds__(WG) void k_flush_BV(SysC s, State st, int g, int d, double* __restrict__ BVout)
 {
   __shared__ double Bs[MAX_K * MAX_K];
   const int w = blockIdx.x;
   const int n = s.n[g], ld = s.ld, k = s.k;
   const double* B = st.Binv[g] + (size_t)w * k * k;
   const double* V = st.V[g] + (size_t)w * ld * k;
-  double* U       = st.U[g] + (size_t)w * ld * k;
+  double* U       = BVout + (size_t)w * ld * k;
   for (int e = threadIdx.x; e < d * d; e += WG)
     Bs[(e / d) * MAX_K + e % d] = B[(e / d) * k + e % d];
   __syncthreads();
@@ -170,7 +171,138 @@ __global__ __launch_bounds__(WG) void k_flush_update(SysC s, State st, int g, in
   }
 }
 
-hipError_t launch_flush(const SysC& s, const State& st, int g, int d, int nw, double* scratchT, hipStream_t stream)
+// Fused flush for n <= 256 on the FP64 matrix cores.  v_mfma_f64_16x16x4f64 operand layout
+// (measured): lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]; result register v of lane l
+// holds D[l / 16 + 4 v][l % 16].  One wave owns 16 rows of Ainv and keeps them in registers for
+// both products, so Ainv is read and written once:
+//   lane (li = l % 16, q = l / 16) holds a[4 jt + v] = Ainv[row li][16 jt + 4 q + v], jt < 16.
+//   1. T (16 x d) = Ainv_rows U^T: the K index of MFMA step kk is column 16 (kk / 4) + 4 q + kk % 4,
+//      which is exactly a[kk]; then T[i][c] -= 1 where delay_list[c] == i (applyW).
+//   2. T goes through LDS to the B-operand layout, and D^T = BV_T T^T is formed per 16-column
+//      tile jt with the BV row of A-operand row m taken as column 16 jt + 4 (m % 4) + m / 4, so
+//      that result register v of lane (li, q) is Delta[row li][16 jt + 4 q + v]: the register
+//      a[4 jt + v] that it updates.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+constexpr int MF_COLS = 256; // n limit of the register-resident path
+
+__global__ __launch_bounds__(WG, 2) void k_flush_mfma(SysC s, State st, int g, int d, const double* __restrict__ BV)
+{
+  __shared__ double sT[NWAVE][16][MAX_K + 1];
+  const int w = blockIdx.y;
+  const int n = s.n[g], ld = s.ld, k = s.k;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, q = lane >> 4, li = lane & 15;
+  const int rbase = blockIdx.x * 64 + wave * 16;
+  const int row   = rbase + li;
+  const bool rok  = row < n;
+  double* A        = st.Ainv[g] + (size_t)w * n * ld;
+  const double* U  = st.U[g] + (size_t)w * ld * k;
+  const double* Bw = BV + (size_t)w * ld * k;
+  const int* dl    = st.dlist[g] + (size_t)w * k;
+  const int njt    = (n + 15) >> 4;
+
+  double a[64];
+#pragma unroll
+  for (int jt = 0; jt < 16; jt++)
+  {
+    const int j0     = jt * 16 + 4 * q;
+    const double* ap = A + (size_t)row * ld + j0;
+    if (rok && j0 + 3 < n)
+    {
+      const double2 x0 = reinterpret_cast<const double2*>(ap)[0];
+      const double2 x1 = reinterpret_cast<const double2*>(ap)[1];
+      a[4 * jt + 0]    = x0.x;
+      a[4 * jt + 1]    = x0.y;
+      a[4 * jt + 2]    = x1.x;
+      a[4 * jt + 3]    = x1.y;
+    }
+    else
+    {
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        a[4 * jt + v] = (rok && j0 + v < n) ? ap[v] : 0.0;
+    }
+  }
+
+  // 1. T = Ainv_rows U^T - W
+  const int nct = (d + 15) >> 4;
+  dbl4 acc[MAX_K / 16];
+#pragma unroll
+  for (int ct = 0; ct < MAX_K / 16; ct++)
+    acc[ct] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int kk = 0; kk < 64; kk++)
+  {
+    if ((kk >> 2) < njt)
+    {
+      const int j = (kk >> 2) * 16 + 4 * q + (kk & 3);
+#pragma unroll
+      for (int ct = 0; ct < MAX_K / 16; ct++)
+        if (ct < nct)
+        {
+          const int c    = ct * 16 + li;
+          const double b = (j < n && c < d) ? U[(size_t)j * k + c] : 0.0;
+          acc[ct]        = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk], b, acc[ct], 0, 0, 0);
+        }
+    }
+  }
+#pragma unroll
+  for (int ct = 0; ct < MAX_K / 16; ct++)
+    if (ct < nct)
+    {
+      const int c = ct * 16 + li;
+      if (c < d)
+      {
+        const int dc = dl[c];
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          sT[wave][q + 4 * v][c] = acc[ct][v] - ((dc == rbase + q + 4 * v) ? 1.0 : 0.0);
+      }
+    }
+  __syncthreads();
+
+  // 2. Ainv_rows -= T BV
+  const int ncc = (d + 3) >> 2;
+#pragma unroll
+  for (int jt = 0; jt < 16; jt++)
+  {
+    if (jt < njt)
+    {
+      const int jrow = jt * 16 + 4 * (li & 3) + (li >> 2);
+      dbl4 o         = dbl4{0.0, 0.0, 0.0, 0.0};
+      for (int cc = 0; cc < ncc; cc++)
+      {
+        const int c     = 4 * cc + q;
+        const double av = (jrow < n && c < d) ? Bw[(size_t)jrow * k + c] : 0.0;
+        const double bv = (c < d) ? sT[wave][li][c] : 0.0;
+        o               = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, o, 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        a[4 * jt + v] = a[4 * jt + v] - o[v];
+    }
+  }
+
+#pragma unroll
+  for (int jt = 0; jt < 16; jt++)
+  {
+    const int j0 = jt * 16 + 4 * q;
+    double* ap   = A + (size_t)row * ld + j0;
+    if (rok && j0 + 3 < n)
+    {
+      reinterpret_cast<double2*>(ap)[0] = double2{a[4 * jt + 0], a[4 * jt + 1]};
+      reinterpret_cast<double2*>(ap)[1] = double2{a[4 * jt + 2], a[4 * jt + 3]};
+    }
+    else
+    {
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        if (rok && j0 + v < n)
+          ap[v] = a[4 * jt + v];
+    }
+  }
+}
+
+hipError_t launch_flush(const SysC& s, const State& st, int g, int d, int nw, double* scratch, hipStream_t stream)
 {
   if (d <= 0)
     return hipSuccess;
@@ -178,9 +310,17 @@ hipError_t launch_flush(const SysC& s, const State& st, int g, int d, int nw, do
     return hipErrorInvalidValue;
   const int n = s.n[g];
   dim3 grid((n + FT_ROWS - 1) / FT_ROWS, nw);
-  k_flush_T<<<grid, WG, 0, stream>>>(s, st, g, d, scratchT);
-  k_flush_BV<<<nw, WG, 0, stream>>>(s, st, g, d);
-  k_flush_update<<<grid, WG, 0, stream>>>(s, st, g, d, scratchT);
+  if (n <= MF_COLS)
+  {
+    // scratch holds BV_T (n x k per walker, walker stride ld * k)
+    k_flush_BV<<<nw, WG, 0, stream>>>(s, st, g, d, scratch);
+    k_flush_mfma<<<grid, WG, 0, stream>>>(s, st, g, d, scratch);
+    return hipGetLastError();
+  }
+  // n > 256: three passes; scratch holds T (n x k per walker)
+  k_flush_T<<<grid, WG, 0, stream>>>(s, st, g, d, scratch);
+  k_flush_BV<<<nw, WG, 0, stream>>>(s, st, g, d, st.U[g]);
+  k_flush_update<<<grid, WG, 0, stream>>>(s, st, g, d, scratch);
   return hipGetLastError();
 }
 

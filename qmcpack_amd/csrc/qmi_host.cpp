@@ -383,7 +383,7 @@ struct qmi_ctx
       ipiv.alloc((size_t)nw * std::max(s.nup, s.ndn));
       info.alloc(nw);
       if (k > 1)
-        flushT.alloc((size_t)nw * std::max(s.nup, s.ndn) * k);
+        flushT.alloc((size_t)nw * ld * k); // T (n x k) or BV_T (ld x k) per walker
     }
     st        = State{};
     st.R      = R.p;

@@ -26,8 +26,22 @@ struct MoveLDS
   double Y[MAX_K]; // new column d of Binv
   double red[9 * WG]; // block_sum / delayed_gemv scratch (up to 9 values per thread)
   double bc[48];      // broadcast scalars
+  double jc[3][MAX_JCOEF]; // Jastrow functor tables: J1, J2 like spins, J2 unlike spins
   SplineLDS spl;      // uniform spline factors of the proposed position
 };
+
+// Copy the functor tables to LDS (published by the caller's next barrier).
+__device__ inline void stage_jastrow(const SysC& s, MoveLDS& sh)
+{
+  for (int e = threadIdx.x; e < 3 * MAX_JCOEF; e += WG)
+  {
+    const int f = e / MAX_JCOEF, i = e - f * MAX_JCOEF;
+    sh.jc[f][i] = f == 0 ? s.j1.c[i] : s.j2[f - 1].c[i];
+  }
+}
+
+// Electrons per thread when every electron of a walker is visited once (N <= 2 MAX_N).
+constexpr int JPT = (2 * MAX_N + WG - 1) / WG;
 
 // The leading d x d block of Binv is staged in dynamically sized LDS with row stride k + 1
 // (column reads by consecutive threads then fall in different banks).
@@ -213,7 +227,7 @@ __device__ inline void phaseC(const SysC& s, int g, int iat, int n, const double
         if (r < s.j1.rcut)
         {
           double u, du, d2u;
-          functor_vgl(s.j1, r, u, du, d2u);
+          functor_vgl_c(sh.jc[0], s.j1.DeltaRInv, s.j1.max_index, r, u, du, d2u);
           acc[0] += u;
           acc[1] += du * dr[0];
           acc[2] += du * dr[1];
@@ -222,23 +236,39 @@ __device__ inline void phaseC(const SysC& s, int g, int iat, int n, const double
         }
       }
     if (s.has_j2)
-      for (int j = t; j < N; j += WG)
+    {
+      // all positions first, so that the loads of every electron are in flight together
+      double px[JPT], py[JPT], pz[JPT];
+#pragma unroll
+      for (int u = 0; u < JPT; u++)
       {
-        if (j == iat)
+        const int j   = t + u * WG;
+        const bool ok = j < N;
+        px[u]         = ok ? R[j] : 0.0;
+        py[u]         = ok ? R[Np + j] : 0.0;
+        pz[u]         = ok ? R[2 * Np + j] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < JPT; u++)
+      {
+        const int j = t + u * WG;
+        if (j >= N || j == iat)
           continue;
         double dr[3];
-        const double r = min_image(s, R[j], R[Np + j], R[2 * Np + j], pos, dr);
-        const FunctorC& f = s.j2[((j < s.nup) ? 0 : 1) == g ? 0 : 1];
+        const double r    = min_image(s, px[u], py[u], pz[u], pos, dr);
+        const int fi      = ((j < s.nup) ? 0 : 1) == g ? 0 : 1;
+        const FunctorC& f = s.j2[fi];
         if (r < f.rcut)
         {
-          double u, du, d2u;
-          functor_vgl(f, r, u, du, d2u);
-          acc[5] += u;
+          double u2, du, d2u;
+          functor_vgl_c(sh.jc[1 + fi], f.DeltaRInv, f.max_index, r, u2, du, d2u);
+          acc[5] += u2;
           acc[6] += du * dr[0];
           acc[7] += du * dr[1];
           acc[8] += du * dr[2];
         }
       }
+    }
     block_sum<9>(acc, sh.red, sh.bc + BC_C);
   }
   double acc[4] = {0, 0, 0, 0};
@@ -472,28 +502,48 @@ __device__ inline void phaseE(const SysC& s, int g, int iat, int n, int ld, int 
       const double pos[3]  = {sh.bc[BC_POS], sh.bc[BC_POS + 1], sh.bc[BC_POS + 2]};
       const double rold[3] = {R[iat], R[Np + iat], R[2 * Np + iat]};
       double cd2[1]        = {0.0};
-      for (int j = t; j < N; j += WG)
+      // positions and the five accumulator rows of every electron of this thread first
+      double px[JPT], py[JPT], pz[JPT], ja[JPT][5];
+#pragma unroll
+      for (int u = 0; u < JPT; u++)
       {
-        if (j == iat)
+        const int j   = t + u * WG;
+        const bool ok = j < N;
+        px[u]         = ok ? R[j] : 0.0;
+        py[u]         = ok ? R[Np + j] : 0.0;
+        pz[u]         = ok ? R[2 * Np + j] : 0.0;
+        ja[u][0]      = ok ? jas[J_UAT * Np + j] : 0.0;
+        ja[u][1]      = ok ? jas[J_D2U * Np + j] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+          ja[u][2 + q] = ok ? jas[(J_DUX + q) * Np + j] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < JPT; u++)
+      {
+        const int j = t + u * WG;
+        if (j >= N || j == iat)
           continue;
         double drn[3], dro[3];
-        const double rn   = min_image(s, R[j], R[Np + j], R[2 * Np + j], pos, drn);
-        const double ro   = min_image(s, R[j], R[Np + j], R[2 * Np + j], rold, dro);
-        const FunctorC& f = s.j2[((j < s.nup) ? 0 : 1) == g ? 0 : 1];
+        const double rn   = min_image(s, px[u], py[u], pz[u], pos, drn);
+        const double ro   = min_image(s, px[u], py[u], pz[u], rold, dro);
+        const int fi      = ((j < s.nup) ? 0 : 1) == g ? 0 : 1;
+        const FunctorC& f = s.j2[fi];
+        const double* fc  = sh.jc[1 + fi];
         double un = 0, dun = 0, d2un = 0, uo = 0, duo = 0, d2uo = 0;
         if (rn < f.rcut)
-          functor_vgl(f, rn, un, dun, d2un);
+          functor_vgl_c(fc, f.DeltaRInv, f.max_index, rn, un, dun, d2un);
         if (ro < f.rcut)
-          functor_vgl(f, ro, uo, duo, d2uo);
+          functor_vgl_c(fc, f.DeltaRInv, f.max_index, ro, uo, duo, d2uo);
         const double newl = d2un + 2.0 * dun;
-        jas[J_UAT * Np + j] += un - uo;
-        jas[J_D2U * Np + j] += d2uo + 2.0 * duo - newl;
+        jas[J_UAT * Np + j] = ja[u][0] + (un - uo);
+        jas[J_D2U * Np + j] = ja[u][1] + (d2uo + 2.0 * duo - newl);
         cd2[0] -= newl;
 #pragma unroll
         for (int q = 0; q < 3; q++)
         {
-          const double newg = dun * drn[q];
-          jas[(J_DUX + q) * Np + j] -= newg - duo * dro[q];
+          const double newg          = dun * drn[q];
+          jas[(J_DUX + q) * Np + j] = ja[u][2 + q] - (newg - duo * dro[q]);
         }
       }
       block_sum<1>(cd2, sh.red, sh.bc + BC_D2U);
@@ -656,6 +706,7 @@ __global__ __launch_bounds__(WG) void k_sweep_segment(SysC s, State st, const ST
 {
   __shared__ MoveLDS sh;
   extern __shared__ double sB[];
+  stage_jastrow(s, sh);
   const int w = blockIdx.x;
   if (w >= a.nw)
     return;
@@ -682,6 +733,7 @@ __global__ __launch_bounds__(WG) void k_move_one(SysC s, State st, const ST* __r
 {
   __shared__ MoveLDS sh;
   extern __shared__ double sB[];
+  stage_jastrow(s, sh);
   const int w = blockIdx.x;
   if (w >= a.nw)
     return;
@@ -704,6 +756,7 @@ __global__ __launch_bounds__(WG) void k_propose(SysC s, State st, const ST* __re
 {
   __shared__ MoveLDS sh;
   extern __shared__ double sB[];
+  stage_jastrow(s, sh);
   const int w = blockIdx.x;
   if (w >= a.nw)
     return;
@@ -755,6 +808,7 @@ __global__ __launch_bounds__(WG) void k_accept_crowd(SysC s, State st, MoveArgs 
 {
   __shared__ MoveLDS sh;
   extern __shared__ double sB[];
+  stage_jastrow(s, sh);
   const int w = blockIdx.x;
   if (w >= a.nw)
     return;
@@ -829,6 +883,7 @@ __global__ __launch_bounds__(WG) void k_api_eval_grad(SysC s, State st, MoveArgs
 {
   __shared__ MoveLDS sh;
   extern __shared__ double sB[];
+  stage_jastrow(s, sh);
   const int w = blockIdx.x;
   if (w >= a.nw)
     return;
@@ -872,6 +927,7 @@ __global__ __launch_bounds__(WG) void k_api_ratio_grad(SysC s, State st, const S
 {
   __shared__ MoveLDS sh;
   extern __shared__ double sB[];
+  stage_jastrow(s, sh);
   const int w = blockIdx.x;
   if (w >= a.nw)
     return;
@@ -937,6 +993,7 @@ __global__ __launch_bounds__(WG) void k_api_accept(SysC s, State st, MoveArgs a)
 {
   __shared__ MoveLDS sh;
   extern __shared__ double sB[];
+  stage_jastrow(s, sh);
   const int w = blockIdx.x;
   if (w >= a.nw)
     return;

@@ -16,7 +16,7 @@ import pytest
 
 from oracle import oracle as orc
 from qmcpack_amd import qmi
-from qmcpack_amd.system import System, config_system, jastrow_params, make_walkers
+from qmcpack_amd.system import System, config_system, jastrow_params, make_system, make_walkers
 
 pytestmark = pytest.mark.gpu
 
@@ -121,10 +121,10 @@ def vmc_parity(system, R, wpc, steps, sub_steps, tau, recompute_period, seed):
     return c, o, E, tr, Eo, tro
 
 
-def check_vmc(c, o, E, tr, Eo, tro):
+def check_vmc(c, o, E, tr, Eo, tro, ratio_tol=REL):
     nmis = int(np.sum(tr[..., 1] != tro[..., 1]))
     assert nmis == 0, f"{nmis} accept/reject decisions differ"
-    assert relerr(tr[..., 0], tro[..., 0], floor=RATIO_FLOOR) < REL, "ratio"
+    assert relerr(tr[..., 0], tro[..., 0], floor=RATIO_FLOOR) < ratio_tol, "ratio"
     assert relerr(tr[..., 5:8], tro[..., 5:8], floor=1.0) < REL, "gradient at the current position"
     assert np.max(np.abs(tr[..., 2:5] - tro[..., 2:5])) < REL * np.max(np.abs(tro[..., 2:5])), "proposal"
     assert relerr(E, Eo, floor=1.0) < REL, "local energy"
@@ -162,3 +162,27 @@ def test_vmc_segment_kernel(monkeypatch):
     R = np.random.default_rng(6).uniform(0, 1, size=(6, s.N, 3)) * s.L
     res = vmc_parity(s, R, 1, steps=2, sub_steps=2, tau=0.4, recompute_period=2, seed=78)
     check_vmc(*res)
+
+
+@pytest.mark.parametrize("n_up,n_dn,nion", [(256, 256, 128), (200, 185, 32)])
+def test_vmc_rank32_large_det(n_up, n_dn, nion):
+    """Determinants of the benchmark's size (256 per spin, delay rank 32) and a ragged size that
+    is not a multiple of the 16-row matrix-core tiles, on a small 16^3 table: every full and
+    partial delay window and the flush at the end of each spin group.
+
+    At n = 256 the rounding of a ratio grows with the conditioning of the Slater matrix, and two
+    exact-arithmetic-equivalent FP64 oracles (delay rank 32 vs Sherman-Morrison) replaying the
+    same trajectory differ by up to ~2e-10 relative near small ratios.  The ratio bound is
+    therefore max(1e-10, 4 x that measured spread); decisions must still agree exactly and the
+    gradients and energies stay within 1e-10."""
+    kw = dict(L=18.0, n_ion=nion, n_up=n_up, n_dn=n_dn, norb=max(n_up, n_dn), M=16, seed=31)
+    s = make_system(delay_rank=32, **kw)
+    R = make_walkers(s, 3, 32)
+    c, o, E, tr, Eo, tro = vmc_parity(s, R, 1, steps=2, sub_steps=1, tau=0.3, recompute_period=0, seed=79)
+    o1 = orc.VMC(make_system(delay_rank=1, **kw), R, 1, 79)
+    o1.evaluate_log()
+    _, tr1 = o1.replay(tr, 1, 0.3, 0)
+    spread = relerr(tr1[..., 0], tro[..., 0], floor=RATIO_FLOOR)
+    err = relerr(tr[..., 0], tro[..., 0], floor=RATIO_FLOOR)
+    print(f"ratio error {err:.3e}, oracle variant spread {spread:.3e}")
+    check_vmc(c, o, E, tr, Eo, tro, ratio_tol=max(REL, 4 * spread))
