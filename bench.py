@@ -24,6 +24,7 @@ sys.path.insert(0, ROOT)
 METRIC = "electron moves/sec/GPU + effective samples/sec, learned+mixed vs plain FP64"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 
 
 def autocorr_kappa(x):
@@ -46,18 +47,19 @@ def autocorr_kappa(x):
     return kappa
 
 
-def segment_bytes_per_move(n, N, n_ion, d, s=8):
-    """Algorithmic HBM bytes of one proposed move in the fused sweep kernel (per walker) at delay d:
-    spline 64 n s; inverse-row update (2 d n + d^2 + n) s; accept (d n + 6 n) s;
-    J2 13 N s; J1 3 N_I s."""
-    return 64 * n * s + (2 * d * n + d * d + n) * s + (d * n + 6 * n) * s + 13 * N * s + 3 * n_ion * s
+def segment_bytes_per_move(n, N, n_ion, d, sc=8, sd=8, s=8):
+    """Algorithmic HBM bytes of one proposed move in the fused move kernel (per walker) at delay d:
+    spline 64 n sc; inverse-row update (2 d n + d^2 + n) sd; accept (d n + 6 n) sd;
+    J2 13 N s; J1 3 N_I s (sc, sd: bytes per table / determinant element; s: FP64 positions and
+    Jastrow accumulators)."""
+    return 64 * n * sc + (2 * d * n + d * d + n) * sd + (d * n + 6 * n) * sd + 13 * N * s + 3 * n_ion * s
 
 
 def segment_flops_per_move(n, N, n_ion, d):
     return 608 * n + (4 * d * n + 2 * d * d) + 8 * n + 2 * d * n + 40 * N + 40 * n_ion
 
 
-def flush_bytes(n, k, s=8):
+def flush_bytes(n, k, s=8):  # s: bytes per determinant element
     """Algorithmic bytes of one rank-k flush of one n x n inverse: read and write Ainv, read U, V,
     Binv and the k x n intermediates."""
     return (3 * n * n + 6 * n * k) * s
@@ -110,6 +112,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-walkers", type=int, default=128)
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--precision", choices=["fp64", "fp32", "bf16"], default="fp64",
+                    help="storage precision: fp64 (the plain form), fp32 (FP32 table and determinant "
+                         "engine), bf16 (BF16 table, FP32 determinant engine)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs (walker sharding, one "
@@ -129,7 +134,9 @@ def main():
     R = make_walkers(system, nw, cfg["seed"])
     log(f"[bench] system built in {time.perf_counter() - t0:.1f}s: N={system.N} n={system.n_up} "
         f"M={system.M[0]} norb={system.norb} ions={len(system.ions)} k={system.delay_rank} nw={nw}")
-    ctx = qmi.Context(0)
+    ctx = qmi.Context(0, precision=args.precision)
+    sc = {"fp64": 8, "fp32": 4, "bf16": 2}[args.precision]
+    sd = 8 if args.precision == "fp64" else 4
     ctx.system_create(system)
     ctx.walkers_load(R, 1)
     t1 = time.perf_counter()
@@ -162,7 +169,7 @@ def main():
     # d = 0..k-1 repeated N/k times.
     kname = "move_one" if "move_one" in tm else "sweep_segment"
     seg = tm.get(kname, dict(ms=0.0, calls=0))
-    sweep_bytes = nw * (N // k) * sum(segment_bytes_per_move(n, N, nion, d) for d in range(k))
+    sweep_bytes = nw * (N // k) * sum(segment_bytes_per_move(n, N, nion, d, sc, sd) for d in range(k))
     sweep_flops = nw * (N // k) * sum(segment_flops_per_move(n, N, nion, d) for d in range(k))
     launches_per_sweep = seg["calls"] / nprof
     bytes_launch = sweep_bytes / max(1.0, launches_per_sweep)
@@ -176,7 +183,9 @@ def main():
     log("[bench] kernels: " + json.dumps(kernels))
 
     traffic, traffic_src = (measured_traffic("qmi::k_move_one" if kname == "move_one" else "qmi::k_sweep_segment")
-                            if nw == 1024 else (None, None))
+                            if nw == 1024 and args.precision == "fp64" else (None, None))
+    workload = {"fp64": "plain FP64", "fp32": "mixed precision (FP32 table, FP32 determinant engine)",
+                "bf16": "mixed precision (BF16 table, FP32 determinant engine)"}[args.precision]
     out = {
         "metric": METRIC,
         "value": value,
@@ -188,14 +197,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": {"fp64": "f64", "fp32": "f32", "bf16": "bf16+f32"}[args.precision],
         "data": "synthetic",
         "config": {
-            "workload": "plain FP64 VMC sweeps: 1024 walkers x 512 electrons (256 per spin), 256 orbitals "
+            "workload": f"{workload} VMC sweeps: 1024 walkers x 512 electrons (256 per spin), 256 orbitals "
                         "on a 64^3 periodic B-spline table, 128 ions, L=18 bohr, J1+J2 B-spline Jastrows, "
                         "delay rank 32, tau 0.3, one RNG stream per walker",
             "walkers": nw, "electrons": N, "orbitals_per_spin": n, "grid": 64, "ions": nion,
-            "delay_rank": k, "precision": "fp64",
+            "delay_rank": k, "precision": args.precision,
         },
         "effective_samples_per_s": eff,
         "kappa": kappa,
@@ -217,9 +226,9 @@ def main():
         "flush_roofline": {
             "kernel": "flush (T = Ainv U^T, BV = Binv V, Ainv -= T BV)",
             "avg_ms": fl_ms,
-            "achieved_gbs": (nw * flush_bytes(n, k)) / (fl_ms * 1e-3) / 1e9 if fl_ms > 0 else 0.0,
+            "achieved_gbs": (nw * flush_bytes(n, k, sd)) / (fl_ms * 1e-3) / 1e9 if fl_ms > 0 else 0.0,
             "achieved_tflops": (nw * flush_flops(n, k)) / (fl_ms * 1e-3) / 1e12 if fl_ms > 0 else 0.0,
-            "fp64_peak_tflops": FP64_PEAK_TFLOPS,
+            "mfma_peak_tflops": FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS,
         },
     }
     if not args.no_cpu:

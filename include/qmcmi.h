@@ -28,10 +28,16 @@ enum
   QMI_ERR_UNSUPPORTED = 4
 };
 
-/* precision of the orbital table and of the delayed-update engine */
+/* Precision of the orbital table and of the delayed-update engine (the reference's full and mixed
+ * precision builds, CMakeLists.txt:83-111).  In the mixed modes the determinant state (inverse,
+ * orbital gradients/Laplacians, delayed rows) is stored in FP32 and the flush runs on FP32
+ * matrix cores; the LU re-inversion and the log determinant stay FP64 (DiracMatrix.h:204-220), as
+ * do positions, Jastrows and every ratio and gradient dot product. */
 enum
 {
-  QMI_PREC_FP64 = 0
+  QMI_PREC_FP64       = 0, /* FP64 table and determinant engine */
+  QMI_PREC_MIXED_FP32 = 1, /* FP32 table, FP32 determinant engine */
+  QMI_PREC_MIXED_BF16 = 2  /* BF16 table storage (FP32 and wider arithmetic), FP32 determinant engine */
 };
 
 /* System description.  Orthorhombic cell (L[d] <= 0: open along d); ions;

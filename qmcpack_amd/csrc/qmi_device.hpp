@@ -17,6 +17,43 @@ constexpr int MAX_N      = 512; // electrons per spin (orbitals per determinant)
 constexpr double DBL_EPS = 2.220446049250313e-16;
 constexpr int TRACE_FIELDS = 8; // ratio, accepted, proposed position[3], gradient at the current position[3]
 
+// Storage precision of a context (SysC::prec).  Arithmetic of the move phases, the ratio and
+// gradient dot products, the Jastrows, positions, the LU re-inversion and log det stay FP64.
+enum Precision
+{
+  PREC_FP64 = 0, // FP64 spline table and determinant engine
+  PREC_FP32 = 1, // FP32 spline table, FP32 determinant engine (Ainv, dpsi, U, V, Binv; FP32 MFMA flush)
+  PREC_BF16 = 2  // BF16 spline table, FP32 determinant engine
+};
+
+// BF16 storage (upper half of an IEEE float); read as double through the float it encodes.
+struct bf16s
+{
+  uint16_t u;
+  __device__ operator double() const { return (double)__uint_as_float((uint32_t)u << 16); }
+};
+
+template<typename T>
+__device__ inline T from_double(double x);
+template<>
+__device__ inline double from_double<double>(double x)
+{
+  return x;
+}
+template<>
+__device__ inline float from_double<float>(double x)
+{
+  return (float)x;
+}
+template<>
+__device__ inline bf16s from_double<bf16s>(double x)
+{
+  // round to nearest even on the float encoding (finite inputs)
+  uint32_t b = __float_as_uint((float)x);
+  b += 0x7fffu + ((b >> 16) & 1u);
+  return bf16s{(uint16_t)(b >> 16)};
+}
+
 // Radial B-spline functor (QMCWaveFunctions/Jastrow/BsplineFunctor.h:117-133 coefficients with
 // the cusp folded into coefs[0]).
 struct FunctorC
@@ -37,6 +74,7 @@ struct SysC
   int ld;   // leading dimension of determinant rows (>= max n, multiple of 8)
   int k;    // delay rank
   int has_det, has_j1, has_j2;
+  int prec; // Precision
   double L[3], Linv[3];
   int periodic[3];
   double G[9], symGG[6];
@@ -49,7 +87,9 @@ struct SysC
   FunctorC j2[2]; // [0] like spins, [1] unlike spins
 };
 
-// Per-walker device state (walker-major arrays).
+// Per-walker device state (walker-major arrays).  The determinant-engine arrays (Ainv, dpsi, U, V,
+// Binv) hold elements of the context's determinant type (double, or float for the mixed
+// precisions); kernels view them through det_ptr<DT>.
 struct State
 {
   double* R;        // [nw][3][Npad]
@@ -68,6 +108,17 @@ struct State
   double* mv;       // [nw][MV_SIZE]
   const double* ions; // [3][nion]
 };
+
+template<typename DT>
+__host__ __device__ inline DT* det_ptr(double* p)
+{
+  return reinterpret_cast<DT*>(p);
+}
+template<typename DT>
+__host__ __device__ inline const DT* det_ptr(const double* p)
+{
+  return reinterpret_cast<const DT*>(p);
+}
 
 enum JasRow
 {

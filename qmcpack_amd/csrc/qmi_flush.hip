@@ -13,6 +13,7 @@ constexpr int FT_ROWS = 64; // rows of Ainv per workgroup
 constexpr int FT_KC   = 32; // contraction chunk
 
 // T = Ainv U^T - W for rows [i0, i0+64).  Thread (row = t/4, cq = t%4) owns columns c = cq + 4 q.
+template<typename DT>
 __global__ __launch_bounds__(WG) void k_flush_T(SysC s, State st, int g, int d, double* __restrict__ T)
 {
   __shared__ double As[FT_ROWS][FT_KC + 1];
@@ -20,8 +21,8 @@ __global__ __launch_bounds__(WG) void k_flush_T(SysC s, State st, int g, int d, 
   const int w = blockIdx.y, i0 = blockIdx.x * FT_ROWS;
   const int n = s.n[g], ld = s.ld, k = s.k;
   const int t = threadIdx.x, row = t >> 2, cq = t & 3;
-  const double* A = st.Ainv[g] + (size_t)w * n * ld;
-  const double* U = st.U[g] + (size_t)w * ld * k;
+  const DT* A = det_ptr<DT>(st.Ainv[g]) + (size_t)w * n * ld;
+  const DT* U = det_ptr<DT>(st.U[g]) + (size_t)w * ld * k;
   double acc[MAX_K / 4];
 #pragma unroll
   for (int q = 0; q < MAX_K / 4; q++)
@@ -71,20 +72,21 @@ __global__ __launch_bounds__(WG) void k_flush_T(SysC s, State st, int g, int d, 
 
 // BV_T[o][c] = sum_e Binv[c][e] V_T[o][e] over the first d delayed rows, written to BVout
 // (orbital-major, row stride k, walker stride ld * k).
-__global__ __launch_bounds__(WG) void k_flush_BV(SysC s, State st, int g, int d, double* __restrict__ BVout)
+template<typename DT>
+__global__ __launch_bounds__(WG) void k_flush_BV(SysC s, State st, int g, int d, DT* __restrict__ BVout)
 {
   __shared__ double Bs[MAX_K * MAX_K];
   const int w = blockIdx.x;
   const int n = s.n[g], ld = s.ld, k = s.k;
-  const double* B = st.Binv[g] + (size_t)w * k * k;
-  const double* V = st.V[g] + (size_t)w * ld * k;
-  double* U       = BVout + (size_t)w * ld * k;
+  const DT* B = det_ptr<DT>(st.Binv[g]) + (size_t)w * k * k;
+  const DT* V = det_ptr<DT>(st.V[g]) + (size_t)w * ld * k;
+  DT* U       = BVout + (size_t)w * ld * k;
   for (int e = threadIdx.x; e < d * d; e += WG)
     Bs[(e / d) * MAX_K + e % d] = B[(e / d) * k + e % d];
   __syncthreads();
   for (int o = threadIdx.x; o < n; o += WG)
   {
-    const double* vo = V + (size_t)o * k;
+    const DT* vo = V + (size_t)o * k;
     for (int c0 = 0; c0 < d; c0 += 16)
     {
       double acc[16];
@@ -102,13 +104,14 @@ __global__ __launch_bounds__(WG) void k_flush_BV(SysC s, State st, int g, int d,
 #pragma unroll
       for (int cc = 0; cc < 16; cc++)
         if (c0 + cc < d)
-          U[(size_t)o * k + c0 + cc] = acc[cc];
+          U[(size_t)o * k + c0 + cc] = (DT)acc[cc];
     }
   }
 }
 
 // Ainv[i][o] = -sum_c T[i][c] BV_T[o][c] + Ainv[i][o] for rows [i0, i0+64); 64x64 output tiles,
 // 4x4 register block per thread.
+template<typename DT>
 __global__ __launch_bounds__(WG) void k_flush_update(SysC s, State st, int g, int d, const double* __restrict__ T)
 {
   __shared__ double Ts[FT_ROWS][MAX_K + 1];
@@ -116,8 +119,8 @@ __global__ __launch_bounds__(WG) void k_flush_update(SysC s, State st, int g, in
   const int w = blockIdx.y, i0 = blockIdx.x * FT_ROWS;
   const int n = s.n[g], ld = s.ld, k = s.k;
   const int t = threadIdx.x, tr = t >> 4, tc = t & 15;
-  double* A        = st.Ainv[g] + (size_t)w * n * ld;
-  const double* BV = st.U[g] + (size_t)w * ld * k;
+  DT* A        = det_ptr<DT>(st.Ainv[g]) + (size_t)w * n * ld;
+  const DT* BV = det_ptr<DT>(st.U[g]) + (size_t)w * ld * k;
   const double* Tw = T + (size_t)w * n * k;
   for (int e = t; e < FT_ROWS * d; e += WG)
   {
@@ -165,120 +168,228 @@ __global__ __launch_bounds__(WG) void k_flush_update(SysC s, State st, int g, in
       {
         const int o = o0 + tc + 16 * b;
         if (o < n)
-          A[(size_t)i * ld + o] = -1.0 * acc[a][b] + A[(size_t)i * ld + o];
+          A[(size_t)i * ld + o] = (DT)(-1.0 * acc[a][b] + (double)A[(size_t)i * ld + o]);
       }
     }
   }
 }
 
-// Fused flush for n <= 256 on the FP64 matrix cores.  v_mfma_f64_16x16x4f64 operand layout
-// (measured): lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]; result register v of lane l
-// holds D[l / 16 + 4 v][l % 16].  One wave owns 16 rows of Ainv and keeps them in registers for
-// both products, so Ainv is read and written once:
+// Fused flush for n <= 256 on the FP64 matrix cores, using the association
+//   Ainv -= ((Ainv U^T - W) Binv) V
+// (the same product as (Ainv U^T - W)(Binv V), without materialising Binv V).
+// v_mfma_f64_16x16x4f64 operand layout (measured): lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16]; result register v of lane l holds D[l / 16 + 4 v][l % 16].
+// One wave owns 16 rows of Ainv and keeps them in registers for the whole update, so Ainv is read
+// and written once:
 //   lane (li = l % 16, q = l / 16) holds a[4 jt + v] = Ainv[row li][16 jt + 4 q + v], jt < 16.
 //   1. T (16 x d) = Ainv_rows U^T: the K index of MFMA step kk is column 16 (kk / 4) + 4 q + kk % 4,
-//      which is exactly a[kk]; then T[i][c] -= 1 where delay_list[c] == i (applyW).
-//   2. T goes through LDS to the B-operand layout, and D^T = BV_T T^T is formed per 16-column
-//      tile jt with the BV row of A-operand row m taken as column 16 jt + 4 (m % 4) + m / 4, so
-//      that result register v of lane (li, q) is Delta[row li][16 jt + 4 q + v]: the register
-//      a[4 jt + v] that it updates.
+//      which is exactly a[kk]; U^T rows are staged in LDS 64 at a time.  T[i][c] -= 1 where
+//      delay_list[c] == i (applyW).
+//   2. T' = T Binv (Binv staged in LDS), through LDS into the B-operand layout.
+//   3. D^T = V_T T'^T per 16-column tile jt, with the V row of A-operand row m taken as column
+//      16 jt + 4 (m % 4) + m / 4, so that result register v of lane (li, q) is
+//      Delta[row li][16 jt + 4 q + v]: the register a[4 jt + v] that it updates.
+// LDS (doubles, LT = 16 ceil(k / 16) + 1): a 64-row chunk of U^T or V^T [64][LT], T per wave
+// [4][16][LT], Binv zero-padded [16 ceil(k / 16)][LT].  Grid: 8 walkers per group of blocks, the
+// row blocks of one walker on one XCD (block b runs on XCD b % 8) so that they share its L2.
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
 constexpr int MF_COLS = 256; // n limit of the register-resident path
 
-__global__ __launch_bounds__(WG, 2) void k_flush_mfma(SysC s, State st, int g, int d, const double* __restrict__ BV)
+// 16x16x4 matrix-core step in the determinant storage type (FP64 or FP32 accumulate); both forms
+// share the operand and result lane layout above.
+template<typename DT>
+struct Mfma;
+template<>
+struct Mfma<double>
 {
-  __shared__ double sT[NWAVE][16][MAX_K + 1];
-  const int w = blockIdx.y;
+  using acc_t = dbl4;
+  __device__ static acc_t mma(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+};
+template<>
+struct Mfma<float>
+{
+  using acc_t = flt4;
+  __device__ static acc_t mma(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+};
+
+__host__ __device__ inline int flush_lt(int k) { return 16 * ((k + 15) / 16) + 1; }
+__host__ __device__ inline size_t flush_lds_bytes(int k, size_t esize)
+{
+  const int lt = flush_lt(k);
+  return (size_t)(64 * lt + 64 * lt + (lt - 1) * lt) * esize;
+}
+
+// Stage rows [j0, j0 + 64) x columns [0, 16 NCT) of an orbital-major delayed block X (row stride
+// k) into sX[64][LT], zero outside j < n, c < d.  Thread (cl = t % 16, r0 = t / 16) loads
+// columns ct * 16 + cl of rows r0 + 16 i; the loads are straight-line code from clamped in-range
+// addresses, so that all of them are in flight together, and the padding is applied by a 0/1
+// factor (the delayed blocks hold finite values only: zeroed at allocation, finite writes).
+template<typename DT, int NCT>
+__device__ inline void stage_rows(const DT* __restrict__ X, int k, int n, int d, int j0, DT* sX, int LT)
+{
+  const int t = threadIdx.x, cl = t & 15, r0 = t >> 4;
+  DT tmp[NCT][4];
+#pragma unroll
+  for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+    {
+      const int r = r0 + 16 * i, c = ct * 16 + cl, j = j0 + r;
+      const DT x = X[(size_t)min(j, n - 1) * k + min(c, d - 1)];
+      tmp[ct][i] = x * ((j < n && c < d) ? DT(1) : DT(0));
+    }
+#pragma unroll
+  for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+      sX[(r0 + 16 * i) * LT + ct * 16 + cl] = tmp[ct][i];
+}
+
+// NCT = ceil(d / 16) column tiles of the delayed block; columns are zero-padded to 16 NCT, so the
+// padded MFMA steps add exact zeros.
+template<typename DT, int NCT>
+__global__ __launch_bounds__(WG, 2) void k_flush_mfma(SysC s, State st, int g, int d, int nw)
+{
+  using MF    = Mfma<DT>;
+  using acc_t = typename MF::acc_t;
+  extern __shared__ double lds_raw[];
+  DT* lds = reinterpret_cast<DT*>(lds_raw);
   const int n = s.n[g], ld = s.ld, k = s.k;
+  const int nblk = (n + 63) >> 6;
+  const int b = blockIdx.x, xcd = b & 7, rest = b >> 3;
+  const int w = (rest / nblk) * 8 + xcd, bx = rest % nblk;
+  if (w >= nw)
+    return;
+  constexpr int DP = 16 * NCT; // padded delay columns
+  const int LT     = flush_lt(k);
+  DT* sU           = lds;           // [64][LT]
+  DT* sT           = lds + 64 * LT; // [NWAVE][16][LT]
+  DT* sB           = sT + 64 * LT;  // [DP][LT]
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6, q = lane >> 4, li = lane & 15;
-  const int rbase = blockIdx.x * 64 + wave * 16;
+  const int rbase = bx * 64 + wave * 16;
   const int row   = rbase + li;
   const bool rok  = row < n;
-  double* A        = st.Ainv[g] + (size_t)w * n * ld;
-  const double* U  = st.U[g] + (size_t)w * ld * k;
-  const double* Bw = BV + (size_t)w * ld * k;
-  const int* dl    = st.dlist[g] + (size_t)w * k;
-  const int njt    = (n + 15) >> 4;
+  DT* A          = det_ptr<DT>(st.Ainv[g]) + (size_t)w * n * ld;
+  const DT* U    = det_ptr<DT>(st.U[g]) + (size_t)w * ld * k;
+  const DT* V    = det_ptr<DT>(st.V[g]) + (size_t)w * ld * k;
+  const DT* Binv = det_ptr<DT>(st.Binv[g]) + (size_t)w * k * k;
+  const int* dl  = st.dlist[g] + (size_t)w * k;
+  DT* sTw        = sT + wave * 16 * LT;
 
-  double a[64];
+  DT a[64];
 #pragma unroll
   for (int jt = 0; jt < 16; jt++)
   {
     const int j0     = jt * 16 + 4 * q;
-    const double* ap = A + (size_t)row * ld + j0;
+    const DT* ap = A + (size_t)row * ld + j0;
     if (rok && j0 + 3 < n)
     {
-      const double2 x0 = reinterpret_cast<const double2*>(ap)[0];
-      const double2 x1 = reinterpret_cast<const double2*>(ap)[1];
-      a[4 * jt + 0]    = x0.x;
-      a[4 * jt + 1]    = x0.y;
-      a[4 * jt + 2]    = x1.x;
-      a[4 * jt + 3]    = x1.y;
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        a[4 * jt + v] = ap[v];
     }
     else
     {
 #pragma unroll
       for (int v = 0; v < 4; v++)
-        a[4 * jt + v] = (rok && j0 + v < n) ? ap[v] : 0.0;
+        a[4 * jt + v] = (rok && j0 + v < n) ? ap[v] : DT(0);
+    }
+  }
+  {
+    constexpr int NB = (DP * DP + WG - 1) / WG;
+    DT tb[NB];
+#pragma unroll
+    for (int i = 0; i < NB; i++)
+    {
+      const int e = t + WG * i, r = e / DP, c = e - r * DP;
+      tb[i]       = Binv[min(r, d - 1) * k + min(c, d - 1)] * ((r < d && c < d) ? DT(1) : DT(0));
+    }
+#pragma unroll
+    for (int i = 0; i < NB; i++)
+    {
+      const int e = t + WG * i, r = e / DP, c = e - r * DP;
+      if (e < DP * DP)
+        sB[r * LT + c] = tb[i];
     }
   }
 
-  // 1. T = Ainv_rows U^T - W
-  const int nct = (d + 15) >> 4;
-  dbl4 acc[MAX_K / 16];
+  // 1. T = Ainv_rows U^T
+  acc_t acc[NCT];
 #pragma unroll
-  for (int ct = 0; ct < MAX_K / 16; ct++)
-    acc[ct] = dbl4{0.0, 0.0, 0.0, 0.0};
+  for (int ct = 0; ct < NCT; ct++)
+    acc[ct] = acc_t{0, 0, 0, 0};
 #pragma unroll
-  for (int kk = 0; kk < 64; kk++)
+  for (int ch = 0; ch < MF_COLS / 64; ch++)
   {
-    if ((kk >> 2) < njt)
+    if (ch * 64 < n)
     {
-      const int j = (kk >> 2) * 16 + 4 * q + (kk & 3);
+      __syncthreads();
+      stage_rows<DT, NCT>(U, k, n, d, ch * 64, sU, LT);
+      __syncthreads();
 #pragma unroll
-      for (int ct = 0; ct < MAX_K / 16; ct++)
-        if (ct < nct)
-        {
-          const int c    = ct * 16 + li;
-          const double b = (j < n && c < d) ? U[(size_t)j * k + c] : 0.0;
-          acc[ct]        = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk], b, acc[ct], 0, 0, 0);
-        }
-    }
-  }
-#pragma unroll
-  for (int ct = 0; ct < MAX_K / 16; ct++)
-    if (ct < nct)
-    {
-      const int c = ct * 16 + li;
-      if (c < d)
+      for (int i = 0; i < 16; i++)
       {
-        const int dc = dl[c];
+        const int jl = (i >> 2) * 16 + 4 * q + (i & 3);
 #pragma unroll
-        for (int v = 0; v < 4; v++)
-          sT[wave][q + 4 * v][c] = acc[ct][v] - ((dc == rbase + q + 4 * v) ? 1.0 : 0.0);
+        for (int ct = 0; ct < NCT; ct++)
+          acc[ct] = MF::mma(a[ch * 16 + i], sU[jl * LT + ct * 16 + li], acc[ct]);
       }
     }
+  }
+  // applyW; T rows q + 4 v of this wave, columns ct * 16 + li (zero beyond d)
+#pragma unroll
+  for (int ct = 0; ct < NCT; ct++)
+  {
+    const int c  = ct * 16 + li;
+    const int dc = c < d ? dl[c] : -2;
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+      sTw[(q + 4 * v) * LT + c] = c < d ? acc[ct][v] - ((dc == rbase + q + 4 * v) ? DT(1) : DT(0)) : DT(0);
+  }
   __syncthreads();
 
-  // 2. Ainv_rows -= T BV
-  const int ncc = (d + 3) >> 2;
+  // 2. T' = T Binv (K index e = 4 ee + q)
 #pragma unroll
-  for (int jt = 0; jt < 16; jt++)
+  for (int ct = 0; ct < NCT; ct++)
+    acc[ct] = acc_t{0, 0, 0, 0};
+#pragma unroll
+  for (int ee = 0; ee < DP / 4; ee++)
   {
-    if (jt < njt)
-    {
-      const int jrow = jt * 16 + 4 * (li & 3) + (li >> 2);
-      dbl4 o         = dbl4{0.0, 0.0, 0.0, 0.0};
-      for (int cc = 0; cc < ncc; cc++)
-      {
-        const int c     = 4 * cc + q;
-        const double av = (jrow < n && c < d) ? Bw[(size_t)jrow * k + c] : 0.0;
-        const double bv = (c < d) ? sT[wave][li][c] : 0.0;
-        o               = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, o, 0, 0, 0);
-      }
+    const DT av = sTw[li * LT + 4 * ee + q];
 #pragma unroll
-      for (int v = 0; v < 4; v++)
-        a[4 * jt + v] = a[4 * jt + v] - o[v];
+    for (int ct = 0; ct < NCT; ct++)
+      acc[ct] = MF::mma(av, sB[(4 * ee + q) * LT + ct * 16 + li], acc[ct]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+      sTw[(q + 4 * v) * LT + ct * 16 + li] = acc[ct][v];
+
+  // 3. Ainv_rows -= T' V
+#pragma unroll
+  for (int ch = 0; ch < MF_COLS / 64; ch++)
+  {
+    if (ch * 64 < n)
+    {
+      __syncthreads();
+      stage_rows<DT, NCT>(V, k, n, d, ch * 64, sU, LT);
+      __syncthreads();
+#pragma unroll
+      for (int j4 = 0; j4 < 4; j4++)
+      {
+        const int jt = ch * 4 + j4;
+        const int jl = j4 * 16 + 4 * (li & 3) + (li >> 2);
+        acc_t o      = acc_t{0, 0, 0, 0};
+#pragma unroll
+        for (int cc = 0; cc < DP / 4; cc++)
+          o = MF::mma(sU[jl * LT + 4 * cc + q], sTw[li * LT + 4 * cc + q], o);
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          a[4 * jt + v] = a[4 * jt + v] - o[v];
+      }
     }
   }
 
@@ -286,11 +397,12 @@ __global__ __launch_bounds__(WG, 2) void k_flush_mfma(SysC s, State st, int g, i
   for (int jt = 0; jt < 16; jt++)
   {
     const int j0 = jt * 16 + 4 * q;
-    double* ap   = A + (size_t)row * ld + j0;
+    DT* ap       = A + (size_t)row * ld + j0;
     if (rok && j0 + 3 < n)
     {
-      reinterpret_cast<double2*>(ap)[0] = double2{a[4 * jt + 0], a[4 * jt + 1]};
-      reinterpret_cast<double2*>(ap)[1] = double2{a[4 * jt + 2], a[4 * jt + 3]};
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        ap[v] = a[4 * jt + v];
     }
     else
     {
@@ -310,18 +422,28 @@ hipError_t launch_flush(const SysC& s, const State& st, int g, int d, int nw, do
     return hipErrorInvalidValue;
   const int n = s.n[g];
   dim3 grid((n + FT_ROWS - 1) / FT_ROWS, nw);
-  if (n <= MF_COLS)
-  {
-    // scratch holds BV_T (n x k per walker, walker stride ld * k)
-    k_flush_BV<<<nw, WG, 0, stream>>>(s, st, g, d, scratch);
-    k_flush_mfma<<<grid, WG, 0, stream>>>(s, st, g, d, scratch);
+  return dispatch_prec(s.prec, [&](auto, auto dtv) {
+    using DT = decltype(dtv);
+    if (n <= MF_COLS)
+    {
+      const int nblk = (n + 63) / 64, ngrp = (nw + 7) / 8;
+      const dim3 grid1(nblk * ngrp * 8);
+      const size_t lds = flush_lds_bytes(s.k, sizeof(DT));
+      switch ((d + 15) / 16)
+      {
+      case 1: k_flush_mfma<DT, 1><<<grid1, WG, lds, stream>>>(s, st, g, d, nw); break;
+      case 2: k_flush_mfma<DT, 2><<<grid1, WG, lds, stream>>>(s, st, g, d, nw); break;
+      case 3: k_flush_mfma<DT, 3><<<grid1, WG, lds, stream>>>(s, st, g, d, nw); break;
+      default: k_flush_mfma<DT, 4><<<grid1, WG, lds, stream>>>(s, st, g, d, nw); break;
+      }
+      return hipGetLastError();
+    }
+    // n > 256: three passes; scratch holds T (n x k per walker, FP64)
+    k_flush_T<DT><<<grid, WG, 0, stream>>>(s, st, g, d, scratch);
+    k_flush_BV<DT><<<nw, WG, 0, stream>>>(s, st, g, d, det_ptr<DT>(st.U[g]));
+    k_flush_update<DT><<<grid, WG, 0, stream>>>(s, st, g, d, scratch);
     return hipGetLastError();
-  }
-  // n > 256: three passes; scratch holds T (n x k per walker)
-  k_flush_T<<<grid, WG, 0, stream>>>(s, st, g, d, scratch);
-  k_flush_BV<<<nw, WG, 0, stream>>>(s, st, g, d, st.U[g]);
-  k_flush_update<<<grid, WG, 0, stream>>>(s, st, g, d, scratch);
-  return hipGetLastError();
+  });
 }
 
 } // namespace qmi

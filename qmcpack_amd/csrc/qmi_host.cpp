@@ -164,7 +164,9 @@ struct qmi_ctx
   // walkers
   int nw = 0, wpc = 1, ncrowd = 0;
   DevBuf<double> R, jas, logdet, stats, invRow, phi, mv, flushT, out, gl3, Gbuf, Lbuf, hostin;
+  // determinant engine, in elements of the determinant storage type (double or float)
   DevBuf<double> Ainv[2], dpsi[2], U[2], V[2], Binv[2];
+  DevBuf<double> lu; // FP64 LU workspace of the mixed precisions ([nw][n][ld])
   DevBuf<int> dlist[2], ipiv, info, ubase, acc;
   DevBuf<double> gauss, unif, trace;
   State st{};
@@ -215,6 +217,12 @@ struct qmi_ctx
 
   int group(int iat) const { return iat < sys.nup ? 0 : 1; }
 
+  // bytes per element of the spline table and of the determinant engine
+  size_t table_esize() const { return precision == QMI_PREC_FP64 ? 8 : precision == QMI_PREC_MIXED_FP32 ? 4 : 2; }
+  size_t det_esize() const { return precision == QMI_PREC_FP64 ? 8 : 4; }
+  // device buffer (in doubles) holding count elements of the determinant type
+  size_t det_words(size_t count) const { return (count * det_esize() + 7) / 8; }
+
   void require_walkers() const
   {
     if (!have_system || nw == 0)
@@ -260,6 +268,7 @@ struct qmi_ctx
     }
     const double symGG[6] = {GGt[0], GGt[1] + GGt[3], GGt[2] + GGt[6], GGt[4], GGt[5] + GGt[7], GGt[8]};
     std::copy(symGG, symGG + 6, s.symGG);
+    s.prec    = precision;
     s.has_det = d->norb > 0 && d->coefs != nullptr;
     if (s.has_det)
     {
@@ -320,7 +329,7 @@ struct qmi_ctx
       DevBuf<double> tmp;
       tmp.alloc(nuniq);
       HIPCHECK(hipMemcpy(tmp.p, d->coefs, nuniq * sizeof(double), hipMemcpyHostToDevice));
-      coefs.alloc(ntab);
+      coefs.alloc((ntab * table_esize() + 7) / 8);
       HIPCHECK(launch_wrap_coefs(s, tmp.p, d->norb, coefs.p, stream));
       HIPCHECK(hipStreamSynchronize(stream));
     }
@@ -370,16 +379,18 @@ struct qmi_ctx
       for (int g = 0; g < 2; g++)
       {
         const int n = s.n[g];
-        Ainv[g].alloc((size_t)nw * std::max(n, 1) * ld);
-        dpsi[g].alloc((size_t)nw * std::max(n, 1) * 4 * ld);
-        U[g].alloc((size_t)nw * ld * k);
-        V[g].alloc((size_t)nw * ld * k);
-        Binv[g].alloc((size_t)nw * k * k);
+        Ainv[g].alloc(det_words((size_t)nw * std::max(n, 1) * ld));
+        dpsi[g].alloc(det_words((size_t)nw * std::max(n, 1) * 4 * ld));
+        U[g].alloc(det_words((size_t)nw * ld * k));
+        V[g].alloc(det_words((size_t)nw * ld * k));
+        Binv[g].alloc(det_words((size_t)nw * k * k));
         dlist[g].alloc((size_t)nw * k);
         HIPCHECK(hipMemset(U[g].p, 0, U[g].n * sizeof(double)));
         HIPCHECK(hipMemset(V[g].p, 0, V[g].n * sizeof(double)));
         HIPCHECK(hipMemset(Binv[g].p, 0, Binv[g].n * sizeof(double)));
       }
+      if (precision != QMI_PREC_FP64)
+        lu.alloc((size_t)nw * std::max(s.nup, s.ndn) * ld);
       ipiv.alloc((size_t)nw * std::max(s.nup, s.ndn));
       info.alloc(nw);
       if (k > 1)
@@ -423,28 +434,31 @@ struct qmi_ctx
       const int n = s.n[g];
       if (n == 0)
         continue;
+      // FP64 LU workspace: Ainv itself in FP64 contexts
+      double* L64 = precision == QMI_PREC_FP64 ? Ainv[g].p : lu.p;
       {
         Scoped t(this, "spo_rows");
-        HIPCHECK(launch_spo_rows(s, st, coefs.p, g, nw, stream));
+        HIPCHECK(launch_spo_rows(s, st, coefs.p, g, nw, L64, stream));
       }
       {
         Scoped t(this, "lu_getrf");
-        if (rocsolver_dgetrf_strided_batched(blas, n, n, Ainv[g].p, s.ld, (rocblas_stride)n * s.ld, ipiv.p, n, info.p,
-                                             nw) != rocblas_status_success)
+        if (rocsolver_dgetrf_strided_batched(blas, n, n, L64, s.ld, (rocblas_stride)n * s.ld, ipiv.p, n, info.p, nw) !=
+            rocblas_status_success)
           throw QmiError(QMI_ERR_DEVICE, "rocsolver_dgetrf_strided_batched failed");
       }
       check_info("LU factorisation: singular Slater matrix");
       {
         Scoped t(this, "logdet");
-        HIPCHECK(launch_logdet(s, st, g, nw, ipiv.p, stream));
+        HIPCHECK(launch_logdet(s, st, g, nw, ipiv.p, L64, stream));
       }
       {
         Scoped t(this, "lu_getri");
-        if (rocsolver_dgetri_strided_batched(blas, n, Ainv[g].p, s.ld, (rocblas_stride)n * s.ld, ipiv.p, n, info.p,
-                                             nw) != rocblas_status_success)
+        if (rocsolver_dgetri_strided_batched(blas, n, L64, s.ld, (rocblas_stride)n * s.ld, ipiv.p, n, info.p, nw) !=
+            rocblas_status_success)
           throw QmiError(QMI_ERR_DEVICE, "rocsolver_dgetri_strided_batched failed");
       }
       check_info("matrix inverse: singular Slater matrix");
+      HIPCHECK(launch_store_inverse(s, st, g, nw, L64, stream));
     }
     invrow_id = -1;
   }
@@ -719,7 +733,7 @@ extern "C"
     if (!out)
       return QMI_ERR_ARG;
     *out = nullptr;
-    if (precision != QMI_PREC_FP64)
+    if (precision != QMI_PREC_FP64 && precision != QMI_PREC_MIXED_FP32 && precision != QMI_PREC_MIXED_BF16)
       return QMI_ERR_UNSUPPORTED;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)

@@ -1,12 +1,15 @@
 // Initialisation, recompute and gradient/Laplacian kernels.
 #include "qmi_kernels.hpp"
 
+#include <algorithm>
+
 namespace qmi
 {
 
 // Padded table from the unique periodic coefficients: table[i][j][k][o] = u[(i-1)%Mx][(j-1)%My][(k-1)%Mz][o]
 // (einspline/bspline_create.c:1027-1034 periodic wrap; MultiBsplineBase.hpp:75-121 layout).
-__global__ void k_wrap_coefs(SysC s, const double* __restrict__ unique, int norb, double* __restrict__ table)
+template<typename ST>
+__global__ void k_wrap_coefs(SysC s, const double* __restrict__ unique, int norb, ST* __restrict__ table)
 {
   const long long Nx = s.M[0] + 3, Ny = s.M[1] + 3, Nz = s.M[2] + 3;
   const long long pt = blockIdx.x;
@@ -15,22 +18,28 @@ __global__ void k_wrap_coefs(SysC s, const double* __restrict__ unique, int norb
   const int i = int(pt / (Ny * Nz)), j = int((pt / Nz) % Ny), kk = int(pt % Nz);
   const int ui = (i - 1 + s.M[0]) % s.M[0], uj = (j - 1 + s.M[1]) % s.M[1], uk = (kk - 1 + s.M[2]) % s.M[2];
   const double* src = unique + (((long long)ui * s.M[1] + uj) * s.M[2] + uk) * norb;
-  double* dst       = table + pt * s.npad;
+  ST* dst           = table + pt * s.npad;
   for (int o = threadIdx.x; o < s.npad; o += blockDim.x)
-    dst[o] = o < norb ? src[o] : 0.0;
+    dst[o] = from_double<ST>(o < norb ? src[o] : 0.0);
 }
 
-hipError_t launch_wrap_coefs(const SysC& s, const double* unique, int norb, double* table, hipStream_t stream)
+hipError_t launch_wrap_coefs(const SysC& s, const double* unique, int norb, void* table, hipStream_t stream)
 {
   const long long npts = (long long)(s.M[0] + 3) * (s.M[1] + 3) * (s.M[2] + 3);
-  k_wrap_coefs<<<dim3((unsigned)npts), 256, 0, stream>>>(s, unique, norb, table);
-  return hipGetLastError();
+  return dispatch_prec(s.prec, [&](auto stv, auto) {
+    using ST = decltype(stv);
+    k_wrap_coefs<ST><<<dim3((unsigned)npts), 256, 0, stream>>>(s, unique, norb, static_cast<ST*>(table));
+    return hipGetLastError();
+  });
 }
 
 // Orbital VGL at every electron of determinant g (BsplineSet::mw_evaluate_notranspose,
-// BsplineSet.h:142-189).  Values are written transposed into Ainv (Ainv[w][j][i] = phi_j(r_i),
-// the column-major psiM handed to the LU), gradients/Laplacians into dpsi[w][i][0..3][j].
-__global__ __launch_bounds__(WG) void k_spo_rows(SysC s, State st, const double* __restrict__ coefs, int g)
+// BsplineSet.h:142-189).  Values are written transposed into the FP64 LU workspace
+// (lu[w][j][i] = phi_j(r_i), the column-major psiM handed to the LU; DiracMatrix.h:204-220 upcasts
+// to FP64 the same way in mixed precision), gradients/Laplacians into dpsi[w][i][0..3][j].
+template<typename ST, typename DT>
+__global__ __launch_bounds__(WG) void k_spo_rows(SysC s, State st, const ST* __restrict__ coefs, int g,
+                                                 double* __restrict__ lu)
 {
   const int i = blockIdx.x, w = blockIdx.y;
   const int n = s.n[g], ld = s.ld, Np = s.Npad;
@@ -39,35 +48,62 @@ __global__ __launch_bounds__(WG) void k_spo_rows(SysC s, State st, const double*
   const double pos[3] = {R[iat], R[Np + iat], R[2 * Np + iat]};
   SplineLoc L;
   spline_locate(s, pos, L);
-  double* A    = st.Ainv[g] + (size_t)w * n * ld;
-  double* drow = st.dpsi[g] + ((size_t)w * n + i) * 4 * ld;
+  double* A = lu + (size_t)w * n * ld;
+  DT* drow  = det_ptr<DT>(st.dpsi[g]) + ((size_t)w * n + i) * 4 * ld;
   for (int o = threadIdx.x; o < n; o += WG)
   {
     double out[5];
-    spline_vgl_orbital<double>(s, coefs, L, o, out);
+    spline_vgl_orbital<ST>(s, coefs, L, o, out);
     A[(size_t)o * ld + i] = out[0];
-    drow[o]               = out[1];
-    drow[ld + o]          = out[2];
-    drow[2 * ld + o]      = out[3];
-    drow[3 * ld + o]      = out[4];
+    drow[o]               = (DT)out[1];
+    drow[ld + o]          = (DT)out[2];
+    drow[2 * ld + o]      = (DT)out[3];
+    drow[3 * ld + o]      = (DT)out[4];
   }
 }
 
-hipError_t launch_spo_rows(const SysC& s, const State& st, const double* coefs, int g, int nw, hipStream_t stream)
+hipError_t launch_spo_rows(const SysC& s, const State& st, const void* coefs, int g, int nw, double* lu,
+                           hipStream_t stream)
 {
-  k_spo_rows<<<dim3(s.n[g], nw), WG, 0, stream>>>(s, st, coefs, g);
+  return dispatch_prec(s.prec, [&](auto stv, auto dtv) {
+    using ST = decltype(stv);
+    using DT = decltype(dtv);
+    k_spo_rows<ST, DT><<<dim3(s.n[g], nw), WG, 0, stream>>>(s, st, static_cast<const ST*>(coefs), g, lu);
+    return hipGetLastError();
+  });
+}
+
+// Ainv (determinant storage type) from the FP64 inverse in the LU workspace.
+template<typename DT>
+__global__ void k_store_inverse(SysC s, State st, int g, const double* __restrict__ lu)
+{
+  const int n = s.n[g], ld = s.ld;
+  const size_t per = (size_t)n * ld;
+  const size_t w   = blockIdx.y;
+  DT* A            = det_ptr<DT>(st.Ainv[g]) + w * per;
+  const double* L  = lu + w * per;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < per; e += (size_t)gridDim.x * blockDim.x)
+    A[e] = (DT)L[e];
+}
+
+hipError_t launch_store_inverse(const SysC& s, const State& st, int g, int nw, const double* lu, hipStream_t stream)
+{
+  if (s.prec == PREC_FP64)
+    return hipSuccess; // the LU workspace is Ainv itself
+  const size_t per = (size_t)s.n[g] * s.ld;
+  k_store_inverse<float><<<dim3((unsigned)std::min<size_t>(64, (per + 255) / 256), nw), 256, 0, stream>>>(s, st, g, lu);
   return hipGetLastError();
 }
 
 // log det from the LU diagonal and pivots (DiracMatrix.h:101-106): sum log(+-U_ii), sign flipped
 // where ipiv[i] != i+1; the phase counts pi per negative factor.
-__global__ void k_logdet(SysC s, State st, int g, int nw, const int* __restrict__ ipiv)
+__global__ void k_logdet(SysC s, State st, int g, int nw, const int* __restrict__ ipiv, const double* __restrict__ lu)
 {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nw)
     return;
   const int n = s.n[g], ld = s.ld;
-  const double* A = st.Ainv[g] + (size_t)w * n * ld;
+  const double* A = lu + (size_t)w * n * ld;
   const int* p    = ipiv + (size_t)w * n;
   double re = 0, im = 0;
   for (int i = 0; i < n; i++)
@@ -82,9 +118,10 @@ __global__ void k_logdet(SysC s, State st, int g, int nw, const int* __restrict_
   st.logdet[(size_t)w * 4 + g * 2 + 1] = im;
 }
 
-hipError_t launch_logdet(const SysC& s, const State& st, int g, int nw, const int* ipiv, hipStream_t stream)
+hipError_t launch_logdet(const SysC& s, const State& st, int g, int nw, const int* ipiv, const double* lu,
+                         hipStream_t stream)
 {
-  k_logdet<<<(nw + 63) / 64, 64, 0, stream>>>(s, st, g, nw, ipiv);
+  k_logdet<<<(nw + 63) / 64, 64, 0, stream>>>(s, st, g, nw, ipiv, lu);
   return hipGetLastError();
 }
 
@@ -163,6 +200,7 @@ hipError_t launch_jastrow_recompute(const SysC& s, const State& st, int nw, hipS
 // TrialWaveFunction::mw_evaluateGL (TrialWaveFunction.cpp:869-918): determinant rows
 // (DiracDeterminantBatched::computeGL, :593-604), J1 (J1OrbitalSoA.h:115-123), J2
 // (TwoBodyJastrow.cpp:769-778); kinetic energy -1/2 (sum G.G + sum L) (BareKineticEnergy.cpp:109-118).
+template<typename DT>
 __global__ __launch_bounds__(WG) void k_eval_gl(SysC s, State st, double* __restrict__ out3, double* __restrict__ Gout,
                                                 double* __restrict__ Lout)
 {
@@ -184,20 +222,20 @@ __global__ __launch_bounds__(WG) void k_eval_gl(SysC s, State st, double* __rest
     for (int g = 0; g < 2; g++)
     {
       const int n     = s.n[g];
-      const double* A = st.Ainv[g] + (size_t)w * n * ld;
-      const double* D = st.dpsi[g] + (size_t)w * n * 4 * ld;
+      const DT* A = det_ptr<DT>(st.Ainv[g]) + (size_t)w * n * ld;
+      const DT* D = det_ptr<DT>(st.dpsi[g]) + (size_t)w * n * 4 * ld;
       for (int i = wave; i < n; i += NWAVE)
       {
         double gx = 0, gy = 0, gz = 0, lp = 0;
-        const double* a = A + (size_t)i * ld;
-        const double* dd = D + (size_t)i * 4 * ld;
+        const DT* a  = A + (size_t)i * ld;
+        const DT* dd = D + (size_t)i * 4 * ld;
         for (int o = lane; o < n; o += 64)
         {
           const double av = a[o];
-          gx += av * dd[o];
-          gy += av * dd[ld + o];
-          gz += av * dd[2 * ld + o];
-          lp += av * dd[3 * ld + o];
+          gx += av * (double)dd[o];
+          gy += av * (double)dd[ld + o];
+          gz += av * (double)dd[2 * ld + o];
+          lp += av * (double)dd[3 * ld + o];
         }
         gx = wave_sum(gx);
         gy = wave_sum(gy);
@@ -271,15 +309,19 @@ hipError_t launch_eval_gl(const SysC& s, const State& st, int nw, double* out3, 
 {
   if (s.N > 1024)
     return hipErrorInvalidValue;
-  k_eval_gl<<<nw, WG, 0, stream>>>(s, st, out3, G, L);
-  return hipGetLastError();
+  return dispatch_prec(s.prec, [&](auto, auto dtv) {
+    using DT = decltype(dtv);
+    k_eval_gl<DT><<<nw, WG, 0, stream>>>(s, st, out3, G, L);
+    return hipGetLastError();
+  });
 }
 
 //------------------------------------------------------------------------------------------------
 // API helpers
 
 // SPOSet::mw_evaluateVGL: out[w][5][n] at pos[w]
-__global__ __launch_bounds__(WG) void k_api_spo_vgl(SysC s, const double* __restrict__ coefs, int n,
+template<typename ST>
+__global__ __launch_bounds__(WG) void k_api_spo_vgl(SysC s, const ST* __restrict__ coefs, int n,
                                                     const double* __restrict__ pos, double* __restrict__ out)
 {
   const int w = blockIdx.x;
@@ -289,22 +331,26 @@ __global__ __launch_bounds__(WG) void k_api_spo_vgl(SysC s, const double* __rest
   for (int o = threadIdx.x; o < n; o += WG)
   {
     double v[5];
-    spline_vgl_orbital<double>(s, coefs, L, o, v);
+    spline_vgl_orbital<ST>(s, coefs, L, o, v);
     for (int q = 0; q < 5; q++)
       out[((size_t)w * 5 + q) * n + o] = v[q];
   }
 }
 
-hipError_t launch_api_spo_vgl(const SysC& s, const double* coefs, int nw, int n, const double* pos, double* out,
+hipError_t launch_api_spo_vgl(const SysC& s, const void* coefs, int nw, int n, const double* pos, double* out,
                               hipStream_t stream)
 {
-  k_api_spo_vgl<<<nw, WG, 0, stream>>>(s, coefs, n, pos, out);
-  return hipGetLastError();
+  return dispatch_prec(s.prec, [&](auto stv, auto) {
+    using ST = decltype(stv);
+    k_api_spo_vgl<ST><<<nw, WG, 0, stream>>>(s, static_cast<const ST*>(coefs), n, pos, out);
+    return hipGetLastError();
+  });
 }
 
 // SPOSet::mw_evaluateDetRatios (SplineR2R.cpp:204-335): value-only orbitals at virtual positions
 // dotted with an inverse row.  Grid (nvp, nw).
-__global__ __launch_bounds__(WG) void k_api_det_ratios(SysC s, const double* __restrict__ coefs, int nvp, int n,
+template<typename ST>
+__global__ __launch_bounds__(WG) void k_api_det_ratios(SysC s, const ST* __restrict__ coefs, int nvp, int n,
                                                        const double* __restrict__ pos, const double* __restrict__ inv,
                                                        double* __restrict__ out)
 {
@@ -320,12 +366,13 @@ __global__ __launch_bounds__(WG) void k_api_det_ratios(SysC s, const double* __r
   {
     // value only: sum_ij a_i b_j sum_k c_k coef
     double val = 0;
-    const double* base = coefs + L.off + o;
+    const ST* base = coefs + L.off + o;
     for (int i = 0; i < 4; i++)
       for (int j = 0; j < 4; j++)
       {
-        const double* q = base + i * s.xs + j * s.ys;
-        const double s0 = L.c[0] * q[0] + L.c[1] * q[s.zs] + L.c[2] * q[2 * s.zs] + L.c[3] * q[3 * s.zs];
+        const ST* q     = base + i * s.xs + j * s.ys;
+        const double s0 = L.c[0] * (double)q[0] + L.c[1] * (double)q[s.zs] + L.c[2] * (double)q[2 * s.zs] +
+            L.c[3] * (double)q[3 * s.zs];
         val += (L.a[i] * L.b[j]) * s0;
       }
     acc[0] += val * inv[(size_t)w * n + o];
@@ -335,11 +382,14 @@ __global__ __launch_bounds__(WG) void k_api_det_ratios(SysC s, const double* __r
     out[(size_t)w * nvp + v] = bc[0];
 }
 
-hipError_t launch_api_det_ratios(const SysC& s, const double* coefs, int nw, int nvp, int n, const double* pos,
+hipError_t launch_api_det_ratios(const SysC& s, const void* coefs, int nw, int nvp, int n, const double* pos,
                                  const double* inv, double* out, hipStream_t stream)
 {
-  k_api_det_ratios<<<dim3(nvp, nw), WG, 0, stream>>>(s, coefs, nvp, n, pos, inv, out);
-  return hipGetLastError();
+  return dispatch_prec(s.prec, [&](auto stv, auto) {
+    using ST = decltype(stv);
+    k_api_det_ratios<ST><<<dim3(nvp, nw), WG, 0, stream>>>(s, static_cast<const ST*>(coefs), nvp, n, pos, inv, out);
+    return hipGetLastError();
+  });
 }
 
 // DistanceTable::mw_move rows: electrons and ions relative to pos[w] (source - target)

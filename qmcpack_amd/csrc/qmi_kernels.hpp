@@ -4,6 +4,20 @@
 
 namespace qmi
 {
+// Calls f(ST{}, DT{}) with the spline storage type ST and determinant storage type DT of a
+// precision mode.
+template<typename F>
+hipError_t dispatch_prec(int prec, F&& f)
+{
+  switch (prec)
+  {
+  case PREC_FP64: return f(double{}, double{});
+  case PREC_FP32: return f(float{}, float{});
+  case PREC_BF16: return f(bf16s{}, float{});
+  default: return hipErrorInvalidValue;
+  }
+}
+
 // arguments of the per-walker move kernels
 struct MoveArgs
 {
@@ -32,14 +46,21 @@ enum MoveKernel
   MK_MOVE_ONE
 };
 
-hipError_t launch_move(int which, const SysC& s, const State& st, const double* coefs, const MoveArgs& a,
+// coefs: the spline table in the storage type of s.prec (double, float or bf16s)
+hipError_t launch_move(int which, const SysC& s, const State& st, const void* coefs, const MoveArgs& a,
                        hipStream_t stream);
 hipError_t launch_make_move(const SysC& s, const State& st, int iat, int nw, const double* displ, hipStream_t stream);
 
 // initialisation / recompute / GL
-hipError_t launch_wrap_coefs(const SysC& s, const double* unique, int norb, double* table, hipStream_t stream);
-hipError_t launch_spo_rows(const SysC& s, const State& st, const double* coefs, int g, int nw, hipStream_t stream);
-hipError_t launch_logdet(const SysC& s, const State& st, int g, int nw, const int* ipiv, hipStream_t stream);
+hipError_t launch_wrap_coefs(const SysC& s, const double* unique, int norb, void* table, hipStream_t stream);
+// orbital rows of determinant g: values transposed into the FP64 LU workspace lu ([nw][n][ld]; Ainv
+// itself in FP64 contexts), gradients/Laplacians into dpsi
+hipError_t launch_spo_rows(const SysC& s, const State& st, const void* coefs, int g, int nw, double* lu,
+                           hipStream_t stream);
+hipError_t launch_logdet(const SysC& s, const State& st, int g, int nw, const int* ipiv, const double* lu,
+                         hipStream_t stream);
+// FP64 inverse in lu -> Ainv in the determinant storage type (mixed precisions)
+hipError_t launch_store_inverse(const SysC& s, const State& st, int g, int nw, const double* lu, hipStream_t stream);
 hipError_t launch_jastrow_recompute(const SysC& s, const State& st, int nw, hipStream_t stream);
 // per-walker outputs: out3[nw][3] = {ekin, logpsi, phase}; G[nw][N][3], L[nw][N] (nullable)
 hipError_t launch_eval_gl(const SysC& s, const State& st, int nw, double* out3, double* G, double* L,
@@ -49,9 +70,9 @@ hipError_t launch_eval_gl(const SysC& s, const State& st, int nw, double* out3, 
 hipError_t launch_flush(const SysC& s, const State& st, int g, int d, int nw, double* scratchT, hipStream_t stream);
 
 // API helpers
-hipError_t launch_api_spo_vgl(const SysC& s, const double* coefs, int nw, int n, const double* pos, double* out,
+hipError_t launch_api_spo_vgl(const SysC& s, const void* coefs, int nw, int n, const double* pos, double* out,
                               hipStream_t stream);
-hipError_t launch_api_det_ratios(const SysC& s, const double* coefs, int nw, int nvp, int n, const double* pos,
+hipError_t launch_api_det_ratios(const SysC& s, const void* coefs, int nw, int nvp, int n, const double* pos,
                                  const double* inv, double* out, hipStream_t stream);
 hipError_t launch_api_distances(const SysC& s, const State& st, int nw, const double* pos, double* ee_r,
                                 double* ee_dr, double* ei_r, double* ei_dr, hipStream_t stream);

@@ -45,9 +45,10 @@ constexpr int JPT = (2 * MAX_N + WG - 1) / WG;
 
 // The leading d x d block of Binv is staged in dynamically sized LDS with row stride k + 1
 // (column reads by consecutive threads then fall in different banks).
-__host__ __device__ constexpr size_t binv_lds_bytes(int k) { return (size_t)k * (k + 1) * sizeof(double); }
+__host__ __device__ constexpr size_t binv_lds_bytes(int k, size_t esize) { return (size_t)k * (k + 1) * esize; }
 
-__device__ inline void stage_binv(const double* __restrict__ Binv, int k, int d, double* sB)
+template<typename DT>
+__device__ inline void stage_binv(const DT* __restrict__ Binv, int k, int d, DT* sB)
 {
   for (int e = threadIdx.x; e < d * d; e += WG)
   {
@@ -60,17 +61,18 @@ __device__ inline void stage_binv(const double* __restrict__ Binv, int k, int d,
 // stride k).  Thread (c = t % k, q = t / k) sums orbitals o = q, q + G, ... (G = WG / k groups),
 // so consecutive threads read consecutive doubles of a row and every load of the unrolled loop is
 // independent; the G partials per column are added through LDS.  Ends with a barrier.
-__device__ inline void delayed_gemv(const double* __restrict__ X, const double* vec, int n, int k, int d,
+template<typename DT>
+__device__ inline void delayed_gemv(const DT* __restrict__ X, const double* vec, int n, int k, int d,
                                     double* red, double* out, double scale)
 {
   const int t = threadIdx.x, G = WG / k, c = t % k, q = t / k;
   double a0 = 0;
   if (q < G && c < d)
   {
-    const double* xp = X + c;
+    const DT* xp = X + c;
 #pragma unroll 8
     for (int o = q; o < n; o += G)
-      a0 += xp[(size_t)o * k] * vec[o];
+      a0 += (double)xp[(size_t)o * k] * vec[o];
   }
   red[t] = a0;
   __syncthreads();
@@ -85,27 +87,42 @@ __device__ inline void delayed_gemv(const double* __restrict__ X, const double* 
 }
 
 // sum_{c<d} v[c] y[c] over one orbital-major row; y is zero-padded to k so that an even-k row can
-// be read in 16-byte pairs (the column past an odd d is a finite stored value times zero).
-__device__ inline double row_dot(const double* __restrict__ v, const double* y, int d, int k)
+// be read in pairs (the column past an odd d is a finite stored value times zero).
+template<typename DT>
+struct Pair2;
+template<>
+struct Pair2<double>
+{
+  using T = double2;
+};
+template<>
+struct Pair2<float>
+{
+  using T = float2;
+};
+
+template<typename DT>
+__device__ inline double row_dot(const DT* __restrict__ v, const double* y, int d, int k)
 {
   double a0 = 0, a1 = 0;
   if ((k & 1) == 0)
   {
-    const double2* v2 = reinterpret_cast<const double2*>(v);
-    const int h       = (d + 1) >> 1;
+    using P2     = typename Pair2<DT>::T;
+    const P2* v2 = reinterpret_cast<const P2*>(v);
+    const int h  = (d + 1) >> 1;
 #pragma unroll 4
     for (int c = 0; c < h; c++)
     {
-      const double2 x = v2[c];
-      a0 += x.x * y[2 * c];
-      a1 += x.y * y[2 * c + 1];
+      const P2 x = v2[c];
+      a0 += (double)x.x * y[2 * c];
+      a1 += (double)x.y * y[2 * c + 1];
     }
   }
   else
   {
 #pragma unroll 4
     for (int c = 0; c < d; c++)
-      a0 += v[c] * y[c];
+      a0 += (double)v[c] * y[c];
   }
   return a0 + a1;
 }
@@ -125,13 +142,13 @@ enum BcField
 };
 
 //------------------------------------------------------------------------------------------------
-template<int MAXM>
-__device__ inline void phaseA(const SysC& s, int n, int ld, int k, int d, int r, const double* __restrict__ Ainv,
-                              const double* __restrict__ U, const double* __restrict__ V, double* __restrict__ Binv,
-                              MoveLDS& sh, double* sB, double (&ainv_r)[MAXM], double (&inv)[MAXM])
+template<typename DT, int MAXM>
+__device__ inline void phaseA(const SysC& s, int n, int ld, int k, int d, int r, const DT* __restrict__ Ainv,
+                              const DT* __restrict__ U, const DT* __restrict__ V, DT* __restrict__ Binv, MoveLDS& sh,
+                              DT* sB, double (&ainv_r)[MAXM], double (&inv)[MAXM])
 {
-  const int t = threadIdx.x;
-  const double* arow = Ainv + (size_t)r * ld;
+  const int t    = threadIdx.x;
+  const DT* arow = Ainv + (size_t)r * ld;
 #pragma unroll
   for (int m = 0; m < MAXM; m++)
   {
@@ -162,7 +179,7 @@ __device__ inline void phaseA(const SysC& s, int n, int ld, int k, int d, int r,
       const double y = -1.0 * acc;
       sh.y[t]        = t < d ? y : 0.0;
       if (t < d)
-        Binv[d * k + t] = y;
+        Binv[d * k + t] = (DT)y;
     }
     __syncthreads();
     // invRow = Ainv[r] + V^T y
@@ -177,22 +194,22 @@ __device__ inline void phaseA(const SysC& s, int n, int ld, int k, int d, int r,
 }
 
 // determinant gradient at the current position: invRow . dpsiM[r]  -> sh.bc[BC_GRAD..+3)
-template<int MAXM>
-__device__ inline void det_grad(int n, int ld, int r, const double* __restrict__ dpsi, const double (&inv)[MAXM],
+template<typename DT, int MAXM>
+__device__ inline void det_grad(int n, int ld, int r, const DT* __restrict__ dpsi, const double (&inv)[MAXM],
                                 MoveLDS& sh)
 {
-  const int t = threadIdx.x;
-  double gv[3] = {0, 0, 0};
-  const double* drow = dpsi + (size_t)r * 4 * ld;
+  const int t    = threadIdx.x;
+  double gv[3]   = {0, 0, 0};
+  const DT* drow = dpsi + (size_t)r * 4 * ld;
 #pragma unroll
   for (int m = 0; m < MAXM; m++)
   {
     const int o = t + WG * m;
     if (o < n)
     {
-      gv[0] += inv[m] * drow[o];
-      gv[1] += inv[m] * drow[ld + o];
-      gv[2] += inv[m] * drow[2 * ld + o];
+      gv[0] += inv[m] * (double)drow[o];
+      gv[1] += inv[m] * (double)drow[ld + o];
+      gv[2] += inv[m] * (double)drow[2 * ld + o];
     }
   }
   block_sum<3>(gv, sh.red, sh.bc + BC_GRAD);
@@ -329,12 +346,12 @@ __device__ inline void combine_ratio(const SysC& s, int iat, const double* __res
 }
 
 // Phase E: accept (sh.bc[BC_ACC] != 0) or reject.  All threads.
-template<int MAXM>
+template<typename DT, int MAXM>
 __device__ inline void phaseE(const SysC& s, int g, int iat, int n, int ld, int k, int d, int r, double* __restrict__ R,
-                              double* __restrict__ Ainv, double* __restrict__ dpsi, double* __restrict__ U,
-                              double* __restrict__ V, double* __restrict__ Binv, int* __restrict__ dl,
-                              double* __restrict__ jas, double* __restrict__ logdet, const double (&ainv_r)[MAXM],
-                              const double (&phv)[MAXM][5], MoveLDS& sh, double* sB, bool stage)
+                              DT* __restrict__ Ainv, DT* __restrict__ dpsi, DT* __restrict__ U, DT* __restrict__ V,
+                              DT* __restrict__ Binv, int* __restrict__ dl, double* __restrict__ jas,
+                              double* __restrict__ logdet, const double (&ainv_r)[MAXM], const double (&phv)[MAXM][5],
+                              MoveLDS& sh, DT* sB, bool stage)
 {
   const int t = threadIdx.x;
   const int N = s.N, Np = s.Npad;
@@ -350,7 +367,7 @@ __device__ inline void phaseE(const SysC& s, int g, int iat, int n, int ld, int 
       {
         const int o = t + WG * m;
         if (o < n)
-          V[(size_t)o * k + d] = ainv_r[m];
+          V[(size_t)o * k + d] = (DT)ainv_r[m];
       }
       if (accepted)
       {
@@ -373,34 +390,34 @@ __device__ inline void phaseE(const SysC& s, int g, int iat, int n, int ld, int 
           for (int j = 0; j < d; j++)
             acc += sB[t * (k + 1) + j] * sh.p[j];
           const double Yv = sigma * acc;
-          Binv[t * k + d] = Yv;
+          Binv[t * k + d] = (DT)Yv;
           sh.Y[t]         = Yv;
         }
         __syncthreads();
         for (int idx = t; idx < d * d; idx += WG)
         {
           const int j = idx / d, i = idx - j * d;
-          Binv[j * k + i] = sB[j * (k + 1) + i] + 1.0 * sh.y[i] * sh.Y[j];
+          Binv[j * k + i] = (DT)((double)sB[j * (k + 1) + i] + 1.0 * sh.y[i] * sh.Y[j]);
         }
         if (t < d)
-          Binv[d * k + t] = sh.y[t] * sigma;
+          Binv[d * k + t] = (DT)(sh.y[t] * sigma);
         if (t == 0)
         {
-          Binv[d * k + d] = sigma;
+          Binv[d * k + d] = (DT)sigma;
           dl[d]           = r;
         }
-        double* drow = dpsi + (size_t)r * 4 * ld;
+        DT* drow = dpsi + (size_t)r * 4 * ld;
 #pragma unroll
         for (int m = 0; m < MAXM; m++)
         {
           const int o = t + WG * m;
           if (o < n)
           {
-            U[(size_t)o * k + d]  = phv[m][0];
-            drow[o]               = phv[m][1];
-            drow[ld + o]          = phv[m][2];
-            drow[2 * ld + o]      = phv[m][3];
-            drow[3 * ld + o]      = phv[m][4];
+            U[(size_t)o * k + d] = (DT)phv[m][0];
+            drow[o]              = (DT)phv[m][1];
+            drow[ld + o]         = (DT)phv[m][2];
+            drow[2 * ld + o]     = (DT)phv[m][3];
+            drow[3 * ld + o]     = (DT)phv[m][4];
           }
         }
       }
@@ -444,9 +461,9 @@ __device__ inline void phaseE(const SysC& s, int g, int iat, int n, int ld, int 
         if (i < n)
         {
           double acc      = 0;
-          const double* ai = Ainv + (size_t)i * ld;
+          const DT* ai = Ainv + (size_t)i * ld;
           for (int o = 0; o < n; o++)
-            acc += ai[o] * sh.vec[o];
+            acc += (double)ai[o] * sh.vec[o];
           sh.row[i] = acc;
         }
       }
@@ -463,20 +480,20 @@ __device__ inline void phaseE(const SysC& s, int g, int iat, int n, int ld, int 
         {
           const int o = t + WG * m;
           if (o < n)
-            Ainv[(size_t)i * ld + o] += ci * ainv_r[m];
+            Ainv[(size_t)i * ld + o] = (DT)((double)Ainv[(size_t)i * ld + o] + ci * ainv_r[m]);
         }
       }
-      double* drow = dpsi + (size_t)r * 4 * ld;
+      DT* drow = dpsi + (size_t)r * 4 * ld;
 #pragma unroll
       for (int m = 0; m < MAXM; m++)
       {
         const int o = t + WG * m;
         if (o < n)
         {
-          drow[o]          = phv[m][1];
-          drow[ld + o]     = phv[m][2];
-          drow[2 * ld + o] = phv[m][3];
-          drow[3 * ld + o] = phv[m][4];
+          drow[o]          = (DT)phv[m][1];
+          drow[ld + o]     = (DT)phv[m][2];
+          drow[2 * ld + o] = (DT)phv[m][3];
+          drow[3 * ld + o] = (DT)phv[m][4];
         }
       }
     }
@@ -564,22 +581,26 @@ __device__ inline void phaseE(const SysC& s, int g, int iat, int n, int ld, int 
 }
 
 //------------------------------------------------------------------------------------------------
+template<typename DT>
 struct WalkerView
 {
-  double *R, *Ainv, *dpsi, *U, *V, *Binv, *jas, *logdet, *stats;
+  double* R;
+  DT *Ainv, *dpsi, *U, *V, *Binv;
+  double *jas, *logdet, *stats;
   int* dl;
 };
 
-__device__ inline WalkerView walker_view(const SysC& s, const State& st, int g, int w)
+template<typename DT>
+__device__ inline WalkerView<DT> walker_view(const SysC& s, const State& st, int g, int w)
 {
-  WalkerView v;
+  WalkerView<DT> v;
   const int n = s.n[g], ld = s.ld, k = s.k;
   v.R      = st.R + (size_t)w * 3 * s.Npad;
-  v.Ainv   = st.Ainv[g] + (size_t)w * n * ld;
-  v.dpsi   = st.dpsi[g] + (size_t)w * n * 4 * ld;
-  v.U      = st.U[g] + (size_t)w * ld * k;
-  v.V      = st.V[g] + (size_t)w * ld * k;
-  v.Binv   = st.Binv[g] + (size_t)w * k * k;
+  v.Ainv   = det_ptr<DT>(st.Ainv[g]) + (size_t)w * n * ld;
+  v.dpsi   = det_ptr<DT>(st.dpsi[g]) + (size_t)w * n * 4 * ld;
+  v.U      = det_ptr<DT>(st.U[g]) + (size_t)w * ld * k;
+  v.V      = det_ptr<DT>(st.V[g]) + (size_t)w * ld * k;
+  v.Binv   = det_ptr<DT>(st.Binv[g]) + (size_t)w * k * k;
   v.dl     = st.dlist[g] + (size_t)w * k;
   v.jas    = st.jas + (size_t)w * J_ROWS * s.Npad;
   v.logdet = st.logdet + (size_t)w * 4;
@@ -642,13 +663,13 @@ __device__ inline double mh_prob(const SysC& s, int iat, const MoveArgs& a, cons
 //------------------------------------------------------------------------------------------------
 // One fused move (phases A-E) of electron iat for walker w at delay d; the walker's uniform
 // counter is carried in ucount.
-template<typename ST, int MAXM>
+template<typename ST, typename DT, int MAXM>
 __device__ inline void fused_move(const SysC& s, const State& st, const ST* __restrict__ coefs, const MoveArgs& a,
-                                  int w, int iat, int d, MoveLDS& sh, double* sB, int& ucount, double& nacc,
+                                  int w, int iat, int d, MoveLDS& sh, DT* sB, int& ucount, double& nacc,
                                   double& nrej)
 {
   const int g = a.g, n = s.n[g], ld = s.ld, k = s.k, N = s.N;
-  WalkerView v = walker_view(s, st, g, w);
+  WalkerView<DT> v = walker_view<DT>(s, st, g, w);
   const int r = iat - s.first[g];
   double ainv_r[MAXM], inv[MAXM], phv[MAXM][5];
 #pragma unroll
@@ -658,8 +679,8 @@ __device__ inline void fused_move(const SysC& s, const State& st, const ST* __re
   }
   if (s.has_det)
   {
-    phaseA<MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, sB, ainv_r, inv);
-    det_grad<MAXM>(n, ld, r, v.dpsi, inv, sh);
+    phaseA<DT, MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, sB, ainv_r, inv);
+    det_grad<DT, MAXM>(n, ld, r, v.dpsi, inv, sh);
   }
   if (threadIdx.x == 0)
     propose(s, iat, w, a, v.R, v.jas, sh);
@@ -695,17 +716,18 @@ __device__ inline void fused_move(const SysC& s, const State& st, const ST* __re
     }
   }
   __syncthreads();
-  phaseE<MAXM>(s, g, iat, n, ld, k, d, r, v.R, v.Ainv, v.dpsi, v.U, v.V, v.Binv, v.dl, v.jas, v.logdet, ainv_r, phv,
+  phaseE<DT, MAXM>(s, g, iat, n, ld, k, d, r, v.R, v.Ainv, v.dpsi, v.U, v.V, v.Binv, v.dl, v.jas, v.logdet, ainv_r, phv,
                sh, sB, false);
 }
 
 // Fused sweep segment: moves [iat0, iat1) of spin group g for every walker (one walker per
 // workgroup, one RNG stream per walker), delay count d0 at iat0 with d0 + (iat1 - iat0) <= k.
-template<typename ST, int MAXM>
+template<typename ST, typename DT, int MAXM>
 __global__ __launch_bounds__(WG) void k_sweep_segment(SysC s, State st, const ST* __restrict__ coefs, MoveArgs a)
 {
   __shared__ MoveLDS sh;
-  extern __shared__ double sB[];
+  extern __shared__ double sB_raw[];
+  DT* sB = reinterpret_cast<DT*>(sB_raw);
   stage_jastrow(s, sh);
   const int w = blockIdx.x;
   if (w >= a.nw)
@@ -716,7 +738,7 @@ __global__ __launch_bounds__(WG) void k_sweep_segment(SysC s, State st, const ST
   for (int iat = a.iat0; iat < a.iat1; ++iat)
   {
     const int d = s.k > 1 ? a.d0 + (iat - a.iat0) : 0;
-    fused_move<ST, MAXM>(s, st, coefs, a, w, iat, d, sh, sB, ucount, nacc, nrej);
+    fused_move<ST, DT, MAXM>(s, st, coefs, a, w, iat, d, sh, sB, ucount, nacc, nrej);
   }
   if (threadIdx.x == 0)
   {
@@ -728,11 +750,12 @@ __global__ __launch_bounds__(WG) void k_sweep_segment(SysC s, State st, const ST
 
 // The same fused move for a single electron a.iat0 (no move loop in the kernel: the register
 // allocation of the loop version is about twice that of one move, which halves occupancy).
-template<typename ST, int MAXM>
+template<typename ST, typename DT, int MAXM>
 __global__ __launch_bounds__(WG) void k_move_one(SysC s, State st, const ST* __restrict__ coefs, MoveArgs a)
 {
   __shared__ MoveLDS sh;
-  extern __shared__ double sB[];
+  extern __shared__ double sB_raw[];
+  DT* sB = reinterpret_cast<DT*>(sB_raw);
   stage_jastrow(s, sh);
   const int w = blockIdx.x;
   if (w >= a.nw)
@@ -740,7 +763,7 @@ __global__ __launch_bounds__(WG) void k_move_one(SysC s, State st, const ST* __r
   const int N = s.N;
   int ucount  = a.ubase[(size_t)w * (N + 1) + a.iat0];
   double nacc = 0, nrej = 0;
-  fused_move<ST, MAXM>(s, st, coefs, a, w, a.iat0, a.d0, sh, sB, ucount, nacc, nrej);
+  fused_move<ST, DT, MAXM>(s, st, coefs, a, w, a.iat0, a.d0, sh, sB, ucount, nacc, nrej);
   if (threadIdx.x == 0)
   {
     a.ubase[(size_t)w * (N + 1) + a.iat0 + 1] = ucount;
@@ -751,23 +774,24 @@ __global__ __launch_bounds__(WG) void k_move_one(SysC s, State st, const ST* __r
 
 // Crowd path, first kernel: phases A-C and the MH probability for one move; the proposed orbitals
 // go to st.phi and the scalars to st.mv.
-template<typename ST, int MAXM>
+template<typename ST, typename DT, int MAXM>
 __global__ __launch_bounds__(WG) void k_propose(SysC s, State st, const ST* __restrict__ coefs, MoveArgs a)
 {
   __shared__ MoveLDS sh;
-  extern __shared__ double sB[];
+  extern __shared__ double sB_raw[];
+  DT* sB = reinterpret_cast<DT*>(sB_raw);
   stage_jastrow(s, sh);
   const int w = blockIdx.x;
   if (w >= a.nw)
     return;
   const int g = a.g, n = s.n[g], ld = s.ld, k = s.k;
   const int iat = a.iat0, r = iat - s.first[g], d = a.d0;
-  WalkerView v = walker_view(s, st, g, w);
+  WalkerView<DT> v = walker_view<DT>(s, st, g, w);
   double ainv_r[MAXM], inv[MAXM], phv[MAXM][5];
   if (s.has_det)
   {
-    phaseA<MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, sB, ainv_r, inv);
-    det_grad<MAXM>(n, ld, r, v.dpsi, inv, sh);
+    phaseA<DT, MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, sB, ainv_r, inv);
+    det_grad<DT, MAXM>(n, ld, r, v.dpsi, inv, sh);
   }
   if (threadIdx.x == 0)
     propose(s, iat, w, a, v.R, v.jas, sh);
@@ -803,11 +827,12 @@ __global__ __launch_bounds__(WG) void k_propose(SysC s, State st, const ST* __re
 
 // Crowd path, second kernel: Metropolis-Hastings with the crowd's shared RNG stream (uniform index =
 // crowd base + number of earlier walkers of the crowd that draw), then phase E.
-template<int MAXM>
+template<typename DT, int MAXM>
 __global__ __launch_bounds__(WG) void k_accept_crowd(SysC s, State st, MoveArgs a)
 {
   __shared__ MoveLDS sh;
-  extern __shared__ double sB[];
+  extern __shared__ double sB_raw[];
+  DT* sB = reinterpret_cast<DT*>(sB_raw);
   stage_jastrow(s, sh);
   const int w = blockIdx.x;
   if (w >= a.nw)
@@ -815,7 +840,7 @@ __global__ __launch_bounds__(WG) void k_accept_crowd(SysC s, State st, MoveArgs 
   const int g = a.g, n = s.n[g], ld = s.ld, k = s.k, N = s.N;
   const int iat = a.iat0, r = iat - s.first[g], d = a.d0;
   const int crowd = w / a.wpc, w0 = crowd * a.wpc, w1 = min(a.nw, w0 + a.wpc);
-  WalkerView v = walker_view(s, st, g, w);
+  WalkerView<DT> v = walker_view<DT>(s, st, g, w);
   const double* mv = st.mv + (size_t)w * MV_SIZE;
   // prefix count of drawing walkers before w in the crowd, and the crowd total
   double cnt[2] = {0, 0};
@@ -870,7 +895,7 @@ __global__ __launch_bounds__(WG) void k_accept_crowd(SysC s, State st, MoveArgs 
       phv[m][q] = o < n ? phi[q * ld + o] : 0.0;
   }
   __syncthreads();
-  phaseE<MAXM>(s, g, iat, n, ld, k, d, r, v.R, v.Ainv, v.dpsi, v.U, v.V, v.Binv, v.dl, v.jas, v.logdet, ainv_r, phv,
+  phaseE<DT, MAXM>(s, g, iat, n, ld, k, d, r, v.R, v.Ainv, v.dpsi, v.U, v.V, v.Binv, v.dl, v.jas, v.logdet, ainv_r, phv,
                sh, sB, true);
 }
 
@@ -878,23 +903,24 @@ __global__ __launch_bounds__(WG) void k_accept_crowd(SysC s, State st, MoveArgs 
 // API kernels (reference call protocol, host-side Metropolis-Hastings)
 
 // mw_evalGrad: inverse row (stored to st.invRow) and total gradient at R[iat] -> out[nw][3]
-template<int MAXM>
+template<typename DT, int MAXM>
 __global__ __launch_bounds__(WG) void k_api_eval_grad(SysC s, State st, MoveArgs a)
 {
   __shared__ MoveLDS sh;
-  extern __shared__ double sB[];
+  extern __shared__ double sB_raw[];
+  DT* sB = reinterpret_cast<DT*>(sB_raw);
   stage_jastrow(s, sh);
   const int w = blockIdx.x;
   if (w >= a.nw)
     return;
   const int g = a.g, n = s.n[g], ld = s.ld, k = s.k, Np = s.Npad;
   const int iat = a.iat0, r = iat - s.first[g], d = a.d0;
-  WalkerView v = walker_view(s, st, g, w);
+  WalkerView<DT> v = walker_view<DT>(s, st, g, w);
   double ainv_r[MAXM], inv[MAXM];
   if (s.has_det)
   {
-    phaseA<MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, sB, ainv_r, inv);
-    det_grad<MAXM>(n, ld, r, v.dpsi, inv, sh);
+    phaseA<DT, MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, sB, ainv_r, inv);
+    det_grad<DT, MAXM>(n, ld, r, v.dpsi, inv, sh);
     double* ir = st.invRow + (size_t)w * ld;
 #pragma unroll
     for (int m = 0; m < MAXM; m++)
@@ -922,25 +948,26 @@ __global__ __launch_bounds__(WG) void k_api_eval_grad(SysC s, State st, MoveArgs
 
 // mw_calcRatioGrad at the position stored by mw_makeMove (st.mv[MV_POS]); prepares the inverse
 // row first when `a.d0 >= 0` and the row is not prepared (a.flag_prepare).
-template<typename ST, int MAXM>
+template<typename ST, typename DT, int MAXM>
 __global__ __launch_bounds__(WG) void k_api_ratio_grad(SysC s, State st, const ST* __restrict__ coefs, MoveArgs a)
 {
   __shared__ MoveLDS sh;
-  extern __shared__ double sB[];
+  extern __shared__ double sB_raw[];
+  DT* sB = reinterpret_cast<DT*>(sB_raw);
   stage_jastrow(s, sh);
   const int w = blockIdx.x;
   if (w >= a.nw)
     return;
   const int g = a.g, n = s.n[g], ld = s.ld, k = s.k;
   const int iat = a.iat0, r = iat - s.first[g], d = a.d0;
-  WalkerView v = walker_view(s, st, g, w);
+  WalkerView<DT> v = walker_view<DT>(s, st, g, w);
   double* mv = st.mv + (size_t)w * MV_SIZE;
   double ainv_r[MAXM], inv[MAXM], phv[MAXM][5];
   if (s.has_det)
   {
     if (a.prepare)
     {
-      phaseA<MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, sB, ainv_r, inv);
+      phaseA<DT, MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, sB, ainv_r, inv);
       double* ir = st.invRow + (size_t)w * ld;
 #pragma unroll
       for (int m = 0; m < MAXM; m++)
@@ -988,18 +1015,19 @@ __global__ __launch_bounds__(WG) void k_api_ratio_grad(SysC s, State st, const S
 }
 
 // mw_accept_rejectMove with host decisions a.host_acc[nw]
-template<int MAXM>
+template<typename DT, int MAXM>
 __global__ __launch_bounds__(WG) void k_api_accept(SysC s, State st, MoveArgs a)
 {
   __shared__ MoveLDS sh;
-  extern __shared__ double sB[];
+  extern __shared__ double sB_raw[];
+  DT* sB = reinterpret_cast<DT*>(sB_raw);
   stage_jastrow(s, sh);
   const int w = blockIdx.x;
   if (w >= a.nw)
     return;
   const int g = a.g, n = s.n[g], ld = s.ld, k = s.k;
   const int iat = a.iat0, r = iat - s.first[g], d = a.d0;
-  WalkerView v = walker_view(s, st, g, w);
+  WalkerView<DT> v = walker_view<DT>(s, st, g, w);
   const double* mv = st.mv + (size_t)w * MV_SIZE;
   if (threadIdx.x == 0)
   {
@@ -1025,7 +1053,7 @@ __global__ __launch_bounds__(WG) void k_api_accept(SysC s, State st, MoveArgs a)
       phv[m][q] = o < n ? phi[q * ld + o] : 0.0;
   }
   __syncthreads();
-  phaseE<MAXM>(s, g, iat, n, ld, k, d, r, v.R, v.Ainv, v.dpsi, v.U, v.V, v.Binv, v.dl, v.jas, v.logdet, ainv_r, phv,
+  phaseE<DT, MAXM>(s, g, iat, n, ld, k, d, r, v.R, v.Ainv, v.dpsi, v.U, v.V, v.Binv, v.dl, v.jas, v.logdet, ainv_r, phv,
                sh, sB, true);
 }
 
@@ -1043,24 +1071,24 @@ __global__ void k_api_make_move(SysC s, State st, int iat, int nw, const double*
 
 //------------------------------------------------------------------------------------------------
 // launchers
-template<typename ST>
+template<typename ST, typename DT>
 static hipError_t launch_move_t(int which, const SysC& s, const State& st, const ST* coefs, const MoveArgs& a,
                                 hipStream_t stream)
 {
   const int maxn = s.n[a.g];
   dim3 grid(a.nw), block(WG);
-  const size_t dyn = binv_lds_bytes(s.k);
+  const size_t dyn = binv_lds_bytes(s.k, sizeof(DT));
   if (maxn <= WG)
   {
     switch (which)
     {
-    case MK_SEGMENT: k_sweep_segment<ST, 1><<<grid, block, dyn, stream>>>(s, st, coefs, a); break;
-    case MK_MOVE_ONE: k_move_one<ST, 1><<<grid, block, dyn, stream>>>(s, st, coefs, a); break;
-    case MK_PROPOSE: k_propose<ST, 1><<<grid, block, dyn, stream>>>(s, st, coefs, a); break;
-    case MK_ACCEPT_CROWD: k_accept_crowd<1><<<grid, block, dyn, stream>>>(s, st, a); break;
-    case MK_API_GRAD: k_api_eval_grad<1><<<grid, block, dyn, stream>>>(s, st, a); break;
-    case MK_API_RATIO: k_api_ratio_grad<ST, 1><<<grid, block, dyn, stream>>>(s, st, coefs, a); break;
-    case MK_API_ACCEPT: k_api_accept<1><<<grid, block, dyn, stream>>>(s, st, a); break;
+    case MK_SEGMENT: k_sweep_segment<ST, DT, 1><<<grid, block, dyn, stream>>>(s, st, coefs, a); break;
+    case MK_MOVE_ONE: k_move_one<ST, DT, 1><<<grid, block, dyn, stream>>>(s, st, coefs, a); break;
+    case MK_PROPOSE: k_propose<ST, DT, 1><<<grid, block, dyn, stream>>>(s, st, coefs, a); break;
+    case MK_ACCEPT_CROWD: k_accept_crowd<DT, 1><<<grid, block, dyn, stream>>>(s, st, a); break;
+    case MK_API_GRAD: k_api_eval_grad<DT, 1><<<grid, block, dyn, stream>>>(s, st, a); break;
+    case MK_API_RATIO: k_api_ratio_grad<ST, DT, 1><<<grid, block, dyn, stream>>>(s, st, coefs, a); break;
+    case MK_API_ACCEPT: k_api_accept<DT, 1><<<grid, block, dyn, stream>>>(s, st, a); break;
     default: return hipErrorInvalidValue;
     }
   }
@@ -1068,13 +1096,13 @@ static hipError_t launch_move_t(int which, const SysC& s, const State& st, const
   {
     switch (which)
     {
-    case MK_SEGMENT: k_sweep_segment<ST, 2><<<grid, block, dyn, stream>>>(s, st, coefs, a); break;
-    case MK_MOVE_ONE: k_move_one<ST, 2><<<grid, block, dyn, stream>>>(s, st, coefs, a); break;
-    case MK_PROPOSE: k_propose<ST, 2><<<grid, block, dyn, stream>>>(s, st, coefs, a); break;
-    case MK_ACCEPT_CROWD: k_accept_crowd<2><<<grid, block, dyn, stream>>>(s, st, a); break;
-    case MK_API_GRAD: k_api_eval_grad<2><<<grid, block, dyn, stream>>>(s, st, a); break;
-    case MK_API_RATIO: k_api_ratio_grad<ST, 2><<<grid, block, dyn, stream>>>(s, st, coefs, a); break;
-    case MK_API_ACCEPT: k_api_accept<2><<<grid, block, dyn, stream>>>(s, st, a); break;
+    case MK_SEGMENT: k_sweep_segment<ST, DT, 2><<<grid, block, dyn, stream>>>(s, st, coefs, a); break;
+    case MK_MOVE_ONE: k_move_one<ST, DT, 2><<<grid, block, dyn, stream>>>(s, st, coefs, a); break;
+    case MK_PROPOSE: k_propose<ST, DT, 2><<<grid, block, dyn, stream>>>(s, st, coefs, a); break;
+    case MK_ACCEPT_CROWD: k_accept_crowd<DT, 2><<<grid, block, dyn, stream>>>(s, st, a); break;
+    case MK_API_GRAD: k_api_eval_grad<DT, 2><<<grid, block, dyn, stream>>>(s, st, a); break;
+    case MK_API_RATIO: k_api_ratio_grad<ST, DT, 2><<<grid, block, dyn, stream>>>(s, st, coefs, a); break;
+    case MK_API_ACCEPT: k_api_accept<DT, 2><<<grid, block, dyn, stream>>>(s, st, a); break;
     default: return hipErrorInvalidValue;
     }
   }
@@ -1083,10 +1111,14 @@ static hipError_t launch_move_t(int which, const SysC& s, const State& st, const
   return hipGetLastError();
 }
 
-hipError_t launch_move(int which, const SysC& s, const State& st, const double* coefs, const MoveArgs& a,
+hipError_t launch_move(int which, const SysC& s, const State& st, const void* coefs, const MoveArgs& a,
                        hipStream_t stream)
 {
-  return launch_move_t<double>(which, s, st, coefs, a, stream);
+  return dispatch_prec(s.prec, [&](auto stv, auto dtv) {
+    using ST = decltype(stv);
+    using DT = decltype(dtv);
+    return launch_move_t<ST, DT>(which, s, st, static_cast<const ST*>(coefs), a, stream);
+  });
 }
 
 hipError_t launch_make_move(const SysC& s, const State& st, int iat, int nw, const double* displ, hipStream_t stream)

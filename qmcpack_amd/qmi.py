@@ -101,13 +101,21 @@ def make_desc(system, keep: list) -> SystemDesc:
     return d
 
 
-class Context:
-    """One qmcmi context: one device stream, one system, one walker population."""
+# storage precisions (include/qmcmi.h)
+PREC_FP64, PREC_MIXED_FP32, PREC_MIXED_BF16 = 0, 1, 2
+PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_MIXED_FP32, "bf16": PREC_MIXED_BF16}
 
-    def __init__(self, device: int = 0):
+
+class Context:
+    """One qmcmi context: one device stream, one system, one walker population.
+
+    precision: PREC_FP64 (default), PREC_MIXED_FP32 or PREC_MIXED_BF16, or their names."""
+
+    def __init__(self, device: int = 0, precision=PREC_FP64):
         self.L = load_library()
         self.h = C.c_void_p()
-        st = self.L.qmi_ctx_create(C.byref(self.h), device, 0)
+        self.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        st = self.L.qmi_ctx_create(C.byref(self.h), device, self.precision)
         if st != 0:
             raise QmiError(st, "qmi_ctx_create failed (no HIP device?)")
         self.nw = 0

@@ -66,6 +66,22 @@ def test_spo_vgl_and_distances():
         assert np.allclose(ei_r[w], r, rtol=1e-14, atol=1e-13) and np.allclose(ei_dr[w], dr, rtol=1e-14, atol=1e-13)
 
 
+
+def test_det_ratios_virtual_positions():
+    """SPOSet::mw_evaluateDetRatios (SplineR2R.cpp:204-335): value-only orbitals at several virtual
+    positions per walker dotted with a given inverse row; oracle = its orbital values (which the
+    spline known answers and finite-difference tests pin) dotted with the same row."""
+    s = small_periodic(norb=37)
+    rng = np.random.default_rng(12)
+    R = rng.uniform(0, 1, size=(3, s.N, 3)) * s.L
+    c = gpu_ctx(s, R, 1)
+    nvp, n = 5, 37
+    pos = rng.uniform(-10, 20, size=(3, nvp, 3))
+    inv = rng.standard_normal((3, n))
+    out = c.mw_evaluateDetRatios(pos, inv)
+    ref = np.array([[orc.spo_vgl(s.L, s.coefs, pos[w, v])[0] @ inv[w] for v in range(nvp)] for w in range(3)])
+    assert relerr(out, ref, floor=1.0) < 1e-13
+
 @pytest.mark.parametrize("k", [1, 3])
 def test_evaluate_log_and_protocol(k):
     """mw_evaluateLog, then the reference call protocol (evalGrad, makeMove, calcRatioGrad,
