@@ -112,9 +112,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-walkers", type=int, default=128)
     ap.add_argument("--quiet", action="store_true")
-    ap.add_argument("--precision", choices=["fp64", "fp32", "bf16"], default="fp64",
+    ap.add_argument("--precision", choices=["fp64", "fp32", "bf16", "t32", "tbf16"], default="fp64",
                     help="storage precision: fp64 (the plain form), fp32 (FP32 table and determinant "
-                         "engine), bf16 (BF16 table, FP32 determinant engine)")
+                         "engine), bf16 (BF16 table, FP32 determinant engine), t32 / tbf16 (FP32 / BF16 "
+                         "table, FP64 determinant engine)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs (walker sharding, one "
@@ -135,8 +136,8 @@ def main():
     log(f"[bench] system built in {time.perf_counter() - t0:.1f}s: N={system.N} n={system.n_up} "
         f"M={system.M[0]} norb={system.norb} ions={len(system.ions)} k={system.delay_rank} nw={nw}")
     ctx = qmi.Context(0, precision=args.precision)
-    sc = {"fp64": 8, "fp32": 4, "bf16": 2}[args.precision]
-    sd = 8 if args.precision == "fp64" else 4
+    sc = {"fp64": 8, "fp32": 4, "bf16": 2, "t32": 4, "tbf16": 2}[args.precision]
+    sd = 4 if args.precision in ("fp32", "bf16") else 8
     ctx.system_create(system)
     ctx.walkers_load(R, 1)
     t1 = time.perf_counter()
@@ -185,7 +186,9 @@ def main():
     traffic, traffic_src = (measured_traffic("qmi::k_move_one" if kname == "move_one" else "qmi::k_sweep_segment")
                             if nw == 1024 and args.precision == "fp64" else (None, None))
     workload = {"fp64": "plain FP64", "fp32": "mixed precision (FP32 table, FP32 determinant engine)",
-                "bf16": "mixed precision (BF16 table, FP32 determinant engine)"}[args.precision]
+                "bf16": "mixed precision (BF16 table, FP32 determinant engine)",
+                "t32": "mixed precision (FP32 table, FP64 determinant engine)",
+                "tbf16": "mixed precision (BF16 table, FP64 determinant engine)"}[args.precision]
     out = {
         "metric": METRIC,
         "value": value,
@@ -197,7 +200,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32", "bf16": "bf16+f32"}[args.precision],
+        "dtype": {"fp64": "f64", "fp32": "f32", "bf16": "bf16+f32", "t32": "f32+f64", "tbf16": "bf16+f64"}[args.precision],
         "data": "synthetic",
         "config": {
             "workload": f"{workload} VMC sweeps: 1024 walkers x 512 electrons (256 per spin), 256 orbitals "
@@ -228,7 +231,7 @@ def main():
             "avg_ms": fl_ms,
             "achieved_gbs": (nw * flush_bytes(n, k, sd)) / (fl_ms * 1e-3) / 1e9 if fl_ms > 0 else 0.0,
             "achieved_tflops": (nw * flush_flops(n, k)) / (fl_ms * 1e-3) / 1e12 if fl_ms > 0 else 0.0,
-            "mfma_peak_tflops": FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS,
+            "mfma_peak_tflops": FP32_PEAK_TFLOPS if sd == 4 else FP64_PEAK_TFLOPS,
         },
     }
     if not args.no_cpu:

@@ -29,15 +29,18 @@ enum
 };
 
 /* Precision of the orbital table and of the delayed-update engine (the reference's full and mixed
- * precision builds, CMakeLists.txt:83-111).  In the mixed modes the determinant state (inverse,
+ * precision builds, CMakeLists.txt:83-111).  In the MIXED modes the determinant state (inverse,
  * orbital gradients/Laplacians, delayed rows) is stored in FP32 and the flush runs on FP32
- * matrix cores; the LU re-inversion and the log determinant stay FP64 (DiracMatrix.h:204-220), as
- * do positions, Jastrows and every ratio and gradient dot product. */
+ * matrix cores; in the TABLE modes only the spline table is narrowed.  The LU re-inversion and the
+ * log determinant stay FP64 (DiracMatrix.h:204-220), as do positions, Jastrows and every ratio and
+ * gradient dot product. */
 enum
 {
   QMI_PREC_FP64       = 0, /* FP64 table and determinant engine */
   QMI_PREC_MIXED_FP32 = 1, /* FP32 table, FP32 determinant engine */
-  QMI_PREC_MIXED_BF16 = 2  /* BF16 table storage (FP32 and wider arithmetic), FP32 determinant engine */
+  QMI_PREC_MIXED_BF16 = 2, /* BF16 table storage (FP32 and wider arithmetic), FP32 determinant engine */
+  QMI_PREC_TABLE_FP32 = 3, /* FP32 table, FP64 determinant engine */
+  QMI_PREC_TABLE_BF16 = 4  /* BF16 table storage, FP64 determinant engine */
 };
 
 /* System description.  Orthorhombic cell (L[d] <= 0: open along d); ions;

@@ -21,9 +21,11 @@ constexpr int TRACE_FIELDS = 8; // ratio, accepted, proposed position[3], gradie
 // gradient dot products, the Jastrows, positions, the LU re-inversion and log det stay FP64.
 enum Precision
 {
-  PREC_FP64 = 0, // FP64 spline table and determinant engine
-  PREC_FP32 = 1, // FP32 spline table, FP32 determinant engine (Ainv, dpsi, U, V, Binv; FP32 MFMA flush)
-  PREC_BF16 = 2  // BF16 spline table, FP32 determinant engine
+  PREC_FP64  = 0, // FP64 spline table and determinant engine
+  PREC_FP32  = 1, // FP32 spline table, FP32 determinant engine (Ainv, dpsi, U, V, Binv; FP32 MFMA flush)
+  PREC_BF16  = 2, // BF16 spline table, FP32 determinant engine
+  PREC_T32   = 3, // FP32 spline table, FP64 determinant engine
+  PREC_TBF16 = 4  // BF16 spline table, FP64 determinant engine
 };
 
 // BF16 storage (upper half of an IEEE float); read as double through the float it encodes.

@@ -177,8 +177,9 @@ __global__ __launch_bounds__(WG) void k_flush_update(SysC s, State st, int g, in
 // Fused flush for n <= 256 on the FP64 matrix cores, using the association
 //   Ainv -= ((Ainv U^T - W) Binv) V
 // (the same product as (Ainv U^T - W)(Binv V), without materialising Binv V).
-// v_mfma_f64_16x16x4f64 operand layout (measured): lane l holds A[l % 16][l / 16] and
-// B[l / 16][l % 16]; result register v of lane l holds D[l / 16 + 4 v][l % 16].
+// Matrix-core operand layout (measured, see Mfma below): lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16]; result register v of lane l holds row drow(l / 16, v) of D, column l % 16
+// (for FP64 drow = l / 16 + 4 v, the case written out below).
 // One wave owns 16 rows of Ainv and keeps them in registers for the whole update, so Ainv is read
 // and written once:
 //   lane (li = l % 16, q = l / 16) holds a[4 jt + v] = Ainv[row li][16 jt + 4 q + v], jt < 16.
@@ -196,8 +197,11 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
 constexpr int MF_COLS = 256; // n limit of the register-resident path
 
-// 16x16x4 matrix-core step in the determinant storage type (FP64 or FP32 accumulate); both forms
-// share the operand and result lane layout above.
+// 16x16x4 matrix-core step in the determinant storage type.  Both forms take A[l % 16][l / 16] and
+// B[l / 16][l % 16]; their results differ (measured): result register v of lane l (q = l / 16) is
+// row drow(q, v) = q + 4 v (FP64) or 4 q + v (FP32) of D, column l % 16.  acol(m) = 4 q + v for the
+// (q, v) with drow(q, v) = m, the offset that maps A-operand row m of step 3 onto the register
+// a[4 jt + v] of lane (li, q).
 template<typename DT>
 struct Mfma;
 template<>
@@ -205,12 +209,16 @@ struct Mfma<double>
 {
   using acc_t = dbl4;
   __device__ static acc_t mma(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ static int drow(int q, int v) { return q + 4 * v; }
+  __device__ static int acol(int m) { return 4 * (m & 3) + (m >> 2); }
 };
 template<>
 struct Mfma<float>
 {
   using acc_t = flt4;
   __device__ static acc_t mma(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static int drow(int q, int v) { return 4 * q + v; }
+  __device__ static int acol(int m) { return m; }
 };
 
 __host__ __device__ inline int flush_lt(int k) { return 16 * ((k + 15) / 16) + 1; }
@@ -345,7 +353,7 @@ __global__ __launch_bounds__(WG, 2) void k_flush_mfma(SysC s, State st, int g, i
     const int dc = c < d ? dl[c] : -2;
 #pragma unroll
     for (int v = 0; v < 4; v++)
-      sTw[(q + 4 * v) * LT + c] = c < d ? acc[ct][v] - ((dc == rbase + q + 4 * v) ? DT(1) : DT(0)) : DT(0);
+      sTw[MF::drow(q, v) * LT + c] = c < d ? acc[ct][v] - ((dc == rbase + MF::drow(q, v)) ? DT(1) : DT(0)) : DT(0);
   }
   __syncthreads();
 
@@ -366,7 +374,7 @@ __global__ __launch_bounds__(WG, 2) void k_flush_mfma(SysC s, State st, int g, i
   for (int ct = 0; ct < NCT; ct++)
 #pragma unroll
     for (int v = 0; v < 4; v++)
-      sTw[(q + 4 * v) * LT + ct * 16 + li] = acc[ct][v];
+      sTw[MF::drow(q, v) * LT + ct * 16 + li] = acc[ct][v];
 
   // 3. Ainv_rows -= T' V
 #pragma unroll
@@ -381,7 +389,7 @@ __global__ __launch_bounds__(WG, 2) void k_flush_mfma(SysC s, State st, int g, i
       for (int j4 = 0; j4 < 4; j4++)
       {
         const int jt = ch * 4 + j4;
-        const int jl = j4 * 16 + 4 * (li & 3) + (li >> 2);
+        const int jl = j4 * 16 + MF::acol(li);
         acc_t o      = acc_t{0, 0, 0, 0};
 #pragma unroll
         for (int cc = 0; cc < DP / 4; cc++)

@@ -218,8 +218,18 @@ struct qmi_ctx
   int group(int iat) const { return iat < sys.nup ? 0 : 1; }
 
   // bytes per element of the spline table and of the determinant engine
-  size_t table_esize() const { return precision == QMI_PREC_FP64 ? 8 : precision == QMI_PREC_MIXED_FP32 ? 4 : 2; }
-  size_t det_esize() const { return precision == QMI_PREC_FP64 ? 8 : 4; }
+  size_t table_esize() const
+  {
+    switch (precision)
+    {
+    case QMI_PREC_FP64: return 8;
+    case QMI_PREC_MIXED_FP32:
+    case QMI_PREC_TABLE_FP32: return 4;
+    default: return 2;
+    }
+  }
+  bool det_fp32() const { return precision == QMI_PREC_MIXED_FP32 || precision == QMI_PREC_MIXED_BF16; }
+  size_t det_esize() const { return det_fp32() ? 4 : 8; }
   // device buffer (in doubles) holding count elements of the determinant type
   size_t det_words(size_t count) const { return (count * det_esize() + 7) / 8; }
 
@@ -389,7 +399,7 @@ struct qmi_ctx
         HIPCHECK(hipMemset(V[g].p, 0, V[g].n * sizeof(double)));
         HIPCHECK(hipMemset(Binv[g].p, 0, Binv[g].n * sizeof(double)));
       }
-      if (precision != QMI_PREC_FP64)
+      if (det_fp32())
         lu.alloc((size_t)nw * std::max(s.nup, s.ndn) * ld);
       ipiv.alloc((size_t)nw * std::max(s.nup, s.ndn));
       info.alloc(nw);
@@ -435,7 +445,7 @@ struct qmi_ctx
       if (n == 0)
         continue;
       // FP64 LU workspace: Ainv itself in FP64 contexts
-      double* L64 = precision == QMI_PREC_FP64 ? Ainv[g].p : lu.p;
+      double* L64 = det_fp32() ? lu.p : Ainv[g].p;
       {
         Scoped t(this, "spo_rows");
         HIPCHECK(launch_spo_rows(s, st, coefs.p, g, nw, L64, stream));
@@ -733,7 +743,7 @@ extern "C"
     if (!out)
       return QMI_ERR_ARG;
     *out = nullptr;
-    if (precision != QMI_PREC_FP64 && precision != QMI_PREC_MIXED_FP32 && precision != QMI_PREC_MIXED_BF16)
+    if (precision < QMI_PREC_FP64 || precision > QMI_PREC_TABLE_BF16)
       return QMI_ERR_UNSUPPORTED;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)

@@ -88,8 +88,8 @@ __global__ void k_store_inverse(SysC s, State st, int g, const double* __restric
 
 hipError_t launch_store_inverse(const SysC& s, const State& st, int g, int nw, const double* lu, hipStream_t stream)
 {
-  if (s.prec == PREC_FP64)
-    return hipSuccess; // the LU workspace is Ainv itself
+  if (s.prec == PREC_FP64 || s.prec == PREC_T32 || s.prec == PREC_TBF16)
+    return hipSuccess; // FP64 determinant engine: the LU workspace is Ainv itself
   const size_t per = (size_t)s.n[g] * s.ld;
   k_store_inverse<float><<<dim3((unsigned)std::min<size_t>(64, (per + 255) / 256), nw), 256, 0, stream>>>(s, st, g, lu);
   return hipGetLastError();

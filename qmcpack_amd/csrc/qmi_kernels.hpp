@@ -14,6 +14,8 @@ hipError_t dispatch_prec(int prec, F&& f)
   case PREC_FP64: return f(double{}, double{});
   case PREC_FP32: return f(float{}, float{});
   case PREC_BF16: return f(bf16s{}, float{});
+  case PREC_T32: return f(float{}, double{});
+  case PREC_TBF16: return f(bf16s{}, double{});
   default: return hipErrorInvalidValue;
   }
 }

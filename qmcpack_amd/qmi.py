@@ -102,8 +102,9 @@ def make_desc(system, keep: list) -> SystemDesc:
 
 
 # storage precisions (include/qmcmi.h)
-PREC_FP64, PREC_MIXED_FP32, PREC_MIXED_BF16 = 0, 1, 2
-PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_MIXED_FP32, "bf16": PREC_MIXED_BF16}
+PREC_FP64, PREC_MIXED_FP32, PREC_MIXED_BF16, PREC_TABLE_FP32, PREC_TABLE_BF16 = 0, 1, 2, 3, 4
+PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_MIXED_FP32, "bf16": PREC_MIXED_BF16, "t32": PREC_TABLE_FP32,
+              "tbf16": PREC_TABLE_BF16}
 
 
 class Context:
