@@ -156,6 +156,37 @@ int qmi_vmc_run(qmi_ctx* ctx, int steps, int sub_steps, double tau, int recomput
 /* accepted / rejected move counts since the context was created: stats[2] */
 int qmi_vmc_stats(qmi_ctx* ctx, double* stats);
 
+/* ---- learned per-electron proposal ----------------------------------------------------------
+ * No counterpart in the reference: an MLP proposal inside the same Metropolis-Hastings step, for
+ * callers that want fewer correlated samples per move.  Features of the moving electron (68): the
+ * 4 nearest ions (dx, dy, dz, |d|), the 16 nearest other electrons (dx, dy, dz), the drift and the
+ * timestep.  Network 68 -> 128 -> 128 -> 128 -> 4 (SiLU; BF16 weights, FP32 accumulation on the
+ * matrix cores); outputs (mu[3], s) give r' = r + drift + mu + sqrt(tau) e^s xi, and the acceptance
+ * carries the asymmetric-proposal correction q(r|r')/q(r'|r) with the reverse network evaluated at
+ * r' (exact sampling of |psi|^2 for any weights).  A zero output layer reproduces plain
+ * drift-diffusion.  Training (optional, on the device): Adam on -mean of
+ * min(1, acceptance) |r' - r|^2 over the moves of the run.  Needs one walker per crowd. */
+#define QMI_MLP_NPARAM 42372 /* W1[128][68], b1, W2[128][128], b2, W3[128][128], b3, W4[4][128], b4 */
+/* set = 1: upload params[QMI_MLP_NPARAM] (FP32); set = 0: download them */
+int qmi_mlp_params(qmi_ctx* ctx, int set, float* params);
+/* forward pass of the current network on X[B][68] -> out[B][4] */
+int qmi_mlp_forward(qmi_ctx* ctx, int B, const float* X, float* out);
+/* gradient of sum_b dout[b] . out[b] with respect to the parameters -> grad[QMI_MLP_NPARAM] */
+int qmi_mlp_gradient(qmi_ctx* ctx, int B, const float* X, const float* dout, float* grad);
+/* learned: use the learned proposal in qmi_vmc_run; train: update the network with Adam (rate lr)
+ * after every move; init_seed != 0: fresh network (uniform fan-in weights, zero output layer) */
+int qmi_proposal_config(qmi_ctx* ctx, int learned, int train, double lr, uint32_t init_seed);
+/* since the last call: out[4] = {mean objective per move, mean acceptance probability, moves,
+ * training steps so far} */
+int qmi_proposal_stats(qmi_ctx* ctx, double* out);
+/* features of electron iat at pos[nw][3] with drift[nw][3] in each walker -> out[nw][68] */
+int qmi_mw_proposal_features(qmi_ctx* ctx, int iat, const double* pos, const double* drift, double tau,
+                             float* out);
+/* qmi_vmc_run with, for the learned proposal, ltrace[steps][sub_steps][N][nw][15] = {mu[3], s,
+ * mu'[3], s', log q_fwd, log q_rev, acceptance probability, objective, drift at r'[3]} (nullable) */
+int qmi_vmc_run_ex(qmi_ctx* ctx, int steps, int sub_steps, double tau, int recompute_period, uint32_t seed,
+                   double* energies, double* trace, double* ltrace);
+
 /* ---- measurement ---------------------------------------------------------------------------- */
 /* Per-kernel device time accumulated with HIP events on the context stream (enabled by
  * qmi_timing_enable(ctx, 1)).  names: '\n'-separated kernel names written to buf (size buflen);

@@ -182,6 +182,15 @@ struct qmi_ctx
   bool rng_seeded     = false;
   PinnedBuf<double> h_gauss, h_unif;
   Timing timing;
+  // learned proposal (no counterpart in the reference)
+  bool learned = false, train = false, mlp_ready = false;
+  float lr     = 1e-3f;
+  int adam_t   = 0;
+  DevBuf<float> mlp_master, mlp_grad, adam_m, adam_v, lX, lout, ldout, lz, ldz_a, ldz_b;
+  DevBuf<uint16_t> w1b, w2b, w3b, w4b;
+  DevBuf<double> lstate, lfsum, ltrace_d;
+  double learn_moves = 0;
+  bool ltrace_on     = false;
 
   // -------------------------------------------------------------------------------------------
   struct Scoped
@@ -382,6 +391,10 @@ struct qmi_ctx
     hostin.alloc((size_t)nw * 3 + 16);
     acc.alloc(nw);
     ubase.alloc((size_t)ncrowd * (s.N + 1));
+    lstate.alloc((size_t)nw * LS_SIZE);
+    lfsum.alloc((size_t)nw * 2);
+    HIPCHECK(hipMemset(lfsum.p, 0, lfsum.n * sizeof(double)));
+    learn_moves = 0;
     if (s.has_det)
     {
       invRow.alloc((size_t)nw * ld);
@@ -592,8 +605,121 @@ struct qmi_ctx
     }
   }
 
+  // ---------------------------------------------------------------------------------------------
+  // learned proposal
+  MlpParams mlp() const { return MlpParams{mlp_master.p, w1b.p, w2b.p, w3b.p, w4b.p}; }
+
+  void mlp_alloc()
+  {
+    if (mlp_master.p)
+      return;
+    mlp_master.alloc(MLP_NPARAM);
+    mlp_grad.alloc(MLP_NPARAM);
+    adam_m.alloc(MLP_NPARAM);
+    adam_v.alloc(MLP_NPARAM);
+    w1b.alloc((size_t)MLP_HID * MLP_K1);
+    w2b.alloc((size_t)MLP_HID * MLP_HID);
+    w3b.alloc((size_t)MLP_HID * MLP_HID);
+    w4b.alloc((size_t)16 * MLP_HID);
+  }
+
+  void mlp_set(const float* hparams)
+  {
+    mlp_alloc();
+    HIPCHECK(hipMemcpyAsync(mlp_master.p, hparams, MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice, stream));
+    HIPCHECK(launch_mlp_pack(mlp(), stream));
+    HIPCHECK(hipMemsetAsync(adam_m.p, 0, MLP_NPARAM * sizeof(float), stream));
+    HIPCHECK(hipMemsetAsync(adam_v.p, 0, MLP_NPARAM * sizeof(float), stream));
+    sync();
+    adam_t    = 0;
+    mlp_ready = true;
+  }
+
+  // uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) weights and biases, zero output layer: the initial
+  // proposal is plain drift-diffusion
+  void mlp_init(uint32_t seed)
+  {
+    std::vector<float> P(MLP_NPARAM, 0.0f);
+    std::mt19937 e(seed);
+    auto fill = [&](int off, int count, int fan_in) {
+      const double lim = 1.0 / std::sqrt((double)fan_in);
+      for (int i = 0; i < count; i++)
+        P[off + i] = (float)((rng_uniform01(e) * 2.0 - 1.0) * lim);
+    };
+    fill(MLP_OFF_W1, MLP_HID * MLP_IN, MLP_IN);
+    fill(MLP_OFF_B1, MLP_HID, MLP_IN);
+    fill(MLP_OFF_W2, MLP_HID * MLP_HID, MLP_HID);
+    fill(MLP_OFF_B2, MLP_HID, MLP_HID);
+    fill(MLP_OFF_W3, MLP_HID * MLP_HID, MLP_HID);
+    fill(MLP_OFF_B3, MLP_HID, MLP_HID);
+    mlp_set(P.data());
+  }
+
+  void learn_alloc(int B)
+  {
+    mlp_alloc();
+    if (lX.n < (size_t)B * MLP_IN)
+    {
+      lX.alloc((size_t)B * MLP_IN);
+      lout.alloc((size_t)B * MLP_OUT);
+      ldout.alloc((size_t)B * MLP_OUT);
+      lz.alloc((size_t)3 * B * MLP_HID);
+      ldz_a.alloc((size_t)B * MLP_HID);
+      ldz_b.alloc((size_t)B * MLP_HID);
+    }
+  }
+
+  LearnArgs learn_args(bool with_train)
+  {
+    LearnArgs la{};
+    la.X    = lX.p;
+    la.out  = lout.p;
+    la.ls   = lstate.p;
+    la.dout = with_train ? ldout.p : nullptr;
+    la.fsum   = lfsum.p;
+    la.ltrace = ltrace_on ? ltrace_d.p : nullptr;
+    return la;
+  }
+
+  // one learned move of electron iat for every walker (a: the sub-step's move arguments)
+  void learned_move(MoveArgs& a)
+  {
+    const SysC& s = sys;
+    LearnArgs la  = learn_args(train);
+    const MlpParams P = mlp();
+    {
+      Scoped t(this, "learn_pre");
+      HIPCHECK(launch_learn(LK_PRE, s, st, coefs.p, a, la, stream));
+    }
+    {
+      Scoped t(this, "mlp_forward");
+      HIPCHECK(launch_mlp_forward(P, nw, lX.p, lout.p, train ? lz.p : nullptr, 2 * nw, 0, stream));
+    }
+    {
+      Scoped t(this, "learn_prop");
+      HIPCHECK(launch_learn(LK_PROP, s, st, coefs.p, a, la, stream));
+    }
+    {
+      Scoped t(this, "mlp_forward");
+      HIPCHECK(launch_mlp_forward(P, nw, lX.p + (size_t)nw * MLP_IN, lout.p + (size_t)nw * MLP_OUT,
+                                  train ? lz.p : nullptr, 2 * nw, nw, stream));
+    }
+    {
+      Scoped t(this, "learn_accept");
+      HIPCHECK(launch_learn(LK_ACCEPT, s, st, coefs.p, a, la, stream));
+    }
+    if (train)
+    {
+      Scoped t(this, "mlp_train");
+      HIPCHECK(launch_mlp_backward(P, 2 * nw, lX.p, lz.p, ldout.p, mlp_grad.p, ldz_a.p, ldz_b.p, stream));
+      HIPCHECK(launch_adam(mlp_master.p, mlp_grad.p, adam_m.p, adam_v.p, MLP_NPARAM, lr, ++adam_t, stream));
+      HIPCHECK(launch_mlp_pack(P, stream));
+    }
+    learn_moves += 1;
+  }
+
   void vmc_run(int steps, int sub_steps, double tau, int recompute_period, uint32_t seed, double* energies,
-               double* htrace)
+               double* htrace, double* hltrace = nullptr)
   {
     require_walkers();
     const SysC& s = sys;
@@ -612,6 +738,19 @@ struct qmi_ctx
     unif.alloc((size_t)ncrowd * N * wpc);
     if (htrace)
       trace.alloc((size_t)N * nw * TRACE_FIELDS);
+    if (learned)
+    {
+      if (wpc != 1)
+        throw QmiError(QMI_ERR_UNSUPPORTED, "the learned proposal needs one walker per crowd (one RNG stream each)");
+      if (!s.has_det || s.nion > 1280)
+        throw QmiError(QMI_ERR_UNSUPPORTED, "the learned proposal needs orbitals and at most 1280 ions");
+      if (!mlp_ready)
+        mlp_init(12345u);
+      learn_alloc(2 * nw);
+      if (hltrace)
+        ltrace_d.alloc((size_t)N * nw * LTRACE_FIELDS);
+      ltrace_on = hltrace != nullptr;
+    }
     std::vector<int> hub((size_t)ncrowd * (N + 1));
     std::vector<double> ek(nw);
     for (int step = 0; step < steps; step++)
@@ -640,7 +779,16 @@ struct qmi_ctx
           {
             a.g  = g;
             a.d0 = s.k > 1 ? dcount[g] : 0;
-            if (wpc == 1 && move_one)
+            if (learned)
+            {
+              a.iat0 = iat;
+              a.iat1 = iat + 1;
+              learned_move(a);
+              iat += 1;
+              if (s.k > 1)
+                dcount[g] += 1;
+            }
+            else if (wpc == 1 && move_one)
             {
               a.iat0 = iat;
               a.iat1 = iat + 1;
@@ -691,6 +839,9 @@ struct qmi_ctx
         if (htrace)
           HIPCHECK(hipMemcpyAsync(htrace + ((size_t)step * sub_steps + sub) * N * nw * TRACE_FIELDS, trace.p,
                                   (size_t)N * nw * TRACE_FIELDS * sizeof(double), hipMemcpyDeviceToHost, stream));
+        if (learned && hltrace)
+          HIPCHECK(hipMemcpyAsync(hltrace + ((size_t)step * sub_steps + sub) * N * nw * LTRACE_FIELDS, ltrace_d.p,
+                                  (size_t)N * nw * LTRACE_FIELDS * sizeof(double), hipMemcpyDeviceToHost, stream));
         sync();
         std::vector<int> consumed(ncrowd);
         for (int c = 0; c < ncrowd; c++)
@@ -997,6 +1148,133 @@ extern "C"
                   double* energies, double* trace)
   {
     return guarded(c, [&] { c->vmc_run(steps, sub_steps, tau, recompute_period, seed, energies, trace); });
+  }
+
+  int qmi_vmc_run_ex(qmi_ctx* c, int steps, int sub_steps, double tau, int recompute_period, uint32_t seed,
+                     double* energies, double* trace, double* ltrace)
+  {
+    return guarded(c, [&] { c->vmc_run(steps, sub_steps, tau, recompute_period, seed, energies, trace, ltrace); });
+  }
+
+  int qmi_mlp_params(qmi_ctx* c, int set, float* params)
+  {
+    return guarded(c, [&] {
+      if (!params)
+        throw QmiError(QMI_ERR_ARG, "null parameter array");
+      if (set)
+        c->mlp_set(params);
+      else
+      {
+        if (!c->mlp_ready)
+          c->mlp_init(12345u);
+        HIPCHECK(hipMemcpyAsync(params, c->mlp_master.p, MLP_NPARAM * sizeof(float), hipMemcpyDeviceToHost,
+                                c->stream));
+        c->sync();
+      }
+    });
+  }
+
+  int qmi_mlp_forward(qmi_ctx* c, int B, const float* X, float* out)
+  {
+    return guarded(c, [&] {
+      if (B <= 0 || !X || !out)
+        throw QmiError(QMI_ERR_ARG, "bad batch");
+      if (!c->mlp_ready)
+        c->mlp_init(12345u);
+      DevBuf<float> dX, dO;
+      dX.alloc((size_t)B * MLP_IN);
+      dO.alloc((size_t)B * MLP_OUT);
+      HIPCHECK(hipMemcpyAsync(dX.p, X, dX.n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+      HIPCHECK(launch_mlp_forward(c->mlp(), B, dX.p, dO.p, nullptr, B, 0, c->stream));
+      HIPCHECK(hipMemcpyAsync(out, dO.p, dO.n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+      c->sync();
+    });
+  }
+
+  int qmi_mlp_gradient(qmi_ctx* c, int B, const float* X, const float* dout, float* grad)
+  {
+    return guarded(c, [&] {
+      if (B <= 0 || !X || !dout || !grad)
+        throw QmiError(QMI_ERR_ARG, "bad batch");
+      if (!c->mlp_ready)
+        c->mlp_init(12345u);
+      DevBuf<float> dX, dO, dD, z, za, zb, gr;
+      dX.alloc((size_t)B * MLP_IN);
+      dO.alloc((size_t)B * MLP_OUT);
+      dD.alloc((size_t)B * MLP_OUT);
+      z.alloc((size_t)3 * B * MLP_HID);
+      za.alloc((size_t)B * MLP_HID);
+      zb.alloc((size_t)B * MLP_HID);
+      gr.alloc(MLP_NPARAM);
+      HIPCHECK(hipMemcpyAsync(dX.p, X, dX.n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+      HIPCHECK(hipMemcpyAsync(dD.p, dout, dD.n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+      HIPCHECK(launch_mlp_forward(c->mlp(), B, dX.p, dO.p, z.p, B, 0, c->stream));
+      HIPCHECK(launch_mlp_backward(c->mlp(), B, dX.p, z.p, dD.p, gr.p, za.p, zb.p, c->stream));
+      HIPCHECK(hipMemcpyAsync(grad, gr.p, MLP_NPARAM * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+      c->sync();
+    });
+  }
+
+  int qmi_proposal_config(qmi_ctx* c, int learned, int train, double lr, uint32_t init_seed)
+  {
+    return guarded(c, [&] {
+      if (init_seed)
+        c->mlp_init(init_seed);
+      c->learned = learned != 0;
+      c->train   = c->learned && train != 0;
+      c->lr      = (float)lr;
+    });
+  }
+
+  int qmi_proposal_stats(qmi_ctx* c, double* out4)
+  {
+    return guarded(c, [&] {
+      c->require_walkers();
+      std::vector<double> h((size_t)c->nw * 2);
+      HIPCHECK(hipMemcpyAsync(h.data(), c->lfsum.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      c->sync();
+      double f = 0, a = 0;
+      for (int w = 0; w < c->nw; w++)
+      {
+        f += h[w * 2];
+        a += h[w * 2 + 1];
+      }
+      const double moves = c->learn_moves * c->nw;
+      out4[0] = moves > 0 ? f / moves : 0.0;
+      out4[1] = moves > 0 ? a / moves : 0.0;
+      out4[2] = moves;
+      out4[3] = c->adam_t;
+      HIPCHECK(hipMemsetAsync(c->lfsum.p, 0, c->lfsum.n * sizeof(double), c->stream));
+      c->sync();
+      c->learn_moves = 0;
+    });
+  }
+
+  int qmi_mw_proposal_features(qmi_ctx* c, int iat, const double* pos, const double* drift, double tau, float* out)
+  {
+    return guarded(c, [&] {
+      c->require_walkers();
+      if (iat < 0 || iat >= c->sys.N || c->sys.nion > 1280)
+        throw QmiError(QMI_ERR_ARG, "bad electron index or too many ions");
+      const int nw = c->nw;
+      std::vector<double> h((size_t)nw * LS_SIZE, 0.0);
+      for (int w = 0; w < nw; w++)
+        for (int q = 0; q < 3; q++)
+        {
+          h[(size_t)w * LS_SIZE + LS_R + q] = pos[w * 3 + q];
+          h[(size_t)w * LS_SIZE + LS_V + q] = drift[w * 3 + q];
+        }
+      c->learn_alloc(2 * nw);
+      HIPCHECK(hipMemcpyAsync(c->lstate.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      MoveArgs a{};
+      a.iat0       = iat;
+      a.nw         = nw;
+      a.tau        = tau;
+      LearnArgs la = c->learn_args(false);
+      HIPCHECK(launch_learn(LK_FEATURES, c->sys, c->st, c->coefs.p, a, la, c->stream));
+      HIPCHECK(hipMemcpyAsync(out, c->lX.p, (size_t)nw * MLP_IN * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+      c->sync();
+    });
   }
 
   int qmi_vmc_stats(qmi_ctx* c, double* stats2)

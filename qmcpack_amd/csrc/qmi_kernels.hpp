@@ -78,4 +78,71 @@ hipError_t launch_api_det_ratios(const SysC& s, const void* coefs, int nw, int n
                                  const double* inv, double* out, hipStream_t stream);
 hipError_t launch_api_distances(const SysC& s, const State& st, int nw, const double* pos, double* ee_r,
                                 double* ee_dr, double* ei_r, double* ei_dr, hipStream_t stream);
+
+// ---- learned proposal (no counterpart in the reference) -----------------------------------------
+constexpr int MLP_IN = 68, MLP_HID = 128, MLP_OUT = 4, MLP_K1 = 96; // K1: input width padded for MFMA
+constexpr int N_ION_NB = 4, N_EL_NB = 16;                          // feature neighbour counts
+// flat FP32 parameter vector: W1 [128][68], b1, W2 [128][128], b2, W3 [128][128], b3, W4 [4][128], b4
+constexpr int MLP_OFF_W1 = 0;
+constexpr int MLP_OFF_B1 = MLP_OFF_W1 + MLP_HID * MLP_IN;
+constexpr int MLP_OFF_W2 = MLP_OFF_B1 + MLP_HID;
+constexpr int MLP_OFF_B2 = MLP_OFF_W2 + MLP_HID * MLP_HID;
+constexpr int MLP_OFF_W3 = MLP_OFF_B2 + MLP_HID;
+constexpr int MLP_OFF_B3 = MLP_OFF_W3 + MLP_HID * MLP_HID;
+constexpr int MLP_OFF_W4 = MLP_OFF_B3 + MLP_HID;
+constexpr int MLP_OFF_B4 = MLP_OFF_W4 + MLP_OUT * MLP_HID;
+constexpr int MLP_NPARAM = MLP_OFF_B4 + MLP_OUT;
+
+struct MlpParams
+{
+  const float* master;              // FP32 parameters (flat layout above)
+  uint16_t *w1b, *w2b, *w3b, *w4b;  // BF16 copies: W1 [128][96], W2/W3 [128][128], W4 [16][128]
+};
+
+// per-walker state of the learned move between its kernels
+enum LearnField
+{
+  LS_R = 0,      // 3 position of the moving electron
+  LS_V = 3,      // 3 drift at r
+  LS_GNOW = 6,   // 3 total gradient at r
+  LS_RP = 9,     // 3 proposed position
+  LS_XI = 12,    // 3 standard normal of the move
+  LS_LQF = 15,   // log q_fwd
+  LS_VR = 16,    // 3 drift at r'
+  LS_GNEW = 19,  // 3 total gradient at r'
+  LS_SIZE = 24
+};
+// mu[3], s, mu'[3], s', log q_fwd, log q_rev, acceptance probability, objective f, drift at r'[3]
+constexpr int LTRACE_FIELDS = 15;
+
+struct LearnArgs
+{
+  float* X;        // [2 nw][68] features at r (rows [0, nw)) and at r' (rows [nw, 2 nw))
+  float* out;      // [2 nw][4] network outputs for those rows
+  double* ls;      // [nw][LS_SIZE]
+  float* dout;     // [2 nw][4] gradient of the training loss with respect to out (nullable)
+  double* fsum;    // [nw][2] running sums of the objective f and of the acceptance probability
+  double* ltrace;  // [N][nw][LTRACE_FIELDS] (nullable)
+};
+
+hipError_t launch_mlp_pack(const MlpParams& P, hipStream_t stream);
+// X [B][68] -> out [B][4]; z (nullable): pre-activations, layer l of sample b at
+// z[(l * zstride + zoff + b) * 128]
+hipError_t launch_mlp_forward(const MlpParams& P, int B, const float* X, float* out, float* z, int zstride, int zoff,
+                              hipStream_t stream);
+// gradient of sum_b dOut[b] . out[b] into grad[MLP_NPARAM]; z from a forward with zstride = B, zoff = 0;
+// dz_a, dz_b: [B][128] scratch
+hipError_t launch_mlp_backward(const MlpParams& P, int B, const float* X, const float* z, const float* dOut,
+                               float* grad, float* dz_a, float* dz_b, hipStream_t stream);
+hipError_t launch_adam(float* p, const float* g, float* m, float* v, int n, float lr, int t, hipStream_t stream);
+
+enum LearnKernel
+{
+  LK_PRE = 0,  // inverse row, gradient, drift and features at r
+  LK_PROP,     // proposal from the network output, ratio / gradient at r', features at r'
+  LK_ACCEPT,   // asymmetric Metropolis-Hastings, objective gradient, accept / reject
+  LK_FEATURES  // features only (API test hook; position in LS_R, drift in LS_V)
+};
+hipError_t launch_learn(int which, const SysC& s, const State& st, const void* coefs, const MoveArgs& a,
+                        const LearnArgs& la, hipStream_t stream);
 } // namespace qmi

@@ -28,6 +28,7 @@ struct MoveLDS
   double bc[48];      // broadcast scalars
   double jc[3][MAX_JCOEF]; // Jastrow functor tables: J1, J2 like spins, J2 unlike spins
   SplineLDS spl;      // uniform spline factors of the proposed position
+  int sel[N_EL_NB + N_ION_NB]; // learned proposal: selected neighbours (electrons, then ions)
 };
 
 // Copy the functor tables to LDS (published by the caller's next barrier).
@@ -1067,6 +1068,424 @@ __global__ void k_api_make_move(SysC s, State st, int iat, int nw, const double*
   double* mv      = st.mv + (size_t)w * MV_SIZE;
   for (int q = 0; q < 3; q++)
     mv[MV_POS + q] = R[q * s.Npad + iat] + displ[(size_t)w * 3 + q];
+}
+
+//------------------------------------------------------------------------------------------------
+// Learned proposal (no counterpart in the reference).
+
+// Features of electron iat at pos with drift vd: the 4 nearest ions (dx, dy, dz, |d|), the 16
+// nearest other electrons (dx, dy, dz) (d = minimum image of source - pos), the drift and tau,
+// written to x[68] (FP32).  Neighbours are ranked by minimum-image distance, ties by the lower
+// index; missing neighbours are zero.  sh.red holds the distances (electrons at [0, N), ions at
+// [1024, 1024 + nion)).  Starts and ends with a barrier.
+__device__ inline void proposal_features(const SysC& s, const double* __restrict__ R, const double* __restrict__ ions,
+                                         int iat, const double pos[3], const double vd[3], double tau, MoveLDS& sh,
+                                         float* __restrict__ x)
+{
+  const int t = threadIdx.x, N = s.N, Np = s.Npad, ni = s.nion;
+  double* de = sh.red;
+  double* di = sh.red + 1024;
+  __syncthreads();
+  for (int j = t; j < N; j += WG)
+  {
+    double dr[3];
+    const double r = min_image(s, R[j], R[Np + j], R[2 * Np + j], pos, dr);
+    de[j]          = j == iat ? INFINITY : r;
+  }
+  for (int j = t; j < ni; j += WG)
+  {
+    double dr[3];
+    di[j] = min_image(s, ions[j], ions[ni + j], ions[2 * ni + j], pos, dr);
+  }
+  if (t < N_EL_NB + N_ION_NB)
+    sh.sel[t] = -1;
+  __syncthreads();
+  for (int j = t; j < N; j += WG)
+  {
+    if (j == iat)
+      continue;
+    const double dj = de[j];
+    int rank        = 0;
+    for (int i = 0; i < N; i++)
+    {
+      const double dv = de[i];
+      rank += (dv < dj) || (dv == dj && i < j);
+    }
+    if (rank < N_EL_NB)
+      sh.sel[rank] = j;
+  }
+  for (int j = t; j < ni; j += WG)
+  {
+    const double dj = di[j];
+    int rank        = 0;
+    for (int i = 0; i < ni; i++)
+    {
+      const double dv = di[i];
+      rank += (dv < dj) || (dv == dj && i < j);
+    }
+    if (rank < N_ION_NB)
+      sh.sel[N_EL_NB + rank] = j;
+  }
+  __syncthreads();
+  if (t < MLP_IN)
+  {
+    double val = 0.0;
+    if (t < 4 * N_ION_NB)
+    {
+      const int j = sh.sel[N_EL_NB + (t >> 2)], c = t & 3;
+      if (j >= 0)
+      {
+        double dr[3];
+        const double r = min_image(s, ions[j], ions[ni + j], ions[2 * ni + j], pos, dr);
+        val            = c < 3 ? dr[c] : r;
+      }
+    }
+    else if (t < 4 * N_ION_NB + 3 * N_EL_NB)
+    {
+      const int u = t - 4 * N_ION_NB, j = sh.sel[u / 3];
+      if (j >= 0)
+      {
+        double dr[3];
+        min_image(s, R[j], R[Np + j], R[2 * Np + j], pos, dr);
+        val = dr[u % 3];
+      }
+    }
+    else if (t < MLP_IN - 1)
+      val = vd[t - (MLP_IN - 4)];
+    else
+      val = tau;
+    x[t] = (float)val;
+  }
+  __syncthreads();
+}
+
+// total gradient at the current position of iat (determinant part from sh.bc[BC_GRAD]), summed in
+// the component order of the plain proposal
+__device__ inline void grad_now(const SysC& s, int iat, const double* __restrict__ jas, const MoveLDS& sh, double g[3])
+{
+  const int Np = s.Npad;
+  for (int q = 0; q < 3; q++)
+    g[q] = 0.0;
+  if (s.has_det)
+    for (int q = 0; q < 3; q++)
+      g[q] += sh.bc[BC_GRAD + q];
+  if (s.has_j1)
+    for (int q = 0; q < 3; q++)
+      g[q] += jas[(J_G1X + q) * Np + iat];
+  if (s.has_j2)
+    for (int q = 0; q < 3; q++)
+      g[q] += jas[(J_DUX + q) * Np + iat];
+}
+
+// Inverse row (to st.invRow), gradient and drift at the current position, features at r (rows
+// [0, nw) of la.X).
+template<typename ST, typename DT, int MAXM>
+__global__ __launch_bounds__(WG) void k_learn_pre(SysC s, State st, const ST* __restrict__ coefs, MoveArgs a,
+                                                  LearnArgs la)
+{
+  __shared__ MoveLDS sh;
+  extern __shared__ double sB_raw[];
+  DT* sB = reinterpret_cast<DT*>(sB_raw);
+  const int w = blockIdx.x;
+  if (w >= a.nw)
+    return;
+  const int g = a.g, n = s.n[g], ld = s.ld, k = s.k, Np = s.Npad;
+  const int iat = a.iat0, r = iat - s.first[g], d = a.d0;
+  WalkerView<DT> v = walker_view<DT>(s, st, g, w);
+  double ainv_r[MAXM], inv[MAXM];
+  if (s.has_det)
+  {
+    phaseA<DT, MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, sB, ainv_r, inv);
+    det_grad<DT, MAXM>(n, ld, r, v.dpsi, inv, sh);
+    double* ir = st.invRow + (size_t)w * ld;
+#pragma unroll
+    for (int m = 0; m < MAXM; m++)
+    {
+      const int o = threadIdx.x + WG * m;
+      if (o < n)
+        ir[o] = inv[m];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+  {
+    double* ls = la.ls + (size_t)w * LS_SIZE;
+    double gn[3], dv[3];
+    grad_now(s, iat, v.jas, sh, gn);
+    unr_drift(a.tau, gn, dv);
+    for (int q = 0; q < 3; q++)
+    {
+      ls[LS_R + q]    = v.R[q * Np + iat];
+      ls[LS_V + q]    = dv[q];
+      ls[LS_GNOW + q] = gn[q];
+      sh.bc[BC_POS + q]  = ls[LS_R + q];
+      sh.bc[BC_MOVE + q] = dv[q];
+    }
+  }
+  __syncthreads();
+  const double pos[3] = {sh.bc[BC_POS], sh.bc[BC_POS + 1], sh.bc[BC_POS + 2]};
+  const double vd[3]  = {sh.bc[BC_MOVE], sh.bc[BC_MOVE + 1], sh.bc[BC_MOVE + 2]};
+  proposal_features(s, v.R, st.ions, iat, pos, vd, a.tau, sh, la.X + (size_t)w * MLP_IN);
+}
+
+// Proposal r' = r + v + mu + sqrt(tau) e^s xi from the network output at r; ratio and gradient at r'
+// (phase C with the stored inverse row); drift and features at r' (rows [nw, 2 nw) of la.X).
+template<typename ST, typename DT, int MAXM>
+__global__ __launch_bounds__(WG) void k_learn_prop(SysC s, State st, const ST* __restrict__ coefs, MoveArgs a,
+                                                   LearnArgs la)
+{
+  __shared__ MoveLDS sh;
+  stage_jastrow(s, sh);
+  const int w = blockIdx.x;
+  if (w >= a.nw)
+    return;
+  const int g = a.g, n = s.n[g], ld = s.ld;
+  const int iat = a.iat0;
+  WalkerView<DT> v = walker_view<DT>(s, st, g, w);
+  double* ls       = la.ls + (size_t)w * LS_SIZE;
+  double* mv       = st.mv + (size_t)w * MV_SIZE;
+  if (threadIdx.x == 0)
+  {
+    const float* of  = la.out + (size_t)w * MLP_OUT;
+    const double* gz = a.gauss + ((size_t)iat * a.nw + w) * 3;
+    const double sig = sqrt(a.tau) * exp((double)of[3]);
+    double xx        = 0;
+    for (int q = 0; q < 3; q++)
+    {
+      const double xi    = gz[q];
+      const double move  = (ls[LS_V + q] + (double)of[q]) + xi * sig;
+      const double rp    = ls[LS_R + q] + move;
+      ls[LS_RP + q]      = rp;
+      ls[LS_XI + q]      = xi;
+      sh.bc[BC_POS + q]  = rp;
+      xx += xi * xi;
+    }
+    ls[LS_LQF] = -0.5 * xx - 3.0 * (double)of[3];
+  }
+  __syncthreads();
+  double inv[MAXM], phv[MAXM][5];
+  const double* ir = st.invRow + (size_t)w * ld;
+#pragma unroll
+  for (int m = 0; m < MAXM; m++)
+  {
+    const int o = threadIdx.x + WG * m;
+    inv[m]      = (s.has_det && o < n) ? ir[o] : 0.0;
+  }
+  phaseC<ST, MAXM>(s, g, iat, n, v.R, st.ions, coefs, inv, phv, sh);
+  if (threadIdx.x == 0)
+  {
+    double ratio, gnew[3], vr[3];
+    combine_ratio(s, iat, v.jas, sh, ratio, gnew);
+    unr_drift(a.tau, gnew, vr);
+    mv[MV_RATIO] = ratio;
+    for (int q = 0; q < 3; q++)
+    {
+      mv[MV_POS + q]  = ls[LS_RP + q];
+      mv[MV_GNOW + q] = ls[LS_GNOW + q];
+      ls[LS_GNEW + q] = gnew[q];
+      ls[LS_VR + q]   = vr[q];
+      sh.bc[BC_MOVE + q] = vr[q];
+    }
+    for (int q = 0; q < 13; q++)
+      mv[MV_C + q] = sh.bc[BC_C + q];
+  }
+  double* phi = st.phi + (size_t)w * 5 * ld;
+#pragma unroll
+  for (int m = 0; m < MAXM; m++)
+  {
+    const int o = threadIdx.x + WG * m;
+    if (o < n)
+      for (int q = 0; q < 5; q++)
+        phi[q * ld + o] = phv[m][q];
+  }
+  __syncthreads();
+  const double pos[3] = {sh.bc[BC_POS], sh.bc[BC_POS + 1], sh.bc[BC_POS + 2]};
+  const double vd[3]  = {sh.bc[BC_MOVE], sh.bc[BC_MOVE + 1], sh.bc[BC_MOVE + 2]};
+  proposal_features(s, v.R, st.ions, iat, pos, vd, a.tau, sh, la.X + ((size_t)a.nw + w) * MLP_IN);
+}
+
+// Metropolis-Hastings with the asymmetric-proposal correction (the walker's own uniform stream),
+// the training objective f = min(1, e^L) |r' - r|^2 and its gradient with respect to both network
+// outputs (loss -mean f), then accept / reject (phase E).
+template<typename DT, int MAXM>
+__global__ __launch_bounds__(WG) void k_learn_accept(SysC s, State st, MoveArgs a, LearnArgs la)
+{
+  __shared__ MoveLDS sh;
+  extern __shared__ double sB_raw[];
+  DT* sB = reinterpret_cast<DT*>(sB_raw);
+  stage_jastrow(s, sh);
+  const int w = blockIdx.x;
+  if (w >= a.nw)
+    return;
+  const int g = a.g, n = s.n[g], ld = s.ld, k = s.k, N = s.N;
+  const int iat = a.iat0, r = iat - s.first[g], d = a.d0;
+  WalkerView<DT> v = walker_view<DT>(s, st, g, w);
+  const double* mv = st.mv + (size_t)w * MV_SIZE;
+  const double* ls = la.ls + (size_t)w * LS_SIZE;
+  if (threadIdx.x == 0)
+  {
+    const float* of = la.out + (size_t)w * MLP_OUT;
+    const float* orr = la.out + ((size_t)a.nw + w) * MLP_OUT;
+    const double tau = a.tau, ratio = mv[MV_RATIO];
+    const double s_r = orr[3], sig_r2 = tau * exp(2.0 * s_r);
+    double e[3], D[3], ee = 0, D2 = 0;
+    for (int q = 0; q < 3; q++)
+    {
+      e[q] = ls[LS_R + q] - ((ls[LS_RP + q] + ls[LS_VR + q]) + (double)orr[q]);
+      D[q] = ls[LS_RP + q] - ls[LS_R + q];
+      ee += e[q] * e[q];
+      D2 += D[q] * D[q];
+    }
+    const double lqf = ls[LS_LQF];
+    const double lqr = -ee / (2.0 * sig_r2) - 3.0 * s_r;
+    const double pr2 = ratio * ratio;
+    const bool flag  = pr2 >= DBL_EPS;
+    const double prob = pr2 * exp(lqr - lqf);
+    int ucount        = a.ubase[(size_t)w * (N + 1) + iat];
+    bool acc          = false;
+    if (flag)
+    {
+      const double u = a.unif[(size_t)w * N + ucount];
+      ucount++;
+      acc = u < prob;
+    }
+    a.ubase[(size_t)w * (N + 1) + iat + 1] = ucount;
+    v.stats[acc ? 0 : 1] += 1;
+    sh.bc[BC_ACC] = acc ? 1.0 : 0.0;
+    // objective and its output gradients
+    const double A     = prob < 1.0 ? prob : 1.0;
+    const bool below   = prob < 1.0;
+    const double f     = A * D2;
+    const double sig_f = sqrt(tau) * exp((double)of[3]);
+    double gf[4], gr[4], dfD[3], dot = 0;
+    for (int q = 0; q < 3; q++)
+    {
+      const double dL = 2.0 * ls[LS_GNEW + q] + e[q] / sig_r2;
+      dfD[q]          = 2.0 * A * D[q] + (below ? A * D2 * dL : 0.0);
+      gf[q]           = dfD[q];
+      gr[q]           = below ? A * D2 * e[q] / sig_r2 : 0.0;
+      dot += dfD[q] * sig_f * ls[LS_XI + q];
+    }
+    gf[3] = dot + (below ? 3.0 * A * D2 : 0.0);
+    gr[3] = below ? A * D2 * (ee / sig_r2 - 3.0) : 0.0;
+    if (la.dout)
+      for (int q = 0; q < 4; q++)
+      {
+        la.dout[(size_t)w * MLP_OUT + q]          = (float)(-gf[q] / a.nw);
+        la.dout[((size_t)a.nw + w) * MLP_OUT + q] = (float)(-gr[q] / a.nw);
+      }
+    if (la.fsum)
+    {
+      la.fsum[(size_t)w * 2 + 0] += f;
+      la.fsum[(size_t)w * 2 + 1] += A;
+    }
+    if (a.trace)
+    {
+      double* tr = a.trace + ((size_t)iat * a.nw + w) * TRACE_FIELDS;
+      tr[0]      = ratio;
+      tr[1]      = acc ? 1.0 : 0.0;
+      for (int q = 0; q < 3; q++)
+      {
+        tr[2 + q] = ls[LS_RP + q];
+        tr[5 + q] = ls[LS_GNOW + q];
+      }
+    }
+    if (la.ltrace)
+    {
+      double* lt = la.ltrace + ((size_t)iat * a.nw + w) * LTRACE_FIELDS;
+      for (int q = 0; q < 4; q++)
+      {
+        lt[q]     = of[q];
+        lt[4 + q] = orr[q];
+      }
+      lt[8]  = lqf;
+      lt[9]  = lqr;
+      lt[10] = prob;
+      lt[11] = f;
+      for (int q = 0; q < 3; q++)
+        lt[12 + q] = ls[LS_VR + q];
+    }
+    for (int q = 0; q < 3; q++)
+      sh.bc[BC_POS + q] = mv[MV_POS + q];
+    for (int q = 0; q < 13; q++)
+      sh.bc[BC_C + q] = mv[MV_C + q];
+  }
+  if (k > 1 && threadIdx.x < d)
+    sh.y[threadIdx.x] = v.Binv[d * k + threadIdx.x];
+  __syncthreads();
+  double ainv_r[MAXM], phv[MAXM][5];
+  const double* phi = st.phi + (size_t)w * 5 * ld;
+#pragma unroll
+  for (int m = 0; m < MAXM; m++)
+  {
+    const int o = threadIdx.x + WG * m;
+    ainv_r[m]   = o < n ? (double)v.Ainv[(size_t)r * ld + o] : 0.0;
+    for (int q = 0; q < 5; q++)
+      phv[m][q] = o < n ? phi[q * ld + o] : 0.0;
+  }
+  __syncthreads();
+  phaseE<DT, MAXM>(s, g, iat, n, ld, k, d, r, v.R, v.Ainv, v.dpsi, v.U, v.V, v.Binv, v.dl, v.jas, v.logdet, ainv_r, phv,
+                   sh, sB, true);
+}
+
+// features of electron a.iat0 at the position LS_R with the drift LS_V of each walker (test hook)
+__global__ __launch_bounds__(WG) void k_learn_features(SysC s, State st, MoveArgs a, LearnArgs la)
+{
+  __shared__ MoveLDS sh;
+  const int w = blockIdx.x;
+  if (w >= a.nw)
+    return;
+  const double* ls    = la.ls + (size_t)w * LS_SIZE;
+  const double pos[3] = {ls[LS_R], ls[LS_R + 1], ls[LS_R + 2]};
+  const double vd[3]  = {ls[LS_V], ls[LS_V + 1], ls[LS_V + 2]};
+  proposal_features(s, st.R + (size_t)w * 3 * s.Npad, st.ions, a.iat0, pos, vd, a.tau, sh, la.X + (size_t)w * MLP_IN);
+}
+
+template<typename ST, typename DT>
+static hipError_t launch_learn_t(int which, const SysC& s, const State& st, const ST* coefs, const MoveArgs& a,
+                                 const LearnArgs& la, hipStream_t stream)
+{
+  const int maxn   = s.n[a.g];
+  const dim3 grid(a.nw), block(WG);
+  const size_t dyn = binv_lds_bytes(s.k, sizeof(DT));
+  if (which == LK_FEATURES)
+  {
+    k_learn_features<<<grid, block, 0, stream>>>(s, st, a, la);
+    return hipGetLastError();
+  }
+  if (maxn <= WG)
+  {
+    switch (which)
+    {
+    case LK_PRE: k_learn_pre<ST, DT, 1><<<grid, block, dyn, stream>>>(s, st, coefs, a, la); break;
+    case LK_PROP: k_learn_prop<ST, DT, 1><<<grid, block, 0, stream>>>(s, st, coefs, a, la); break;
+    case LK_ACCEPT: k_learn_accept<DT, 1><<<grid, block, dyn, stream>>>(s, st, a, la); break;
+    default: return hipErrorInvalidValue;
+    }
+  }
+  else if (maxn <= 2 * WG)
+  {
+    switch (which)
+    {
+    case LK_PRE: k_learn_pre<ST, DT, 2><<<grid, block, dyn, stream>>>(s, st, coefs, a, la); break;
+    case LK_PROP: k_learn_prop<ST, DT, 2><<<grid, block, 0, stream>>>(s, st, coefs, a, la); break;
+    case LK_ACCEPT: k_learn_accept<DT, 2><<<grid, block, dyn, stream>>>(s, st, a, la); break;
+    default: return hipErrorInvalidValue;
+    }
+  }
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_learn(int which, const SysC& s, const State& st, const void* coefs, const MoveArgs& a,
+                        const LearnArgs& la, hipStream_t stream)
+{
+  return dispatch_prec(s.prec, [&](auto stv, auto dtv) {
+    using ST = decltype(stv);
+    using DT = decltype(dtv);
+    return launch_learn_t<ST, DT>(which, s, st, static_cast<const ST*>(coefs), a, la, stream);
+  });
 }
 
 //------------------------------------------------------------------------------------------------

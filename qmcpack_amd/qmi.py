@@ -218,6 +218,62 @@ class Context:
                                        C.c_uint32(seed), _p(E), _p(tr)))
         return E, tr
 
+    # ---------------------------------------------------------------- learned proposal
+    MLP_NPARAM = 42372
+
+    def mlp_params(self, params=None):
+        """Upload params (FP32, MLP_NPARAM) when given; return the current parameters."""
+        if params is not None:
+            p = np.ascontiguousarray(params, dtype=np.float32)
+            assert p.size == self.MLP_NPARAM
+            self._check(self.L.qmi_mlp_params(self.h, 1, p.ctypes.data_as(C.POINTER(C.c_float))))
+        out = np.empty(self.MLP_NPARAM, dtype=np.float32)
+        self._check(self.L.qmi_mlp_params(self.h, 0, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def mlp_forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        out = np.empty((X.shape[0], 4), dtype=np.float32)
+        self._check(self.L.qmi_mlp_forward(self.h, X.shape[0], X.ctypes.data_as(C.POINTER(C.c_float)),
+                                           out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def mlp_gradient(self, X, dout):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        dout = np.ascontiguousarray(dout, dtype=np.float32)
+        g = np.empty(self.MLP_NPARAM, dtype=np.float32)
+        self._check(self.L.qmi_mlp_gradient(self.h, X.shape[0], X.ctypes.data_as(C.POINTER(C.c_float)),
+                                            dout.ctypes.data_as(C.POINTER(C.c_float)),
+                                            g.ctypes.data_as(C.POINTER(C.c_float))))
+        return g
+
+    def proposal_config(self, learned=True, train=False, lr=1e-3, init_seed=0):
+        self._check(self.L.qmi_proposal_config(self.h, int(learned), int(train), C.c_double(lr),
+                                               C.c_uint32(init_seed)))
+
+    def proposal_stats(self):
+        out = np.empty(4)
+        self._check(self.L.qmi_proposal_stats(self.h, _p(out)))
+        return dict(objective=out[0], acceptance=out[1], moves=out[2], train_steps=out[3])
+
+    def proposal_features(self, iat, pos, drift, tau):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.nw, 3)
+        drift = np.ascontiguousarray(drift, dtype=np.float64).reshape(self.nw, 3)
+        out = np.empty((self.nw, 68), dtype=np.float32)
+        self._check(self.L.qmi_mw_proposal_features(self.h, int(iat), _p(pos), _p(drift), C.c_double(tau),
+                                                    out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def vmc_run_learned(self, steps, sub_steps=1, tau=0.3, recompute_period=0, seed=1, trace=False):
+        """vmc_run that also returns the learned-proposal trace [steps][sub][N][nw][15]: mu[3], s,
+        mu'[3], s', log q_fwd, log q_rev, acceptance probability, objective, drift at r'[3]."""
+        E = np.empty((steps, self.nw))
+        tr = np.empty((steps, sub_steps, self.N, self.nw, 8)) if trace else None
+        lt = np.empty((steps, sub_steps, self.N, self.nw, 15)) if trace else None
+        self._check(self.L.qmi_vmc_run_ex(self.h, int(steps), int(sub_steps), C.c_double(tau), int(recompute_period),
+                                          C.c_uint32(seed), _p(E), _p(tr), _p(lt)))
+        return E, tr, lt
+
     def vmc_stats(self):
         s = np.empty(2)
         self._check(self.L.qmi_vmc_stats(self.h, _p(s)))
