@@ -6,14 +6,14 @@ recomputes every quantity.  Two references are used:
   * the oracle on the table rounded exactly as the device stores it (FP32, or BF16 by round to
     nearest even of the FP32 value): the difference is the device's arithmetic and storage;
   * the oracle on the original FP64 table: adds the table quantization (about 3e-8 relative for
-    FP32 and 4e-3 for BF16 coefficients).
+    FP32 and 2e-3 for BF16 coefficients).
 Errors are relative to max(|x|, floor): ratios floor 1e-2, gradients / Laplacians / energies 1.
 
 An FP32 determinant engine stores the inverse Slater matrix in FP32, so a ratio (a dot product of an
 inverse row with the orbital values) carries an error of about cond(A) x 6e-8 times its
 cancellation.  The synthetic random-coefficient orbitals have cond(A) = 4e2..1e4 at n = 128, where
 that is ~1e-3 absolute near small ratios; the FP32-engine bounds are therefore checked on a
-well-conditioned plane-wave system (cond 40..200 at n = 16), and on the random system only the
+well-conditioned plane-wave system (cond up to 1e3 at n = 16), and on the random system only the
 measured errors are printed with gross bounds.  The FP64-engine (table-only) modes keep FP64
 parity against the same-table oracle.
 """
