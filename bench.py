@@ -103,19 +103,68 @@ def cpu_baseline(system, cfg, nwalk=16, nthreads=None):
                        f"({dt:.1f} s), delay rank {system.delay_rank}, oracle in C++ FP64")
 
 
+def chain_summary(E, dt, nw, N, steps):
+    """moves/s, kappa (mean over walkers), effective samples/s and the mean local energy with its
+    error bar sqrt(var kappa / (nw steps))."""
+    kappa = float(np.mean([autocorr_kappa(E[:, w]) for w in range(nw)])) if steps >= 3 else 1.0
+    emean = float(np.mean(E))
+    eerr = float(np.sqrt(np.var(E) * kappa / E.size))
+    return dict(moves_per_s=nw * N * steps / dt, ms_per_sweep=dt / steps * 1e3, kappa=kappa,
+                effective_samples_per_s=nw * steps / kappa / dt, energy=emean, energy_err=eerr)
+
+
+def run_variant(qmi, system, R, cfg, precision, learned, steps, warmup, train_sweeps, lr, log):
+    """One context: optional training of the learned proposal, warmup, then `steps` timed sweeps."""
+    nw, N, tau, seed = R.shape[0], system.N, cfg["tau"], cfg["seed"]
+    ctx = qmi.Context(0, precision=precision)
+    ctx.system_create(system)
+    ctx.walkers_load(R, 1)
+    ctx.mw_evaluateLog()
+    extra = {}
+    if learned:
+        ctx.proposal_config(learned=True, train=True, lr=lr, init_seed=7)
+        t1 = time.perf_counter()
+        ctx.vmc_run(train_sweeps, 1, tau, 0, seed + 11, energies=False)
+        st = ctx.proposal_stats()
+        extra = dict(train_samples=int(st["moves"]), train_steps=int(st["train_steps"]),
+                     train_s=time.perf_counter() - t1)
+        ctx.proposal_config(learned=True, train=False)
+    if warmup > 0:
+        ctx.vmc_run(warmup, 1, tau, 0, seed)
+    if learned:
+        ctx.proposal_stats()
+    t1 = time.perf_counter()
+    E, _ = ctx.vmc_run(steps, 1, tau, 0, seed)
+    dt = time.perf_counter() - t1
+    out = chain_summary(E, dt, nw, N, steps)
+    if learned:
+        st = ctx.proposal_stats()
+        out.update(extra, mean_acceptance=st["acceptance"], mean_objective=st["objective"])
+    else:
+        acc = ctx.vmc_stats()
+        out.update(mean_acceptance=acc["accepted"] / max(1, acc["accepted"] + acc["rejected"]))
+    log(f"[bench] {precision}{' + learned' if learned else ''}: " + json.dumps(out))
+    return ctx, out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=50, help="timed sweeps (the workload's 50)")
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--walkers", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-walkers", type=int, default=128)
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--precision", choices=["fp64", "fp32", "bf16", "t32", "tbf16"], default="fp64",
-                    help="storage precision: fp64 (the plain form), fp32 (FP32 table and determinant "
-                         "engine), bf16 (BF16 table, FP32 determinant engine), t32 / tbf16 (FP32 / BF16 "
-                         "table, FP64 determinant engine)")
+                    help="storage precision of the headline run: fp64 (the plain form), fp32 (FP32 table "
+                         "and determinant engine), bf16 (BF16 table, FP32 determinant engine), t32 / tbf16 "
+                         "(FP32 / BF16 table, FP64 determinant engine)")
+    ap.add_argument("--variants", default="fp32,t32,learned:fp32,learned:t32",
+                    help="comma-separated extra runs (precision, or learned:<precision>); empty for none")
+    ap.add_argument("--train-sweeps", type=int, default=2,
+                    help="training sweeps of the learned proposal (2 x 1024 x 512 = 1.05e6 moves)")
+    ap.add_argument("--lr", type=float, default=1e-3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs (walker sharding, one "
@@ -135,27 +184,11 @@ def main():
     R = make_walkers(system, nw, cfg["seed"])
     log(f"[bench] system built in {time.perf_counter() - t0:.1f}s: N={system.N} n={system.n_up} "
         f"M={system.M[0]} norb={system.norb} ions={len(system.ions)} k={system.delay_rank} nw={nw}")
-    ctx = qmi.Context(0, precision=args.precision)
     sc = {"fp64": 8, "fp32": 4, "bf16": 2, "t32": 4, "tbf16": 2}[args.precision]
     sd = 4 if args.precision in ("fp32", "bf16") else 8
-    ctx.system_create(system)
-    ctx.walkers_load(R, 1)
-    t1 = time.perf_counter()
-    ctx.mw_evaluateLog()
-    log(f"[bench] upload + initial evaluateLog {time.perf_counter() - t1:.1f}s")
-    tau = cfg["tau"]
-    seed = cfg["seed"]
-    if args.warmup > 0:
-        ctx.vmc_run(args.warmup, 1, tau, 0, seed)
-        log(f"[bench] warmup {args.warmup} steps done")
-    t1 = time.perf_counter()
-    E, _ = ctx.vmc_run(args.steps, 1, tau, 0, seed)
-    dt = time.perf_counter() - t1
-    moves = nw * system.N * args.steps
-    value = moves / dt
-    log(f"[bench] timed {args.steps} steps in {dt:.3f}s -> {value:.4e} moves/s")
-    kappa = float(np.mean([autocorr_kappa(E[:, w]) for w in range(nw)])) if args.steps >= 3 else 1.0
-    eff = nw * args.steps / kappa / dt
+    ctx, head = run_variant(qmi, system, R, cfg, args.precision, False, args.steps, args.warmup, 0, args.lr, log)
+    value = head["moves_per_s"]
+    tau, seed = cfg["tau"], cfg["seed"]
 
     # per-kernel device time (HIP events on the library stream), a separate short pass
     ctx.timing_enable(True)
@@ -164,6 +197,7 @@ def main():
     ctx.vmc_run(nprof, 1, tau, 0, seed)
     tm = ctx.timing()
     ctx.timing_enable(False)
+    del ctx
     n, N, nion, k = system.n_up, system.N, len(system.ions), system.delay_rank
     # the fused move kernel runs either once per move (move_one) or once per delay window of k
     # moves (sweep_segment); either way one sweep performs N moves per walker at delays
@@ -183,6 +217,18 @@ def main():
                for name, v in tm.items()}
     log("[bench] kernels: " + json.dumps(kernels))
 
+    variants = {}
+    for v in [x for x in args.variants.split(",") if x]:
+        learned = v.startswith("learned:")
+        prec = v.split(":", 1)[1] if learned else v
+        c2, variants[v] = run_variant(qmi, system, R, cfg, prec, learned, args.steps, args.warmup,
+                                      args.train_sweeps, args.lr, log)
+        del c2
+    for v, r in variants.items():
+        r["effective_samples_vs_plain"] = r["effective_samples_per_s"] / head["effective_samples_per_s"]
+        d = abs(r["energy"] - head["energy"])
+        r["energy_diff_sigmas"] = d / np.sqrt(r["energy_err"] ** 2 + head["energy_err"] ** 2)
+
     traffic, traffic_src = (measured_traffic("qmi::k_move_one" if kname == "move_one" else "qmi::k_sweep_segment")
                             if nw == 1024 and args.precision == "fp64" else (None, None))
     workload = {"fp64": "plain FP64", "fp32": "mixed precision (FP32 table, FP32 determinant engine)",
@@ -196,7 +242,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": dt / args.steps * 1e3,
+        "ms_per_step": head["ms_per_sweep"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -209,8 +255,11 @@ def main():
             "walkers": nw, "electrons": N, "orbitals_per_spin": n, "grid": 64, "ions": nion,
             "delay_rank": k, "precision": args.precision,
         },
-        "effective_samples_per_s": eff,
-        "kappa": kappa,
+        "effective_samples_per_s": head["effective_samples_per_s"],
+        "kappa": head["kappa"],
+        "energy": [head["energy"], head["energy_err"]],
+        "acceptance": head["mean_acceptance"],
+        "variants": variants,
         "kernels": kernels,
         "roofline": {
             "kernel": ("k_move_one" if kname == "move_one" else "k_sweep_segment")
@@ -227,7 +276,7 @@ def main():
             "flops_per_launch": flops_launch,
         },
         "flush_roofline": {
-            "kernel": "flush (T = Ainv U^T, BV = Binv V, Ainv -= T BV)",
+            "kernel": "k_flush_mfma (T = Ainv U^T - W, T' = T Binv, Ainv -= T' V on the matrix cores)",
             "avg_ms": fl_ms,
             "achieved_gbs": (nw * flush_bytes(n, k, sd)) / (fl_ms * 1e-3) / 1e9 if fl_ms > 0 else 0.0,
             "achieved_tflops": (nw * flush_flops(n, k)) / (fl_ms * 1e-3) / 1e12 if fl_ms > 0 else 0.0,

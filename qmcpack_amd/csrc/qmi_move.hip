@@ -1073,11 +1073,104 @@ __global__ void k_api_make_move(SysC s, State st, int iat, int nw, const double*
 //------------------------------------------------------------------------------------------------
 // Learned proposal (no counterpart in the reference).
 
+// Indices of the K smallest values of r[0, count) (ties by the lower index) into sel[0, K) (-1
+// where fewer than K are finite).  The largest finite value sets the range of a 64-bin histogram;
+// the bin holding the K-th value bounds the candidates, which are compacted and ranked among
+// themselves (a full ranking when more than SEL_CAP fall below the bound).  scratch: at least
+// SEL_CAP + 68 ints of LDS.  Call with all threads; ends with a barrier.
+constexpr int SEL_BINS = 64, SEL_CAP = 256;
+__device__ inline void select_nearest(const double* r, int count, int K, int* sel, int* scratch)
+{
+  const int t   = threadIdx.x;
+  int* hist     = scratch;              // [SEL_BINS]
+  int* cnt      = scratch + SEL_BINS;   // candidate count
+  int* rmaxbits = scratch + SEL_BINS + 1;
+  int* bstar    = scratch + SEL_BINS + 2;
+  int* list     = scratch + SEL_BINS + 4; // [SEL_CAP]
+  if (t < SEL_BINS)
+    hist[t] = 0;
+  if (t == 0)
+  {
+    *cnt      = 0;
+    *rmaxbits = 0;
+  }
+  if (t < K)
+    sel[t] = -1;
+  __syncthreads();
+  for (int j = t; j < count; j += WG)
+    if (r[j] < INFINITY)
+      atomicMax(rmaxbits, __float_as_int((float)r[j]));
+  __syncthreads();
+  const double rcap = (double)__int_as_float(*rmaxbits) * (1.0 + 1e-6) + 1e-30;
+  for (int j = t; j < count; j += WG)
+    if (r[j] < INFINITY)
+      atomicAdd(&hist[min(SEL_BINS - 1, (int)(r[j] / rcap * SEL_BINS))], 1);
+  __syncthreads();
+  if (t == 0)
+  {
+    int c = 0, b = 0;
+    for (; b < SEL_BINS; b++)
+    {
+      c += hist[b];
+      if (c >= K)
+        break;
+    }
+    *bstar = min(b, SEL_BINS - 1);
+  }
+  __syncthreads();
+  const int bs = *bstar;
+  for (int j = t; j < count; j += WG)
+    if (r[j] < INFINITY && min(SEL_BINS - 1, (int)(r[j] / rcap * SEL_BINS)) <= bs)
+    {
+      const int slot = atomicAdd(cnt, 1);
+      if (slot < SEL_CAP)
+        list[slot] = j;
+    }
+  __syncthreads();
+  const int m = *cnt;
+  if (m <= SEL_CAP)
+  {
+    if (t < m)
+    {
+      const int j     = list[t];
+      const double rj = r[j];
+      int rank        = 0;
+      for (int i = 0; i < m; i++)
+      {
+        const int ji    = list[i];
+        const double ri = r[ji];
+        rank += (ri < rj) || (ri == rj && ji < j);
+      }
+      if (rank < K)
+        sel[rank] = j;
+    }
+  }
+  else
+  {
+    for (int j = t; j < count; j += WG)
+    {
+      const double rj = r[j];
+      if (!(rj < INFINITY))
+        continue;
+      int rank = 0;
+      for (int i = 0; i < count; i++)
+      {
+        const double ri = r[i];
+        rank += (ri < rj) || (ri == rj && i < j);
+      }
+      if (rank < K)
+        sel[rank] = j;
+    }
+  }
+  __syncthreads();
+}
+
 // Features of electron iat at pos with drift vd: the 4 nearest ions (dx, dy, dz, |d|), the 16
 // nearest other electrons (dx, dy, dz) (d = minimum image of source - pos), the drift and tau,
 // written to x[68] (FP32).  Neighbours are ranked by minimum-image distance, ties by the lower
 // index; missing neighbours are zero.  sh.red holds the distances (electrons at [0, N), ions at
-// [1024, 1024 + nion)).  Starts and ends with a barrier.
+// [1024, 1024 + nion)); sh.row / sh.vec serve as the selection scratch.  Starts and ends with a
+// barrier.
 __device__ inline void proposal_features(const SysC& s, const double* __restrict__ R, const double* __restrict__ ions,
                                          int iat, const double pos[3], const double vd[3], double tau, MoveLDS& sh,
                                          float* __restrict__ x)
@@ -1097,36 +1190,8 @@ __device__ inline void proposal_features(const SysC& s, const double* __restrict
     double dr[3];
     di[j] = min_image(s, ions[j], ions[ni + j], ions[2 * ni + j], pos, dr);
   }
-  if (t < N_EL_NB + N_ION_NB)
-    sh.sel[t] = -1;
-  __syncthreads();
-  for (int j = t; j < N; j += WG)
-  {
-    if (j == iat)
-      continue;
-    const double dj = de[j];
-    int rank        = 0;
-    for (int i = 0; i < N; i++)
-    {
-      const double dv = de[i];
-      rank += (dv < dj) || (dv == dj && i < j);
-    }
-    if (rank < N_EL_NB)
-      sh.sel[rank] = j;
-  }
-  for (int j = t; j < ni; j += WG)
-  {
-    const double dj = di[j];
-    int rank        = 0;
-    for (int i = 0; i < ni; i++)
-    {
-      const double dv = di[i];
-      rank += (dv < dj) || (dv == dj && i < j);
-    }
-    if (rank < N_ION_NB)
-      sh.sel[N_EL_NB + rank] = j;
-  }
-  __syncthreads();
+  select_nearest(de, N, N_EL_NB, sh.sel, reinterpret_cast<int*>(sh.row));
+  select_nearest(di, ni, N_ION_NB, sh.sel + N_EL_NB, reinterpret_cast<int*>(sh.vec));
   if (t < MLP_IN)
   {
     double val = 0.0;
