@@ -60,9 +60,9 @@ def segment_flops_per_move(n, N, n_ion, d):
 
 
 def flush_bytes(n, k, s=8):  # s: bytes per determinant element
-    """Algorithmic bytes of one rank-k flush of one n x n inverse: read and write Ainv, read U, V,
-    Binv and the k x n intermediates."""
-    return (3 * n * n + 6 * n * k) * s
+    """Compulsory bytes of one rank-k flush of one n x n inverse in a single pass: read and write
+    Ainv, read U and V (n x k each) and Binv (k x k)."""
+    return (2 * n * n + 2 * n * k + k * k) * s
 
 
 def flush_flops(n, k):
@@ -154,7 +154,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--walkers", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-walkers", type=int, default=128)
+    ap.add_argument("--cpu-walkers", type=int, default=1024)
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--precision", choices=["fp64", "fp32", "bf16", "t32", "tbf16"], default="fp64",
                     help="storage precision of the headline run: fp64 (the plain form), fp32 (FP32 table "
