@@ -69,6 +69,13 @@ typedef struct
   const double* j2_ud;
   double j2_rcut, j2_cusp_uu, j2_cusp_ud;
   int delay_rank; /* 1: Sherman-Morrison (Fahy) update; k > 1: rank-k delayed Woodbury */
+  /* Complex orbitals (SplineC2C, QMCWaveFunctions/BsplineFactory/SplineC2C.cpp:200-279):
+   * psi_j(r) = e^{-i k_j.r} u_j(r); coefs then hold [Mx][My][Mz][norb][2] (real and imaginary
+   * parts of u_j interleaved) and kpoints the Cartesian k_j.  With complex orbitals every
+   * value-typed array crossing this interface (ratios, gradients, G, L, orbital values,
+   * inverse rows) holds (re, im) pairs, and logpsi holds (log|psi|, phase) per walker. */
+  int complex_orbitals;
+  const double* kpoints; /* [norb][3] */
 } qmi_system_desc;
 
 /* ---- context and data ---------------------------------------------------------------------- */

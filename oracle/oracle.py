@@ -28,6 +28,7 @@ class OrcSystemDesc(C.Structure):
         ("j2_np", C.c_int), ("j2_uu", _dp), ("j2_ud", _dp), ("j2_rcut", C.c_double),
         ("j2_cusp_uu", C.c_double), ("j2_cusp_ud", C.c_double),
         ("delay_rank", C.c_int),
+        ("complex_orbitals", C.c_int), ("kpoints", _dp),
     ]
 
 
@@ -52,7 +53,7 @@ def lib():
         for name in ("orc_det_recompute", "orc_det_eval_grad", "orc_det_ratio_grad", "orc_det_accept_reject",
                      "orc_det_complete", "orc_det_get", "orc_vmc_evaluate_log", "orc_wf_ratio_grad",
                      "orc_wf_accept_reject", "orc_wf_eval_grad", "orc_wf_evaluate_gl", "orc_vmc_run", "orc_vmc_replay",
-                     "orc_vmc_get"):
+                     "orc_vmc_get", "orc_vmc_is_complex", "orc_spo_vgl_c", "orc_invert_transpose_c"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -118,6 +119,21 @@ def spo_vgl(L, coefs, r, n=None):
     return out
 
 
+def spo_vgl_c(L, coefs, kpoints, r, n=None):
+    """Complex (C2C) orbitals: coefs [Mx][My][Mz][norb][2], kpoints [norb][3]; returns complex
+    [5][n] (value, gradient x/y/z, Laplacian)."""
+    coefs = np.ascontiguousarray(coefs, dtype=np.float64)
+    M = (C.c_int * 3)(*coefs.shape[:3])
+    norb = coefs.shape[3]
+    n = norb if n is None else n
+    Lv = np.ascontiguousarray(L, dtype=np.float64)
+    kp = np.ascontiguousarray(kpoints, dtype=np.float64)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    out = np.empty((5, n, 2))
+    lib().orc_spo_vgl_c(ptr(Lv), M, C.c_int(norb), ptr(coefs), ptr(kp), ptr(r), C.c_int(n), ptr(out))
+    return out[..., 0] + 1j * out[..., 1]
+
+
 def functor_vgl(params, rcut, cusp, r):
     params = np.ascontiguousarray(params, dtype=np.float64)
     out = np.empty(3)
@@ -143,6 +159,22 @@ def invert_transpose(a):
     ld = np.empty(2)
     info = lib().orc_invert_transpose(C.c_int(n), ptr(a), ptr(ainv), ptr(ld))
     return ainv, complex(ld[0], ld[1]), info
+
+
+def invert_transpose_c(a):
+    """Complex LU inverse of the transpose with log determinant; a complex [n][n]."""
+    a = np.asarray(a, dtype=np.complex128)
+    n = a.shape[0]
+    ar = np.ascontiguousarray(np.stack([a.real, a.imag], axis=-1))
+    ainv = np.empty_like(ar)
+    ld = np.empty(2)
+    info = lib().orc_invert_transpose_c(C.c_int(n), ptr(ar), ptr(ainv), ptr(ld))
+    return ainv[..., 0] + 1j * ainv[..., 1], complex(ld[0], ld[1]), info
+
+
+def _cv(a):
+    """(re, im) pairs in the last axis -> complex array."""
+    return a[..., 0] + 1j * a[..., 1]
 
 
 class Det:
@@ -200,27 +232,33 @@ class VMC:
                                                   C.c_uint32(seed)))
         if not self.h:
             raise RuntimeError("orc_vmc_create failed")
+        self.cplx = bool(lib().orc_vmc_is_complex(self.h))
+        self.vw = 2 if self.cplx else 1
 
     def __del__(self):
         if getattr(self, "h", None):
             lib().orc_vmc_destroy(self.h)
             self.h = None
 
+    def _vals(self, *shape):
+        return np.empty(shape + ((2,) if self.cplx else ()))
+
+    def _out(self, a):
+        return _cv(a) if self.cplx else a
+
     def evaluate_log(self):
-        logpsi = np.empty(self.nw)
-        G = np.empty((self.nw, self.N, 3))
-        L = np.empty((self.nw, self.N))
+        """logpsi (log|psi|, or log|psi| + i phase for complex orbitals), G, L (value type)."""
+        logpsi, G, L = self._vals(self.nw), self._vals(self.nw, self.N, 3), self._vals(self.nw, self.N)
         st = lib().orc_vmc_evaluate_log(self.h, ptr(logpsi), ptr(G), ptr(L))
         if st != 0:
             raise RuntimeError("oracle evaluate_log failed")
-        return logpsi, G, L
+        return self._out(logpsi), self._out(G), self._out(L)
 
     def ratio_grad(self, iw, iat, newpos):
         newpos = np.ascontiguousarray(newpos, dtype=np.float64)
-        r = C.c_double()
-        g = np.empty(3)
-        lib().orc_wf_ratio_grad(self.h, iw, iat, ptr(newpos), C.byref(r), ptr(g))
-        return r.value, g
+        r, g = self._vals(1), self._vals(3)
+        lib().orc_wf_ratio_grad(self.h, iw, iat, ptr(newpos), ptr(r), ptr(g))
+        return self._out(r)[0], self._out(g)
 
     def accept_reject(self, iw, iat, accepted):
         st = lib().orc_wf_accept_reject(self.h, iw, iat, int(accepted))
@@ -228,16 +266,15 @@ class VMC:
             raise RuntimeError("oracle accept failed")
 
     def eval_grad(self, iw, iat):
-        g = np.empty(3)
+        g = self._vals(3)
         lib().orc_wf_eval_grad(self.h, iw, iat, ptr(g))
-        return g
+        return self._out(g)
 
     def evaluate_gl(self, iw, fromscratch=False):
-        lp = C.c_double()
-        G = np.empty((self.N, 3))
-        L = np.empty(self.N)
-        lib().orc_wf_evaluate_gl(self.h, iw, int(fromscratch), C.byref(lp), ptr(G), ptr(L))
-        return lp.value, G, L
+        lp, G, L = self._vals(1), self._vals(self.N, 3), self._vals(self.N)
+        if lib().orc_wf_evaluate_gl(self.h, iw, int(fromscratch), ptr(lp), ptr(G), ptr(L)) != 0:
+            raise RuntimeError("oracle evaluate_gl failed")
+        return self._out(lp)[0], self._out(G), self._out(L)
 
     def run(self, steps, sub_steps=1, tau=0.3, recompute_period=0, nthreads=0, trace=False):
         E = np.empty((steps, self.nw))
@@ -263,12 +300,11 @@ class VMC:
 
     def get(self):
         R = np.empty((self.nw, self.N, 3))
-        logpsi = np.empty(self.nw)
-        G = np.empty((self.nw, self.N, 3))
-        L = np.empty((self.nw, self.N))
+        logpsi, G, L = self._vals(self.nw), self._vals(self.nw, self.N, 3), self._vals(self.nw, self.N)
         stats = np.empty(2)
         lib().orc_vmc_get(self.h, ptr(R), ptr(logpsi), ptr(G), ptr(L), ptr(stats))
-        return dict(R=R, logpsi=logpsi, G=G, L=L, accepted=stats[0], rejected=stats[1])
+        return dict(R=R, logpsi=self._out(logpsi), G=self._out(G), L=self._out(L), accepted=stats[0],
+                    rejected=stats[1])
 
 
 def make_desc(system, keep):
@@ -300,4 +336,7 @@ def make_desc(system, keep):
     d.j2_rcut = float(system.j2_rcut)
     d.j2_cusp_uu, d.j2_cusp_ud = float(system.j2_cusp_uu), float(system.j2_cusp_ud)
     d.delay_rank = int(system.delay_rank)
+    kp = getattr(system, "kpoints", None)
+    d.complex_orbitals = int(kp is not None and d.norb > 0)
+    d.kpoints = arr(kp) if d.complex_orbitals else None
     return d

@@ -36,6 +36,8 @@ typedef struct
   const double* j2_ud;
   double j2_rcut, j2_cusp_uu, j2_cusp_ud;
   int delay_rank; /* k: 1 = Sherman-Morrison, >1 delayed Woodbury */
+  int complex_orbitals;   /* 1: C2C orbitals, coefs [Mx][My][Mz][norb][2] (re, im) */
+  const double* kpoints;  /* [norb][3] Cartesian k of each orbital (complex orbitals) */
 } orc_system_desc;
 
 /* RNG: mt19937 + boost-style uniform + Box-Muller */

@@ -37,6 +37,7 @@ class SystemDesc(C.Structure):
         ("j2_np", C.c_int), ("j2_uu", _dp), ("j2_ud", _dp), ("j2_rcut", C.c_double),
         ("j2_cusp_uu", C.c_double), ("j2_cusp_ud", C.c_double),
         ("delay_rank", C.c_int),
+        ("complex_orbitals", C.c_int), ("kpoints", _dp),
     ]
 
 
@@ -98,6 +99,9 @@ def make_desc(system, keep: list) -> SystemDesc:
     d.j2_rcut = float(system.j2_rcut)
     d.j2_cusp_uu, d.j2_cusp_ud = float(system.j2_cusp_uu), float(system.j2_cusp_ud)
     d.delay_rank = int(system.delay_rank)
+    kp = getattr(system, "kpoints", None)
+    d.complex_orbitals = int(kp is not None and d.norb > 0)
+    d.kpoints = arr(kp) if d.complex_orbitals else None
     return d
 
 

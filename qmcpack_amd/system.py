@@ -50,6 +50,9 @@ class System:
     j2_cusp_uu: float
     j2_cusp_ud: float
     delay_rank: int
+    # complex (C2C) orbitals: coefs [Mx][My][Mz][norb][2] (real, imaginary) and the Cartesian k-point
+    # of each orbital; psi_j(r) = exp(-i k_j . r) u_j(r) (SplineC2C.cpp:200-279)
+    kpoints: np.ndarray | None = None
 
     @property
     def N(self) -> int:
@@ -90,6 +93,26 @@ def make_system(L: float, n_ion: int, n_up: int, n_dn: int, norb: int, M: int, d
                   j1_params=p.copy(), j1_rcut=rcut, j1_cusp=0.0,
                   j2_uu=p.copy(), j2_ud=p.copy(), j2_rcut=rcut, j2_cusp_uu=-0.25, j2_cusp_ud=-0.5,
                   delay_rank=delay_rank)
+
+
+def make_complex_system(L: float, n_ion: int, n_up: int, n_dn: int, norb: int, M: int, delay_rank: int,
+                        seed: int, n_jastrow: int = 8, **_ignored) -> System:
+    """Complex (C2C) orbitals at a seeded twist: u_j = a_j + i b_j with a_j, b_j drawn as the real
+    tables, and k_j = 2 pi (t + n_j) / L for the twist t in [-1/2, 1/2)^3 and the integer vectors n_j
+    of smallest |t + n_j| (distinct per orbital)."""
+    s = make_system(L, n_ion, n_up, n_dn, norb, M, delay_rank, seed, with_coefs=False, n_jastrow=n_jastrow)
+    rng = np.random.default_rng(seed + 104729)
+    re = make_coefs(M, norb, rng)
+    im = make_coefs(M, norb, rng)
+    coefs = np.ascontiguousarray(np.stack([re, im], axis=-1))
+    twist = rng.uniform(-0.5, 0.5, size=3)
+    r = int(np.ceil(np.cbrt(norb))) + 1
+    n = np.array([(a, b, c) for a in range(-r, r + 1) for b in range(-r, r + 1) for c in range(-r, r + 1)],
+                 dtype=np.float64)
+    kv = twist[None, :] + n
+    order = np.lexsort((kv[:, 2], kv[:, 1], kv[:, 0], np.round(np.sum(kv * kv, axis=1), 12)))
+    kpoints = 2.0 * np.pi * kv[order[:norb]] / np.asarray(s.L)[None, :]
+    return dataclasses.replace(s, coefs=coefs, kpoints=np.ascontiguousarray(kpoints))
 
 
 def make_walkers(system: System, nw: int, seed: int) -> np.ndarray:
