@@ -77,6 +77,8 @@ struct SysC
   int k;    // delay rank
   int has_det, has_j1, has_j2;
   int prec; // Precision
+  int cplx; // complex (C2C) orbitals: complex FP64 determinant engine, table of 2 norb splines
+  int kld;  // row stride of the k-point table (>= norb)
   double L[3], Linv[3];
   int periodic[3];
   double G[9], symGG[6];
@@ -109,6 +111,7 @@ struct State
   double* phi;      // [nw][5][ld]
   double* mv;       // [nw][MV_SIZE]
   const double* ions; // [3][nion]
+  const double* kpts; // [3][kld] Cartesian k of each orbital (complex orbitals)
 };
 
 template<typename DT>
@@ -142,7 +145,8 @@ enum MoveField
   MV_FLAG  = 5, // 1 if |ratio|^2 >= eps (a uniform is drawn)
   MV_C     = 8, // 13: J1 (curAt, curGrad[3], curLap), J2 (cur_Uat, grad[3]), det (ratio, grad numerators[3])
   MV_GNOW  = 21, // 3: gradient at the current position (drift input)
-  MV_SIZE  = 24
+  MV_CDET  = 24, // 8: complex orbitals: det (ratio, grad numerators[3]) as (re, im)
+  MV_SIZE  = 32
 };
 
 //------------------------------------------------------------------------------------------------

@@ -159,8 +159,9 @@ struct qmi_ctx
   rocblas_handle blas = nullptr;
   std::string err;
   bool have_system = false;
+  bool cplx        = false; // complex (C2C) orbitals
   SysC sys{};
-  DevBuf<double> coefs, ions;
+  DevBuf<double> coefs, ions, kpts;
   // walkers
   int nw = 0, wpc = 1, ncrowd = 0;
   DevBuf<double> R, jas, logdet, stats, invRow, phi, mv, flushT, out, gl3, Gbuf, Lbuf, hostin;
@@ -238,7 +239,14 @@ struct qmi_ctx
     }
   }
   bool det_fp32() const { return precision == QMI_PREC_MIXED_FP32 || precision == QMI_PREC_MIXED_BF16; }
-  size_t det_esize() const { return det_fp32() ? 4 : 8; }
+  size_t det_esize() const { return cplx ? 16 : det_fp32() ? 4 : 8; }
+  // doubles per value crossing the interface (ratios, gradients, G, L, orbitals)
+  int vw() const { return cplx ? 2 : 1; }
+
+  hipError_t move_kernel(int which, const MoveArgs& a)
+  {
+    return cplx ? launch_move_c(which, sys, st, coefs.p, a, stream) : launch_move(which, sys, st, coefs.p, a, stream);
+  }
   // device buffer (in doubles) holding count elements of the determinant type
   size_t det_words(size_t count) const { return (count * det_esize() + 7) / 8; }
 
@@ -289,6 +297,14 @@ struct qmi_ctx
     std::copy(symGG, symGG + 6, s.symGG);
     s.prec    = precision;
     s.has_det = d->norb > 0 && d->coefs != nullptr;
+    const bool zc = s.has_det && d->complex_orbitals;
+    if (zc && precision != QMI_PREC_FP64)
+      throw QmiError(QMI_ERR_UNSUPPORTED, "complex orbitals need the FP64 precision mode");
+    if (zc && !d->kpoints)
+      throw QmiError(QMI_ERR_ARG, "complex orbitals need k-points");
+    s.cplx = zc ? 1 : 0;
+    // spline columns: norb, or 2 norb (real and imaginary parts adjacent) for complex orbitals
+    const int ncol = zc ? 2 * d->norb : d->norb;
     if (s.has_det)
     {
       if (d->norb < nmax)
@@ -301,7 +317,7 @@ struct qmi_ctx
         const double delta = (1.0 - 0.0) / (double)(s.M[i]);
         s.dinv[i]          = 1.0 / delta;
       }
-      s.npad = (d->norb + 7) / 8 * 8;
+      s.npad = (ncol + 7) / 8 * 8;
       s.zs   = s.npad;
       s.ys   = (long long)(s.M[2] + 3) * s.npad;
       s.xs   = (long long)(s.M[1] + 3) * s.ys;
@@ -343,15 +359,26 @@ struct qmi_ctx
     }
     if (s.has_det)
     {
-      const size_t nuniq = (size_t)s.M[0] * s.M[1] * s.M[2] * d->norb;
+      const size_t nuniq = (size_t)s.M[0] * s.M[1] * s.M[2] * ncol;
       const size_t ntab  = (size_t)(s.M[0] + 3) * (s.M[1] + 3) * (s.M[2] + 3) * s.npad;
       DevBuf<double> tmp;
       tmp.alloc(nuniq);
       HIPCHECK(hipMemcpy(tmp.p, d->coefs, nuniq * sizeof(double), hipMemcpyHostToDevice));
       coefs.alloc((ntab * table_esize() + 7) / 8);
-      HIPCHECK(launch_wrap_coefs(s, tmp.p, d->norb, coefs.p, stream));
+      HIPCHECK(launch_wrap_coefs(s, tmp.p, ncol, coefs.p, stream));
       HIPCHECK(hipStreamSynchronize(stream));
     }
+    s.kld = std::max(s.ld, (std::max(d->norb, 1) + 7) / 8 * 8);
+    kpts.alloc((size_t)3 * s.kld);
+    {
+      std::vector<double> t((size_t)3 * s.kld, 0.0);
+      if (zc)
+        for (int j = 0; j < d->norb; j++)
+          for (int q = 0; q < 3; q++)
+            t[(size_t)q * s.kld + j] = d->kpoints[3 * j + q];
+      HIPCHECK(hipMemcpy(kpts.p, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    cplx        = zc;
     sys         = s;
     have_system = true;
     nw          = 0;
@@ -386,7 +413,7 @@ struct qmi_ctx
     HIPCHECK(hipMemset(stats.p, 0, stats.n * sizeof(double)));
     mv.alloc((size_t)nw * MV_SIZE);
     HIPCHECK(hipMemset(mv.p, 0, mv.n * sizeof(double)));
-    out.alloc((size_t)nw * 4 + 16);
+    out.alloc((size_t)nw * 8 + 16);
     gl3.alloc((size_t)nw * 3);
     hostin.alloc((size_t)nw * 3 + 16);
     acc.alloc(nw);
@@ -397,8 +424,8 @@ struct qmi_ctx
     learn_moves = 0;
     if (s.has_det)
     {
-      invRow.alloc((size_t)nw * ld);
-      phi.alloc((size_t)nw * 5 * ld);
+      invRow.alloc((size_t)nw * ld * vw());
+      phi.alloc((size_t)nw * 5 * ld * vw());
       for (int g = 0; g < 2; g++)
       {
         const int n = s.n[g];
@@ -417,7 +444,7 @@ struct qmi_ctx
       ipiv.alloc((size_t)nw * std::max(s.nup, s.ndn));
       info.alloc(nw);
       if (k > 1)
-        flushT.alloc((size_t)nw * ld * k); // T (n x k) or BV_T (ld x k) per walker
+        flushT.alloc((size_t)nw * ld * k * vw()); // T (n x k) or BV_T (ld x k) per walker
     }
     st        = State{};
     st.R      = R.p;
@@ -428,6 +455,7 @@ struct qmi_ctx
     st.phi    = phi.p;
     st.mv     = mv.p;
     st.ions   = ions.p;
+    st.kpts   = kpts.p;
     for (int g = 0; g < 2; g++)
     {
       st.Ainv[g]  = Ainv[g].p;
@@ -459,26 +487,35 @@ struct qmi_ctx
         continue;
       // FP64 LU workspace: Ainv itself in FP64 contexts
       double* L64 = det_fp32() ? lu.p : Ainv[g].p;
+      const rocblas_stride stride = (rocblas_stride)n * s.ld;
+      auto* Lz = reinterpret_cast<rocblas_double_complex*>(L64);
       {
         Scoped t(this, "spo_rows");
-        HIPCHECK(launch_spo_rows(s, st, coefs.p, g, nw, L64, stream));
+        HIPCHECK(cplx ? launch_spo_rows_c(s, st, coefs.p, g, nw, L64, stream)
+                      : launch_spo_rows(s, st, coefs.p, g, nw, L64, stream));
       }
       {
         Scoped t(this, "lu_getrf");
-        if (rocsolver_dgetrf_strided_batched(blas, n, n, L64, s.ld, (rocblas_stride)n * s.ld, ipiv.p, n, info.p, nw) !=
-            rocblas_status_success)
-          throw QmiError(QMI_ERR_DEVICE, "rocsolver_dgetrf_strided_batched failed");
+        const rocblas_status rs = cplx ? rocsolver_zgetrf_strided_batched(blas, n, n, Lz, s.ld, stride, ipiv.p, n,
+                                                                          info.p, nw)
+                                       : rocsolver_dgetrf_strided_batched(blas, n, n, L64, s.ld, stride, ipiv.p, n,
+                                                                          info.p, nw);
+        if (rs != rocblas_status_success)
+          throw QmiError(QMI_ERR_DEVICE, "rocsolver getrf_strided_batched failed");
       }
       check_info("LU factorisation: singular Slater matrix");
       {
         Scoped t(this, "logdet");
-        HIPCHECK(launch_logdet(s, st, g, nw, ipiv.p, L64, stream));
+        HIPCHECK(cplx ? launch_logdet_c(s, st, g, nw, ipiv.p, L64, stream)
+                      : launch_logdet(s, st, g, nw, ipiv.p, L64, stream));
       }
       {
         Scoped t(this, "lu_getri");
-        if (rocsolver_dgetri_strided_batched(blas, n, L64, s.ld, (rocblas_stride)n * s.ld, ipiv.p, n, info.p, nw) !=
-            rocblas_status_success)
-          throw QmiError(QMI_ERR_DEVICE, "rocsolver_dgetri_strided_batched failed");
+        const rocblas_status rs =
+            cplx ? rocsolver_zgetri_strided_batched(blas, n, Lz, s.ld, stride, ipiv.p, n, info.p, nw)
+                 : rocsolver_dgetri_strided_batched(blas, n, L64, s.ld, stride, ipiv.p, n, info.p, nw);
+        if (rs != rocblas_status_success)
+          throw QmiError(QMI_ERR_DEVICE, "rocsolver getri_strided_batched failed");
       }
       check_info("matrix inverse: singular Slater matrix");
       HIPCHECK(launch_store_inverse(s, st, g, nw, L64, stream));
@@ -501,32 +538,38 @@ struct qmi_ctx
     const SysC& s = sys;
     double* dG    = nullptr;
     double* dL    = nullptr;
+    const size_t nG = (size_t)nw * s.N * 3 * vw(), nL = (size_t)nw * s.N * vw();
     if (G)
     {
-      Gbuf.alloc((size_t)nw * s.N * 3);
+      Gbuf.alloc(nG);
       dG = Gbuf.p;
     }
     if (L)
     {
-      Lbuf.alloc((size_t)nw * s.N);
+      Lbuf.alloc(nL);
       dL = Lbuf.p;
     }
     {
       Scoped t(this, "eval_gl");
-      HIPCHECK(launch_eval_gl(s, st, nw, gl3.p, dG, dL, stream));
+      HIPCHECK(cplx ? launch_eval_gl_c(s, st, nw, gl3.p, dG, dL, stream) : launch_eval_gl(s, st, nw, gl3.p, dG, dL, stream));
     }
     std::vector<double> h((size_t)nw * 3);
     HIPCHECK(hipMemcpyAsync(h.data(), gl3.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
     if (G)
-      HIPCHECK(hipMemcpyAsync(G, dG, (size_t)nw * s.N * 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+      HIPCHECK(hipMemcpyAsync(G, dG, nG * sizeof(double), hipMemcpyDeviceToHost, stream));
     if (L)
-      HIPCHECK(hipMemcpyAsync(L, dL, (size_t)nw * s.N * sizeof(double), hipMemcpyDeviceToHost, stream));
+      HIPCHECK(hipMemcpyAsync(L, dL, nL * sizeof(double), hipMemcpyDeviceToHost, stream));
     sync();
     for (int w = 0; w < nw; w++)
     {
       if (ekin)
         ekin[w] = h[w * 3 + 0];
-      if (logpsi)
+      if (logpsi && cplx)
+      {
+        logpsi[2 * w]     = h[w * 3 + 1];
+        logpsi[2 * w + 1] = h[w * 3 + 2];
+      }
+      else if (logpsi)
         logpsi[w] = h[w * 3 + 1];
       if (!std::isfinite(h[w * 3 + 0]) || !std::isfinite(h[w * 3 + 1]))
         throw QmiError(QMI_ERR_NUMERIC, "non-finite log psi or local energy (walker " + std::to_string(w) + ")");
@@ -539,7 +582,8 @@ struct qmi_ctx
       return;
     {
       Scoped t(this, "flush");
-      HIPCHECK(launch_flush(sys, st, g, dcount[g], nw, flushT.p, stream));
+      HIPCHECK(cplx ? launch_flush_c(sys, st, g, dcount[g], nw, flushT.p, stream)
+                    : launch_flush(sys, st, g, dcount[g], nw, flushT.p, stream));
     }
     dcount[g] = 0;
     invrow_id = -1;
@@ -738,6 +782,8 @@ struct qmi_ctx
     unif.alloc((size_t)ncrowd * N * wpc);
     if (htrace)
       trace.alloc((size_t)N * nw * TRACE_FIELDS);
+    if (learned && cplx)
+      throw QmiError(QMI_ERR_UNSUPPORTED, "the learned proposal needs real orbitals");
     if (learned)
     {
       if (wpc != 1)
@@ -788,13 +834,13 @@ struct qmi_ctx
               if (s.k > 1)
                 dcount[g] += 1;
             }
-            else if (wpc == 1 && move_one)
+            else if (wpc == 1 && (move_one || cplx))
             {
               a.iat0 = iat;
               a.iat1 = iat + 1;
               {
                 Scoped t(this, "move_one");
-                HIPCHECK(launch_move(MK_MOVE_ONE, s, st, coefs.p, a, stream));
+                HIPCHECK(move_kernel(MK_MOVE_ONE, a));
               }
               iat += 1;
               if (s.k > 1)
@@ -819,11 +865,11 @@ struct qmi_ctx
               a.iat1 = iat + 1;
               {
                 Scoped t(this, "propose");
-                HIPCHECK(launch_move(MK_PROPOSE, s, st, coefs.p, a, stream));
+                HIPCHECK(move_kernel(MK_PROPOSE, a));
               }
               {
                 Scoped t(this, "accept_crowd");
-                HIPCHECK(launch_move(MK_ACCEPT_CROWD, s, st, coefs.p, a, stream));
+                HIPCHECK(move_kernel(MK_ACCEPT_CROWD, a));
               }
               iat += 1;
               if (s.k > 1)
@@ -984,13 +1030,14 @@ extern "C"
       a.out      = c->out.p;
       {
         qmi_ctx::Scoped t(c, "api_eval_grad");
-        HIPCHECK(launch_move(MK_API_GRAD, c->sys, c->st, c->coefs.p, a, c->stream));
+        HIPCHECK(c->move_kernel(MK_API_GRAD, a));
       }
-      HIPCHECK(hipMemcpyAsync(grads, c->out.p, (size_t)c->nw * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIPCHECK(hipMemcpyAsync(grads, c->out.p, (size_t)c->nw * 3 * c->vw() * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
       c->sync();
       if (c->sys.k > 1)
         c->invrow_id = iat;
-      for (int i = 0; i < 3 * c->nw; i++)
+      for (int i = 0; i < 3 * c->nw * c->vw(); i++)
         if (!std::isfinite(grads[i]))
           throw QmiError(QMI_ERR_NUMERIC, "NaN gradient in mw_evalGrad");
     });
@@ -1019,22 +1066,26 @@ extern "C"
       a.prepare  = (c->sys.k == 1 || c->invrow_id != iat) ? 1 : 0;
       {
         qmi_ctx::Scoped t(c, "api_ratio_grad");
-        HIPCHECK(launch_move(MK_API_RATIO, c->sys, c->st, c->coefs.p, a, c->stream));
+        HIPCHECK(c->move_kernel(MK_API_RATIO, a));
       }
-      std::vector<double> h((size_t)c->nw * 4);
+      const int vw = c->vw();
+      std::vector<double> h((size_t)c->nw * 4 * vw);
       HIPCHECK(hipMemcpyAsync(h.data(), c->out.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
       c->sync();
       if (c->sys.k > 1)
         c->invrow_id = iat;
       for (int w = 0; w < c->nw; w++)
       {
-        if (ratios)
-          ratios[w] = h[w];
-        if (grads)
-          for (int q = 0; q < 3; q++)
-            grads[w * 3 + q] = h[c->nw + w * 3 + q];
-        if (std::isnan(h[w]))
-          throw QmiError(QMI_ERR_NUMERIC, "NaN ratio in mw_calcRatioGrad");
+        for (int e = 0; e < vw; e++)
+        {
+          if (ratios)
+            ratios[w * vw + e] = h[w * vw + e];
+          if (grads)
+            for (int q = 0; q < 3; q++)
+              grads[(w * 3 + q) * vw + e] = h[(c->nw + w * 3 + q) * vw + e];
+          if (std::isnan(h[w * vw + e]))
+            throw QmiError(QMI_ERR_NUMERIC, "NaN ratio in mw_calcRatioGrad");
+        }
       }
     });
   }
@@ -1050,7 +1101,7 @@ extern "C"
       a.host_acc = c->acc.p;
       {
         qmi_ctx::Scoped t(c, "api_accept");
-        HIPCHECK(launch_move(MK_API_ACCEPT, c->sys, c->st, c->coefs.p, a, c->stream));
+        HIPCHECK(c->move_kernel(MK_API_ACCEPT, a));
       }
       c->invrow_id = -1;
       const int g  = c->group(iat);
@@ -1086,15 +1137,16 @@ extern "C"
   {
     return guarded(c, [&] {
       c->require_walkers();
-      if (!c->sys.has_det || n <= 0 || n > c->sys.npad)
+      if (!c->sys.has_det || n <= 0 || n > c->sys.npad / c->vw())
         throw QmiError(QMI_ERR_ARG, "no orbital table or bad orbital count");
       DevBuf<double> dpos, dout;
       dpos.alloc((size_t)c->nw * 3);
-      dout.alloc((size_t)c->nw * 5 * n);
+      dout.alloc((size_t)c->nw * 5 * n * c->vw());
       HIPCHECK(hipMemcpyAsync(dpos.p, pos, dpos.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
       {
         qmi_ctx::Scoped t(c, "api_spo_vgl");
-        HIPCHECK(launch_api_spo_vgl(c->sys, c->coefs.p, c->nw, n, dpos.p, dout.p, c->stream));
+        HIPCHECK(c->cplx ? launch_api_spo_vgl_c(c->sys, c->coefs.p, c->kpts.p, c->nw, n, dpos.p, dout.p, c->stream)
+                         : launch_api_spo_vgl(c->sys, c->coefs.p, c->nw, n, dpos.p, dout.p, c->stream));
       }
       HIPCHECK(hipMemcpyAsync(outv, dout.p, dout.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
       c->sync();
@@ -1105,15 +1157,17 @@ extern "C"
   {
     return guarded(c, [&] {
       c->require_walkers();
-      if (!c->sys.has_det || n <= 0 || n > c->sys.npad || nvp <= 0)
+      if (!c->sys.has_det || n <= 0 || n > c->sys.npad / c->vw() || nvp <= 0)
         throw QmiError(QMI_ERR_ARG, "no orbital table or bad sizes");
       DevBuf<double> dpos, dinv, dout;
       dpos.alloc((size_t)c->nw * nvp * 3);
-      dinv.alloc((size_t)c->nw * n);
-      dout.alloc((size_t)c->nw * nvp);
+      dinv.alloc((size_t)c->nw * n * c->vw());
+      dout.alloc((size_t)c->nw * nvp * c->vw());
       HIPCHECK(hipMemcpyAsync(dpos.p, pos, dpos.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
       HIPCHECK(hipMemcpyAsync(dinv.p, inv_row, dinv.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-      HIPCHECK(launch_api_det_ratios(c->sys, c->coefs.p, c->nw, nvp, n, dpos.p, dinv.p, dout.p, c->stream));
+      HIPCHECK(c->cplx ? launch_api_det_ratios_c(c->sys, c->coefs.p, c->kpts.p, c->nw, nvp, n, dpos.p, dinv.p, dout.p,
+                                                 c->stream)
+                       : launch_api_det_ratios(c->sys, c->coefs.p, c->nw, nvp, n, dpos.p, dinv.p, dout.p, c->stream));
       HIPCHECK(hipMemcpyAsync(ratios, dout.p, dout.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
       c->sync();
     });

@@ -76,6 +76,20 @@ hipError_t launch_api_spo_vgl(const SysC& s, const void* coefs, int nw, int n, c
                               hipStream_t stream);
 hipError_t launch_api_det_ratios(const SysC& s, const void* coefs, int nw, int nvp, int n, const double* pos,
                                  const double* inv, double* out, hipStream_t stream);
+// complex (C2C) orbitals: complex FP64 determinant engine; value arrays hold (re, im) pairs
+hipError_t launch_move_c(int which, const SysC& s, const State& st, const void* coefs, const MoveArgs& a,
+                         hipStream_t stream);
+hipError_t launch_spo_rows_c(const SysC& s, const State& st, const void* coefs, int g, int nw, double* lu,
+                             hipStream_t stream);
+hipError_t launch_logdet_c(const SysC& s, const State& st, int g, int nw, const int* ipiv, const double* lu,
+                           hipStream_t stream);
+hipError_t launch_eval_gl_c(const SysC& s, const State& st, int nw, double* out3, double* G, double* L,
+                            hipStream_t stream);
+hipError_t launch_flush_c(const SysC& s, const State& st, int g, int d, int nw, double* scratchT, hipStream_t stream);
+hipError_t launch_api_spo_vgl_c(const SysC& s, const void* coefs, const double* kpts, int nw, int n,
+                                const double* pos, double* out, hipStream_t stream);
+hipError_t launch_api_det_ratios_c(const SysC& s, const void* coefs, const double* kpts, int nw, int nvp, int n,
+                                   const double* pos, const double* inv, double* out, hipStream_t stream);
 hipError_t launch_api_distances(const SysC& s, const State& st, int nw, const double* pos, double* ee_r,
                                 double* ee_dr, double* ei_r, double* ei_dr, hipStream_t stream);
 

@@ -12,51 +12,10 @@
 //   E  accept (rank-1 growth of the delayed-update block, DelayedUpdateBatched.h:542-670, or the
 //      Sherman-Morrison update for delay rank 1, :252-340; Jastrow accumulators,
 //      TwoBodyJastrow.cpp:581-628, J1OrbitalSoA.h:456-468) or pseudo-accept on reject.
-#include "qmi_kernels.hpp"
+#include "qmi_move_common.hpp"
 
 namespace qmi
 {
-
-struct MoveLDS
-{
-  double row[MAX_N]; // Ainv row / SM-1 temp vector
-  double vec[MAX_N]; // proposed orbital values
-  double p[MAX_K];
-  double y[MAX_K]; // row d of Binv (from phase A), zero beyond d
-  double Y[MAX_K]; // new column d of Binv
-  double red[9 * WG]; // block_sum / delayed_gemv scratch (up to 9 values per thread)
-  double bc[48];      // broadcast scalars
-  double jc[3][MAX_JCOEF]; // Jastrow functor tables: J1, J2 like spins, J2 unlike spins
-  SplineLDS spl;      // uniform spline factors of the proposed position
-  int sel[N_EL_NB + N_ION_NB]; // learned proposal: selected neighbours (electrons, then ions)
-};
-
-// Copy the functor tables to LDS (published by the caller's next barrier).
-__device__ inline void stage_jastrow(const SysC& s, MoveLDS& sh)
-{
-  for (int e = threadIdx.x; e < 3 * MAX_JCOEF; e += WG)
-  {
-    const int f = e / MAX_JCOEF, i = e - f * MAX_JCOEF;
-    sh.jc[f][i] = f == 0 ? s.j1.c[i] : s.j2[f - 1].c[i];
-  }
-}
-
-// Electrons per thread when every electron of a walker is visited once (N <= 2 MAX_N).
-constexpr int JPT = (2 * MAX_N + WG - 1) / WG;
-
-// The leading d x d block of Binv is staged in dynamically sized LDS with row stride k + 1
-// (column reads by consecutive threads then fall in different banks).
-__host__ __device__ constexpr size_t binv_lds_bytes(int k, size_t esize) { return (size_t)k * (k + 1) * esize; }
-
-template<typename DT>
-__device__ inline void stage_binv(const DT* __restrict__ Binv, int k, int d, DT* sB)
-{
-  for (int e = threadIdx.x; e < d * d; e += WG)
-  {
-    const int j = e / d, i = e - j * d;
-    sB[j * (k + 1) + i] = Binv[j * k + i];
-  }
-}
 
 // out[c] = scale * sum_o X[o][c] vec[o] for c < d over an orbital-major delayed block X (row
 // stride k).  Thread (c = t % k, q = t / k) sums orbitals o = q, q + G, ... (G = WG / k groups),
@@ -127,20 +86,6 @@ __device__ inline double row_dot(const DT* __restrict__ v, const double* y, int 
   }
   return a0 + a1;
 }
-
-enum BcField
-{
-  BC_GRAD = 0,  // 3 determinant gradient at the current position
-  BC_POS  = 3,  // 3 proposed position
-  BC_MOVE = 6,  // 3 displacement
-  BC_LOGGF = 9,
-  BC_C    = 10, // 13 phase-C sums: J1 (u, g3, lap), J2 (u, g3), det (ratio, g3)
-  BC_ACC  = 23,
-  BC_RATIO = 24,
-  BC_D2U  = 25,
-  BC_OLD  = 26, // 2 crowd prefix counts
-  BC_GNOW = 29  // 3 gradient at the current position
-};
 
 //------------------------------------------------------------------------------------------------
 template<typename DT, int MAXM>
@@ -232,63 +177,7 @@ __device__ inline void phaseC(const SysC& s, int g, int iat, int n, const double
     spline_locate(s, pos, L);
     spline_stage(L, sh.spl);
   }
-  {
-    double acc[9];
-#pragma unroll
-    for (int i = 0; i < 9; i++)
-      acc[i] = 0;
-    if (s.has_j1)
-      for (int j = t; j < s.nion; j += WG)
-      {
-        double dr[3];
-        const double r = min_image(s, ions[j], ions[s.nion + j], ions[2 * s.nion + j], pos, dr);
-        if (r < s.j1.rcut)
-        {
-          double u, du, d2u;
-          functor_vgl_c(sh.jc[0], s.j1.DeltaRInv, s.j1.max_index, r, u, du, d2u);
-          acc[0] += u;
-          acc[1] += du * dr[0];
-          acc[2] += du * dr[1];
-          acc[3] += du * dr[2];
-          acc[4] += d2u + 2.0 * du;
-        }
-      }
-    if (s.has_j2)
-    {
-      // all positions first, so that the loads of every electron are in flight together
-      double px[JPT], py[JPT], pz[JPT];
-#pragma unroll
-      for (int u = 0; u < JPT; u++)
-      {
-        const int j   = t + u * WG;
-        const bool ok = j < N;
-        px[u]         = ok ? R[j] : 0.0;
-        py[u]         = ok ? R[Np + j] : 0.0;
-        pz[u]         = ok ? R[2 * Np + j] : 0.0;
-      }
-#pragma unroll
-      for (int u = 0; u < JPT; u++)
-      {
-        const int j = t + u * WG;
-        if (j >= N || j == iat)
-          continue;
-        double dr[3];
-        const double r    = min_image(s, px[u], py[u], pz[u], pos, dr);
-        const int fi      = ((j < s.nup) ? 0 : 1) == g ? 0 : 1;
-        const FunctorC& f = s.j2[fi];
-        if (r < f.rcut)
-        {
-          double u2, du, d2u;
-          functor_vgl_c(sh.jc[1 + fi], f.DeltaRInv, f.max_index, r, u2, du, d2u);
-          acc[5] += u2;
-          acc[6] += du * dr[0];
-          acc[7] += du * dr[1];
-          acc[8] += du * dr[2];
-        }
-      }
-    }
-    block_sum<9>(acc, sh.red, sh.bc + BC_C);
-  }
+  jastrow_ratio_sums(s, g, iat, R, ions, pos, sh);
   double acc[4] = {0, 0, 0, 0};
   if (s.has_det)
   {
@@ -506,142 +395,11 @@ __device__ inline void phaseE(const SysC& s, int g, int iat, int n, int ld, int 
     }
   }
   if (accepted)
-  {
-    if (s.has_j1 && t == 0)
-    {
-      jas[J_VAT * Np + iat]      = C[0];
-      jas[(J_G1X + 0) * Np + iat] = C[1];
-      jas[(J_G1X + 1) * Np + iat] = C[2];
-      jas[(J_G1X + 2) * Np + iat] = C[3];
-      jas[J_LAP1 * Np + iat]     = C[4];
-    }
-    if (s.has_j2)
-    {
-      const double pos[3]  = {sh.bc[BC_POS], sh.bc[BC_POS + 1], sh.bc[BC_POS + 2]};
-      const double rold[3] = {R[iat], R[Np + iat], R[2 * Np + iat]};
-      double cd2[1]        = {0.0};
-      // positions and the five accumulator rows of every electron of this thread first
-      double px[JPT], py[JPT], pz[JPT], ja[JPT][5];
-#pragma unroll
-      for (int u = 0; u < JPT; u++)
-      {
-        const int j   = t + u * WG;
-        const bool ok = j < N;
-        px[u]         = ok ? R[j] : 0.0;
-        py[u]         = ok ? R[Np + j] : 0.0;
-        pz[u]         = ok ? R[2 * Np + j] : 0.0;
-        ja[u][0]      = ok ? jas[J_UAT * Np + j] : 0.0;
-        ja[u][1]      = ok ? jas[J_D2U * Np + j] : 0.0;
-#pragma unroll
-        for (int q = 0; q < 3; q++)
-          ja[u][2 + q] = ok ? jas[(J_DUX + q) * Np + j] : 0.0;
-      }
-#pragma unroll
-      for (int u = 0; u < JPT; u++)
-      {
-        const int j = t + u * WG;
-        if (j >= N || j == iat)
-          continue;
-        double drn[3], dro[3];
-        const double rn   = min_image(s, px[u], py[u], pz[u], pos, drn);
-        const double ro   = min_image(s, px[u], py[u], pz[u], rold, dro);
-        const int fi      = ((j < s.nup) ? 0 : 1) == g ? 0 : 1;
-        const FunctorC& f = s.j2[fi];
-        const double* fc  = sh.jc[1 + fi];
-        double un = 0, dun = 0, d2un = 0, uo = 0, duo = 0, d2uo = 0;
-        if (rn < f.rcut)
-          functor_vgl_c(fc, f.DeltaRInv, f.max_index, rn, un, dun, d2un);
-        if (ro < f.rcut)
-          functor_vgl_c(fc, f.DeltaRInv, f.max_index, ro, uo, duo, d2uo);
-        const double newl = d2un + 2.0 * dun;
-        jas[J_UAT * Np + j] = ja[u][0] + (un - uo);
-        jas[J_D2U * Np + j] = ja[u][1] + (d2uo + 2.0 * duo - newl);
-        cd2[0] -= newl;
-#pragma unroll
-        for (int q = 0; q < 3; q++)
-        {
-          const double newg          = dun * drn[q];
-          jas[(J_DUX + q) * Np + j] = ja[u][2 + q] - (newg - duo * dro[q]);
-        }
-      }
-      block_sum<1>(cd2, sh.red, sh.bc + BC_D2U);
-      if (t == 0)
-      {
-        jas[J_UAT * Np + iat]       = C[5];
-        jas[(J_DUX + 0) * Np + iat] = C[6];
-        jas[(J_DUX + 1) * Np + iat] = C[7];
-        jas[(J_DUX + 2) * Np + iat] = C[8];
-        jas[J_D2U * Np + iat]       = sh.bc[BC_D2U];
-      }
-    }
-    __syncthreads();
-    if (t < 3)
-      R[t * Np + iat] = sh.bc[BC_POS + t];
-  }
+    jastrow_accept(s, g, iat, R, jas, sh);
   __syncthreads();
 }
 
 //------------------------------------------------------------------------------------------------
-template<typename DT>
-struct WalkerView
-{
-  double* R;
-  DT *Ainv, *dpsi, *U, *V, *Binv;
-  double *jas, *logdet, *stats;
-  int* dl;
-};
-
-template<typename DT>
-__device__ inline WalkerView<DT> walker_view(const SysC& s, const State& st, int g, int w)
-{
-  WalkerView<DT> v;
-  const int n = s.n[g], ld = s.ld, k = s.k;
-  v.R      = st.R + (size_t)w * 3 * s.Npad;
-  v.Ainv   = det_ptr<DT>(st.Ainv[g]) + (size_t)w * n * ld;
-  v.dpsi   = det_ptr<DT>(st.dpsi[g]) + (size_t)w * n * 4 * ld;
-  v.U      = det_ptr<DT>(st.U[g]) + (size_t)w * ld * k;
-  v.V      = det_ptr<DT>(st.V[g]) + (size_t)w * ld * k;
-  v.Binv   = det_ptr<DT>(st.Binv[g]) + (size_t)w * k * k;
-  v.dl     = st.dlist[g] + (size_t)w * k;
-  v.jas    = st.jas + (size_t)w * J_ROWS * s.Npad;
-  v.logdet = st.logdet + (size_t)w * 4;
-  v.stats  = st.stats + (size_t)w * 2;
-  return v;
-}
-
-// Phase B (thread 0): proposal from the gradient at the current position.
-__device__ inline void propose(const SysC& s, int iat, int w, const MoveArgs& a, const double* __restrict__ R,
-                               const double* __restrict__ jas, MoveLDS& sh)
-{
-  const int Np = s.Npad;
-  const double tauovermass = a.tau;
-  const double sqrttau     = sqrt(tauovermass);
-  const double oneover2tau = 0.5 / tauovermass;
-  double gnow[3] = {0, 0, 0};
-  if (s.has_det)
-    for (int q = 0; q < 3; q++)
-      gnow[q] += sh.bc[BC_GRAD + q];
-  if (s.has_j1)
-    for (int q = 0; q < 3; q++)
-      gnow[q] += jas[(J_G1X + q) * Np + iat];
-  if (s.has_j2)
-    for (int q = 0; q < 3; q++)
-      gnow[q] += jas[(J_DUX + q) * Np + iat];
-  double delta[3], drift[3];
-  const double* gz = a.gauss + ((size_t)iat * a.nw + w) * 3;
-  for (int q = 0; q < 3; q++)
-    delta[q] = gz[q] * sqrttau;
-  unr_drift(tauovermass, gnow, drift);
-  for (int q = 0; q < 3; q++)
-  {
-    sh.bc[BC_GNOW + q] = gnow[q];
-    drift[q] += delta[q];
-    sh.bc[BC_MOVE + q] = drift[q];
-    sh.bc[BC_POS + q]  = R[q * Np + iat] + drift[q];
-  }
-  sh.bc[BC_LOGGF] = -oneover2tau * (delta[0] * delta[0] + delta[1] * delta[1] + delta[2] * delta[2]);
-}
-
 // Phase D core (thread 0): returns prob_eff; flag = |ratio|^2 >= eps.
 __device__ inline double mh_prob(const SysC& s, int iat, const MoveArgs& a, const double* __restrict__ jas,
                                  MoveLDS& sh, double& ratio, bool& flag)

@@ -125,6 +125,21 @@ class Context:
             raise QmiError(st, "qmi_ctx_create failed (no HIP device?)")
         self.nw = 0
         self.N = 0
+        self.cplx = False
+
+    def _vals(self, *shape):
+        """Host buffer for value-typed outputs: (re, im) pairs for complex orbitals."""
+        return np.empty(shape + ((2,) if self.cplx else ()))
+
+    def _out(self, a):
+        return a[..., 0] + 1j * a[..., 1] if self.cplx else a
+
+    def _vin(self, a):
+        """Value-typed input: complex arrays as (re, im) pairs."""
+        if self.cplx:
+            a = np.asarray(a, dtype=np.complex128)
+            return np.ascontiguousarray(np.stack([a.real, a.imag], axis=-1))
+        return np.ascontiguousarray(a, dtype=np.float64)
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
@@ -148,6 +163,7 @@ class Context:
         self._check(self.L.qmi_system_create(self.h, C.byref(d)))
         self.N = system.n_up + system.n_dn
         self.n_up = system.n_up
+        self.cplx = getattr(system, "kpoints", None) is not None and system.norb > 0
 
     def walkers_load(self, R, walkers_per_crowd: int = 1):
         R = np.ascontiguousarray(R, dtype=np.float64)
@@ -161,23 +177,24 @@ class Context:
 
     # ---------------------------------------------------------------- multi-walker API
     def mw_evaluateLog(self):
-        lp, G, L = np.empty(self.nw), np.empty((self.nw, self.N, 3)), np.empty((self.nw, self.N))
+        """log psi (log|psi|, or log|psi| + i phase for complex orbitals), G, L (value type)."""
+        lp, G, L = self._vals(self.nw), self._vals(self.nw, self.N, 3), self._vals(self.nw, self.N)
         self._check(self.L.qmi_mw_evaluate_log(self.h, _p(lp), _p(G), _p(L)))
-        return lp, G, L
+        return self._out(lp), self._out(G), self._out(L)
 
     def mw_evalGrad(self, iat):
-        g = np.empty((self.nw, 3))
+        g = self._vals(self.nw, 3)
         self._check(self.L.qmi_mw_eval_grad(self.h, int(iat), _p(g)))
-        return g
+        return self._out(g)
 
     def mw_makeMove(self, iat, displ):
         displ = np.ascontiguousarray(displ, dtype=np.float64).reshape(self.nw, 3)
         self._check(self.L.qmi_mw_make_move(self.h, int(iat), _p(displ)))
 
     def mw_calcRatioGrad(self, iat):
-        r, g = np.empty(self.nw), np.empty((self.nw, 3))
+        r, g = self._vals(self.nw), self._vals(self.nw, 3)
         self._check(self.L.qmi_mw_ratio_grad(self.h, int(iat), _p(r), _p(g)))
-        return r, g
+        return self._out(r), self._out(g)
 
     def mw_accept_rejectMove(self, iat, accepted, safe_to_delay=True):
         acc = np.ascontiguousarray(np.asarray(accepted, dtype=np.int32).reshape(self.nw))
@@ -187,25 +204,24 @@ class Context:
         self._check(self.L.qmi_mw_complete_updates(self.h))
 
     def mw_evaluateGL(self, fromscratch=False):
-        lp, G, L, ek = (np.empty(self.nw), np.empty((self.nw, self.N, 3)), np.empty((self.nw, self.N)),
-                        np.empty(self.nw))
+        lp, G, L, ek = self._vals(self.nw), self._vals(self.nw, self.N, 3), self._vals(self.nw, self.N), np.empty(self.nw)
         self._check(self.L.qmi_mw_evaluate_gl(self.h, int(bool(fromscratch)), _p(lp), _p(G), _p(L), _p(ek)))
-        return lp, G, L, ek
+        return self._out(lp), self._out(G), self._out(L), ek
 
     def mw_evaluateVGL(self, pos, n):
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.nw, 3)
-        out = np.empty((self.nw, 5, n))
+        out = self._vals(self.nw, 5, n)
         self._check(self.L.qmi_mw_spo_vgl(self.h, _p(pos), int(n), _p(out)))
-        return out
+        return self._out(out)
 
     def mw_evaluateDetRatios(self, pos, inv_row):
         pos = np.ascontiguousarray(pos, dtype=np.float64)
         nvp = pos.shape[1]
-        inv_row = np.ascontiguousarray(inv_row, dtype=np.float64)
-        n = inv_row.shape[1]
-        out = np.empty((self.nw, nvp))
+        n = np.asarray(inv_row).shape[1]
+        inv_row = self._vin(inv_row)
+        out = self._vals(self.nw, nvp)
         self._check(self.L.qmi_mw_det_ratios(self.h, nvp, _p(pos), n, _p(inv_row), _p(out)))
-        return out
+        return self._out(out)
 
     def mw_distanceRows(self, pos, n_ion):
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.nw, 3)
