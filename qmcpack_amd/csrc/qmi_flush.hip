@@ -422,7 +422,8 @@ __global__ __launch_bounds__(WG, 2) void k_flush_mfma(SysC s, State st, int g, i
   }
 }
 
-hipError_t launch_flush(const SysC& s, const State& st, int g, int d, int nw, double* scratch, hipStream_t stream)
+hipError_t launch_flush(const SysC& s, const State& st, int g, int d, int nw, double* scratch, hipStream_t stream,
+                        int variant)
 {
   if (d <= 0)
     return hipSuccess;
@@ -432,7 +433,7 @@ hipError_t launch_flush(const SysC& s, const State& st, int g, int d, int nw, do
   dim3 grid((n + FT_ROWS - 1) / FT_ROWS, nw);
   return dispatch_prec(s.prec, [&](auto, auto dtv) {
     using DT = decltype(dtv);
-    if (n <= MF_COLS)
+    if (n <= MF_COLS && variant == 0)
     {
       const int nblk = (n + 63) / 64, ngrp = (nw + 7) / 8;
       const dim3 grid1(nblk * ngrp * 8);

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <map>
 #include <random>
@@ -176,6 +177,13 @@ struct qmi_ctx
   // fused move kernel: 1 = one launch per move (k_move_one), 0 = one launch per delay window
   // (k_sweep_segment).  QMI_MOVE_KERNEL=segment selects the latter.
   int move_one = 1;
+  int flush_variant = 0; // 0: matrix-core flush for n <= 256; 1: three LDS-tiled passes
+  // sub-step replayed from a captured HIP graph (no trace, no learned proposal)
+  bool graph_on              = false;
+  hipGraphExec_t sweep_exec  = nullptr;
+  double graph_tau           = 0;
+  hipEvent_t ev_done         = nullptr;
+  PinnedBuf<int> h_ubase;
   long long step_count = 0;
   // RNG: one stream per crowd
   std::vector<std::mt19937> rng;
@@ -223,6 +231,20 @@ struct qmi_ctx
   {
     HIPCHECK(hipStreamSynchronize(stream));
     timing.resolve();
+  }
+
+  void invalidate_graph()
+  {
+    if (sweep_exec)
+      (void)hipGraphExecDestroy(sweep_exec);
+    sweep_exec = nullptr;
+  }
+
+  ~qmi_ctx()
+  {
+    invalidate_graph();
+    if (ev_done)
+      (void)hipEventDestroy(ev_done);
   }
 
   int group(int iat) const { return iat < sys.nup ? 0 : 1; }
@@ -382,6 +404,7 @@ struct qmi_ctx
     sys         = s;
     have_system = true;
     nw          = 0;
+    invalidate_graph();
   }
 
   // ---------------------------------------------------------------------------------------------
@@ -469,6 +492,44 @@ struct qmi_ctx
     invrow_id             = -1;
     step_count            = 0;
     rng_seeded            = false;
+    invalidate_graph();
+  }
+
+  // delayed-update rank of the engine (between sub-steps: no pending delayed rows); the delayed
+  // block buffers are reallocated and cleared
+  void set_delay_rank(int k)
+  {
+    require_walkers();
+    const int nmax = std::max(sys.nup, sys.ndn);
+    if (k < 1 || k > MAX_K || (k > 1 && k > nmax))
+      throw QmiError(QMI_ERR_ARG, "delay rank must be in [1, 64] and not exceed the determinant size");
+    if (dcount[0] || dcount[1])
+      throw QmiError(QMI_ERR_ARG, "delay rank change with pending delayed updates");
+    if (k == sys.k)
+      return;
+    sys.k = k;
+    if (sys.has_det)
+    {
+      const int ld = sys.ld;
+      for (int g = 0; g < 2; g++)
+      {
+        U[g].alloc(det_words((size_t)nw * ld * k));
+        V[g].alloc(det_words((size_t)nw * ld * k));
+        Binv[g].alloc(det_words((size_t)nw * k * k));
+        dlist[g].alloc((size_t)nw * k);
+        HIPCHECK(hipMemset(U[g].p, 0, U[g].n * sizeof(double)));
+        HIPCHECK(hipMemset(V[g].p, 0, V[g].n * sizeof(double)));
+        HIPCHECK(hipMemset(Binv[g].p, 0, Binv[g].n * sizeof(double)));
+        st.U[g]     = U[g].p;
+        st.V[g]     = V[g].p;
+        st.Binv[g]  = Binv[g].p;
+        st.dlist[g] = dlist[g].p;
+      }
+      if (k > 1)
+        flushT.alloc((size_t)nw * ld * k * vw());
+    }
+    invrow_id = -1;
+    invalidate_graph();
   }
 
   // ---------------------------------------------------------------------------------------------
@@ -583,7 +644,7 @@ struct qmi_ctx
     {
       Scoped t(this, "flush");
       HIPCHECK(cplx ? launch_flush_c(sys, st, g, dcount[g], nw, flushT.p, stream)
-                    : launch_flush(sys, st, g, dcount[g], nw, flushT.p, stream));
+                    : launch_flush(sys, st, g, dcount[g], nw, flushT.p, stream, flush_variant));
     }
     dcount[g] = 0;
     invrow_id = -1;
@@ -762,14 +823,23 @@ struct qmi_ctx
     learn_moves += 1;
   }
 
-  void vmc_run(int steps, int sub_steps, double tau, int recompute_period, uint32_t seed, double* energies,
-               double* htrace, double* hltrace = nullptr)
+  // device buffers and pinned host buffers of one sub-step (allocated before any capture)
+  void substep_buffers(bool with_trace)
   {
-    require_walkers();
-    const SysC& s = sys;
-    const int N   = s.N;
-    if (steps < 0 || sub_steps < 1 || !(tau > 0))
-      throw QmiError(QMI_ERR_ARG, "invalid steps / sub_steps / tau");
+    const int N = sys.N;
+    if (gauss.n != (size_t)N * nw * 3)
+      gauss.alloc((size_t)N * nw * 3);
+    if (unif.n != (size_t)ncrowd * N * wpc)
+      unif.alloc((size_t)ncrowd * N * wpc);
+    if (with_trace && trace.n != (size_t)N * nw * TRACE_FIELDS)
+      trace.alloc((size_t)N * nw * TRACE_FIELDS);
+    h_gauss.alloc((size_t)N * nw * 3);
+    h_unif.alloc((size_t)ncrowd * N * wpc);
+    h_ubase.alloc(ubase.n);
+  }
+
+  void seed_rng(uint32_t seed)
+  {
     if (!rng_seeded || seed != rng_seed)
     {
       rng.clear();
@@ -778,10 +848,153 @@ struct qmi_ctx
       rng_seed   = seed;
       rng_seeded = true;
     }
-    gauss.alloc((size_t)N * nw * 3);
-    unif.alloc((size_t)ncrowd * N * wpc);
-    if (htrace)
-      trace.alloc((size_t)N * nw * TRACE_FIELDS);
+  }
+
+  // One sub-step on the context's stream: RNG upload (from the pinned buffers filled by
+  // rng_generate), every move of both spin groups with their flushes, and the per-crowd uniform
+  // counts back to pinned memory.  Contains no memset and no pageable copy, so it can be captured.
+  void enqueue_substep(double tau, bool with_trace)
+  {
+    const SysC& s = sys;
+    const int N   = s.N;
+    HIPCHECK(hipMemcpyAsync(gauss.p, h_gauss.p, (size_t)N * nw * 3 * sizeof(double), hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(unif.p, h_unif.p, (size_t)ncrowd * N * wpc * sizeof(double), hipMemcpyHostToDevice,
+                            stream));
+    HIPCHECK(launch_fill_i32(ubase.p, ubase.n, 0, stream));
+    MoveArgs a{};
+    a.nw    = nw;
+    a.wpc   = wpc;
+    a.tau   = tau;
+    a.gauss = gauss.p;
+    a.unif  = unif.p;
+    a.ubase = ubase.p;
+    a.trace = with_trace ? trace.p : nullptr;
+    for (int g = 0; g < 2; g++)
+    {
+      int iat = s.first[g];
+      const int last = s.first[g] + s.n[g];
+      while (iat < last)
+      {
+        a.g  = g;
+        a.d0 = s.k > 1 ? dcount[g] : 0;
+        if (learned)
+        {
+          a.iat0 = iat;
+          a.iat1 = iat + 1;
+          learned_move(a);
+          iat += 1;
+          if (s.k > 1)
+            dcount[g] += 1;
+        }
+        else if (wpc == 1 && (move_one || cplx))
+        {
+          a.iat0 = iat;
+          a.iat1 = iat + 1;
+          {
+            Scoped t(this, "move_one");
+            HIPCHECK(move_kernel(MK_MOVE_ONE, a));
+          }
+          iat += 1;
+          if (s.k > 1)
+            dcount[g] += 1;
+        }
+        else if (wpc == 1)
+        {
+          const int seg = s.k > 1 ? std::min(s.k - dcount[g], last - iat) : last - iat;
+          a.iat0        = iat;
+          a.iat1        = iat + seg;
+          {
+            Scoped t(this, "sweep_segment");
+            HIPCHECK(launch_move(MK_SEGMENT, s, st, coefs.p, a, stream));
+          }
+          iat += seg;
+          if (s.k > 1)
+            dcount[g] += seg;
+        }
+        else
+        {
+          a.iat0 = iat;
+          a.iat1 = iat + 1;
+          {
+            Scoped t(this, "propose");
+            HIPCHECK(move_kernel(MK_PROPOSE, a));
+          }
+          {
+            Scoped t(this, "accept_crowd");
+            HIPCHECK(move_kernel(MK_ACCEPT_CROWD, a));
+          }
+          iat += 1;
+          if (s.k > 1)
+            dcount[g] += 1;
+        }
+        if (s.k > 1 && dcount[g] == s.k)
+          flush(g);
+      }
+    }
+    flush(0);
+    flush(1);
+    HIPCHECK(hipMemcpyAsync(h_ubase.p, ubase.p, ubase.n * sizeof(int), hipMemcpyDeviceToHost, stream));
+  }
+
+  // after the sub-step's work has completed: rewind each crowd's generator to the uniforms used
+  void finish_substep()
+  {
+    const int N = sys.N;
+    std::vector<int> consumed(ncrowd);
+    for (int c = 0; c < ncrowd; c++)
+      consumed[c] = h_ubase.p[(size_t)c * (N + 1) + N];
+    rng_rewind(consumed);
+  }
+
+  // capture enqueue_substep(tau) into sweep_exec (kept until the system, walkers, tau or a tuning
+  // knob change)
+  void ensure_graph(double tau)
+  {
+    if (sweep_exec && graph_tau == tau)
+      return;
+    invalidate_graph();
+    hipGraph_t graph = nullptr;
+    capture(stream, [&] { enqueue_substep(tau, false); }, &graph);
+    const hipError_t e = hipGraphInstantiate(&sweep_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHECK(e);
+    graph_tau = tau;
+  }
+
+  // record f()'s work on stream s (and the streams it joins) into *graph
+  template<typename F>
+  void capture(hipStream_t s, F&& f, hipGraph_t* graph)
+  {
+    const bool t_on = timing.on;
+    timing.on       = false;
+    HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try
+    {
+      f();
+    }
+    catch (...)
+    {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(s, &g);
+      if (g)
+        (void)hipGraphDestroy(g);
+      timing.on = t_on;
+      throw;
+    }
+    timing.on = t_on;
+    HIPCHECK(hipStreamEndCapture(s, graph));
+  }
+
+  void vmc_run(int steps, int sub_steps, double tau, int recompute_period, uint32_t seed, double* energies,
+               double* htrace, double* hltrace = nullptr)
+  {
+    require_walkers();
+    const SysC& s = sys;
+    const int N   = s.N;
+    if (steps < 0 || sub_steps < 1 || !(tau > 0))
+      throw QmiError(QMI_ERR_ARG, "invalid steps / sub_steps / tau");
+    seed_rng(seed);
+    substep_buffers(htrace != nullptr);
     if (learned && cplx)
       throw QmiError(QMI_ERR_UNSUPPORTED, "the learned proposal needs real orbitals");
     if (learned)
@@ -797,91 +1010,20 @@ struct qmi_ctx
         ltrace_d.alloc((size_t)N * nw * LTRACE_FIELDS);
       ltrace_on = hltrace != nullptr;
     }
-    std::vector<int> hub((size_t)ncrowd * (N + 1));
     std::vector<double> ek(nw);
+    const bool use_graph = graph_on && !htrace && !learned;
     for (int step = 0; step < steps; step++)
     {
       for (int sub = 0; sub < sub_steps; sub++)
       {
         rng_generate();
-        HIPCHECK(hipMemcpyAsync(gauss.p, h_gauss.p, (size_t)N * nw * 3 * sizeof(double), hipMemcpyHostToDevice,
-                                stream));
-        HIPCHECK(hipMemcpyAsync(unif.p, h_unif.p, (size_t)ncrowd * N * wpc * sizeof(double), hipMemcpyHostToDevice,
-                                stream));
-        HIPCHECK(hipMemsetAsync(ubase.p, 0, ubase.n * sizeof(int), stream));
-        MoveArgs a{};
-        a.nw    = nw;
-        a.wpc   = wpc;
-        a.tau   = tau;
-        a.gauss = gauss.p;
-        a.unif  = unif.p;
-        a.ubase = ubase.p;
-        a.trace = htrace ? trace.p : nullptr;
-        for (int g = 0; g < 2; g++)
+        if (use_graph)
         {
-          int iat = s.first[g];
-          const int last = s.first[g] + s.n[g];
-          while (iat < last)
-          {
-            a.g  = g;
-            a.d0 = s.k > 1 ? dcount[g] : 0;
-            if (learned)
-            {
-              a.iat0 = iat;
-              a.iat1 = iat + 1;
-              learned_move(a);
-              iat += 1;
-              if (s.k > 1)
-                dcount[g] += 1;
-            }
-            else if (wpc == 1 && (move_one || cplx))
-            {
-              a.iat0 = iat;
-              a.iat1 = iat + 1;
-              {
-                Scoped t(this, "move_one");
-                HIPCHECK(move_kernel(MK_MOVE_ONE, a));
-              }
-              iat += 1;
-              if (s.k > 1)
-                dcount[g] += 1;
-            }
-            else if (wpc == 1)
-            {
-              const int seg = s.k > 1 ? std::min(s.k - dcount[g], last - iat) : last - iat;
-              a.iat0        = iat;
-              a.iat1        = iat + seg;
-              {
-                Scoped t(this, "sweep_segment");
-                HIPCHECK(launch_move(MK_SEGMENT, s, st, coefs.p, a, stream));
-              }
-              iat += seg;
-              if (s.k > 1)
-                dcount[g] += seg;
-            }
-            else
-            {
-              a.iat0 = iat;
-              a.iat1 = iat + 1;
-              {
-                Scoped t(this, "propose");
-                HIPCHECK(move_kernel(MK_PROPOSE, a));
-              }
-              {
-                Scoped t(this, "accept_crowd");
-                HIPCHECK(move_kernel(MK_ACCEPT_CROWD, a));
-              }
-              iat += 1;
-              if (s.k > 1)
-                dcount[g] += 1;
-            }
-            if (s.k > 1 && dcount[g] == s.k)
-              flush(g);
-          }
+          ensure_graph(tau);
+          HIPCHECK(hipGraphLaunch(sweep_exec, stream));
         }
-        flush(0);
-        flush(1);
-        HIPCHECK(hipMemcpyAsync(hub.data(), ubase.p, hub.size() * sizeof(int), hipMemcpyDeviceToHost, stream));
+        else
+          enqueue_substep(tau, htrace != nullptr);
         if (htrace)
           HIPCHECK(hipMemcpyAsync(htrace + ((size_t)step * sub_steps + sub) * N * nw * TRACE_FIELDS, trace.p,
                                   (size_t)N * nw * TRACE_FIELDS * sizeof(double), hipMemcpyDeviceToHost, stream));
@@ -889,10 +1031,7 @@ struct qmi_ctx
           HIPCHECK(hipMemcpyAsync(hltrace + ((size_t)step * sub_steps + sub) * N * nw * LTRACE_FIELDS, ltrace_d.p,
                                   (size_t)N * nw * LTRACE_FIELDS * sizeof(double), hipMemcpyDeviceToHost, stream));
         sync();
-        std::vector<int> consumed(ncrowd);
-        for (int c = 0; c < ncrowd; c++)
-          consumed[c] = hub[(size_t)c * (N + 1) + N];
-        rng_rewind(consumed);
+        finish_substep();
       }
       const bool recompute = recompute_period > 0 && ((step_count + 1) % recompute_period == 0);
       if (recompute)
@@ -901,6 +1040,214 @@ struct qmi_ctx
       if (energies)
         std::copy(ek.begin(), ek.end(), energies + (size_t)step * nw);
       step_count++;
+    }
+  }
+};
+
+//--------------------------------------------------------------------------------------------------
+// Several contexts (walker populations of different systems: electron counts, cells, tables, real
+// or complex orbitals) advanced together: one sub-step of every context is captured into one HIP
+// graph, each context on its own stream (concurrent branches) or chained one after the other.
+struct qmi_batch
+{
+  std::vector<qmi_ctx*> ctxs;
+  int device = 0;
+  hipStream_t origin = nullptr;
+  hipEvent_t ev_fork = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double exec_tau     = 0;
+  bool concurrent     = true;
+  std::string err;
+
+  void invalidate()
+  {
+    if (exec)
+      (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+  }
+
+  ~qmi_batch()
+  {
+    invalidate();
+    if (ev_fork)
+      (void)hipEventDestroy(ev_fork);
+    if (origin)
+      (void)hipStreamDestroy(origin);
+  }
+
+  void prepare(double tau)
+  {
+    for (auto* c : ctxs)
+    {
+      c->require_walkers();
+      if (c->learned)
+        throw QmiError(QMI_ERR_UNSUPPORTED, "batched sweeps run the drift-diffusion proposal");
+      c->substep_buffers(false);
+      if (!c->ev_done)
+        HIPCHECK(hipEventCreateWithFlags(&c->ev_done, hipEventDisableTiming));
+    }
+    // a context's own tuning changes drop its graph; the batch graph is rebuilt when any did
+    bool stale = !exec || exec_tau != tau;
+    for (auto* c : ctxs)
+      stale = stale || !c->sweep_exec || c->graph_tau != tau;
+    if (!stale)
+      return;
+    invalidate();
+    for (auto* c : ctxs)
+      c->ensure_graph(tau); // marks the context's current knobs as captured
+    hipGraph_t graph = nullptr;
+    ctxs[0]->capture(origin, [&] {
+      HIPCHECK(hipEventRecord(ev_fork, origin));
+      hipEvent_t prev = ev_fork;
+      for (auto* c : ctxs)
+      {
+        HIPCHECK(hipStreamWaitEvent(c->stream, concurrent ? ev_fork : prev, 0));
+        c->enqueue_substep(tau, false);
+        HIPCHECK(hipEventRecord(c->ev_done, c->stream));
+        prev = c->ev_done;
+      }
+      for (auto* c : ctxs)
+        HIPCHECK(hipStreamWaitEvent(origin, c->ev_done, 0));
+    }, &graph);
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHECK(e);
+    exec_tau = tau;
+  }
+
+  // one sub-step of every context: host RNG, one graph launch, rewind
+  void substep(double tau)
+  {
+    prepare(tau);
+    for (auto* c : ctxs)
+      c->rng_generate();
+    HIPCHECK(hipGraphLaunch(exec, origin));
+    HIPCHECK(hipStreamSynchronize(origin));
+    for (auto* c : ctxs)
+      c->finish_substep();
+  }
+
+  void run(int steps, int sub_steps, double tau, int recompute_period, const uint32_t* seeds,
+           double* const* energies)
+  {
+    if (steps < 0 || sub_steps < 1 || !(tau > 0))
+      throw QmiError(QMI_ERR_ARG, "invalid steps / sub_steps / tau");
+    for (size_t i = 0; i < ctxs.size(); i++)
+    {
+      ctxs[i]->require_walkers();
+      ctxs[i]->seed_rng(seeds ? seeds[i] : ctxs[i]->rng_seed);
+    }
+    std::vector<std::vector<double>> ek(ctxs.size());
+    for (int step = 0; step < steps; step++)
+    {
+      for (int sub = 0; sub < sub_steps; sub++)
+        substep(tau);
+      for (size_t i = 0; i < ctxs.size(); i++)
+      {
+        qmi_ctx* c           = ctxs[i];
+        const bool recompute = recompute_period > 0 && ((c->step_count + 1) % recompute_period == 0);
+        if (recompute)
+          c->recompute_dets();
+        ek[i].resize(c->nw);
+        c->eval_gl(nullptr, nullptr, nullptr, ek[i].data());
+        if (energies && energies[i])
+          std::copy(ek[i].begin(), ek[i].end(), energies[i] + (size_t)step * c->nw);
+        c->step_count++;
+      }
+    }
+  }
+
+  // wall time of one batched sub-step (ms), median of `trials` after one warm-up
+  double time_substep(double tau, int trials)
+  {
+    substep(tau);
+    std::vector<double> t;
+    for (int i = 0; i < trials; i++)
+    {
+      const auto a = std::chrono::steady_clock::now();
+      substep(tau);
+      t.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count());
+    }
+    std::sort(t.begin(), t.end());
+    return t[t.size() / 2];
+  }
+
+  // Coordinate search per context (the others' knobs fixed): delay rank, then the fused-move form
+  // (one launch per move / per delay window; real orbitals, one walker per crowd), then the flush
+  // form (matrix-core pass / three LDS passes; real orbitals, n <= 256).  Each candidate is timed as
+  // the batched sub-step it would run in; finally concurrent vs chained branches.  The chains
+  // advance during tuning (equilibration moves of a valid Markov chain).
+  void autotune(double tau, int trials, int* choice, double* best_ms)
+  {
+    trials = std::max(1, trials);
+    for (size_t i = 0; i < ctxs.size(); i++)
+    {
+      qmi_ctx* c = ctxs[i];
+      c->require_walkers();
+      if (!c->rng_seeded)
+        c->seed_rng(1u + uint32_t(i));
+    }
+    auto apply = [&](qmi_ctx* c, int k, int mv, int fl) {
+      c->set_delay_rank(k);
+      if (c->move_one != mv || c->flush_variant != fl)
+      {
+        c->move_one      = mv;
+        c->flush_variant = fl;
+        c->invalidate_graph();
+      }
+    };
+    for (size_t i = 0; i < ctxs.size(); i++)
+    {
+      qmi_ctx* c     = ctxs[i];
+      const int nmax = std::max(c->sys.nup, c->sys.ndn);
+      int bk = c->sys.k, bm = c->move_one, bf = c->flush_variant;
+      double best = std::numeric_limits<double>::infinity();
+      auto trial  = [&](int k, int mv, int fl) {
+        apply(c, k, mv, fl);
+        const double t = time_substep(tau, trials);
+        if (t < best)
+        {
+          best = t;
+          bk   = k;
+          bm   = mv;
+          bf   = fl;
+        }
+      };
+      if (c->sys.has_det)
+      {
+        for (int k : {1, 8, 16, 32, 64})
+          if (k == 1 || k <= nmax)
+            trial(k, bm, bf);
+        if (!c->cplx && c->wpc == 1 && bk > 1)
+          trial(bk, 1 - bm, bf);
+        if (!c->cplx && bk > 1 && nmax <= 256)
+          trial(bk, bm, 1 - bf);
+      }
+      else
+        trial(bk, bm, bf);
+      apply(c, bk, bm, bf);
+      if (choice)
+      {
+        choice[3 * i + 0] = bk;
+        choice[3 * i + 1] = 1 - bm; // 0: one launch per move, 1: one launch per delay window
+        choice[3 * i + 2] = bf;
+      }
+      if (best_ms)
+        best_ms[i] = best;
+    }
+    double tb[2];
+    for (int cc = 0; cc < 2; cc++)
+    {
+      concurrent = cc == 1;
+      invalidate();
+      tb[cc] = time_substep(tau, trials);
+    }
+    concurrent = tb[1] <= tb[0];
+    invalidate();
+    if (best_ms)
+    {
+      best_ms[ctxs.size()]     = tb[0];
+      best_ms[ctxs.size() + 1] = tb[1];
     }
   }
 };
@@ -1383,6 +1730,136 @@ extern "C"
         buf[buflen - 1] = 0;
       }
       *nk = n;
+    });
+  }
+
+  // ---- tuning knobs, graph replay, batches of contexts
+  int qmi_tune_set(qmi_ctx* c, int delay_rank, int move_kernel, int flush_variant)
+  {
+    return guarded(c, [&] {
+      if (move_kernel < 0 || move_kernel > 1 || flush_variant < 0 || flush_variant > 1)
+        throw QmiError(QMI_ERR_ARG, "move_kernel and flush_variant are 0 or 1");
+      if (delay_rank > 0)
+        c->set_delay_rank(delay_rank);
+      if (c->move_one != 1 - move_kernel || c->flush_variant != flush_variant)
+      {
+        c->move_one      = 1 - move_kernel;
+        c->flush_variant = flush_variant;
+        c->invalidate_graph();
+      }
+    });
+  }
+
+  int qmi_tune_get(qmi_ctx* c, int* out3)
+  {
+    return guarded(c, [&] {
+      if (!out3)
+        throw QmiError(QMI_ERR_ARG, "null output");
+      out3[0] = c->sys.k;
+      out3[1] = 1 - c->move_one;
+      out3[2] = c->flush_variant;
+    });
+  }
+
+  int qmi_graph_enable(qmi_ctx* c, int on)
+  {
+    return guarded(c, [&] {
+      c->graph_on = on != 0;
+      if (!c->graph_on)
+        c->invalidate_graph();
+    });
+  }
+
+  int qmi_batch_create(qmi_batch** out, qmi_ctx* const* ctxs, int nctx)
+  {
+    if (!out || !ctxs || nctx <= 0)
+      return QMI_ERR_ARG;
+    *out = nullptr;
+    for (int i = 0; i < nctx; i++)
+    {
+      if (!ctxs[i] || ctxs[i]->device != ctxs[0]->device)
+        return QMI_ERR_ARG;
+      for (int j = 0; j < i; j++)
+        if (ctxs[j] == ctxs[i])
+          return QMI_ERR_ARG;
+    }
+    auto* b   = new qmi_batch;
+    b->ctxs   = std::vector<qmi_ctx*>(ctxs, ctxs + nctx);
+    b->device = ctxs[0]->device;
+    if (hipSetDevice(b->device) != hipSuccess || hipStreamCreate(&b->origin) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess)
+    {
+      delete b;
+      return QMI_ERR_DEVICE;
+    }
+    *out = b;
+    return QMI_OK;
+  }
+
+  int qmi_batch_destroy(qmi_batch* b)
+  {
+    if (!b)
+      return QMI_ERR_ARG;
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->origin);
+    delete b;
+    return QMI_OK;
+  }
+
+  const char* qmi_batch_last_error(const qmi_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+  static int batch_guarded(qmi_batch* b, const std::function<void()>& f)
+  {
+    if (!b)
+      return QMI_ERR_ARG;
+    try
+    {
+      HIPCHECK(hipSetDevice(b->device));
+      f();
+      b->err.clear();
+      return QMI_OK;
+    }
+    catch (const QmiError& e)
+    {
+      b->err = e.what();
+      return e.code;
+    }
+    catch (const std::exception& e)
+    {
+      b->err = e.what();
+      return QMI_ERR_DEVICE;
+    }
+  }
+
+  int qmi_batch_vmc_run(qmi_batch* b, int steps, int sub_steps, double tau, int recompute_period,
+                        const uint32_t* seeds, double* const* energies)
+  {
+    return batch_guarded(b, [&] { b->run(steps, sub_steps, tau, recompute_period, seeds, energies); });
+  }
+
+  int qmi_batch_autotune(qmi_batch* b, double tau, int trials, int* choice, double* ms)
+  {
+    return batch_guarded(b, [&] {
+      if (!(tau > 0))
+        throw QmiError(QMI_ERR_ARG, "invalid tau");
+      b->autotune(tau, trials, choice, ms);
+    });
+  }
+
+  int qmi_batch_set_concurrent(qmi_batch* b, int concurrent)
+  {
+    return batch_guarded(b, [&] {
+      b->concurrent = concurrent != 0;
+      b->invalidate();
+    });
+  }
+
+  int qmi_batch_get_concurrent(qmi_batch* b, int* concurrent)
+  {
+    return batch_guarded(b, [&] {
+      if (!concurrent)
+        throw QmiError(QMI_ERR_ARG, "null output");
+      *concurrent = b->concurrent ? 1 : 0;
     });
   }
 }

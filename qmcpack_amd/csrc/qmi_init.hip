@@ -429,4 +429,18 @@ hipError_t launch_api_distances(const SysC& s, const State& st, int nw, const do
   return hipGetLastError();
 }
 
+__global__ void k_fill_i32(int* __restrict__ p, size_t n, int v)
+{
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+hipError_t launch_fill_i32(int* p, size_t n, int v, hipStream_t stream)
+{
+  if (n == 0)
+    return hipSuccess;
+  k_fill_i32<<<(unsigned)std::min<size_t>(1024, (n + 255) / 256), 256, 0, stream>>>(p, n, v);
+  return hipGetLastError();
+}
+
 } // namespace qmi

@@ -68,8 +68,12 @@ hipError_t launch_jastrow_recompute(const SysC& s, const State& st, int nw, hipS
 hipError_t launch_eval_gl(const SysC& s, const State& st, int nw, double* out3, double* G, double* L,
                           hipStream_t stream);
 
-// delayed-update flush of determinant g with d delayed rows (Ainv -= (Ainv U^T - W) Binv V)
-hipError_t launch_flush(const SysC& s, const State& st, int g, int d, int nw, double* scratchT, hipStream_t stream);
+// delayed-update flush of determinant g with d delayed rows (Ainv -= (Ainv U^T - W) Binv V).
+// variant 0: one matrix-core pass for n <= 256, three LDS-tiled passes above; 1: always the three passes
+hipError_t launch_flush(const SysC& s, const State& st, int g, int d, int nw, double* scratchT, hipStream_t stream,
+                        int variant = 0);
+// p[0, n) = v (a kernel, so that it can be captured in a graph)
+hipError_t launch_fill_i32(int* p, size_t n, int v, hipStream_t stream);
 
 // API helpers
 hipError_t launch_api_spo_vgl(const SysC& s, const void* coefs, int nw, int n, const double* pos, double* out,

@@ -54,6 +54,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L = C.CDLL(path)
     L.qmi_last_error.restype = C.c_char_p
     L.qmi_last_error.argtypes = [C.c_void_p]
+    L.qmi_batch_last_error.restype = C.c_char_p
+    L.qmi_batch_last_error.argtypes = [C.c_void_p]
     _lib = L
     return L
 
@@ -299,6 +301,20 @@ class Context:
         self._check(self.L.qmi_vmc_stats(self.h, _p(s)))
         return dict(accepted=s[0], rejected=s[1])
 
+    # ---------------------------------------------------------------- tuning and graph replay
+    def tune_set(self, delay_rank=0, move_kernel=0, flush_variant=0):
+        """delay_rank (0: unchanged), move_kernel (0: launch per move, 1: per delay window),
+        flush_variant (0: matrix-core flush for n <= 256, 1: three LDS passes)."""
+        self._check(self.L.qmi_tune_set(self.h, int(delay_rank), int(move_kernel), int(flush_variant)))
+
+    def tune_get(self):
+        out = np.empty(3, dtype=np.int32)
+        self._check(self.L.qmi_tune_get(self.h, out.ctypes.data_as(_ip)))
+        return dict(delay_rank=int(out[0]), move_kernel=int(out[1]), flush_variant=int(out[2]))
+
+    def graph_enable(self, on=True):
+        self._check(self.L.qmi_graph_enable(self.h, int(bool(on))))
+
     # ---------------------------------------------------------------- timing
     def timing_enable(self, on=True):
         self._check(self.L.qmi_timing_enable(self.h, int(bool(on))))
@@ -313,3 +329,59 @@ class Context:
         self._check(self.L.qmi_timing_get(self.h, buf, 4096, _p(ms), _p(calls), 64, C.byref(nk)))
         names = buf.value.decode().split("\n")[: nk.value]
         return {n: dict(ms=float(ms[i]), calls=int(calls[i])) for i, n in enumerate(names)}
+
+
+class Batch:
+    """Several contexts advanced together in one captured sub-step graph (uneven batching)."""
+
+    def __init__(self, ctxs):
+        self.ctxs = list(ctxs)
+        self.L = load_library()
+        self.h = C.c_void_p()
+        arr = (C.c_void_p * len(self.ctxs))(*[c.h.value for c in self.ctxs])
+        st = self.L.qmi_batch_create(C.byref(self.h), arr, len(self.ctxs))
+        if st != 0:
+            raise QmiError(st, "qmi_batch_create failed")
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.L.qmi_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != 0:
+            raise QmiError(st, self.L.qmi_batch_last_error(self.h).decode())
+
+    def vmc_run(self, steps, sub_steps=1, tau=0.3, recompute_period=0, seeds=None):
+        """Returns the kinetic local energies [steps][nw] of each context."""
+        seeds = np.ascontiguousarray(seeds if seeds is not None else np.arange(1, len(self.ctxs) + 1),
+                                     dtype=np.uint32)
+        E = [np.empty((steps, c.nw)) for c in self.ctxs]
+        ptrs = (_dp * len(E))(*[_p(e) for e in E])
+        self._check(self.L.qmi_batch_vmc_run(self.h, int(steps), int(sub_steps), C.c_double(tau),
+                                             int(recompute_period), seeds.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                             ptrs))
+        return E
+
+    def autotune(self, tau=0.3, trials=3):
+        """Returns (choice [nctx][3] = delay rank, move_kernel, flush_variant; ms [nctx + 2])."""
+        n = len(self.ctxs)
+        choice = np.empty((n, 3), dtype=np.int32)
+        ms = np.empty(n + 2)
+        self._check(self.L.qmi_batch_autotune(self.h, C.c_double(tau), int(trials), choice.ctypes.data_as(_ip),
+                                              _p(ms)))
+        return choice, ms
+
+    def set_concurrent(self, on=True):
+        self._check(self.L.qmi_batch_set_concurrent(self.h, int(bool(on))))
+
+    def concurrent(self):
+        v = C.c_int()
+        self._check(self.L.qmi_batch_get_concurrent(self.h, C.byref(v)))
+        return bool(v.value)

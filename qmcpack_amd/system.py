@@ -115,6 +115,30 @@ def make_complex_system(L: float, n_ion: int, n_up: int, n_dn: int, norb: int, M
     return dataclasses.replace(s, coefs=coefs, kpoints=np.ascontiguousarray(kpoints))
 
 
+# Uneven batching: 1024 walkers over four systems of 128 / 256 / 512 / 1024 electrons, each in its
+# own cubic cell at the electron density of the 512-electron system (L = 18 (N / 512)^(1/3)) with one
+# ion per four electrons and its own table (48^3 / 64^3 / 64^3 / 80^3); a quarter of each system's
+# walkers use complex orbitals at a seeded twist (same cell and ions, own complex table).
+UNEVEN = dict(nw=1024, sizes=(128, 256, 512, 1024), grids=(48, 64, 64, 80), complex_fraction=0.25, tau=0.3,
+              sweeps=5, seed=20242)
+
+
+def uneven_populations(nw: int = 1024, sizes=(128, 256, 512, 1024), grids=(48, 64, 64, 80),
+                       complex_fraction: float = 0.25, seed: int = 20242, delay_rank: int = 32, **_ignored):
+    """[(System, walkers, label)]: per size a real population and a complex one."""
+    pops = []
+    per = nw // len(sizes)
+    for i, (N, M) in enumerate(zip(sizes, grids)):
+        L = 18.0 * (N / 512.0) ** (1.0 / 3.0)
+        kw = dict(L=L, n_ion=max(1, N // 4), n_up=N // 2, n_dn=N - N // 2, norb=N - N // 2, M=M,
+                  delay_rank=min(delay_rank, N // 2), seed=seed + 10 * i)
+        nc = int(round(per * complex_fraction))
+        pops.append((make_system(**kw), per - nc, f"{N}e real"))
+        if nc:
+            pops.append((make_complex_system(**kw), nc, f"{N}e complex"))
+    return pops
+
+
 def make_walkers(system: System, nw: int, seed: int) -> np.ndarray:
     """Electron positions [nw][N][3], uniform in the cell, seeded."""
     rng = np.random.default_rng(seed + 7919)
