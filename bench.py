@@ -7,6 +7,11 @@ One step = one sweep (every electron of every walker proposes one move) + evalua
 kinetic local energy.  Inputs are synthetic (seeded) and resident in HBM before timing.
 
 Prints one JSON line with `value` = electron moves per second of the whole job.
+
+`--workload uneven`: 1024 walkers over four systems of 128 / 256 / 512 / 1024 electrons (cells at
+the same electron density, 48^3 / 64^3 / 64^3 / 80^3 tables, a quarter of each system's walkers
+with complex orbitals at a seeded twist), FP64, advanced together in one captured graph per
+sub-step, delay rank and kernel forms chosen by the autotuner.
 """
 from __future__ import annotations
 
@@ -147,6 +152,125 @@ def run_variant(qmi, system, R, cfg, precision, learned, steps, warmup, train_sw
     return ctx, out
 
 
+def bench_uneven(args, log):
+    """Uneven batching: eight populations (four systems, real and complex orbitals) in one batch."""
+    from qmcpack_amd import qmi
+    from qmcpack_amd.system import UNEVEN, make_walkers, uneven_populations
+    t0 = time.perf_counter()
+    pops = uneven_populations(**UNEVEN)
+    log(f"[bench] uneven populations built in {time.perf_counter() - t0:.1f}s: "
+        + ", ".join(f"{lab} x {n}" for _, n, lab in pops))
+    tau = UNEVEN["tau"]
+    ctxs = []
+    for i, (s, n, lab) in enumerate(pops):
+        c = qmi.Context(0)
+        c.system_create(s)
+        c.walkers_load(make_walkers(s, n, UNEVEN["seed"] + i), 1)
+        c.mw_evaluateLog()
+        ctxs.append(c)
+    b = qmi.Batch(ctxs)
+    seeds = [UNEVEN["seed"] + 100 * i for i in range(len(ctxs))]
+    b.vmc_run(1, 1, tau, 0, seeds)
+    t1 = time.perf_counter()
+    choice, tms = b.autotune(tau, trials=2)
+    tune_s = time.perf_counter() - t1
+    tuned = {lab: dict(delay_rank=int(ch[0]), move_kernel=("per delay window" if ch[1] else "per move"),
+                       flush=("three LDS passes" if ch[2] else "auto (matrix cores for n <= 256)"),
+                       substep_ms=float(tms[i]))
+             for i, ((_, _, lab), ch) in enumerate(zip(pops, choice))}
+    log(f"[bench] autotune ({tune_s:.1f}s): " + json.dumps(tuned) +
+        f"; chained {tms[-2]:.2f} ms, concurrent {tms[-1]:.2f} ms -> {'concurrent' if b.concurrent() else 'chained'}")
+    if args.warmup > 0:
+        b.vmc_run(args.warmup, 1, tau, 0, seeds)
+    t1 = time.perf_counter()
+    E = b.vmc_run(args.steps, 1, tau, 0, seeds)
+    dt = time.perf_counter() - t1
+    moves = sum(n * s.N for s, n, _ in pops) * args.steps
+    value = moves / dt
+    per_pop = {lab: dict(walkers=n, electrons=s.N, energy=float(np.mean(e)),
+                         moves_per_s_share=n * s.N * args.steps / dt)
+               for (s, n, lab), e in zip(pops, E)}
+    # per-kernel device time: one eager sweep per context with HIP events
+    kernels, dominant = {}, None
+    for (s, n, lab), c, ch in zip(pops, ctxs, choice):
+        c.timing_enable(True)
+        c.timing_reset()
+        c.graph_enable(False)
+        c.vmc_run(1, 1, tau, 0, seeds[0])
+        tm = c.timing()
+        c.timing_enable(False)
+        for name, v in tm.items():
+            kernels[f"{lab}: {name}"] = dict(ms_per_call=v["ms"] / max(1, v["calls"]), calls_per_step=v["calls"])
+            if name in ("move_one", "sweep_segment") and (dominant is None or v["ms"] > dominant[2]["ms"]):
+                dominant = (lab, name, v, s, n, int(ch[0]), ctxs.index(c))
+    log("[bench] kernels: " + json.dumps(kernels))
+    lab, kname, v, s, n, k, ci = dominant
+    sz = 16 if ctxs[ci].cplx else 8
+    nn = max(s.n_up, s.n_dn)
+    sweep_bytes = n * (s.N // k) * sum(segment_bytes_per_move(nn, s.N, len(s.ions), d, sz, sz) for d in range(k))
+    bytes_launch = sweep_bytes / max(1.0, v["calls"])
+    avg_ms = v["ms"] / max(1, v["calls"])
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "electron moves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "uneven batching VMC sweeps: 1024 walkers over four systems of 128 / 256 / 512 / 1024 "
+                        "electrons at one electron density (L = 11.34 / 14.29 / 18 / 22.68 bohr, N/4 ions, "
+                        "48^3 / 64^3 / 64^3 / 80^3 tables), a quarter of each system's walkers with complex "
+                        "orbitals at a seeded twist, FP64, one captured graph per sub-step, autotuned",
+            "populations": {lab: dict(walkers=n, electrons=s.N, grid=s.M[0], complex=s.kpoints is not None)
+                            for s, n, lab in pops},
+            "autotune": tuned,
+            "branches": "concurrent" if b.concurrent() else "chained",
+            "autotune_s": tune_s,
+        },
+        "populations": per_pop,
+        "kernels": kernels,
+        "roofline": {
+            "kernel": f"{kname} of the {lab} population (largest device time)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_launch": bytes_launch,
+            "avg_launch_ms": avg_ms,
+        },
+    }
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        nthreads = min(16, os.cpu_count() or 1)
+        tot_moves, tot_t, parts = 0, 0.0, []
+        for s, n, lab in pops:
+            if s.kpoints is not None:
+                continue
+            nwalk = 16 if s.N <= 512 else 8
+            o = orc.VMC(s, make_walkers(s, nwalk, 7), 1, 123)
+            o.evaluate_log()
+            t1 = time.perf_counter()
+            o.run(1, 1, tau, 0, nthreads=nthreads)
+            d = time.perf_counter() - t1
+            tot_moves += nwalk * s.N
+            tot_t += d
+            parts.append(f"{nwalk} x {s.N}e ({d:.1f} s)")
+        out["cpu_baseline"] = dict(value=tot_moves / tot_t, unit="electron moves/s", cores=nthreads, kind="port",
+                                   sample="one sweep of " + ", ".join(parts) + " (real-orbital systems, delay rank "
+                                          "of the system, oracle in C++ FP64, one walker per thread)")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +289,9 @@ def main():
     ap.add_argument("--train-sweeps", type=int, default=2,
                     help="training sweeps of the learned proposal (2 x 1024 x 512 = 1.05e6 moves)")
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--workload", choices=["plain", "uneven"], default="plain",
+                    help="plain: the 1024 x 512-electron FP64 workload with variants; uneven: 1024 walkers over "
+                         "four systems of 128-1024 electrons, a quarter complex, in one captured graph")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs (walker sharding, one "
@@ -177,6 +304,14 @@ def main():
     def log(*a):
         if not args.quiet:
             print(*a, file=sys.stderr, flush=True)
+
+    if args.workload == "uneven":
+        if "--steps" not in sys.argv:
+            args.steps = 20
+        if "--warmup" not in sys.argv:
+            args.warmup = 2
+        print(json.dumps(bench_uneven(args, log)))
+        return
 
     t0 = time.perf_counter()
     system, cfg = config_system(1)

@@ -202,7 +202,7 @@ int qmi_vmc_run_ex(qmi_ctx* ctx, int steps, int sub_steps, double tau, int recom
  * sub-step.  Results of each context are those of qmi_vmc_run on it alone. */
 /* delay_rank (<= 0: unchanged; else 1..64, at most the determinant size), move_kernel (0: one
  * launch per move, 1: one launch per delay window, real orbitals with one walker per crowd),
- * flush_variant (0: matrix-core flush for n <= 256, 1: three LDS-tiled passes).  Between sweeps. */
+ * flush_variant (0: matrix-core flush for n <= 512, 1: three LDS-tiled passes).  Between sweeps. */
 int qmi_tune_set(qmi_ctx* ctx, int delay_rank, int move_kernel, int flush_variant);
 int qmi_tune_get(qmi_ctx* ctx, int* out3);
 /* on = 1: qmi_vmc_run replays each sub-step from a captured graph (runs without trace) */
@@ -215,10 +215,11 @@ const char* qmi_batch_last_error(const qmi_batch* batch);
  * energies[nctx] -> [steps][nw of that context] kinetic local energies (entries nullable) */
 int qmi_batch_vmc_run(qmi_batch* batch, int steps, int sub_steps, double tau, int recompute_period,
                       const uint32_t* seeds, double* const* energies);
-/* Autotuner: per context, delay rank, move-kernel form and flush form by timing batched sub-steps
- * (median of `trials`), then concurrent vs chained branches.  choice[nctx][3] = {delay rank,
- * move_kernel, flush_variant}; ms[nctx + 2] = best sub-step time per context's search, then the
- * chained and the concurrent batch (nullable).  The chains advance while tuning. */
+/* Autotuner: per context, delay rank, move-kernel form and flush form by timing the context's
+ * sub-step alone (its own graph, median of `trials`), then concurrent vs chained branches of the
+ * batch.  choice[nctx][3] = {delay rank, move_kernel, flush_variant}; ms[nctx + 2] = best sub-step
+ * time of each context, then the chained and the concurrent batch (nullable).  The chains advance
+ * while tuning. */
 int qmi_batch_autotune(qmi_batch* batch, double tau, int trials, int* choice, double* ms);
 int qmi_batch_set_concurrent(qmi_batch* batch, int concurrent);
 int qmi_batch_get_concurrent(qmi_batch* batch, int* concurrent);

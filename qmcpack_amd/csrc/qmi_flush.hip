@@ -174,14 +174,14 @@ __global__ __launch_bounds__(WG) void k_flush_update(SysC s, State st, int g, in
   }
 }
 
-// Fused flush for n <= 256 on the FP64 matrix cores, using the association
+// Fused flush for n <= 512 on the FP64 / FP32 matrix cores, using the association
 //   Ainv -= ((Ainv U^T - W) Binv) V
 // (the same product as (Ainv U^T - W)(Binv V), without materialising Binv V).
 // Matrix-core operand layout (measured, see Mfma below): lane l holds A[l % 16][l / 16] and
 // B[l / 16][l % 16]; result register v of lane l holds row drow(l / 16, v) of D, column l % 16
 // (for FP64 drow = l / 16 + 4 v, the case written out below).
-// One wave owns 16 rows of Ainv and keeps them in registers for the whole update, so Ainv is read
-// and written once:
+// One wave owns 16 rows of Ainv and keeps a 256-column block of them in registers (for n <= 256 the
+// whole row, so Ainv is read and written once):
 //   lane (li = l % 16, q = l / 16) holds a[4 jt + v] = Ainv[row li][16 jt + 4 q + v], jt < 16.
 //   1. T (16 x d) = Ainv_rows U^T: the K index of MFMA step kk is column 16 (kk / 4) + 4 q + kk % 4,
 //      which is exactly a[kk]; U^T rows are staged in LDS 64 at a time.  T[i][c] -= 1 where
@@ -255,8 +255,10 @@ __device__ inline void stage_rows(const DT* __restrict__ X, int k, int n, int d,
 }
 
 // NCT = ceil(d / 16) column tiles of the delayed block; columns are zero-padded to 16 NCT, so the
-// padded MFMA steps add exact zeros.
-template<typename DT, int NCT>
+// padded MFMA steps add exact zeros.  NCB = ceil(n / 256) column blocks of 256: the register slab
+// holds one block; step 1 accumulates T over the blocks in order, step 3 updates the last block
+// (still in registers) first and reloads the earlier ones (Ainv read 2 - 1/NCB times, written once).
+template<typename DT, int NCT, int NCB>
 __global__ __launch_bounds__(WG, 2) void k_flush_mfma(SysC s, State st, int g, int d, int nw)
 {
   using MF    = Mfma<DT>;
@@ -284,26 +286,50 @@ __global__ __launch_bounds__(WG, 2) void k_flush_mfma(SysC s, State st, int g, i
   const DT* Binv = det_ptr<DT>(st.Binv[g]) + (size_t)w * k * k;
   const int* dl  = st.dlist[g] + (size_t)w * k;
   DT* sTw        = sT + wave * 16 * LT;
+  const int ncb  = min(NCB, (n + MF_COLS - 1) / MF_COLS);
 
   DT a[64];
+  auto load_block = [&](int cb) {
 #pragma unroll
-  for (int jt = 0; jt < 16; jt++)
-  {
-    const int j0     = jt * 16 + 4 * q;
-    const DT* ap = A + (size_t)row * ld + j0;
-    if (rok && j0 + 3 < n)
+    for (int jt = 0; jt < 16; jt++)
     {
+      const int j0 = cb * MF_COLS + jt * 16 + 4 * q;
+      const DT* ap = A + (size_t)row * ld + j0;
+      if (rok && j0 + 3 < n)
+      {
 #pragma unroll
-      for (int v = 0; v < 4; v++)
-        a[4 * jt + v] = ap[v];
+        for (int v = 0; v < 4; v++)
+          a[4 * jt + v] = ap[v];
+      }
+      else
+      {
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          a[4 * jt + v] = (rok && j0 + v < n) ? ap[v] : DT(0);
+      }
     }
-    else
+  };
+  auto store_block = [&](int cb) {
+#pragma unroll
+    for (int jt = 0; jt < 16; jt++)
     {
+      const int j0 = cb * MF_COLS + jt * 16 + 4 * q;
+      DT* ap       = A + (size_t)row * ld + j0;
+      if (rok && j0 + 3 < n)
+      {
 #pragma unroll
-      for (int v = 0; v < 4; v++)
-        a[4 * jt + v] = (rok && j0 + v < n) ? ap[v] : DT(0);
+        for (int v = 0; v < 4; v++)
+          ap[v] = a[4 * jt + v];
+      }
+      else
+      {
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          if (rok && j0 + v < n)
+            ap[v] = a[4 * jt + v];
+      }
     }
-  }
+  };
   {
     constexpr int NB = (DP * DP + WG - 1) / WG;
     DT tb[NB];
@@ -322,26 +348,31 @@ __global__ __launch_bounds__(WG, 2) void k_flush_mfma(SysC s, State st, int g, i
     }
   }
 
-  // 1. T = Ainv_rows U^T
+  // 1. T = Ainv_rows U^T over the column blocks
   acc_t acc[NCT];
 #pragma unroll
   for (int ct = 0; ct < NCT; ct++)
     acc[ct] = acc_t{0, 0, 0, 0};
-#pragma unroll
-  for (int ch = 0; ch < MF_COLS / 64; ch++)
+  for (int cb = 0; cb < ncb; cb++)
   {
-    if (ch * 64 < n)
+    load_block(cb);
+#pragma unroll
+    for (int ch = 0; ch < MF_COLS / 64; ch++)
     {
-      __syncthreads();
-      stage_rows<DT, NCT>(U, k, n, d, ch * 64, sU, LT);
-      __syncthreads();
-#pragma unroll
-      for (int i = 0; i < 16; i++)
+      const int j0 = cb * MF_COLS + ch * 64;
+      if (j0 < n)
       {
-        const int jl = (i >> 2) * 16 + 4 * q + (i & 3);
+        __syncthreads();
+        stage_rows<DT, NCT>(U, k, n, d, j0, sU, LT);
+        __syncthreads();
 #pragma unroll
-        for (int ct = 0; ct < NCT; ct++)
-          acc[ct] = MF::mma(a[ch * 16 + i], sU[jl * LT + ct * 16 + li], acc[ct]);
+        for (int i = 0; i < 16; i++)
+        {
+          const int jl = (i >> 2) * 16 + 4 * q + (i & 3);
+#pragma unroll
+          for (int ct = 0; ct < NCT; ct++)
+            acc[ct] = MF::mma(a[ch * 16 + i], sU[jl * LT + ct * 16 + li], acc[ct]);
+        }
       }
     }
   }
@@ -376,49 +407,36 @@ __global__ __launch_bounds__(WG, 2) void k_flush_mfma(SysC s, State st, int g, i
     for (int v = 0; v < 4; v++)
       sTw[MF::drow(q, v) * LT + ct * 16 + li] = acc[ct][v];
 
-  // 3. Ainv_rows -= T' V
-#pragma unroll
-  for (int ch = 0; ch < MF_COLS / 64; ch++)
+  // 3. Ainv_rows -= T' V, last column block first (its values are still in a[])
+  for (int cb = ncb - 1; cb >= 0; cb--)
   {
-    if (ch * 64 < n)
+    if (cb != ncb - 1)
+      load_block(cb);
+#pragma unroll
+    for (int ch = 0; ch < MF_COLS / 64; ch++)
     {
-      __syncthreads();
-      stage_rows<DT, NCT>(V, k, n, d, ch * 64, sU, LT);
-      __syncthreads();
-#pragma unroll
-      for (int j4 = 0; j4 < 4; j4++)
+      const int j0 = cb * MF_COLS + ch * 64;
+      if (j0 < n)
       {
-        const int jt = ch * 4 + j4;
-        const int jl = j4 * 16 + MF::acol(li);
-        acc_t o      = acc_t{0, 0, 0, 0};
+        __syncthreads();
+        stage_rows<DT, NCT>(V, k, n, d, j0, sU, LT);
+        __syncthreads();
 #pragma unroll
-        for (int cc = 0; cc < DP / 4; cc++)
-          o = MF::mma(sU[jl * LT + 4 * cc + q], sTw[li * LT + 4 * cc + q], o);
+        for (int j4 = 0; j4 < 4; j4++)
+        {
+          const int jt = ch * 4 + j4;
+          const int jl = j4 * 16 + MF::acol(li);
+          acc_t o      = acc_t{0, 0, 0, 0};
 #pragma unroll
-        for (int v = 0; v < 4; v++)
-          a[4 * jt + v] = a[4 * jt + v] - o[v];
+          for (int cc = 0; cc < DP / 4; cc++)
+            o = MF::mma(sU[jl * LT + 4 * cc + q], sTw[li * LT + 4 * cc + q], o);
+#pragma unroll
+          for (int v = 0; v < 4; v++)
+            a[4 * jt + v] = a[4 * jt + v] - o[v];
+        }
       }
     }
-  }
-
-#pragma unroll
-  for (int jt = 0; jt < 16; jt++)
-  {
-    const int j0 = jt * 16 + 4 * q;
-    DT* ap       = A + (size_t)row * ld + j0;
-    if (rok && j0 + 3 < n)
-    {
-#pragma unroll
-      for (int v = 0; v < 4; v++)
-        ap[v] = a[4 * jt + v];
-    }
-    else
-    {
-#pragma unroll
-      for (int v = 0; v < 4; v++)
-        if (rok && j0 + v < n)
-          ap[v] = a[4 * jt + v];
-    }
+    store_block(cb);
   }
 }
 
@@ -433,21 +451,28 @@ hipError_t launch_flush(const SysC& s, const State& st, int g, int d, int nw, do
   dim3 grid((n + FT_ROWS - 1) / FT_ROWS, nw);
   return dispatch_prec(s.prec, [&](auto, auto dtv) {
     using DT = decltype(dtv);
-    if (n <= MF_COLS && variant == 0)
+    if (n <= 2 * MF_COLS && variant == 0)
     {
       const int nblk = (n + 63) / 64, ngrp = (nw + 7) / 8;
       const dim3 grid1(nblk * ngrp * 8);
       const size_t lds = flush_lds_bytes(s.k, sizeof(DT));
-      switch ((d + 15) / 16)
-      {
-      case 1: k_flush_mfma<DT, 1><<<grid1, WG, lds, stream>>>(s, st, g, d, nw); break;
-      case 2: k_flush_mfma<DT, 2><<<grid1, WG, lds, stream>>>(s, st, g, d, nw); break;
-      case 3: k_flush_mfma<DT, 3><<<grid1, WG, lds, stream>>>(s, st, g, d, nw); break;
-      default: k_flush_mfma<DT, 4><<<grid1, WG, lds, stream>>>(s, st, g, d, nw); break;
-      }
+      auto go          = [&](auto cbt) {
+        constexpr int CB = decltype(cbt)::value;
+        switch ((d + 15) / 16)
+        {
+        case 1: k_flush_mfma<DT, 1, CB><<<grid1, WG, lds, stream>>>(s, st, g, d, nw); break;
+        case 2: k_flush_mfma<DT, 2, CB><<<grid1, WG, lds, stream>>>(s, st, g, d, nw); break;
+        case 3: k_flush_mfma<DT, 3, CB><<<grid1, WG, lds, stream>>>(s, st, g, d, nw); break;
+        default: k_flush_mfma<DT, 4, CB><<<grid1, WG, lds, stream>>>(s, st, g, d, nw); break;
+        }
+      };
+      if (n <= MF_COLS)
+        go(std::integral_constant<int, 1>{});
+      else
+        go(std::integral_constant<int, 2>{});
       return hipGetLastError();
     }
-    // n > 256: three passes; scratch holds T (n x k per walker, FP64)
+    // three passes; scratch holds T (n x k per walker)
     k_flush_T<DT><<<grid, WG, 0, stream>>>(s, st, g, d, scratch);
     k_flush_BV<DT><<<nw, WG, 0, stream>>>(s, st, g, d, det_ptr<DT>(st.U[g]));
     k_flush_update<DT><<<grid, WG, 0, stream>>>(s, st, g, d, scratch);

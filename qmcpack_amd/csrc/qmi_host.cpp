@@ -1172,11 +1172,33 @@ struct qmi_batch
     return t[t.size() / 2];
   }
 
-  // Coordinate search per context (the others' knobs fixed): delay rank, then the fused-move form
-  // (one launch per move / per delay window; real orbitals, one walker per crowd), then the flush
-  // form (matrix-core pass / three LDS passes; real orbitals, n <= 256).  Each candidate is timed as
-  // the batched sub-step it would run in; finally concurrent vs chained branches.  The chains
-  // advance during tuning (equilibration moves of a valid Markov chain).
+  // wall time of one sub-step of context c alone (its own captured graph), median of `trials`
+  double time_ctx(qmi_ctx* c, double tau, int trials)
+  {
+    auto one = [&] {
+      c->ensure_graph(tau);
+      c->rng_generate();
+      HIPCHECK(hipGraphLaunch(c->sweep_exec, c->stream));
+      c->sync();
+      c->finish_substep();
+    };
+    one();
+    std::vector<double> t;
+    for (int i = 0; i < trials; i++)
+    {
+      const auto a = std::chrono::steady_clock::now();
+      one();
+      t.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count());
+    }
+    std::sort(t.begin(), t.end());
+    return t[t.size() / 2];
+  }
+
+  // Coordinate search per context, each timed alone on its own graph (the time it holds the GPU's
+  // resources): delay rank, then the fused-move form (one launch per move / per delay window; real
+  // orbitals, one walker per crowd), then the flush form (matrix-core pass / three LDS passes;
+  // real orbitals, n <= 512).  Finally concurrent vs chained branches for the whole batch.  The
+  // chains advance during tuning (equilibration moves of a valid Markov chain).
   void autotune(double tau, int trials, int* choice, double* best_ms)
   {
     trials = std::max(1, trials);
@@ -1184,8 +1206,11 @@ struct qmi_batch
     {
       qmi_ctx* c = ctxs[i];
       c->require_walkers();
+      if (c->learned)
+        throw QmiError(QMI_ERR_UNSUPPORTED, "batched sweeps run the drift-diffusion proposal");
       if (!c->rng_seeded)
         c->seed_rng(1u + uint32_t(i));
+      c->substep_buffers(false);
     }
     auto apply = [&](qmi_ctx* c, int k, int mv, int fl) {
       c->set_delay_rank(k);
@@ -1204,7 +1229,7 @@ struct qmi_batch
       double best = std::numeric_limits<double>::infinity();
       auto trial  = [&](int k, int mv, int fl) {
         apply(c, k, mv, fl);
-        const double t = time_substep(tau, trials);
+        const double t = time_ctx(c, tau, trials);
         if (t < best)
         {
           best = t;
@@ -1220,7 +1245,7 @@ struct qmi_batch
             trial(k, bm, bf);
         if (!c->cplx && c->wpc == 1 && bk > 1)
           trial(bk, 1 - bm, bf);
-        if (!c->cplx && bk > 1 && nmax <= 256)
+        if (!c->cplx && bk > 1 && nmax <= 512)
           trial(bk, bm, 1 - bf);
       }
       else

@@ -137,7 +137,7 @@ def vmc_parity(system, R, wpc, steps, sub_steps, tau, recompute_period, seed):
     return c, o, E, tr, Eo, tro
 
 
-def check_vmc(c, o, E, tr, Eo, tro, ratio_floor=RATIO_FLOOR, ratio_tol=REL):
+def check_vmc(c, o, E, tr, Eo, tro, ratio_floor=RATIO_FLOOR, ratio_tol=REL, grad_tol=REL):
     err = dict(decisions=int(np.sum(tr[..., 1] != tro[..., 1])),
                ratio=relerr(tr[..., 0], tro[..., 0], floor=ratio_floor),
                grad=relerr(tr[..., 5:8], tro[..., 5:8], floor=1.0),
@@ -146,7 +146,7 @@ def check_vmc(c, o, E, tr, Eo, tro, ratio_floor=RATIO_FLOOR, ratio_tol=REL):
     print("errors vs oracle:", err)
     assert err["decisions"] == 0, f"{err['decisions']} accept/reject decisions differ"
     assert err["ratio"] < ratio_tol, "ratio"
-    assert err["grad"] < REL, "gradient at the current position"
+    assert err["grad"] < grad_tol, "gradient at the current position"
     assert err["proposal"] < REL, "proposal"
     assert err["energy"] < REL, "local energy"
     assert np.array_equal(c.walkers_get(), o.get()["R"])
@@ -185,8 +185,8 @@ def test_vmc_segment_kernel(monkeypatch):
     check_vmc(*res)
 
 
-@pytest.mark.parametrize("n_up,n_dn,nion", [(256, 256, 128), (200, 185, 32)])
-def test_vmc_rank32_large_det(n_up, n_dn, nion):
+@pytest.mark.parametrize("n_up,n_dn,nion,k", [(256, 256, 128, 32), (200, 185, 32, 32), (300, 270, 32, 64)])
+def test_vmc_rank32_large_det(n_up, n_dn, nion, k):
     """Determinants of the benchmark's size (256 per spin, delay rank 32) and a ragged size that
     is not a multiple of the 16-row matrix-core tiles, on a small 16^3 table: every full and
     partial delay window and the flush at the end of each spin group.
@@ -199,9 +199,13 @@ def test_vmc_rank32_large_det(n_up, n_dn, nion):
     reassociating the oracle's flush as (T Binv) V moves it by 4.5e-11.  Ratios are therefore
     compared relative to max(|ratio|, 0.1) at 1e-9 here; decisions must agree exactly, gradients,
     proposals and energies stay within 1e-10.  The oracle-variant spread and the worst point are
-    printed."""
+    printed.  The third case has determinants of 257..512 orbitals (the matrix-core flush over two
+    256-column blocks) at delay rank 64; 300 random orbitals on a 16^3 table are so poorly
+    conditioned that the two oracle variants themselves differ by ~1e-9 in ratios and ~5e-10 in
+    gradients, so the bound there is three times that measured spread (or the general bound, if
+    larger)."""
     kw = dict(L=18.0, n_ion=nion, n_up=n_up, n_dn=n_dn, norb=max(n_up, n_dn), M=16, seed=31)
-    s = make_system(delay_rank=32, **kw)
+    s = make_system(delay_rank=k, **kw)
     R = make_walkers(s, 3, 32)
     c, o, E, tr, Eo, tro = vmc_parity(s, R, 1, steps=2, sub_steps=1, tau=0.3, recompute_period=0, seed=79)
     o1 = orc.VMC(make_system(delay_rank=1, **kw), R, 1, 79)
@@ -209,8 +213,10 @@ def test_vmc_rank32_large_det(n_up, n_dn, nion):
     _, tr1 = o1.replay(tr, 1, 0.3, 0)
     floor = 0.1
     spread = relerr(tr1[..., 0], tro[..., 0], floor=floor)
+    gspread = relerr(tr1[..., 5:8], tro[..., 5:8], floor=1.0)
     rel = np.abs(tr[..., 0] - tro[..., 0]) / np.maximum(np.abs(tro[..., 0]), floor)
     i = np.unravel_index(np.argmax(rel), rel.shape)
     print(f"ratio error {rel[i]:.3e} at ratio {tro[i][0]:.4g} (abs {abs(tr[i][0] - tro[i][0]):.2e}); "
-          f"oracle variant spread {spread:.3e}")
-    check_vmc(c, o, E, tr, Eo, tro, ratio_floor=floor, ratio_tol=1e-9)
+          f"oracle variant spread: ratio {spread:.3e}, gradient {gspread:.3e}")
+    check_vmc(c, o, E, tr, Eo, tro, ratio_floor=floor, ratio_tol=max(1e-9, 3 * spread),
+              grad_tol=max(REL, 3 * gspread))
