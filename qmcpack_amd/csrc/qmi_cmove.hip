@@ -1125,55 +1125,75 @@ hipError_t launch_eval_gl_c(const SysC& s, const State& st, int nw, double* out3
 
 //------------------------------------------------------------------------------------------------
 // Complex delayed-update flush (DelayedUpdateBatched::mw_updateInvMat): T = Ainv U^T - W (n x d),
-// BV_T[o][c] = sum_e Binv[c][e] V_T[o][e] (over U), Ainv -= T BV.  Plain LDS-tiled passes.
-constexpr int CF_ROWS = 32;
+// BV_T[o][c] = sum_e Binv[c][e] V_T[o][e] (over U), Ainv -= T BV.  The two n x n x d products are
+// register-blocked: a workgroup owns a 64 x 64 output tile, thread (tr = t / 16, tc = t % 16) the
+// 4 x 4 block of rows 4 tr + a and columns tc + 16 b, with 32-wide K chunks of both operands in LDS.
+constexpr int CF_ROWS = 64;
+constexpr int CF_KC   = 32;
 
+// T = Ainv U^T - W for rows [i0, i0 + 64), columns c < d (<= 64)
 __global__ __launch_bounds__(WG) void k_flush_T_c(SysC s, State st, int g, int d, double2* __restrict__ T)
 {
-  __shared__ double2 Us[32][MAX_K + 1];
+  __shared__ double2 As[CF_ROWS][CF_KC + 1];
+  __shared__ double2 Us[CF_KC][MAX_K + 1];
   const int w = blockIdx.y, i0 = blockIdx.x * CF_ROWS;
   const int n = s.n[g], ld = s.ld, k = s.k;
-  const int t = threadIdx.x, row = t >> 3, cq = t & 7;
+  const int t = threadIdx.x, tr = t >> 4, tc = t & 15;
   const double2* A = det_ptr<double2>(st.Ainv[g]) + (size_t)w * n * ld;
   const double2* U = det_ptr<double2>(st.U[g]) + (size_t)w * ld * k;
-  double2 acc[MAX_K / 8];
+  double2 acc[4][4];
 #pragma unroll
-  for (int q = 0; q < MAX_K / 8; q++)
-    acc[q] = zmk(0, 0);
-  const int i = i0 + row;
-  for (int j0 = 0; j0 < n; j0 += 32)
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+      acc[a][b] = zmk(0, 0);
+  for (int j0 = 0; j0 < n; j0 += CF_KC)
   {
     __syncthreads();
-    for (int e = t; e < 32 * d; e += WG)
+    for (int e = t; e < CF_ROWS * CF_KC; e += WG)
     {
-      const int jj = e / d, c = e % d;
+      const int rr = e / CF_KC, jj = e % CF_KC;
+      const int i = i0 + rr, j = j0 + jj;
+      As[rr][jj]  = (i < n && j < n) ? A[(size_t)i * ld + j] : zmk(0, 0);
+    }
+    for (int e = t; e < CF_KC * MAX_K; e += WG)
+    {
+      const int jj = e / MAX_K, c = e % MAX_K;
       const int j  = j0 + jj;
-      Us[jj][c]    = j < n ? U[(size_t)j * k + c] : zmk(0, 0);
+      Us[jj][c]    = (j < n && c < d) ? U[(size_t)j * k + c] : zmk(0, 0);
     }
     __syncthreads();
-    if (i < n)
-      for (int jj = 0; jj < 32 && j0 + jj < n; jj++)
-      {
-        const double2 a = A[(size_t)i * ld + j0 + jj];
-#pragma unroll
-        for (int q = 0; q < MAX_K / 8; q++)
-        {
-          const int c = cq + 8 * q;
-          if (c < d)
-            acc[q] = zfma(a, Us[jj][c], acc[q]);
-        }
-      }
-  }
-  if (i < n)
-  {
-    const int* dl = st.dlist[g] + (size_t)w * k;
-    double2* Tw   = T + (size_t)w * n * k;
-#pragma unroll
-    for (int q = 0; q < MAX_K / 8; q++)
+#pragma unroll 4
+    for (int jj = 0; jj < CF_KC; jj++)
     {
-      const int c = cq + 8 * q;
+      double2 av[4], uv[4];
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+        av[a] = As[4 * tr + a][jj];
+#pragma unroll
+      for (int b = 0; b < 4; b++)
+        uv[b] = Us[jj][tc + 16 * b];
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+          acc[a][b] = zfma(av[a], uv[b], acc[a][b]);
+    }
+  }
+  const int* dl = st.dlist[g] + (size_t)w * k;
+  double2* Tw   = T + (size_t)w * n * k;
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+  {
+    const int i = i0 + 4 * tr + a;
+    if (i >= n)
+      continue;
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+    {
+      const int c = tc + 16 * b;
       if (c < d)
-        Tw[(size_t)i * k + c] = zmk(acc[q].x - ((dl[c] == i) ? 1.0 : 0.0), acc[q].y);
+        Tw[(size_t)i * k + c] = zmk(acc[a][b].x - ((dl[c] == i) ? 1.0 : 0.0), acc[a][b].y);
     }
   }
 }
@@ -1214,43 +1234,74 @@ __global__ __launch_bounds__(WG) void k_flush_BV_c(SysC s, State st, int g, int 
   }
 }
 
-// Ainv[i][o] -= sum_c T[i][c] BV_T[o][c]; thread per (i, o) over a 32-row band, BV staged per
-// 64-orbital tile.
+// Ainv[i][o] -= sum_c T[i][c] BV_T[o][c] for rows [i0, i0 + 64), every 64-column tile of the row band
 __global__ __launch_bounds__(WG) void k_flush_update_c(SysC s, State st, int g, int d, const double2* __restrict__ T)
 {
-  __shared__ double2 Ts[CF_ROWS][MAX_K + 1];
-  __shared__ double2 Bv[MAX_K][64 + 1];
+  __shared__ double2 Ts[CF_ROWS][CF_KC + 1];
+  __shared__ double2 Bv[CF_KC][64 + 1];
   const int w = blockIdx.y, i0 = blockIdx.x * CF_ROWS;
   const int n = s.n[g], ld = s.ld, k = s.k;
-  const int t = threadIdx.x, tc = t & 63, tr = t >> 6;
+  const int t = threadIdx.x, tr = t >> 4, tc = t & 15;
   double2* A        = det_ptr<double2>(st.Ainv[g]) + (size_t)w * n * ld;
   const double2* BV = det_ptr<double2>(st.U[g]) + (size_t)w * ld * k;
   const double2* Tw = T + (size_t)w * n * k;
-  for (int e = t; e < CF_ROWS * d; e += WG)
-  {
-    const int rr = e / d, c = e % d;
-    Ts[rr][c]    = (i0 + rr < n) ? Tw[(size_t)(i0 + rr) * k + c] : zmk(0, 0);
-  }
   for (int o0 = 0; o0 < n; o0 += 64)
   {
-    __syncthreads();
-    for (int e = t; e < 64 * d; e += WG)
+    double2 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int b = 0; b < 4; b++)
+        acc[a][b] = zmk(0, 0);
+    for (int c0 = 0; c0 < d; c0 += CF_KC)
     {
-      const int oo = e / d, c = e % d;
-      Bv[c][oo]    = (o0 + oo < n) ? BV[(size_t)(o0 + oo) * k + c] : zmk(0, 0);
+      __syncthreads();
+      for (int e = t; e < CF_ROWS * CF_KC; e += WG)
+      {
+        const int rr = e / CF_KC, cc = e % CF_KC;
+        const int i = i0 + rr, c = c0 + cc;
+        Ts[rr][cc]  = (i < n && c < d) ? Tw[(size_t)i * k + c] : zmk(0, 0);
+      }
+      for (int e = t; e < 64 * CF_KC; e += WG)
+      {
+        const int oo = e / CF_KC, cc = e % CF_KC;
+        const int o = o0 + oo, c = c0 + cc;
+        Bv[cc][oo]  = (o < n && c < d) ? BV[(size_t)o * k + c] : zmk(0, 0);
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int cc = 0; cc < CF_KC; cc++)
+      {
+        double2 tv[4], bv[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+          tv[a] = Ts[4 * tr + a][cc];
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+          bv[b] = Bv[cc][tc + 16 * b];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+          for (int b = 0; b < 4; b++)
+            acc[a][b] = zfma(tv[a], bv[b], acc[a][b]);
+      }
     }
-    __syncthreads();
-    const int o = o0 + tc;
-    for (int rr = tr; rr < CF_ROWS; rr += NWAVE)
+#pragma unroll
+    for (int a = 0; a < 4; a++)
     {
-      const int i = i0 + rr;
-      if (i >= n || o >= n)
+      const int i = i0 + 4 * tr + a;
+      if (i >= n)
         continue;
-      double2 acc = zmk(0, 0);
-      for (int c = 0; c < d; c++)
-        acc = zfma(Ts[rr][c], Bv[c][tc], acc);
-      double2* ap = A + (size_t)i * ld + o;
-      *ap         = zsub(*ap, acc);
+#pragma unroll
+      for (int b = 0; b < 4; b++)
+      {
+        const int o = o0 + tc + 16 * b;
+        if (o < n)
+        {
+          double2* ap = A + (size_t)i * ld + o;
+          *ap         = zsub(*ap, acc[a][b]);
+        }
+      }
     }
   }
 }
