@@ -172,14 +172,19 @@ def bench_uneven(args, log):
     seeds = [UNEVEN["seed"] + 100 * i for i in range(len(ctxs))]
     b.vmc_run(1, 1, tau, 0, seeds)
     t1 = time.perf_counter()
-    choice, tms = b.autotune(tau, trials=2)
+    if args.no_autotune:
+        choice = np.array([[c.tune_get()[x] for x in ("delay_rank", "move_kernel", "flush_variant")] for c in ctxs])
+        tms = np.full(len(ctxs) + 2, np.nan)
+    else:
+        choice, tms = b.autotune(tau, trials=2)
     tune_s = time.perf_counter() - t1
     tuned = {lab: dict(delay_rank=int(ch[0]), move_kernel=("per delay window" if ch[1] else "per move"),
-                       flush=("three LDS passes" if ch[2] else "auto (matrix cores for n <= 256)"),
-                       substep_ms=float(tms[i]))
+                       flush=("three LDS passes" if ch[2] else "auto (matrix cores for n <= 512)"),
+                       substep_ms=None if np.isnan(tms[i]) else float(tms[i]))
              for i, ((_, _, lab), ch) in enumerate(zip(pops, choice))}
     log(f"[bench] autotune ({tune_s:.1f}s): " + json.dumps(tuned) +
         f"; chained {tms[-2]:.2f} ms, concurrent {tms[-1]:.2f} ms -> {'concurrent' if b.concurrent() else 'chained'}")
+    tune_times = None if args.no_autotune else dict(chained_ms=float(tms[-2]), concurrent_ms=float(tms[-1]))
     if args.warmup > 0:
         b.vmc_run(args.warmup, 1, tau, 0, seeds)
     t1 = time.perf_counter()
@@ -233,6 +238,7 @@ def bench_uneven(args, log):
                             for s, n, lab in pops},
             "autotune": tuned,
             "branches": "concurrent" if b.concurrent() else "chained",
+            "branch_substep_ms": tune_times,
             "autotune_s": tune_s,
         },
         "populations": per_pop,
@@ -289,6 +295,7 @@ def main():
     ap.add_argument("--train-sweeps", type=int, default=2,
                     help="training sweeps of the learned proposal (2 x 1024 x 512 = 1.05e6 moves)")
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--no-autotune", action="store_true", help="uneven workload: keep the default knobs")
     ap.add_argument("--workload", choices=["plain", "uneven"], default="plain",
                     help="plain: the 1024 x 512-electron FP64 workload with variants; uneven: 1024 walkers over "
                          "four systems of 128-1024 electrons, a quarter complex, in one captured graph")

@@ -184,6 +184,9 @@ struct qmi_ctx
   double graph_tau           = 0;
   hipEvent_t ev_done         = nullptr;
   PinnedBuf<int> h_ubase;
+  // generation of the device buffers and knobs baked into captured graphs: every change that
+  // invalidates a graph (reallocation, tuning, new walkers) increments it
+  unsigned long long graph_gen = 0;
   long long step_count = 0;
   // RNG: one stream per crowd
   std::vector<std::mt19937> rng;
@@ -238,6 +241,7 @@ struct qmi_ctx
     if (sweep_exec)
       (void)hipGraphExecDestroy(sweep_exec);
     sweep_exec = nullptr;
+    graph_gen++;
   }
 
   ~qmi_ctx()
@@ -1057,6 +1061,7 @@ struct qmi_batch
   hipGraphExec_t exec = nullptr;
   double exec_tau     = 0;
   bool concurrent     = true;
+  std::vector<unsigned long long> exec_gen; // graph_gen of every context when exec was captured
   std::string err;
 
   void invalidate()
@@ -1086,15 +1091,13 @@ struct qmi_batch
       if (!c->ev_done)
         HIPCHECK(hipEventCreateWithFlags(&c->ev_done, hipEventDisableTiming));
     }
-    // a context's own tuning changes drop its graph; the batch graph is rebuilt when any did
-    bool stale = !exec || exec_tau != tau;
-    for (auto* c : ctxs)
-      stale = stale || !c->sweep_exec || c->graph_tau != tau;
+    // the batch graph is rebuilt when any context changed buffers or knobs since it was captured
+    bool stale = !exec || exec_tau != tau || exec_gen.size() != ctxs.size();
+    for (size_t i = 0; !stale && i < ctxs.size(); i++)
+      stale = exec_gen[i] != ctxs[i]->graph_gen;
     if (!stale)
       return;
     invalidate();
-    for (auto* c : ctxs)
-      c->ensure_graph(tau); // marks the context's current knobs as captured
     hipGraph_t graph = nullptr;
     ctxs[0]->capture(origin, [&] {
       HIPCHECK(hipEventRecord(ev_fork, origin));
@@ -1113,6 +1116,9 @@ struct qmi_batch
     (void)hipGraphDestroy(graph);
     HIPCHECK(e);
     exec_tau = tau;
+    exec_gen.clear();
+    for (auto* c : ctxs)
+      exec_gen.push_back(c->graph_gen);
   }
 
   // one sub-step of every context: host RNG, one graph launch, rewind

@@ -108,3 +108,35 @@ def test_autotune_small_uneven_batch():
     assert np.all(np.isfinite(ms)) and np.all(ms > 0)
     E = b.vmc_run(2, 1, 0.3, 0, [1 + i for i in range(len(ctxs))])
     assert all(np.all(np.isfinite(e)) for e in E)
+
+
+def test_batch_graph_rebuilt_after_context_changes():
+    """A context retuned (buffers reallocated) and run alone between two batched runs: the batch
+    must capture again rather than replay kernels bound to the freed buffers.  Reference: the same
+    sequence with every sub-step launched eagerly, context by context."""
+    pops = small_pops()[:3]
+    Rs = [make_walkers(s, n, 50 + i) for i, (s, n, _) in enumerate(pops)]
+    seeds = [7, 8, 9]
+
+    def sequence(batched):
+        ctxs = [ctx_for(s, R) for (s, n, _), R in zip(pops, Rs)]
+        out = []
+        if batched:
+            b = qmi.Batch(ctxs)
+            out += b.vmc_run(1, 1, 0.3, 0, seeds)
+        else:
+            out += [c.vmc_run(1, 1, 0.3, 0, sd)[0] for c, sd in zip(ctxs, seeds)]
+        ctxs[0].tune_set(4, 0, 1)
+        ctxs[0].graph_enable(batched)
+        out.append(ctxs[0].vmc_run(1, 1, 0.3, 0, seeds[0])[0])
+        if batched:
+            out += b.vmc_run(1, 1, 0.3, 0, seeds)
+        else:
+            out += [c.vmc_run(1, 1, 0.3, 0, sd)[0] for c, sd in zip(ctxs, seeds)]
+        return out, [c.walkers_get() for c in ctxs]
+
+    (Eb, Rb), (Ee, Re) = sequence(True), sequence(False)
+    for a, b in zip(Eb, Ee):
+        assert np.array_equal(a, b)
+    for a, b in zip(Rb, Re):
+        assert np.array_equal(a, b)
