@@ -28,6 +28,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "electron moves/sec/GPU + effective samples/sec, learned+mixed vs plain FP64"
 HBM_PEAK_GBS = 8000.0
+GRAPH = True  # replay each sweep from a captured HIP graph (--eager: launch kernel by kernel)
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 
@@ -125,6 +126,8 @@ def run_variant(qmi, system, R, cfg, precision, learned, steps, warmup, train_sw
     ctx.system_create(system)
     ctx.walkers_load(R, 1)
     ctx.mw_evaluateLog()
+    if GRAPH and not learned:
+        ctx.graph_enable(True)
     extra = {}
     if learned:
         ctx.proposal_config(learned=True, train=True, lr=lr, init_seed=7)
@@ -296,10 +299,14 @@ def main():
                     help="training sweeps of the learned proposal (2 x 1024 x 512 = 1.05e6 moves)")
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--no-autotune", action="store_true", help="uneven workload: keep the default knobs")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch the sweeps kernel by kernel instead of replaying a captured graph")
     ap.add_argument("--workload", choices=["plain", "uneven"], default="plain",
                     help="plain: the 1024 x 512-electron FP64 workload with variants; uneven: 1024 walkers over "
                          "four systems of 128-1024 electrons, a quarter complex, in one captured graph")
     args = ap.parse_args()
+    global GRAPH
+    GRAPH = not args.eager
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs (walker sharding, one "
                                    "rank per GPU) are not part of it", "n_gpus": args.gpus}))
@@ -332,7 +339,8 @@ def main():
     value = head["moves_per_s"]
     tau, seed = cfg["tau"], cfg["seed"]
 
-    # per-kernel device time (HIP events on the library stream), a separate short pass
+    # per-kernel device time (HIP events on the library stream; eager launches), a separate short pass
+    ctx.graph_enable(False)
     ctx.timing_enable(True)
     ctx.timing_reset()
     nprof = 2

@@ -328,7 +328,7 @@ __device__ inline void phaseC_c(const SysC& s, int g, int iat, int n, const doub
     spline_locate(s, pos, L);
     spline_stage(L, sh.spl);
   }
-  jastrow_ratio_sums(s, g, iat, R, ions, pos, sh);
+  jastrow_ratio_sums<2 * MAXM>(s, g, iat, R, ions, pos, sh);
   double2 acc[4] = {zmk(0, 0), zmk(0, 0), zmk(0, 0), zmk(0, 0)};
   if (s.has_det)
   {
@@ -558,7 +558,7 @@ __device__ inline void phaseE_c(const SysC& s, int g, int iat, int n, int ld, in
     }
   }
   if (accepted)
-    jastrow_accept(s, g, iat, v.R, v.jas, sh);
+    jastrow_accept<2 * MAXM>(s, g, iat, v.R, v.jas, sh);
   __syncthreads();
 }
 
@@ -906,7 +906,9 @@ __global__ __launch_bounds__(WG) void k_api_accept_c(SysC s, State st, MoveArgs 
 hipError_t launch_move_c(int which, const SysC& s, const State& st, const void* coefs, const MoveArgs& a,
                          hipStream_t stream)
 {
-  const int maxn   = s.n[a.g];
+  const int maxn   = move_maxm(s, a.g) * WG;
+  if (maxn == 0)
+    return hipErrorInvalidValue;
   const dim3 grid(a.nw), block(WG);
   const size_t dyn = binv_lds_bytes(s.k, sizeof(double2));
   const double* c  = static_cast<const double*>(coefs);

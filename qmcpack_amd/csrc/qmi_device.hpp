@@ -375,7 +375,7 @@ __device__ inline void spline_vgl_orbital_lds(const SysC& s, const ST* __restric
   }
   const ST* base = coefs + S.off + o;
   const long long zs = s.zs;
-#pragma unroll 1
+#pragma unroll 2
   for (int i = 0; i < 4; i++)
   {
     const ST* pi = base + i * s.xs;

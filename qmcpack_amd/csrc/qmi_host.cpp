@@ -187,6 +187,7 @@ struct qmi_ctx
   // generation of the device buffers and knobs baked into captured graphs: every change that
   // invalidates a graph (reallocation, tuning, new walkers) increments it
   unsigned long long graph_gen = 0;
+  hipEvent_t ev_ubase = nullptr; // recorded when a sub-step's uniform counts have reached the host
   long long step_count = 0;
   // RNG: one stream per crowd
   std::vector<std::mt19937> rng;
@@ -249,6 +250,8 @@ struct qmi_ctx
     invalidate_graph();
     if (ev_done)
       (void)hipEventDestroy(ev_done);
+    if (ev_ubase)
+      (void)hipEventDestroy(ev_ubase);
   }
 
   int group(int iat) const { return iat < sys.nup ? 0 : 1; }
@@ -598,6 +601,32 @@ struct qmi_ctx
         throw QmiError(QMI_ERR_NUMERIC, std::string(what) + " (walker " + std::to_string(w) + ")");
   }
 
+  // mw_evaluateGL in two halves: the kernel and the copies (per-walker {ekin, log psi, phase} to
+  // pinned memory) on the stream, then, after a synchronisation, the outputs
+  PinnedBuf<double> h_gl3;
+  void eval_gl_enqueue()
+  {
+    h_gl3.alloc((size_t)nw * 3);
+    {
+      Scoped t(this, "eval_gl");
+      HIPCHECK(cplx ? launch_eval_gl_c(sys, st, nw, gl3.p, nullptr, nullptr, stream)
+                    : launch_eval_gl(sys, st, nw, gl3.p, nullptr, nullptr, stream));
+    }
+    HIPCHECK(hipMemcpyAsync(h_gl3.p, gl3.p, (size_t)nw * 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+  }
+
+  void eval_gl_finish(double* ekin)
+  {
+    for (int w = 0; w < nw; w++)
+    {
+      const double* h = h_gl3.p + (size_t)w * 3;
+      if (ekin)
+        ekin[w] = h[0];
+      if (!std::isfinite(h[0]) || !std::isfinite(h[1]))
+        throw QmiError(QMI_ERR_NUMERIC, "non-finite log psi or local energy (walker " + std::to_string(w) + ")");
+    }
+  }
+
   void eval_gl(double* logpsi, double* G, double* L, double* ekin)
   {
     const SysC& s = sys;
@@ -840,6 +869,8 @@ struct qmi_ctx
     h_gauss.alloc((size_t)N * nw * 3);
     h_unif.alloc((size_t)ncrowd * N * wpc);
     h_ubase.alloc(ubase.n);
+    if (!ev_ubase)
+      HIPCHECK(hipEventCreateWithFlags(&ev_ubase, hipEventDisableTiming));
   }
 
   void seed_rng(uint32_t seed)
@@ -857,7 +888,7 @@ struct qmi_ctx
   // One sub-step on the context's stream: RNG upload (from the pinned buffers filled by
   // rng_generate), every move of both spin groups with their flushes, and the per-crowd uniform
   // counts back to pinned memory.  Contains no memset and no pageable copy, so it can be captured.
-  void enqueue_substep(double tau, bool with_trace)
+  void enqueue_substep(double tau, bool with_trace, bool mark_counts = false)
   {
     const SysC& s = sys;
     const int N   = s.N;
@@ -935,9 +966,13 @@ struct qmi_ctx
           flush(g);
       }
     }
+    // the uniform counts are final after the last move: copy them before the closing flushes, so
+    // that the host can rewind and draw the next sub-step's numbers while the flushes run
+    HIPCHECK(hipMemcpyAsync(h_ubase.p, ubase.p, ubase.n * sizeof(int), hipMemcpyDeviceToHost, stream));
+    if (mark_counts)
+      HIPCHECK(hipEventRecord(ev_ubase, stream));
     flush(0);
     flush(1);
-    HIPCHECK(hipMemcpyAsync(h_ubase.p, ubase.p, ubase.n * sizeof(int), hipMemcpyDeviceToHost, stream));
   }
 
   // after the sub-step's work has completed: rewind each crowd's generator to the uniforms used
@@ -1016,31 +1051,54 @@ struct qmi_ctx
     }
     std::vector<double> ek(nw);
     const bool use_graph = graph_on && !htrace && !learned;
+    // The host draws a sub-step's random numbers while the device finishes the previous one (its
+    // closing flushes, evaluateGL): the draws depend only on the uniforms already consumed, whose
+    // counts arrive before those kernels run.
+    bool rng_ready = false;
     for (int step = 0; step < steps; step++)
     {
       for (int sub = 0; sub < sub_steps; sub++)
       {
-        rng_generate();
+        if (!rng_ready)
+          rng_generate();
+        rng_ready = false;
         if (use_graph)
         {
           ensure_graph(tau);
           HIPCHECK(hipGraphLaunch(sweep_exec, stream));
         }
         else
-          enqueue_substep(tau, htrace != nullptr);
+          enqueue_substep(tau, htrace != nullptr, true);
         if (htrace)
           HIPCHECK(hipMemcpyAsync(htrace + ((size_t)step * sub_steps + sub) * N * nw * TRACE_FIELDS, trace.p,
                                   (size_t)N * nw * TRACE_FIELDS * sizeof(double), hipMemcpyDeviceToHost, stream));
         if (learned && hltrace)
           HIPCHECK(hipMemcpyAsync(hltrace + ((size_t)step * sub_steps + sub) * N * nw * LTRACE_FIELDS, ltrace_d.p,
                                   (size_t)N * nw * LTRACE_FIELDS * sizeof(double), hipMemcpyDeviceToHost, stream));
-        sync();
+        // a graph replay is waited for as a whole: an event recorded during capture does not
+        // signal the replay here (measured: the host read the counts before they were written)
+        if (use_graph || htrace || hltrace)
+          sync();
+        else
+          HIPCHECK(hipEventSynchronize(ev_ubase));
         finish_substep();
+        if (sub + 1 < sub_steps)
+        {
+          rng_generate();
+          rng_ready = true;
+        }
       }
       const bool recompute = recompute_period > 0 && ((step_count + 1) % recompute_period == 0);
       if (recompute)
         recompute_dets();
-      eval_gl(nullptr, nullptr, nullptr, ek.data());
+      eval_gl_enqueue();
+      if (step + 1 < steps)
+      {
+        rng_generate();
+        rng_ready = true;
+      }
+      sync();
+      eval_gl_finish(ek.data());
       if (energies)
         std::copy(ek.begin(), ek.end(), energies + (size_t)step * nw);
       step_count++;

@@ -18,31 +18,36 @@ namespace qmi
 {
 
 // out[c] = scale * sum_o X[o][c] vec[o] for c < d over an orbital-major delayed block X (row
-// stride k).  Thread (c = t % k, q = t / k) sums orbitals o = q, q + G, ... (G = WG / k groups),
-// so consecutive threads read consecutive doubles of a row and every load of the unrolled loop is
-// independent; the G partials per column are added through LDS.  Ends with a barrier.
+// stride k).  With dd = d rounded up to a power of two (<= 64), thread t works on column
+// c = t % dd for the orbitals o = q, q + G, ... of its group q = t / dd (G = WG / dd groups), so
+// every thread of the workgroup is busy at any d and each runs n dd / WG independent loads.  The
+// group partials of a column are added by xor shuffles inside a wave (lanes of equal c differ by
+// multiples of dd) and across the four waves through LDS.  Ends with a barrier.
 template<typename DT>
 __device__ inline void delayed_gemv(const DT* __restrict__ X, const double* vec, int n, int k, int d,
                                     double* red, double* out, double scale)
 {
-  const int t = threadIdx.x, G = WG / k, c = t % k, q = t / k;
+  const int t = threadIdx.x;
+  int dd      = 1;
+  while (dd < d)
+    dd <<= 1;
+  const int G = WG / dd, c = t & (dd - 1), q = t / dd;
   double a0 = 0;
-  if (q < G && c < d)
+  if (c < d)
   {
     const DT* xp = X + c;
 #pragma unroll 8
     for (int o = q; o < n; o += G)
       a0 += (double)xp[(size_t)o * k] * vec[o];
   }
-  red[t] = a0;
+  for (int off = 32; off >= dd; off >>= 1)
+    a0 += __shfl_xor(a0, off, 64);
+  const int lane = t & 63, wave = t >> 6;
+  if (lane < dd)
+    red[wave * 64 + lane] = a0;
   __syncthreads();
   if (t < d)
-  {
-    double s = 0;
-    for (int j = 0; j < G; j++)
-      s += red[j * k + t];
-    out[t] = scale * s;
-  }
+    out[t] = scale * (red[t] + red[64 + t] + red[128 + t] + red[192 + t]);
   __syncthreads();
 }
 
@@ -177,7 +182,7 @@ __device__ inline void phaseC(const SysC& s, int g, int iat, int n, const double
     spline_locate(s, pos, L);
     spline_stage(L, sh.spl);
   }
-  jastrow_ratio_sums(s, g, iat, R, ions, pos, sh);
+  jastrow_ratio_sums<2 * MAXM>(s, g, iat, R, ions, pos, sh);
   double acc[4] = {0, 0, 0, 0};
   if (s.has_det)
   {
@@ -395,7 +400,7 @@ __device__ inline void phaseE(const SysC& s, int g, int iat, int n, int ld, int 
     }
   }
   if (accepted)
-    jastrow_accept(s, g, iat, R, jas, sh);
+    jastrow_accept<2 * MAXM>(s, g, iat, R, jas, sh);
   __syncthreads();
 }
 
@@ -482,7 +487,7 @@ __device__ inline void fused_move(const SysC& s, const State& st, const ST* __re
 // Fused sweep segment: moves [iat0, iat1) of spin group g for every walker (one walker per
 // workgroup, one RNG stream per walker), delay count d0 at iat0 with d0 + (iat1 - iat0) <= k.
 template<typename ST, typename DT, int MAXM>
-__global__ __launch_bounds__(WG) void k_sweep_segment(SysC s, State st, const ST* __restrict__ coefs, MoveArgs a)
+__global__ __launch_bounds__(WG, 4) void k_sweep_segment(SysC s, State st, const ST* __restrict__ coefs, MoveArgs a)
 {
   __shared__ MoveLDS sh;
   extern __shared__ double sB_raw[];
@@ -494,6 +499,7 @@ __global__ __launch_bounds__(WG) void k_sweep_segment(SysC s, State st, const ST
   const int N = s.N;
   int ucount  = a.ubase[(size_t)w * (N + 1) + a.iat0];
   double nacc = 0, nrej = 0;
+#pragma unroll 1
   for (int iat = a.iat0; iat < a.iat1; ++iat)
   {
     const int d = s.k > 1 ? a.d0 + (iat - a.iat0) : 0;
@@ -1268,7 +1274,9 @@ template<typename ST, typename DT>
 static hipError_t launch_learn_t(int which, const SysC& s, const State& st, const ST* coefs, const MoveArgs& a,
                                  const LearnArgs& la, hipStream_t stream)
 {
-  const int maxn   = s.n[a.g];
+  const int maxn   = move_maxm(s, a.g) * WG;
+  if (maxn == 0)
+    return hipErrorInvalidValue;
   const dim3 grid(a.nw), block(WG);
   const size_t dyn = binv_lds_bytes(s.k, sizeof(DT));
   if (which == LK_FEATURES)
@@ -1317,7 +1325,9 @@ template<typename ST, typename DT>
 static hipError_t launch_move_t(int which, const SysC& s, const State& st, const ST* coefs, const MoveArgs& a,
                                 hipStream_t stream)
 {
-  const int maxn = s.n[a.g];
+  const int maxn = move_maxm(s, a.g) * WG;
+  if (maxn == 0)
+    return hipErrorInvalidValue;
   dim3 grid(a.nw), block(WG);
   const size_t dyn = binv_lds_bytes(s.k, sizeof(DT));
   if (maxn <= WG)

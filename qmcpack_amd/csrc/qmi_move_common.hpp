@@ -31,8 +31,18 @@ __device__ inline void stage_jastrow(const SysC& s, SH& sh)
   }
 }
 
-// Electrons per thread when every electron of a walker is visited once (N <= 2 MAX_N).
+// Electrons per thread when every electron of a walker is visited once: a kernel instantiated for
+// MAXM orbitals per thread runs systems with N <= 2 MAXM WG electrons (move_maxm below), so
+// 2 MAXM electrons per thread cover them all.
 constexpr int JPT = (2 * MAX_N + WG - 1) / WG;
+
+// orbitals per thread (1 or 2; 0 if unsupported) of the move kernels for the moving spin group g:
+// enough for its n orbitals and for 2 MAXM electrons per thread in the Jastrow loops
+__host__ __device__ inline int move_maxm(const SysC& s, int g)
+{
+  const int need = s.n[g] > (s.N + 1) / 2 ? s.n[g] : (s.N + 1) / 2;
+  return need <= WG ? 1 : need <= 2 * WG ? 2 : 0;
+}
 
 // The leading d x d block of Binv is staged in dynamically sized LDS with row stride k + 1
 // (column reads by consecutive threads then fall in different banks).
@@ -65,7 +75,7 @@ enum BcField
 
 // J1/J2 ratio sums at the proposed position pos -> sh.bc[BC_C..BC_C+9): J1 (u, g3, lap), J2 (u, g3)
 // (J1OrbitalSoA.h:136-186 computeU3; TwoBodyJastrow.cpp:420-446).  All threads; ends with a barrier.
-template<typename SH>
+template<int NJ, typename SH>
 __device__ inline void jastrow_ratio_sums(const SysC& s, int g, int iat, const double* __restrict__ R,
                                           const double* __restrict__ ions, const double pos[3], SH& sh)
 {
@@ -94,9 +104,9 @@ __device__ inline void jastrow_ratio_sums(const SysC& s, int g, int iat, const d
   if (s.has_j2)
   {
     // all positions first, so that the loads of every electron are in flight together
-    double px[JPT], py[JPT], pz[JPT];
+    double px[NJ], py[NJ], pz[NJ];
 #pragma unroll
-    for (int u = 0; u < JPT; u++)
+    for (int u = 0; u < NJ; u++)
     {
       const int j   = t + u * WG;
       const bool ok = j < N;
@@ -105,7 +115,7 @@ __device__ inline void jastrow_ratio_sums(const SysC& s, int g, int iat, const d
       pz[u]         = ok ? R[2 * Np + j] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < JPT; u++)
+    for (int u = 0; u < NJ; u++)
     {
       const int j = t + u * WG;
       if (j >= N || j == iat)
@@ -131,7 +141,7 @@ __device__ inline void jastrow_ratio_sums(const SysC& s, int g, int iat, const d
 // Accepted move, Jastrow side and the particle position (J1OrbitalSoA.h:456-468;
 // TwoBodyJastrow::acceptMove, TwoBodyJastrow.cpp:581-628; ParticleSet::acceptMove).  Reads the
 // phase-C sums at sh.bc[BC_C]; all threads.
-template<typename SH>
+template<int NJ, typename SH>
 __device__ inline void jastrow_accept(const SysC& s, int g, int iat, double* __restrict__ R,
                                       double* __restrict__ jas, SH& sh)
 {
@@ -152,9 +162,9 @@ __device__ inline void jastrow_accept(const SysC& s, int g, int iat, double* __r
     const double rold[3] = {R[iat], R[Np + iat], R[2 * Np + iat]};
     double cd2[1]        = {0.0};
     // positions and the five accumulator rows of every electron of this thread first
-    double px[JPT], py[JPT], pz[JPT], ja[JPT][5];
+    double px[NJ], py[NJ], pz[NJ], ja[NJ][5];
 #pragma unroll
-    for (int u = 0; u < JPT; u++)
+    for (int u = 0; u < NJ; u++)
     {
       const int j   = t + u * WG;
       const bool ok = j < N;
@@ -168,7 +178,7 @@ __device__ inline void jastrow_accept(const SysC& s, int g, int iat, double* __r
         ja[u][2 + q] = ok ? jas[(J_DUX + q) * Np + j] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < JPT; u++)
+    for (int u = 0; u < NJ; u++)
     {
       const int j = t + u * WG;
       if (j >= N || j == iat)
