@@ -197,6 +197,9 @@ struct qmi_ctx
   Timing timing;
   // learned proposal (no counterpart in the reference)
   bool learned = false, train = false, mlp_ready = false;
+  // frozen network: the whole learned move in one kernel (QMI_LEARN_FUSED=0 selects the five-launch
+  // form used for training)
+  bool fused_learn = true;
   float lr     = 1e-3f;
   int adam_t   = 0;
   DevBuf<float> mlp_master, mlp_grad, adam_m, adam_v, lX, lout, ldout, lz, ldz_a, ldz_b;
@@ -825,6 +828,15 @@ struct qmi_ctx
     const SysC& s = sys;
     LearnArgs la  = learn_args(train);
     const MlpParams P = mlp();
+    if (!train && fused_learn)
+    {
+      {
+        Scoped t(this, "learn_move");
+        HIPCHECK(launch_learn(LK_FUSED, s, st, coefs.p, a, la, stream, &P));
+      }
+      learn_moves += 1;
+      return;
+    }
     {
       Scoped t(this, "learn_pre");
       HIPCHECK(launch_learn(LK_PRE, s, st, coefs.p, a, la, stream));
@@ -1391,6 +1403,8 @@ extern "C"
     }
     if (const char* mk = std::getenv("QMI_MOVE_KERNEL"))
       c->move_one = std::string(mk) != "segment";
+    if (const char* lf = std::getenv("QMI_LEARN_FUSED"))
+      c->fused_learn = std::string(lf) != "0";
     if (rocblas_create_handle(&c->blas) != rocblas_status_success ||
         rocblas_set_stream(c->blas, c->stream) != rocblas_status_success)
     {

@@ -159,8 +159,9 @@ enum LearnKernel
   LK_PRE = 0,  // inverse row, gradient, drift and features at r
   LK_PROP,     // proposal from the network output, ratio / gradient at r', features at r'
   LK_ACCEPT,   // asymmetric Metropolis-Hastings, objective gradient, accept / reject
-  LK_FEATURES  // features only (API test hook; position in LS_R, drift in LS_V)
+  LK_FEATURES, // features only (API test hook; position in LS_R, drift in LS_V)
+  LK_FUSED     // the whole move with a frozen network in one kernel (network evaluated per walker)
 };
 hipError_t launch_learn(int which, const SysC& s, const State& st, const void* coefs, const MoveArgs& a,
-                        const LearnArgs& la, hipStream_t stream);
+                        const LearnArgs& la, hipStream_t stream, const MlpParams* P = nullptr);
 } // namespace qmi

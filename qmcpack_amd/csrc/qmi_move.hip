@@ -12,6 +12,7 @@
 //   E  accept (rank-1 growth of the delayed-update block, DelayedUpdateBatched.h:542-670, or the
 //      Sherman-Morrison update for delay rank 1, :252-340; Jastrow accumulators,
 //      TwoBodyJastrow.cpp:581-628, J1OrbitalSoA.h:456-468) or pseudo-accept on reject.
+#include "qmi_mlp.hpp"
 #include "qmi_move_common.hpp"
 
 namespace qmi
@@ -1257,6 +1258,188 @@ __global__ __launch_bounds__(WG) void k_learn_accept(SysC s, State st, MoveArgs 
                    sh, sB, true);
 }
 
+// The frozen network on one sample per workgroup: x[68] (FP32, LDS) -> out[4].  The 16-row
+// matrix-core tiles of the batched forward kernel carry the sample in row 0 (rows 1..15 zero), so
+// row 0 of every layer is computed exactly as there.  Activations in sh.red; all threads, ends with
+// a barrier.
+__device__ inline void mlp_single(const MlpParams& P, const float* x, MoveLDS& sh, float* out)
+{
+  uint16_t* sA = reinterpret_cast<uint16_t*>(sh.red);
+  uint16_t* sB = sA + 16 * MLP_LDA;
+  const int t  = threadIdx.x;
+  for (int e = t; e < 16 * MLP_LDA; e += WG)
+    sA[e] = (e < MLP_IN) ? bf16_bits(x[e]) : (uint16_t)0;
+  __syncthreads();
+  const float* bias = P.master;
+  mlp_layer<MLP_K1>(sA, P.w1b, bias + MLP_OFF_B1, sB, nullptr, 1);
+  __syncthreads();
+  mlp_layer<MLP_HID>(sB, P.w2b, bias + MLP_OFF_B2, sA, nullptr, 1);
+  __syncthreads();
+  mlp_layer<MLP_HID>(sA, P.w3b, bias + MLP_OFF_B3, sB, nullptr, 1);
+  __syncthreads();
+  if ((t >> 6) == 0)
+    mlp_out_layer(sB, P, out, MLP_OUT, 1);
+  __syncthreads();
+}
+
+// One learned move with a frozen network, start to end in one workgroup per walker: the same
+// steps and arithmetic as k_learn_pre, k_mlp_forward, k_learn_prop, k_mlp_forward and
+// k_learn_accept (without the objective's gradient), with the inverse row, the proposed orbitals
+// and the move scalars kept in registers and LDS instead of global memory.
+template<typename ST, typename DT, int MAXM>
+__global__ __launch_bounds__(WG, 4) void k_learn_move(SysC s, State st, const ST* __restrict__ coefs, MoveArgs a,
+                                                   LearnArgs la, MlpParams P)
+{
+  __shared__ MoveLDS sh;
+  extern __shared__ double sB_raw[];
+  DT* sB = reinterpret_cast<DT*>(sB_raw);
+  stage_jastrow(s, sh);
+  const int w = blockIdx.x;
+  if (w >= a.nw)
+    return;
+  const int g = a.g, n = s.n[g], ld = s.ld, k = s.k, N = s.N, Np = s.Npad;
+  const int iat = a.iat0, r = iat - s.first[g], d = a.d0;
+  // move scalars in sh.bc beyond the plain move's fields
+  constexpr int L_R = 32, L_V = 35, L_XI = 38, L_LQF = 41, L_GNEW = 42, L_VR = 45;
+  static_assert(L_VR + 3 <= 48, "broadcast slots");
+  WalkerView<DT> v = walker_view<DT>(s, st, g, w);
+  double ainv_r[MAXM], inv[MAXM], phv[MAXM][5];
+#pragma unroll
+  for (int m = 0; m < MAXM; m++)
+    ainv_r[m] = inv[m] = 0.0;
+  if (s.has_det)
+  {
+    phaseA<DT, MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, sB, ainv_r, inv);
+    det_grad<DT, MAXM>(n, ld, r, v.dpsi, inv, sh);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+  {
+    double gn[3], dv[3];
+    grad_now(s, iat, v.jas, sh, gn);
+    unr_drift(a.tau, gn, dv);
+    for (int q = 0; q < 3; q++)
+    {
+      sh.bc[L_R + q]     = v.R[q * Np + iat];
+      sh.bc[L_V + q]     = dv[q];
+      sh.bc[BC_GNOW + q] = gn[q];
+    }
+  }
+  __syncthreads();
+  float* xf = reinterpret_cast<float*>(sh.Y); // features (68 FP32)
+  float* mo = reinterpret_cast<float*>(sh.p); // network outputs at r [0, 4) and at r' [4, 8)
+  {
+    const double pos[3] = {sh.bc[L_R], sh.bc[L_R + 1], sh.bc[L_R + 2]};
+    const double vd[3]  = {sh.bc[L_V], sh.bc[L_V + 1], sh.bc[L_V + 2]};
+    proposal_features(s, v.R, st.ions, iat, pos, vd, a.tau, sh, xf);
+  }
+  mlp_single(P, xf, sh, mo);
+  if (threadIdx.x == 0)
+  {
+    const double* gz = a.gauss + ((size_t)iat * a.nw + w) * 3;
+    const double sig = sqrt(a.tau) * exp((double)mo[3]);
+    double xx        = 0;
+    for (int q = 0; q < 3; q++)
+    {
+      const double xi   = gz[q];
+      const double move = (sh.bc[L_V + q] + (double)mo[q]) + xi * sig;
+      sh.bc[BC_POS + q] = sh.bc[L_R + q] + move;
+      sh.bc[L_XI + q]   = xi;
+      xx += xi * xi;
+    }
+    sh.bc[L_LQF] = -0.5 * xx - 3.0 * (double)mo[3];
+  }
+  __syncthreads();
+  phaseC<ST, MAXM>(s, g, iat, n, v.R, st.ions, coefs, inv, phv, sh);
+  if (threadIdx.x == 0)
+  {
+    double ratio, gnew[3], vr[3];
+    combine_ratio(s, iat, v.jas, sh, ratio, gnew);
+    unr_drift(a.tau, gnew, vr);
+    sh.bc[BC_RATIO] = ratio;
+    for (int q = 0; q < 3; q++)
+    {
+      sh.bc[L_GNEW + q]  = gnew[q];
+      sh.bc[L_VR + q]    = vr[q];
+      sh.bc[BC_MOVE + q] = vr[q];
+    }
+  }
+  __syncthreads();
+  {
+    const double pos[3] = {sh.bc[BC_POS], sh.bc[BC_POS + 1], sh.bc[BC_POS + 2]};
+    const double vd[3]  = {sh.bc[BC_MOVE], sh.bc[BC_MOVE + 1], sh.bc[BC_MOVE + 2]};
+    proposal_features(s, v.R, st.ions, iat, pos, vd, a.tau, sh, xf);
+  }
+  mlp_single(P, xf, sh, mo + MLP_OUT);
+  if (threadIdx.x == 0)
+  {
+    const float* of  = mo;
+    const float* orr = mo + MLP_OUT;
+    const double tau = a.tau, ratio = sh.bc[BC_RATIO];
+    const double s_r = orr[3], sig_r2 = tau * exp(2.0 * s_r);
+    double e[3], D[3], ee = 0, D2 = 0;
+    for (int q = 0; q < 3; q++)
+    {
+      e[q] = sh.bc[L_R + q] - ((sh.bc[BC_POS + q] + sh.bc[L_VR + q]) + (double)orr[q]);
+      D[q] = sh.bc[BC_POS + q] - sh.bc[L_R + q];
+      ee += e[q] * e[q];
+      D2 += D[q] * D[q];
+    }
+    const double lqf  = sh.bc[L_LQF];
+    const double lqr  = -ee / (2.0 * sig_r2) - 3.0 * s_r;
+    const double pr2  = ratio * ratio;
+    const bool flag   = pr2 >= DBL_EPS;
+    const double prob = pr2 * exp(lqr - lqf);
+    int ucount        = a.ubase[(size_t)w * (N + 1) + iat];
+    bool acc          = false;
+    if (flag)
+    {
+      const double u = a.unif[(size_t)w * N + ucount];
+      ucount++;
+      acc = u < prob;
+    }
+    a.ubase[(size_t)w * (N + 1) + iat + 1] = ucount;
+    v.stats[acc ? 0 : 1] += 1;
+    sh.bc[BC_ACC]    = acc ? 1.0 : 0.0;
+    const double A   = prob < 1.0 ? prob : 1.0;
+    const double f   = A * D2;
+    if (la.fsum)
+    {
+      la.fsum[(size_t)w * 2 + 0] += f;
+      la.fsum[(size_t)w * 2 + 1] += A;
+    }
+    if (a.trace)
+    {
+      double* tr = a.trace + ((size_t)iat * a.nw + w) * TRACE_FIELDS;
+      tr[0]      = ratio;
+      tr[1]      = acc ? 1.0 : 0.0;
+      for (int q = 0; q < 3; q++)
+      {
+        tr[2 + q] = sh.bc[BC_POS + q];
+        tr[5 + q] = sh.bc[BC_GNOW + q];
+      }
+    }
+    if (la.ltrace)
+    {
+      double* lt = la.ltrace + ((size_t)iat * a.nw + w) * LTRACE_FIELDS;
+      for (int q = 0; q < 4; q++)
+      {
+        lt[q]     = of[q];
+        lt[4 + q] = orr[q];
+      }
+      lt[8]  = lqf;
+      lt[9]  = lqr;
+      lt[10] = prob;
+      lt[11] = f;
+      for (int q = 0; q < 3; q++)
+        lt[12 + q] = sh.bc[L_VR + q];
+    }
+  }
+  __syncthreads();
+  phaseE<DT, MAXM>(s, g, iat, n, ld, k, d, r, v.R, v.Ainv, v.dpsi, v.U, v.V, v.Binv, v.dl, v.jas, v.logdet, ainv_r, phv,
+                   sh, sB, false);
+}
+
 // features of electron a.iat0 at the position LS_R with the drift LS_V of each walker (test hook)
 __global__ __launch_bounds__(WG) void k_learn_features(SysC s, State st, MoveArgs a, LearnArgs la)
 {
@@ -1272,7 +1455,7 @@ __global__ __launch_bounds__(WG) void k_learn_features(SysC s, State st, MoveArg
 
 template<typename ST, typename DT>
 static hipError_t launch_learn_t(int which, const SysC& s, const State& st, const ST* coefs, const MoveArgs& a,
-                                 const LearnArgs& la, hipStream_t stream)
+                                 const LearnArgs& la, hipStream_t stream, const MlpParams* P)
 {
   const int maxn   = move_maxm(s, a.g) * WG;
   if (maxn == 0)
@@ -1291,6 +1474,7 @@ static hipError_t launch_learn_t(int which, const SysC& s, const State& st, cons
     case LK_PRE: k_learn_pre<ST, DT, 1><<<grid, block, dyn, stream>>>(s, st, coefs, a, la); break;
     case LK_PROP: k_learn_prop<ST, DT, 1><<<grid, block, 0, stream>>>(s, st, coefs, a, la); break;
     case LK_ACCEPT: k_learn_accept<DT, 1><<<grid, block, dyn, stream>>>(s, st, a, la); break;
+    case LK_FUSED: k_learn_move<ST, DT, 1><<<grid, block, dyn, stream>>>(s, st, coefs, a, la, *P); break;
     default: return hipErrorInvalidValue;
     }
   }
@@ -1301,6 +1485,7 @@ static hipError_t launch_learn_t(int which, const SysC& s, const State& st, cons
     case LK_PRE: k_learn_pre<ST, DT, 2><<<grid, block, dyn, stream>>>(s, st, coefs, a, la); break;
     case LK_PROP: k_learn_prop<ST, DT, 2><<<grid, block, 0, stream>>>(s, st, coefs, a, la); break;
     case LK_ACCEPT: k_learn_accept<DT, 2><<<grid, block, dyn, stream>>>(s, st, a, la); break;
+    case LK_FUSED: k_learn_move<ST, DT, 2><<<grid, block, dyn, stream>>>(s, st, coefs, a, la, *P); break;
     default: return hipErrorInvalidValue;
     }
   }
@@ -1310,12 +1495,14 @@ static hipError_t launch_learn_t(int which, const SysC& s, const State& st, cons
 }
 
 hipError_t launch_learn(int which, const SysC& s, const State& st, const void* coefs, const MoveArgs& a,
-                        const LearnArgs& la, hipStream_t stream)
+                        const LearnArgs& la, hipStream_t stream, const MlpParams* P)
 {
+  if (which == LK_FUSED && !P)
+    return hipErrorInvalidValue;
   return dispatch_prec(s.prec, [&](auto stv, auto dtv) {
     using ST = decltype(stv);
     using DT = decltype(dtv);
-    return launch_learn_t<ST, DT>(which, s, st, static_cast<const ST*>(coefs), a, la, stream);
+    return launch_learn_t<ST, DT>(which, s, st, static_cast<const ST*>(coefs), a, la, stream, P);
   });
 }
 

@@ -13,7 +13,7 @@ struct MoveLDS
   double p[MAX_K];
   double y[MAX_K]; // row d of Binv (from phase A), zero beyond d
   double Y[MAX_K]; // new column d of Binv
-  double red[9 * WG]; // block_sum / delayed_gemv scratch (up to 9 values per thread)
+  alignas(16) double red[9 * WG]; // block_sum / delayed_gemv scratch (up to 9 values per thread)
   double bc[48];      // broadcast scalars
   double jc[3][MAX_JCOEF]; // Jastrow functor tables: J1, J2 like spins, J2 unlike spins
   SplineLDS spl;      // uniform spline factors of the proposed position
