@@ -146,6 +146,13 @@ int qmi_mw_det_ratios(qmi_ctx* ctx, int nvp, const double* pos, int n, const dou
  * (SoaDistanceTableAA.h:58-75, SoaDistanceTableAB.h:59-78): distances and displacements
  * (source - target, minimum image) from pos[nw][3] to all electrons and all ions.
  * ee_r[nw][N], ee_dr[nw][N][3], ei_r[nw][n_ion], ei_dr[nw][n_ion][3] (nullable). */
+/* TrialWaveFunction::mw_evaluateRatios (TrialWaveFunction.cpp:1079-1110) over one
+ * VirtualParticleSet per walker (Particle/VirtualParticleSet.cpp; the virtual moves of non-local
+ * pseudopotentials, NonLocalECPComponent.cpp:208-228): ratios[w][v] = psi with electron iat at
+ * pos[w][v] over psi (determinant: stored inverse row, DiracDeterminantBatched.cpp:812-846;
+ * J1OrbitalSoA.h:295-299; TwoBodyJastrow.cpp:166-171).  Needs no pending delayed update in iat's
+ * spin group (call qmi_mw_complete_updates first). */
+int qmi_mw_evaluate_ratios(qmi_ctx* ctx, int iat, int nvp, const double* pos, double* ratios);
 int qmi_mw_distance_rows(qmi_ctx* ctx, const double* pos, double* ee_r, double* ee_dr, double* ei_r,
                          double* ei_dr);
 

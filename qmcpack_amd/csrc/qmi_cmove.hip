@@ -1376,6 +1376,66 @@ __global__ __launch_bounds__(WG) void k_api_det_ratios_c(SysC s, const double* _
     out[(size_t)w * nvp + v] = zmk(bc[0], bc[1]);
 }
 
+// TrialWaveFunction::mw_evaluateRatios with complex orbitals: out[w][nvp] complex
+struct VpLDSc
+{
+  alignas(16) double red[9 * WG];
+  double bc[64];
+  double jc[3][MAX_JCOEF];
+};
+
+template<int NJ>
+__global__ __launch_bounds__(WG) void k_api_vp_ratios_c(SysC s, State st, const double* __restrict__ coefs, int iat,
+                                                        int nvp, const double* __restrict__ pos, double2* __restrict__ out)
+{
+  __shared__ VpLDSc sh;
+  stage_jastrow(s, sh);
+  const int v = blockIdx.x, w = blockIdx.y, t = threadIdx.x;
+  const int g = iat < s.nup ? 0 : 1, n = s.n[g], ld = s.ld, Np = s.Npad;
+  const int r       = iat - s.first[g];
+  const double* R   = st.R + (size_t)w * 3 * Np;
+  const double* jas = st.jas + (size_t)w * J_ROWS * Np;
+  const double* pp  = pos + ((size_t)w * nvp + v) * 3;
+  const double p[3] = {pp[0], pp[1], pp[2]};
+  jastrow_ratio_sums<NJ>(s, g, iat, R, st.ions, p, sh);
+  double2 acc[1] = {zmk(0, 0)};
+  if (s.has_det)
+  {
+    SplineLoc L;
+    spline_locate(s, p, L);
+    const double2* arow = det_ptr<double2>(st.Ainv[g]) + ((size_t)w * n + r) * ld;
+    for (int o = t; o < n; o += WG)
+      acc[0] = zfma(spline_v_orbital_c(s, coefs, L, o, st.kpts, p), arow[o], acc[0]);
+  }
+  block_sum_z<1>(acc, sh.red, sh.bc + 56);
+  if (t == 0)
+  {
+    const double* C = sh.bc + BC_C;
+    double2 ratio   = s.has_det ? bcz(sh.bc, 56) : zmk(1, 0);
+    if (s.has_j1)
+      ratio = zscale(ratio, exp(jas[J_VAT * Np + iat] - C[0]));
+    if (s.has_j2)
+      ratio = zscale(ratio, exp(jas[J_UAT * Np + iat] - C[5]));
+    out[(size_t)w * nvp + v] = ratio;
+  }
+}
+
+hipError_t launch_api_vp_ratios_c(const SysC& s, const State& st, const void* coefs, int nw, int iat, int nvp,
+                                  const double* pos, double* out, hipStream_t stream)
+{
+  const int mm = move_maxm(s, iat < s.nup ? 0 : 1);
+  if (mm == 0 || nvp <= 0)
+    return hipErrorInvalidValue;
+  const dim3 grid(nvp, nw);
+  const double* c = static_cast<const double*>(coefs);
+  double2* o      = reinterpret_cast<double2*>(out);
+  if (mm == 1)
+    k_api_vp_ratios_c<2><<<grid, WG, 0, stream>>>(s, st, c, iat, nvp, pos, o);
+  else
+    k_api_vp_ratios_c<4><<<grid, WG, 0, stream>>>(s, st, c, iat, nvp, pos, o);
+  return hipGetLastError();
+}
+
 hipError_t launch_api_det_ratios_c(const SysC& s, const void* coefs, const double* kpts, int nw, int nvp, int n,
                                    const double* pos, const double* inv, double* out, hipStream_t stream)
 {

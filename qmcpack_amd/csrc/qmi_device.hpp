@@ -324,6 +324,24 @@ __device__ inline void spline_vgl_orbital(const SysC& s, const ST* __restrict__ 
       hzz * s.symGG[5];
 }
 
+// Value only of orbital o (SplineR2R::evaluateValue / assign_v, SplineR2R.cpp:162-202):
+// sum_ij a_i b_j sum_k c_k coef.
+template<typename ST>
+__device__ inline double spline_v_orbital(const SysC& s, const ST* __restrict__ coefs, const SplineLoc& L, int o)
+{
+  double val     = 0;
+  const ST* base = coefs + L.off + o;
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++)
+    {
+      const ST* q     = base + i * s.xs + j * s.ys;
+      const double s0 = L.c[0] * (double)q[0] + L.c[1] * (double)q[s.zs] + L.c[2] * (double)q[2 * s.zs] +
+          L.c[3] * (double)q[3 * s.zs];
+      val += (L.a[i] * L.b[j]) * s0;
+    }
+  return val;
+}
+
 // Uniform per-position spline factors staged in LDS: the 16 (i,j) products
 // {d2a_i b_j, da_i b_j, a_i b_j, da_i db_j, a_i db_j, a_i d2b_j} and the z factors c, dc, d2c.
 struct SplineLDS

@@ -1623,6 +1623,31 @@ extern "C"
     });
   }
 
+  int qmi_mw_evaluate_ratios(qmi_ctx* c, int iat, int nvp, const double* pos, double* ratios)
+  {
+    return guarded(c, [&] {
+      c->require_walkers();
+      if (iat < 0 || iat >= c->sys.N || nvp <= 0 || !pos || !ratios)
+        throw QmiError(QMI_ERR_ARG, "bad electron index or virtual positions");
+      if (c->dcount[c->group(iat)] != 0)
+        throw QmiError(QMI_ERR_ARG, "delayed updates pending: call qmi_mw_complete_updates first");
+      DevBuf<double> dpos, dout;
+      dpos.alloc((size_t)c->nw * nvp * 3);
+      dout.alloc((size_t)c->nw * nvp * c->vw());
+      HIPCHECK(hipMemcpyAsync(dpos.p, pos, dpos.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      {
+        qmi_ctx::Scoped t(c, "api_vp_ratios");
+        HIPCHECK(c->cplx ? launch_api_vp_ratios_c(c->sys, c->st, c->coefs.p, c->nw, iat, nvp, dpos.p, dout.p, c->stream)
+                         : launch_api_vp_ratios(c->sys, c->st, c->coefs.p, c->nw, iat, nvp, dpos.p, dout.p, c->stream));
+      }
+      HIPCHECK(hipMemcpyAsync(ratios, dout.p, dout.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      c->sync();
+      for (size_t i = 0; i < dout.n; i++)
+        if (std::isnan(ratios[i]))
+          throw QmiError(QMI_ERR_NUMERIC, "NaN ratio in mw_evaluateRatios");
+    });
+  }
+
   int qmi_mw_distance_rows(qmi_ctx* c, const double* pos, double* ee_r, double* ee_dr, double* ei_r, double* ei_dr)
   {
     return guarded(c, [&] {

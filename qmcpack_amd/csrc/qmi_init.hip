@@ -363,20 +363,7 @@ __global__ __launch_bounds__(WG) void k_api_det_ratios(SysC s, const ST* __restr
   spline_locate(s, p, L);
   double acc[1] = {0};
   for (int o = threadIdx.x; o < n; o += WG)
-  {
-    // value only: sum_ij a_i b_j sum_k c_k coef
-    double val = 0;
-    const ST* base = coefs + L.off + o;
-    for (int i = 0; i < 4; i++)
-      for (int j = 0; j < 4; j++)
-      {
-        const ST* q     = base + i * s.xs + j * s.ys;
-        const double s0 = L.c[0] * (double)q[0] + L.c[1] * (double)q[s.zs] + L.c[2] * (double)q[2 * s.zs] +
-            L.c[3] * (double)q[3 * s.zs];
-        val += (L.a[i] * L.b[j]) * s0;
-      }
-    acc[0] += val * inv[(size_t)w * n + o];
-  }
+    acc[0] += spline_v_orbital<ST>(s, coefs, L, o) * inv[(size_t)w * n + o];
   block_sum<1>(acc, red, bc);
   if (threadIdx.x == 0)
     out[(size_t)w * nvp + v] = bc[0];

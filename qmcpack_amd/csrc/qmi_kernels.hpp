@@ -94,6 +94,12 @@ hipError_t launch_api_spo_vgl_c(const SysC& s, const void* coefs, const double* 
                                 const double* pos, double* out, hipStream_t stream);
 hipError_t launch_api_det_ratios_c(const SysC& s, const void* coefs, const double* kpts, int nw, int nvp, int n,
                                    const double* pos, const double* inv, double* out, hipStream_t stream);
+// TrialWaveFunction::mw_evaluateRatios for virtual moves of electron iat: pos [nw][nvp][3] ->
+// out [nw][nvp] (complex pairs for complex orbitals); no delayed update pending for iat's group
+hipError_t launch_api_vp_ratios(const SysC& s, const State& st, const void* coefs, int nw, int iat, int nvp,
+                                const double* pos, double* out, hipStream_t stream);
+hipError_t launch_api_vp_ratios_c(const SysC& s, const State& st, const void* coefs, int nw, int iat, int nvp,
+                                  const double* pos, double* out, hipStream_t stream);
 hipError_t launch_api_distances(const SysC& s, const State& st, int nw, const double* pos, double* ee_r,
                                 double* ee_dr, double* ei_r, double* ei_dr, hipStream_t stream);
 

@@ -823,6 +823,76 @@ __global__ __launch_bounds__(WG) void k_api_accept(SysC s, State st, MoveArgs a)
                sh, sB, true);
 }
 
+// LDS of the virtual-move ratio kernels: the Jastrow reductions only
+struct VpLDS
+{
+  alignas(16) double red[9 * WG];
+  double bc[48];
+  double jc[3][MAX_JCOEF];
+};
+
+// TrialWaveFunction::mw_evaluateRatios (TrialWaveFunction.cpp:1079-1110) for virtual moves of
+// electron iat (one VirtualParticleSet per walker, positions pos[w][nvp][3]): psi with r_iat at
+// each virtual position over psi, as the product of the determinant ratio (DiracDeterminantBatched::
+// mw_evaluateRatios, DiracDeterminantBatched.cpp:812-846: value-only orbitals dotted with the stored
+// inverse row, so no delayed update may be pending), the J1 ratio (J1OrbitalSoA.h:295-299) and the
+// J2 ratio (TwoBodyJastrow.cpp:166-171, the pair with iat itself excluded).  Grid (nvp, nw).
+template<typename ST, typename DT, int NJ>
+__global__ __launch_bounds__(WG) void k_api_vp_ratios(SysC s, State st, const ST* __restrict__ coefs, int iat, int nvp,
+                                                      const double* __restrict__ pos, double* __restrict__ out)
+{
+  __shared__ VpLDS sh;
+  stage_jastrow(s, sh);
+  const int v = blockIdx.x, w = blockIdx.y, t = threadIdx.x;
+  const int g = iat < s.nup ? 0 : 1, n = s.n[g], ld = s.ld, Np = s.Npad;
+  const int r       = iat - s.first[g];
+  const double* R   = st.R + (size_t)w * 3 * Np;
+  const double* jas = st.jas + (size_t)w * J_ROWS * Np;
+  const double* pp  = pos + ((size_t)w * nvp + v) * 3;
+  const double p[3] = {pp[0], pp[1], pp[2]};
+  jastrow_ratio_sums<NJ>(s, g, iat, R, st.ions, p, sh);
+  double acc[1] = {0};
+  if (s.has_det)
+  {
+    SplineLoc L;
+    spline_locate(s, p, L);
+    const DT* arow = det_ptr<DT>(st.Ainv[g]) + ((size_t)w * n + r) * ld;
+    for (int o = t; o < n; o += WG)
+      acc[0] += spline_v_orbital<ST>(s, coefs, L, o) * (double)arow[o];
+  }
+  block_sum<1>(acc, sh.red, sh.bc + 40);
+  if (t == 0)
+  {
+    const double* C = sh.bc + BC_C;
+    double ratio    = 1.0;
+    if (s.has_det)
+      ratio *= sh.bc[40];
+    if (s.has_j1)
+      ratio *= exp(jas[J_VAT * Np + iat] - C[0]);
+    if (s.has_j2)
+      ratio *= exp(jas[J_UAT * Np + iat] - C[5]);
+    out[(size_t)w * nvp + v] = ratio;
+  }
+}
+
+hipError_t launch_api_vp_ratios(const SysC& s, const State& st, const void* coefs, int nw, int iat, int nvp,
+                                const double* pos, double* out, hipStream_t stream)
+{
+  const int mm = move_maxm(s, iat < s.nup ? 0 : 1);
+  if (mm == 0 || nvp <= 0)
+    return hipErrorInvalidValue;
+  return dispatch_prec(s.prec, [&](auto stv, auto dtv) {
+    using ST = decltype(stv);
+    using DT = decltype(dtv);
+    const dim3 grid(nvp, nw);
+    if (mm == 1)
+      k_api_vp_ratios<ST, DT, 2><<<grid, WG, 0, stream>>>(s, st, static_cast<const ST*>(coefs), iat, nvp, pos, out);
+    else
+      k_api_vp_ratios<ST, DT, 4><<<grid, WG, 0, stream>>>(s, st, static_cast<const ST*>(coefs), iat, nvp, pos, out);
+    return hipGetLastError();
+  });
+}
+
 // mw_makeMove: proposed position R[iat] + displ
 __global__ void k_api_make_move(SysC s, State st, int iat, int nw, const double* __restrict__ displ)
 {

@@ -225,6 +225,15 @@ class Context:
         self._check(self.L.qmi_mw_det_ratios(self.h, nvp, _p(pos), n, _p(inv_row), _p(out)))
         return self._out(out)
 
+    def mw_evaluateRatios(self, iat, pos):
+        """TrialWaveFunction::mw_evaluateRatios over virtual positions pos [nw][nvp][3] of electron
+        iat -> ratios [nw][nvp] (complex for complex orbitals)."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        nvp = pos.shape[1]
+        out = self._vals(self.nw, nvp)
+        self._check(self.L.qmi_mw_evaluate_ratios(self.h, int(iat), nvp, _p(pos), _p(out)))
+        return self._out(out)
+
     def mw_distanceRows(self, pos, n_ion):
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.nw, 3)
         ee_r, ee_dr = np.empty((self.nw, self.N)), np.empty((self.nw, self.N, 3))

@@ -16,7 +16,7 @@ from oracle import oracle as orc
 from qmcpack_amd import qmi
 from qmcpack_amd.system import System, jastrow_params, make_complex_system, make_walkers
 
-from test_gpu_parity import REL, RATIO_FLOOR, check_vmc, relerr
+from test_gpu_parity import REL, RATIO_FLOOR, check_vmc, quadrature_positions, relerr
 
 pytestmark = pytest.mark.gpu
 
@@ -138,3 +138,21 @@ def test_c2c_vmc_large(n_up, n_dn, k):
     R = make_walkers(s, 2, 42)
     res = vmc_parity_c(s, R, 1, steps=1, sub_steps=1, tau=0.3, recompute_period=0, seed=80)
     check_vmc(*res, ratio_floor=0.1, ratio_tol=1e-9)
+
+
+def test_c2c_evaluate_ratios_virtual_moves():
+    """mw_evaluateRatios with complex orbitals (complex ratios) after a delayed-update VMC run."""
+    s = complex_small(n_up=12, n_dn=10, k=4, norb=14, seed=5)
+    nw = 4
+    R = np.random.default_rng(6).uniform(0, 1, size=(nw, s.N, 3)) * s.L
+    c, o, E, tr, Eo, tro = vmc_parity_c(s, R, 1, steps=2, sub_steps=1, tau=0.4, recompute_period=0, seed=82)
+    Rc = c.walkers_get()
+    rng = np.random.default_rng(8)
+    worst = 0.0
+    for iat in (1, 11, 20):
+        vp = quadrature_positions(Rc, iat, s.ions, rng)
+        rat = c.mw_evaluateRatios(iat, vp)
+        ref = np.array([[o.ratio_grad(w, iat, vp[w, v])[0] for v in range(vp.shape[1])] for w in range(nw)])
+        worst = max(worst, relerr(rat, ref, floor=RATIO_FLOOR))
+    print("complex virtual-move ratio error", worst)
+    assert worst < REL

@@ -220,3 +220,50 @@ def test_vmc_rank32_large_det(n_up, n_dn, nion, k):
           f"oracle variant spread: ratio {spread:.3e}, gradient {gspread:.3e}")
     check_vmc(c, o, E, tr, Eo, tro, ratio_floor=floor, ratio_tol=max(1e-9, 3 * spread),
               grad_tol=max(REL, 3 * gspread))
+
+
+def quadrature_positions(R, iat, ions, rng, nvp=12):
+    """Virtual positions of a non-local pseudopotential: the 12 icosahedron directions, randomly
+    rotated, at the electron's distance from its nearest ion (NonLocalECPComponent.cpp:208-228)."""
+    g = (1 + 5 ** 0.5) / 2
+    q = np.array([[0, 1, g], [0, -1, g], [0, 1, -g], [0, -1, -g], [1, g, 0], [-1, g, 0], [1, -g, 0], [-1, -g, 0],
+                  [g, 0, 1], [-g, 0, 1], [g, 0, -1], [-g, 0, -1]], dtype=float)[:nvp]
+    q /= np.linalg.norm(q, axis=1)[:, None]
+    out = []
+    for w in range(R.shape[0]):
+        A, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        d = R[w, iat] - ions
+        I = ions[np.argmin(np.linalg.norm(d, axis=1))]
+        rad = np.linalg.norm(R[w, iat] - I)
+        out.append(I + rad * q @ A.T)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("k", [1, 4])
+def test_evaluate_ratios_virtual_moves(k):
+    """mw_evaluateRatios at quadrature-like virtual positions after a VMC run (completed delayed
+    updates), against the oracle's ratio at each virtual position (itself pinned against full
+    recomputation of log psi, tests/test_oracle_math.py); a pending delayed update is refused."""
+    s = small_periodic(n_up=12, n_dn=10, k=k, norb=14, seed=5)
+    nw = 4
+    R = np.random.default_rng(6).uniform(0, 1, size=(nw, s.N, 3)) * s.L
+    c, o, E, tr, Eo, tro = vmc_parity(s, R, 1, steps=2, sub_steps=1, tau=0.4, recompute_period=0, seed=81)
+    Rc = c.walkers_get()
+    rng = np.random.default_rng(7)
+    worst = 0.0
+    for iat in (0, 5, 13, 21):
+        vp = quadrature_positions(Rc, iat, s.ions, rng)
+        rat = c.mw_evaluateRatios(iat, vp)
+        ref = np.array([[o.ratio_grad(w, iat, vp[w, v])[0] for v in range(vp.shape[1])] for w in range(nw)])
+        worst = max(worst, relerr(rat, ref, floor=RATIO_FLOOR))
+    print("virtual-move ratio error", worst)
+    assert worst < REL
+    if k > 1:
+        c.mw_evalGrad(2)
+        c.mw_makeMove(2, np.full((nw, 3), 0.1))
+        c.mw_calcRatioGrad(2)
+        c.mw_accept_rejectMove(2, np.ones(nw, dtype=int), safe_to_delay=True)
+        with pytest.raises(qmi.QmiError):
+            c.mw_evaluateRatios(3, quadrature_positions(Rc, 3, s.ions, rng))
+        c.mw_completeUpdates()
+        c.mw_evaluateRatios(3, quadrature_positions(Rc, 3, s.ions, rng))
