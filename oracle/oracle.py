@@ -46,6 +46,8 @@ def lib():
             build()
         L = C.CDLL(LIB_PATH)
         L.orc_symtrace.restype = C.c_double
+        L.orc_coulomb_aa.restype = C.c_double
+        L.orc_coulomb_ab.restype = C.c_double
         L.orc_det_create.restype = C.c_void_p
         L.orc_vmc_create.restype = C.c_void_p
         for name in ("orc_det_destroy", "orc_vmc_destroy"):
@@ -53,7 +55,8 @@ def lib():
         for name in ("orc_det_recompute", "orc_det_eval_grad", "orc_det_ratio_grad", "orc_det_accept_reject",
                      "orc_det_complete", "orc_det_get", "orc_vmc_evaluate_log", "orc_wf_ratio_grad",
                      "orc_wf_accept_reject", "orc_wf_eval_grad", "orc_wf_evaluate_gl", "orc_vmc_run", "orc_vmc_replay",
-                     "orc_vmc_get", "orc_vmc_is_complex", "orc_spo_vgl_c", "orc_invert_transpose_c"):
+                     "orc_vmc_get", "orc_vmc_is_complex", "orc_spo_vgl_c", "orc_invert_transpose_c",
+                     "orc_ewald_params", "orc_vmc_potential"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -177,6 +180,37 @@ def _cv(a):
     return a[..., 0] + 1j * a[..., 1]
 
 
+def ewald_params(L, dim_cutoff=15.0):
+    """(r_c, k_c, sigma, number of k vectors) of the Ewald breakup of an orthorhombic cell."""
+    Lv = np.ascontiguousarray(L, dtype=np.float64)
+    out = np.empty(4)
+    lib().orc_ewald_params(ptr(Lv), C.c_double(dim_cutoff), ptr(out))
+    return out[0], out[1], out[2], int(out[3])
+
+
+def coulomb_aa(L, r, q, dim_cutoff=15.0):
+    """CoulombPBCAA energy of charges q at r [n][3]; returns (total, (short, long, constants))."""
+    Lv = np.ascontiguousarray(L, dtype=np.float64)
+    r = np.ascontiguousarray(r, dtype=np.float64).reshape(-1, 3)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    parts = np.empty(3)
+    v = lib().orc_coulomb_aa(ptr(Lv), C.c_double(dim_cutoff), r.shape[0], ptr(r), ptr(q), ptr(parts))
+    return v, parts
+
+
+def coulomb_ab(L, ra, qa, rb, qb, dim_cutoff=15.0):
+    """CoulombPBCAB energy between charges qa at ra and qb at rb."""
+    Lv = np.ascontiguousarray(L, dtype=np.float64)
+    ra = np.ascontiguousarray(ra, dtype=np.float64).reshape(-1, 3)
+    rb = np.ascontiguousarray(rb, dtype=np.float64).reshape(-1, 3)
+    qa = np.ascontiguousarray(qa, dtype=np.float64)
+    qb = np.ascontiguousarray(qb, dtype=np.float64)
+    parts = np.empty(3)
+    v = lib().orc_coulomb_ab(ptr(Lv), C.c_double(dim_cutoff), ra.shape[0], ptr(ra), ptr(qa), rb.shape[0], ptr(rb),
+                             ptr(qb), ptr(parts))
+    return v, parts
+
+
 class Det:
     """Batched determinant + delayed-update engine (nw walkers in lock-step)."""
 
@@ -297,6 +331,13 @@ class VMC:
         if st != 0:
             raise RuntimeError("oracle VMC replay failed")
         return E, tr
+
+    def potential(self, ion_charge):
+        """[nw][3]: electron-electron, electron-ion, ion-ion Coulomb energies (Ewald)."""
+        out = np.empty((self.nw, 3))
+        if lib().orc_vmc_potential(self.h, C.c_double(ion_charge), ptr(out)) != 0:
+            raise RuntimeError("oracle potential failed")
+        return out
 
     def get(self):
         R = np.empty((self.nw, self.N, 3))
