@@ -89,6 +89,14 @@ def measured_traffic(kernel_prefix="qmi::k_sweep_segment"):
     return None, None
 
 
+def coulomb_cost(N, n_ion, nk, pair_flops=40, k_flops=30):
+    """Floating-point operations per walker of the Ewald energy: each electron-electron and
+    electron-ion pair costs ~40 (minimum image 15, distance 6, spline 19) and each (k, electron)
+    term ~30 (phase 5, sincos ~20, accumulate 4); the rest is O(nk)."""
+    pairs = N * (N - 1) // 2 + N * n_ion
+    return dict(pairs=pairs, k_terms=nk * N, flops=pairs * pair_flops + nk * N * k_flops)
+
+
 def cpu_baseline(system, cfg, nwalk=16, nthreads=None):
     """The CPU FP64 oracle (a from-scratch restatement of the reference's CPU batched path) on a
     bounded sample: nwalk walkers of the same system, one crowd per thread, one sweep."""
@@ -257,6 +265,16 @@ def bench_uneven(args, log):
             "bytes_per_launch": bytes_launch,
             "avg_launch_ms": avg_ms,
         },
+        "coulomb_roofline": {
+            "kernel": "k_coulomb (Ewald: short-range pair sums from an LDS spline, long-range rho_k)",
+            "avg_ms": tc["ms"] / max(1, tc["calls"]),
+            "walkers": nw,
+            "k_vectors": coul["k_terms"] // N,
+            "flops_per_walker": coul["flops"],
+            "achieved_tflops": nw * coul["flops"] / (tc["ms"] / max(1, tc["calls"]) * 1e-3) / 1e12
+                               if tc["ms"] > 0 else 0.0,
+            "fp64_peak_tflops": FP64_PEAK_TFLOPS,
+        },
     }
     if not args.no_cpu:
         from oracle import oracle as orc
@@ -346,6 +364,15 @@ def main():
     nprof = 2
     ctx.vmc_run(nprof, 1, tau, 0, seed)
     tm = ctx.timing()
+    ctx.timing_reset()
+    # periodic Coulomb energy (Ewald) of every walker once per step, as the Hamiltonian would be
+    # evaluated; timed apart from the sweep (the sweep's local energy is the kinetic part)
+    zion = system.N / len(system.ions)
+    ctx.coulomb_setup(zion)
+    for _ in range(3):
+        ctx.mw_evaluatePotential()
+    tc = ctx.timing().get("coulomb", dict(ms=0.0, calls=0))
+    coul = coulomb_cost(system.N, len(system.ions), ctx.coulomb_nk())
     ctx.timing_enable(False)
     del ctx
     n, N, nion, k = system.n_up, system.N, len(system.ions), system.delay_rank
@@ -431,6 +458,16 @@ def main():
             "achieved_gbs": (nw * flush_bytes(n, k, sd)) / (fl_ms * 1e-3) / 1e9 if fl_ms > 0 else 0.0,
             "achieved_tflops": (nw * flush_flops(n, k)) / (fl_ms * 1e-3) / 1e12 if fl_ms > 0 else 0.0,
             "mfma_peak_tflops": FP32_PEAK_TFLOPS if sd == 4 else FP64_PEAK_TFLOPS,
+        },
+        "coulomb_roofline": {
+            "kernel": "k_coulomb (Ewald: short-range pair sums from an LDS spline, long-range rho_k)",
+            "avg_ms": tc["ms"] / max(1, tc["calls"]),
+            "walkers": nw,
+            "k_vectors": coul["k_terms"] // N,
+            "flops_per_walker": coul["flops"],
+            "achieved_tflops": nw * coul["flops"] / (tc["ms"] / max(1, tc["calls"]) * 1e-3) / 1e12
+                               if tc["ms"] > 0 else 0.0,
+            "fp64_peak_tflops": FP64_PEAK_TFLOPS,
         },
     }
     if not args.no_cpu:

@@ -153,6 +153,16 @@ int qmi_mw_det_ratios(qmi_ctx* ctx, int nvp, const double* pos, int n, const dou
  * J1OrbitalSoA.h:295-299; TwoBodyJastrow.cpp:166-171).  Needs no pending delayed update in iat's
  * spin group (call qmi_mw_complete_updates first). */
 int qmi_mw_evaluate_ratios(qmi_ctx* ctx, int iat, int nvp, const double* pos, double* ratios);
+/* Periodic Coulomb energy with the Ewald breakup (CoulombPBCAA / CoulombPBCAB with
+ * Particle/LongRange/EwaldHandler3D; QMCHamiltonians/CoulombPBCAA.cpp:568-840,
+ * CoulombPBCAB.cpp:373-453): electrons carry charge -1, the ions ion_charge; dim_cutoff = k_c r_c
+ * (the reference's LR_dim_cutoff, 15 by default).  Periodic orthorhombic cells; tables built once
+ * per system. */
+int qmi_coulomb_setup(qmi_ctx* ctx, double ion_charge, double dim_cutoff);
+/* number of k vectors of the long-range sum (0 before qmi_coulomb_setup) */
+int qmi_coulomb_nk(qmi_ctx* ctx);
+/* v[nw][3] = {electron-electron, electron-ion, ion-ion} energy of every walker */
+int qmi_mw_evaluate_potential(qmi_ctx* ctx, double* v);
 int qmi_mw_distance_rows(qmi_ctx* ctx, const double* pos, double* ee_r, double* ee_dr, double* ei_r,
                          double* ei_dr);
 

@@ -415,6 +415,7 @@ struct qmi_ctx
     have_system = true;
     nw          = 0;
     invalidate_graph();
+    coul_ready = false;
   }
 
   // ---------------------------------------------------------------------------------------------
@@ -696,6 +697,168 @@ struct qmi_ctx
     a.nw   = nw;
     a.wpc  = wpc;
     return a;
+  }
+
+  // ---------------------------------------------------------------------------------------------
+  // Periodic Coulomb tables (Ewald breakup, Particle/LongRange/EwaldHandler3D.{h,cpp} as used by
+  // CoulombPBCAA / CoulombPBCAB): r_c = min L / 2, k_c = dim_cutoff / r_c, sigma = sqrt(k_c / 2 r_c);
+  // k vectors 2 pi n / L with |k|^2 <= k_c^2 grouped in shells (KContainer.cpp:54-230); the
+  // clamped cubic spline of r erfc(sigma r) / r on 1001 points (LRCoulombSingleton.cpp:137-158).
+  bool coul_ready = false;
+  CoulombC coul{};
+  DevBuf<double> coul_k, coul_fk, coul_rhoi, coul_y, coul_y2, coul_out;
+
+  static void clamped_spline(const std::vector<double>& x, const std::vector<double>& y, double yp0, double ypn,
+                             std::vector<double>& y2)
+  {
+    const int n = (int)x.size();
+    std::vector<double> c(n), d(n);
+    const double h0 = x[1] - x[0];
+    c[0]            = 0.5;
+    d[0]            = 3.0 * ((y[1] - y[0]) / h0 - yp0) / h0;
+    for (int i = 1; i < n - 1; i++)
+    {
+      const double hl = x[i] - x[i - 1], hr = x[i + 1] - x[i];
+      const double den = 2.0 * (hl + hr) - hl * c[i - 1];
+      c[i]             = hr / den;
+      d[i]             = (6.0 * ((y[i + 1] - y[i]) / hr - (y[i] - y[i - 1]) / hl) - hl * d[i - 1]) / den;
+    }
+    const double hn = x[n - 1] - x[n - 2];
+    y2[n - 1]       = (6.0 * (ypn - (y[n - 1] - y[n - 2]) / hn) - hn * d[n - 2]) / (2.0 * hn - hn * c[n - 2]);
+    for (int i = n - 2; i >= 0; i--)
+      y2[i] = d[i] - c[i] * y2[i + 1];
+  }
+
+  void coulomb_setup(double zion, double dim_cutoff)
+  {
+    if (!have_system)
+      throw QmiError(QMI_ERR_ARG, "system not created");
+    const SysC& s = sys;
+    if (!s.periodic[0] || !s.periodic[1] || !s.periodic[2])
+      throw QmiError(QMI_ERR_UNSUPPORTED, "the Ewald energy needs a periodic cell");
+    if (!(dim_cutoff > 0))
+      throw QmiError(QMI_ERR_ARG, "dim_cutoff must be positive");
+    const double* L  = s.L;
+    const double rc  = 0.5 * std::min(L[0], std::min(L[1], L[2]));
+    const double kc  = dim_cutoff / rc;
+    const double sig = std::sqrt(kc / (2.0 * rc));
+    const double vol = L[0] * L[1] * L[2];
+    struct K
+    {
+      double x, y, z, k2;
+    };
+    std::vector<K> ks;
+    int mmax[3];
+    for (int d = 0; d < 3; d++)
+      mmax[d] = (int)std::floor(L[d] * kc / (2 * M_PI)) + 1;
+    for (int i = -mmax[0]; i <= mmax[0]; i++)
+      for (int j = -mmax[1]; j <= mmax[1]; j++)
+        for (int k = -mmax[2]; k <= mmax[2]; k++)
+        {
+          if (i == 0 && j == 0 && k == 0)
+            continue;
+          const double x = 2 * M_PI * i / L[0], y = 2 * M_PI * j / L[1], z = 2 * M_PI * k / L[2];
+          const double k2 = x * x + y * y + z * z;
+          if (k2 <= kc * kc)
+            ks.push_back({x, y, z, k2});
+        }
+    std::stable_sort(ks.begin(), ks.end(),
+                     [](const K& a, const K& b) { return std::llround(a.k2 * 1e8) < std::llround(b.k2 * 1e8); });
+    const int nk = (int)ks.size();
+    std::vector<double> kv(3 * (size_t)nk), fk(nk), rhoi(2 * (size_t)nk, 0.0);
+    const double kgauss = 1.0 / (4 * sig * sig), knorm = 4 * M_PI / vol;
+    long long key = -1;
+    double fshell = 0;
+    std::vector<double> ions_h(3 * (size_t)s.nion);
+    if (s.nion)
+      HIPCHECK(hipMemcpy(ions_h.data(), ions.p, ions_h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = 0; k < nk; k++)
+    {
+      const long long kk = std::llround(ks[k].k2 * 1e8);
+      if (kk != key)
+      {
+        key    = kk;
+        fshell = knorm * std::exp(-ks[k].k2 * kgauss) / ks[k].k2;
+      }
+      kv[k]                   = ks[k].x;
+      kv[nk + k]              = ks[k].y;
+      kv[2 * (size_t)nk + k]  = ks[k].z;
+      fk[k]                   = fshell;
+      for (int a = 0; a < s.nion; a++)
+      {
+        const double ph = ks[k].x * ions_h[a] + ks[k].y * ions_h[s.nion + a] + ks[k].z * ions_h[2 * s.nion + a];
+        rhoi[k] += zion * std::cos(ph);
+        rhoi[nk + k] += zion * std::sin(ph);
+      }
+    }
+    const int ng = 1001;
+    std::vector<double> gx(ng), gy(ng), gy2(ng);
+    const double dr = rc / (ng - 1);
+    for (int i = 0; i < ng; i++)
+      gx[i] = i * dr;
+    for (int i = 1; i < ng - 1; i++)
+      gy[i] = gx[i] * (std::erfc(sig * gx[i]) / gx[i]);
+    gy[0]      = 2.0 * gy[1] - gy[2];
+    gy[ng - 1] = 0.0;
+    clamped_spline(gx, gy, (gy[1] - gy[0]) / dr, 0.0, gy2);
+    auto vsr = [&](double r) {
+      if (r >= gx[ng - 1])
+        return 0.0;
+      const int loc     = (int)(r / dr);
+      const double dist = r - gx[loc];
+      const double cl = dist / dr, cr = (dr - dist) / dr, h6 = dr / 6.0;
+      const double q1 = cr * (cr * cr - 1.0) * h6 * dr, q2 = cl * (cl * cl - 1.0) * h6 * dr;
+      return (cr * gy[loc] + cl * gy[loc + 1] + q1 * gy2[loc] + q2 * gy2[loc + 1]) / r;
+    };
+    const double vlr0 = 2.0 * sig / std::sqrt(M_PI), vsk0 = M_PI / (sig * sig * vol);
+    const int N = s.N, ni = s.nion;
+    // ion-ion energy (CoulombPBCAA on the ions)
+    double vii = 0;
+    if (ni > 0)
+    {
+      double sr = 0, lr = 0;
+      for (int a = 1; a < ni; a++)
+        for (int b = 0; b < a; b++)
+        {
+          double d2 = 0;
+          for (int q = 0; q < 3; q++)
+          {
+            const double u  = (ions_h[(size_t)q * ni + b] - ions_h[(size_t)q * ni + a]) * s.Linv[q];
+            const double dd = s.L[q] * (u - std::round(u));
+            d2 += dd * dd;
+          }
+          sr += zion * zion * vsr(std::sqrt(d2));
+        }
+      for (int k = 0; k < nk; k++)
+        lr += fk[k] * (rhoi[k] * rhoi[k] + rhoi[nk + k] * rhoi[nk + k]);
+      vii = sr + 0.5 * lr + ni * (-0.5 * zion * zion * vlr0 - 0.5 * zion * (zion * ni) * vsk0);
+    }
+    coul_k.alloc(kv.size());
+    coul_fk.alloc(nk);
+    coul_rhoi.alloc(rhoi.size());
+    coul_y.alloc(ng);
+    coul_y2.alloc(ng);
+    HIPCHECK(hipMemcpy(coul_k.p, kv.data(), kv.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(coul_fk.p, fk.data(), fk.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(coul_rhoi.p, rhoi.data(), rhoi.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(coul_y.p, gy.data(), ng * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(coul_y2.p, gy2.data(), ng * sizeof(double), hipMemcpyHostToDevice));
+    coul          = CoulombC{};
+    coul.kvec     = coul_k.p;
+    coul.fk       = coul_fk.p;
+    coul.rhoi     = coul_rhoi.p;
+    coul.y        = coul_y.p;
+    coul.y2       = coul_y2.p;
+    coul.nk       = nk;
+    coul.ng       = ng;
+    coul.dr       = dr;
+    coul.drinv    = 1.0 / dr;
+    coul.rmax     = gx[ng - 1];
+    coul.zion     = zion;
+    coul.const_ee = N * (-0.5 * vlr0) - 0.5 * (double)N * (double)N * vsk0;
+    coul.const_ei = (double)N * zion * ni * vsk0;
+    coul.vii      = vii;
+    coul_ready    = true;
   }
 
   // ---------------------------------------------------------------------------------------------
@@ -1645,6 +1808,31 @@ extern "C"
       for (size_t i = 0; i < dout.n; i++)
         if (std::isnan(ratios[i]))
           throw QmiError(QMI_ERR_NUMERIC, "NaN ratio in mw_evaluateRatios");
+    });
+  }
+
+  int qmi_coulomb_setup(qmi_ctx* c, double ion_charge, double dim_cutoff)
+  {
+    return guarded(c, [&] { c->coulomb_setup(ion_charge, dim_cutoff); });
+  }
+
+  int qmi_coulomb_nk(qmi_ctx* c) { return c && c->coul_ready ? c->coul.nk : 0; }
+
+  int qmi_mw_evaluate_potential(qmi_ctx* c, double* v)
+  {
+    return guarded(c, [&] {
+      c->require_walkers();
+      if (!c->coul_ready)
+        throw QmiError(QMI_ERR_ARG, "call qmi_coulomb_setup first");
+      if (!v)
+        throw QmiError(QMI_ERR_ARG, "null output");
+      c->coul_out.alloc((size_t)c->nw * 3);
+      {
+        qmi_ctx::Scoped t(c, "coulomb");
+        HIPCHECK(launch_coulomb(c->sys, c->st, c->coul, c->nw, c->coul_out.p, c->stream));
+      }
+      HIPCHECK(hipMemcpyAsync(v, c->coul_out.p, (size_t)c->nw * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      c->sync();
     });
   }
 

@@ -94,6 +94,22 @@ hipError_t launch_api_spo_vgl_c(const SysC& s, const void* coefs, const double* 
                                 const double* pos, double* out, hipStream_t stream);
 hipError_t launch_api_det_ratios_c(const SysC& s, const void* coefs, const double* kpts, int nw, int nvp, int n,
                                    const double* pos, const double* inv, double* out, hipStream_t stream);
+// Ewald tables of the periodic Coulomb energy (built on the host, device pointers)
+constexpr int COUL_MAX_NG = 1024;
+struct CoulombC
+{
+  const double* kvec; // [3][nk] k vectors with |k| <= k_c (k and -k), k != 0
+  const double* fk;   // [nk] F_k
+  const double* rhoi; // [2][nk] rho_k of the ions (charges included): real, imaginary parts
+  const double* y;    // [ng] r v_sr(r) on the linear grid [0, r_c]
+  const double* y2;   // [ng] its spline second derivatives
+  int nk, ng;
+  double dr, drinv, rmax, zion;
+  double const_ee, const_ei, vii; // constant terms of V_ee and V_eI, the ion-ion energy
+};
+// out[nw][3] = {V_ee, V_eI, V_II}
+hipError_t launch_coulomb(const SysC& s, const State& st, const CoulombC& c, int nw, double* out, hipStream_t stream);
+
 // TrialWaveFunction::mw_evaluateRatios for virtual moves of electron iat: pos [nw][nvp][3] ->
 // out [nw][nvp] (complex pairs for complex orbitals); no delayed update pending for iat's group
 hipError_t launch_api_vp_ratios(const SysC& s, const State& st, const void* coefs, int nw, int iat, int nvp,

@@ -234,6 +234,19 @@ class Context:
         self._check(self.L.qmi_mw_evaluate_ratios(self.h, int(iat), nvp, _p(pos), _p(out)))
         return self._out(out)
 
+    def coulomb_setup(self, ion_charge, dim_cutoff=15.0):
+        self._check(self.L.qmi_coulomb_setup(self.h, C.c_double(ion_charge), C.c_double(dim_cutoff)))
+
+    def coulomb_nk(self):
+        """Number of k vectors of the Ewald long-range sum (after coulomb_setup)."""
+        return int(self.L.qmi_coulomb_nk(self.h))
+
+    def mw_evaluatePotential(self):
+        """[nw][3]: electron-electron, electron-ion, ion-ion Coulomb energy (Ewald)."""
+        v = np.empty((self.nw, 3))
+        self._check(self.L.qmi_mw_evaluate_potential(self.h, _p(v)))
+        return v
+
     def mw_distanceRows(self, pos, n_ion):
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.nw, 3)
         ee_r, ee_dr = np.empty((self.nw, self.N)), np.empty((self.nw, self.N, 3))
