@@ -302,7 +302,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50, help="timed sweeps (the workload's 50)")
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=100,
+                    help="untimed equilibration sweeps from the uniform start; with fewer than ~100 the "
+                         "energies of the variants still drift and their comparison is biased")
     ap.add_argument("--walkers", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-walkers", type=int, default=1024)

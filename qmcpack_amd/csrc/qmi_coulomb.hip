@@ -10,25 +10,36 @@
 namespace qmi
 {
 
-// v_sr(r) from the spline values y / second derivatives y2 on the grid of spacing dr (ng points)
-__device__ inline double ewald_vsr(const CoulombC& c, const double* __restrict__ y, const double* __restrict__ y2,
-                                   double r)
+// v_sr(r) for r^2 = r2 < r_c^2: the spline interval's cubic in t = (r - i dr) / dr, divided by r
+__device__ inline double ewald_vsr(const CoulombC& c, const double4* __restrict__ sp, double r2)
 {
-  if (r >= c.rmax)
-    return 0.0;
-  const int loc     = (int)(r * c.drinv);
-  const double dist = r - loc * c.dr;
-  const double cl = dist * c.drinv, cr = (c.dr - dist) * c.drinv, h6 = c.dr / 6.0;
-  const double q1 = cr * (cr * cr - 1.0) * h6 * c.dr, q2 = cl * (cl * cl - 1.0) * h6 * c.dr;
-  return (cr * y[loc] + cl * y[loc + 1] + q1 * y2[loc] + q2 * y2[loc + 1]) / r;
+  const double r    = sqrt(r2);
+  const int loc     = min((int)(r * c.drinv), c.ng - 2); // r < r_c; rounding can reach ng - 1
+  const double t    = (r - loc * c.dr) * c.drinv;
+  const double4 q   = sp[loc];
+  return (q.x + t * (q.y + t * (q.z + t * q.w))) / r;
+}
+
+__device__ inline double min_image2(const SysC& s, double ax, double ay, double az, double bx, double by, double bz)
+{
+  const double d[3] = {ax - bx, ay - by, az - bz};
+  double r2         = 0;
+#pragma unroll
+  for (int q = 0; q < 3; q++)
+  {
+    const double u  = d[q] * s.Linv[q];
+    const double dd = s.L[q] * (u - rint(u));
+    r2 += dd * dd;
+  }
+  return r2;
 }
 
 // out[w][3] = {V_ee, V_eI, V_II}; electrons carry charge -1, ions c.zion.  LDS: electron positions
-// (N <= 1024) and the spline (ng <= COUL_MAX_NG).
+// (N <= 1024) and the spline (ng <= COUL_MAX_NG).  Periodic cells only (the host checks).
 __global__ __launch_bounds__(WG) void k_coulomb(SysC s, State st, CoulombC c, double* __restrict__ out)
 {
   __shared__ double px[1024], py[1024], pz[1024];
-  __shared__ double sy[COUL_MAX_NG], sy2[COUL_MAX_NG];
+  __shared__ double4 sp[COUL_MAX_NG - 1];
   __shared__ double red[4 * WG];
   __shared__ double bc[4];
   const int w = blockIdx.x, t = threadIdx.x;
@@ -40,13 +51,12 @@ __global__ __launch_bounds__(WG) void k_coulomb(SysC s, State st, CoulombC c, do
     py[i] = R[Np + i];
     pz[i] = R[2 * Np + i];
   }
-  for (int i = t; i < c.ng; i += WG)
-  {
-    sy[i]  = c.y[i];
-    sy2[i] = c.y2[i];
-  }
+  const double4* gp = reinterpret_cast<const double4*>(c.poly);
+  for (int i = t; i < c.ng - 1; i += WG)
+    sp[i] = gp[i];
   __syncthreads();
-  double acc[4] = {0, 0, 0, 0}; // short range e-e, e-I; long range e-e, e-I
+  const double rc2 = c.rmax * c.rmax;
+  double acc[4]    = {0, 0, 0, 0}; // short range e-e, e-I; long range e-e, e-I
   // e-e pairs j < i, rows i and N - 1 - i together (equal work per thread)
   const int half = (N + 1) / 2;
   for (int rr = t; rr < half; rr += WG)
@@ -56,13 +66,13 @@ __global__ __launch_bounds__(WG) void k_coulomb(SysC s, State st, CoulombC c, do
       const int i = pass == 0 ? rr : N - 1 - rr;
       if (pass == 1 && i == rr)
         break;
-      const double ri[3] = {px[i], py[i], pz[i]};
-      double esum        = 0;
+      const double xi = px[i], yi = py[i], zi = pz[i];
+      double esum     = 0;
       for (int j = 0; j < i; j++)
       {
-        double dr[3];
-        const double r = min_image(s, px[j], py[j], pz[j], ri, dr);
-        esum += ewald_vsr(c, sy, sy2, r);
+        const double r2 = min_image2(s, px[j], py[j], pz[j], xi, yi, zi);
+        if (r2 < rc2)
+          esum += ewald_vsr(c, sp, r2);
       }
       acc[0] += esum; // q_i q_j = 1
     }
@@ -70,13 +80,13 @@ __global__ __launch_bounds__(WG) void k_coulomb(SysC s, State st, CoulombC c, do
   // e-I pairs
   for (int i = t; i < N; i += WG)
   {
-    const double ri[3] = {px[i], py[i], pz[i]};
-    double esum        = 0;
+    const double xi = px[i], yi = py[i], zi = pz[i];
+    double esum     = 0;
     for (int j = 0; j < ni; j++)
     {
-      double dr[3];
-      const double r = min_image(s, st.ions[j], st.ions[ni + j], st.ions[2 * ni + j], ri, dr);
-      esum += ewald_vsr(c, sy, sy2, r);
+      const double r2 = min_image2(s, st.ions[j], st.ions[ni + j], st.ions[2 * ni + j], xi, yi, zi);
+      if (r2 < rc2)
+        esum += ewald_vsr(c, sp, r2);
     }
     acc[1] += -c.zion * esum;
   }
@@ -106,7 +116,7 @@ __global__ __launch_bounds__(WG) void k_coulomb(SysC s, State st, CoulombC c, do
 
 hipError_t launch_coulomb(const SysC& s, const State& st, const CoulombC& c, int nw, double* out, hipStream_t stream)
 {
-  if (s.N > 1024 || c.ng > COUL_MAX_NG)
+  if (s.N > 1024 || c.ng > COUL_MAX_NG || c.ng < 2)
     return hipErrorInvalidValue;
   k_coulomb<<<nw, WG, 0, stream>>>(s, st, c, out);
   return hipGetLastError();

@@ -706,7 +706,7 @@ struct qmi_ctx
   // clamped cubic spline of r erfc(sigma r) / r on 1001 points (LRCoulombSingleton.cpp:137-158).
   bool coul_ready = false;
   CoulombC coul{};
-  DevBuf<double> coul_k, coul_fk, coul_rhoi, coul_y, coul_y2, coul_out;
+  DevBuf<double> coul_k, coul_fk, coul_rhoi, coul_poly, coul_out;
 
   static void clamped_spline(const std::vector<double>& x, const std::vector<double>& y, double yp0, double ypn,
                              std::vector<double>& y2)
@@ -746,6 +746,7 @@ struct qmi_ctx
     struct K
     {
       double x, y, z, k2;
+      int i, j, k;
     };
     std::vector<K> ks;
     int mmax[3];
@@ -760,33 +761,50 @@ struct qmi_ctx
           const double x = 2 * M_PI * i / L[0], y = 2 * M_PI * j / L[1], z = 2 * M_PI * k / L[2];
           const double k2 = x * x + y * y + z * z;
           if (k2 <= kc * kc)
-            ks.push_back({x, y, z, k2});
+            ks.push_back({x, y, z, k2, i, j, k});
         }
     std::stable_sort(ks.begin(), ks.end(),
                      [](const K& a, const K& b) { return std::llround(a.k2 * 1e8) < std::llround(b.k2 * 1e8); });
-    const int nk = (int)ks.size();
+    // F_k by shell over the full list (the reference's grouping), then one k of each pair (k, -k):
+    // for real charges rho_{-k} = conj(rho_k), so the pair contributes twice the kept term
+    std::vector<double> fall(ks.size());
+    {
+      const double kgauss = 1.0 / (4 * sig * sig), knorm = 4 * M_PI / vol;
+      long long key = -1;
+      double fshell = 0;
+      for (size_t k = 0; k < ks.size(); k++)
+      {
+        const long long kk = std::llround(ks[k].k2 * 1e8);
+        if (kk != key)
+        {
+          key    = kk;
+          fshell = knorm * std::exp(-ks[k].k2 * kgauss) / ks[k].k2;
+        }
+        fall[k] = fshell;
+      }
+    }
+    std::vector<K> kh;
+    std::vector<double> fh;
+    for (size_t k = 0; k < ks.size(); k++)
+      if (ks[k].i > 0 || (ks[k].i == 0 && (ks[k].j > 0 || (ks[k].j == 0 && ks[k].k > 0))))
+      {
+        kh.push_back(ks[k]);
+        fh.push_back(2.0 * fall[k]);
+      }
+    const int nk = (int)kh.size();
     std::vector<double> kv(3 * (size_t)nk), fk(nk), rhoi(2 * (size_t)nk, 0.0);
-    const double kgauss = 1.0 / (4 * sig * sig), knorm = 4 * M_PI / vol;
-    long long key = -1;
-    double fshell = 0;
     std::vector<double> ions_h(3 * (size_t)s.nion);
     if (s.nion)
       HIPCHECK(hipMemcpy(ions_h.data(), ions.p, ions_h.size() * sizeof(double), hipMemcpyDeviceToHost));
     for (int k = 0; k < nk; k++)
     {
-      const long long kk = std::llround(ks[k].k2 * 1e8);
-      if (kk != key)
-      {
-        key    = kk;
-        fshell = knorm * std::exp(-ks[k].k2 * kgauss) / ks[k].k2;
-      }
-      kv[k]                   = ks[k].x;
-      kv[nk + k]              = ks[k].y;
-      kv[2 * (size_t)nk + k]  = ks[k].z;
-      fk[k]                   = fshell;
+      kv[k]                  = kh[k].x;
+      kv[nk + k]             = kh[k].y;
+      kv[2 * (size_t)nk + k] = kh[k].z;
+      fk[k]                  = fh[k];
       for (int a = 0; a < s.nion; a++)
       {
-        const double ph = ks[k].x * ions_h[a] + ks[k].y * ions_h[s.nion + a] + ks[k].z * ions_h[2 * s.nion + a];
+        const double ph = kh[k].x * ions_h[a] + kh[k].y * ions_h[s.nion + a] + kh[k].z * ions_h[2 * s.nion + a];
         rhoi[k] += zion * std::cos(ph);
         rhoi[nk + k] += zion * std::sin(ph);
       }
@@ -833,22 +851,30 @@ struct qmi_ctx
         lr += fk[k] * (rhoi[k] * rhoi[k] + rhoi[nk + k] * rhoi[nk + k]);
       vii = sr + 0.5 * lr + ni * (-0.5 * zion * zion * vlr0 - 0.5 * zion * (zion * ni) * vsk0);
     }
+    // r v_sr on interval i as a cubic in t = (r - i dr) / dr: the spline's
+    // (1 - t) y_i + t y_{i+1} + h ((1 - t)^3 - (1 - t)) y2_i + h (t^3 - t) y2_{i+1}, h = dr^2 / 6
+    std::vector<double> poly(4 * (size_t)(ng - 1));
+    const double hh = dr * dr / 6.0;
+    for (int i = 0; i < ng - 1; i++)
+    {
+      poly[4 * i + 0] = gy[i];
+      poly[4 * i + 1] = (gy[i + 1] - gy[i]) - hh * (2.0 * gy2[i] + gy2[i + 1]);
+      poly[4 * i + 2] = 3.0 * hh * gy2[i];
+      poly[4 * i + 3] = hh * (gy2[i + 1] - gy2[i]);
+    }
     coul_k.alloc(kv.size());
     coul_fk.alloc(nk);
     coul_rhoi.alloc(rhoi.size());
-    coul_y.alloc(ng);
-    coul_y2.alloc(ng);
+    coul_poly.alloc(poly.size());
     HIPCHECK(hipMemcpy(coul_k.p, kv.data(), kv.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(coul_fk.p, fk.data(), fk.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(coul_rhoi.p, rhoi.data(), rhoi.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy(coul_y.p, gy.data(), ng * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy(coul_y2.p, gy2.data(), ng * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(coul_poly.p, poly.data(), poly.size() * sizeof(double), hipMemcpyHostToDevice));
     coul          = CoulombC{};
     coul.kvec     = coul_k.p;
     coul.fk       = coul_fk.p;
     coul.rhoi     = coul_rhoi.p;
-    coul.y        = coul_y.p;
-    coul.y2       = coul_y2.p;
+    coul.poly     = coul_poly.p;
     coul.nk       = nk;
     coul.ng       = ng;
     coul.dr       = dr;

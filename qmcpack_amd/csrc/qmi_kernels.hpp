@@ -98,11 +98,10 @@ hipError_t launch_api_det_ratios_c(const SysC& s, const void* coefs, const doubl
 constexpr int COUL_MAX_NG = 1024;
 struct CoulombC
 {
-  const double* kvec; // [3][nk] k vectors with |k| <= k_c (k and -k), k != 0
-  const double* fk;   // [nk] F_k
+  const double* kvec; // [3][nk] one k of every pair (k, -k) with |k| <= k_c, k != 0
+  const double* fk;   // [nk] 2 F_k (the pair's two terms are equal for real charges)
   const double* rhoi; // [2][nk] rho_k of the ions (charges included): real, imaginary parts
-  const double* y;    // [ng] r v_sr(r) on the linear grid [0, r_c]
-  const double* y2;   // [ng] its spline second derivatives
+  const double* poly; // [ng - 1][4] r v_sr(r) = a + t (b + t (c + t d)) on [i dr, (i + 1) dr]
   int nk, ng;
   double dr, drinv, rmax, zion;
   double const_ee, const_ei, vii; // constant terms of V_ee and V_eI, the ion-ion energy
