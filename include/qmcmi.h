@@ -159,6 +159,35 @@ int qmi_mw_evaluate_ratios(qmi_ctx* ctx, int iat, int nvp, const double* pos, do
  * (the reference's LR_dim_cutoff, 15 by default).  Periodic orthorhombic cells; tables built once
  * per system. */
 int qmi_coulomb_setup(qmi_ctx* ctx, double ion_charge, double dim_cutoff);
+/* DMC driver: DMCBatched::advanceWalkers / runDMCStep (QMCDrivers/DMC/DMCBatched.cpp:67-414),
+ * SFNBranch (QMCDrivers/SFNBranch.{h,cpp}: branch weight, trial / reference energy, effective
+ * time step, branching cutoff) and WalkerControl::branch with a fluctuating population
+ * (QMCDrivers/DMC/WalkerControl.cpp:150-260).  Needs one walker per crowd and
+ * qmi_coulomb_setup; the local energy is kinetic + V_ee + V_eI + V_II. */
+typedef struct qmi_dmc_params
+{
+  double tau;             /* time step */
+  double feedback;        /* population feedback (DMC input "feedback", 1.0) */
+  double sigma_bound;     /* SFNBranch sigmaBound (10) */
+  int warmup_steps;       /* SFNBranch warmupSteps (200) */
+  int refenergy_history;  /* 0: limited_history (default), 1: unlimited_history */
+  int cutoff_scheme;      /* branching_cutoff_scheme: 0 classic, 1 DRV, 2 ZSGMA, 3 YL */
+  int capacity;           /* walker slots; a population beyond it is an error */
+  int target_walkers;     /* 0: the population at qmi_dmc_init */
+  int recompute_period;   /* full re-inversion every this many steps (0: never) */
+  uint32_t seed;          /* walker slot w draws from seed + w, walker control from seed - 1 */
+} qmi_dmc_params;
+#define QMI_DMC_STATS 10
+/* re-evaluates every walker (log psi, local energy), sets E_T = E_ref = mean energy */
+int qmi_dmc_init(qmi_ctx* ctx, const qmi_dmc_params* params);
+/* stats[steps][QMI_DMC_STATS] per step: weighted mean energy and variance before branching,
+ * total weight, walkers before and after branching, E_trial and E_ref after the update,
+ * effective time step, accepted and rejected moves */
+int qmi_dmc_run(qmi_ctx* ctx, int steps, double* stats);
+/* current population (walkers in slots 0 .. n-1) */
+int qmi_num_walkers(qmi_ctx* ctx);
+/* weights[n], local energies[n] of the current population (nullable outputs) */
+int qmi_dmc_walker_data(qmi_ctx* ctx, double* weights, double* energies);
 /* number of k vectors of the long-range sum (0 before qmi_coulomb_setup) */
 int qmi_coulomb_nk(qmi_ctx* ctx);
 /* v[nw][3] = {electron-electron, electron-ion, ion-ion} energy of every walker */

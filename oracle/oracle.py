@@ -252,6 +252,13 @@ class Det:
         return ainv, complex(ld[0], ld[1])
 
 
+class OrcDmcParams(C.Structure):
+    _fields_ = [("tau", C.c_double), ("feedback", C.c_double), ("sigma_bound", C.c_double),
+                ("warmup_steps", C.c_int), ("refenergy_history", C.c_int), ("cutoff_scheme", C.c_int),
+                ("capacity", C.c_int), ("target_walkers", C.c_int), ("recompute_period", C.c_int),
+                ("seed", C.c_uint32)]
+
+
 class VMC:
     """Wave function + VMCBatched driver over nw walkers split in crowds."""
 
@@ -287,6 +294,26 @@ class VMC:
         if st != 0:
             raise RuntimeError("oracle evaluate_log failed")
         return self._out(logpsi), self._out(G), self._out(L)
+
+    def dmc_init(self, ion_charge, tau, capacity, seed, feedback=1.0, sigma_bound=10.0, warmup_steps=200,
+                 refenergy_history=0, cutoff_scheme=0, target_walkers=0, recompute_period=0, dim_cutoff=15.0):
+        p = OrcDmcParams(tau, feedback, sigma_bound, warmup_steps, refenergy_history, cutoff_scheme, capacity,
+                         target_walkers, recompute_period, seed)
+        if lib().orc_dmc_init(self.h, C.c_double(ion_charge), C.c_double(dim_cutoff), C.byref(p)) != 0:
+            raise RuntimeError("oracle dmc_init failed")
+
+    def dmc_run(self, steps):
+        out = np.empty((steps, 10))
+        st = lib().orc_dmc_run(self.h, int(steps), ptr(out))
+        if st != 0:
+            raise RuntimeError(f"oracle dmc_run failed ({st})")
+        self.nw = int(lib().orc_num_walkers(self.h))
+        return out
+
+    def dmc_walker_data(self):
+        w, e = np.empty(self.nw), np.empty(self.nw)
+        lib().orc_dmc_walker_data(self.h, ptr(w), ptr(e))
+        return w, e
 
     def ratio_grad(self, iw, iat, newpos):
         newpos = np.ascontiguousarray(newpos, dtype=np.float64)

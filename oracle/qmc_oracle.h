@@ -95,6 +95,20 @@ int orc_vmc_replay(void* h, int steps, int sub_steps, double tau, int recompute_
 /* R[nw][N][3], logpsi[nw], G, L, stats[2] = {accepted, rejected} (nullable each) */
 int orc_vmc_get(void* h, double* R, double* logpsi, double* G, double* L, double* stats);
 
+/* DMC (DMCBatched, SFNBranch, WalkerControl with a fluctuating population) on the walkers of an
+ * orc_vmc_create handle with one walker per crowd.  Mirrors qmi_dmc_params. */
+typedef struct
+{
+  double tau, feedback, sigma_bound;
+  int warmup_steps, refenergy_history, cutoff_scheme, capacity, target_walkers, recompute_period;
+  uint32_t seed;
+} orc_dmc_params;
+int orc_dmc_init(void* h, double ion_charge, double dim_cutoff, const orc_dmc_params* p);
+/* stats[steps][10]: as qmi_dmc_run */
+int orc_dmc_run(void* h, int steps, double* stats);
+int orc_num_walkers(void* h);
+int orc_dmc_walker_data(void* h, double* weights, double* energies);
+
 #ifdef __cplusplus
 }
 #endif

@@ -595,7 +595,9 @@ __global__ __launch_bounds__(WG) void k_move_one_c(SysC s, State st, const doubl
     double ratio_abs;
     bool flag;
     const double pe = mh_prob_c(s, iat, a, v.jas, sh, ratio_abs, flag);
-    bool acc        = false;
+    if (a.dmc) // complex wave function: no node check (SFNBranch.h:161-169), prob_eff against eps
+      flag = pe >= DBL_EPS;
+    bool acc = false;
     if (flag)
     {
       const double u = a.unif[(size_t)w * N + ucount];
@@ -603,6 +605,8 @@ __global__ __launch_bounds__(WG) void k_move_one_c(SysC s, State st, const doubl
       acc = u < pe;
     }
     sh.bc[BC_ACC] = acc ? 1.0 : 0.0;
+    if (a.dmc)
+      dmc_record(a, iat, w, acc);
     v.stats[acc ? 0 : 1] += 1;
     a.ubase[(size_t)w * (N + 1) + iat + 1] = ucount;
     if (a.trace)

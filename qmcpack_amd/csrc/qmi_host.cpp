@@ -8,6 +8,8 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <array>
+#include <deque>
 #include <chrono>
 #include <cstdlib>
 #include <cmath>
@@ -165,6 +167,7 @@ struct qmi_ctx
   DevBuf<double> coefs, ions, kpts;
   // walkers
   int nw = 0, wpc = 1, ncrowd = 0;
+  int cap = 0; // walker slots allocated (>= nw)
   DevBuf<double> R, jas, logdet, stats, invRow, phi, mv, flushT, out, gl3, Gbuf, Lbuf, hostin;
   // determinant engine, in elements of the determinant storage type (double or float)
   DevBuf<double> Ainv[2], dpsi[2], U[2], V[2], Binv[2];
@@ -419,7 +422,8 @@ struct qmi_ctx
   }
 
   // ---------------------------------------------------------------------------------------------
-  void walkers_load(int nw_in, const double* Rin, int wpc_in)
+  // capacity: walker slots to allocate (>= nw; DMC populations grow into them)
+  void walkers_load(int nw_in, const double* Rin, int wpc_in, int capacity = 0)
   {
     if (!have_system)
       throw QmiError(QMI_ERR_ARG, "system not created");
@@ -427,58 +431,60 @@ struct qmi_ctx
       throw QmiError(QMI_ERR_ARG, "need at least one walker");
     const SysC& s = sys;
     nw            = nw_in;
+    cap           = std::max(nw, capacity);
     wpc           = std::max(1, std::min(wpc_in, nw));
     ncrowd        = (nw + wpc - 1) / wpc;
+    dmc.on        = false;
     const int Np = s.Npad, ld = s.ld, k = s.k;
-    R.alloc((size_t)nw * 3 * Np);
+    R.alloc((size_t)cap * 3 * Np);
     {
-      std::vector<double> t((size_t)nw * 3 * Np, 0.0);
+      std::vector<double> t((size_t)cap * 3 * Np, 0.0);
       for (int w = 0; w < nw; w++)
         for (int i = 0; i < s.N; i++)
           for (int q = 0; q < 3; q++)
             t[((size_t)w * 3 + q) * Np + i] = Rin[((size_t)w * s.N + i) * 3 + q];
       HIPCHECK(hipMemcpy(R.p, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
     }
-    jas.alloc((size_t)nw * J_ROWS * Np);
+    jas.alloc((size_t)cap * J_ROWS * Np);
     HIPCHECK(hipMemset(jas.p, 0, jas.n * sizeof(double)));
-    logdet.alloc((size_t)nw * 4);
+    logdet.alloc((size_t)cap * 4);
     HIPCHECK(hipMemset(logdet.p, 0, logdet.n * sizeof(double)));
-    stats.alloc((size_t)nw * 2);
+    stats.alloc((size_t)cap * 2);
     HIPCHECK(hipMemset(stats.p, 0, stats.n * sizeof(double)));
-    mv.alloc((size_t)nw * MV_SIZE);
+    mv.alloc((size_t)cap * MV_SIZE);
     HIPCHECK(hipMemset(mv.p, 0, mv.n * sizeof(double)));
-    out.alloc((size_t)nw * 8 + 16);
-    gl3.alloc((size_t)nw * 3);
-    hostin.alloc((size_t)nw * 3 + 16);
-    acc.alloc(nw);
-    ubase.alloc((size_t)ncrowd * (s.N + 1));
-    lstate.alloc((size_t)nw * LS_SIZE);
-    lfsum.alloc((size_t)nw * 2);
+    out.alloc((size_t)cap * 8 + 16);
+    gl3.alloc((size_t)cap * 3);
+    hostin.alloc((size_t)cap * 3 + 16);
+    acc.alloc(cap);
+    ubase.alloc((size_t)((cap + wpc - 1) / wpc) * (s.N + 1));
+    lstate.alloc((size_t)cap * LS_SIZE);
+    lfsum.alloc((size_t)cap * 2);
     HIPCHECK(hipMemset(lfsum.p, 0, lfsum.n * sizeof(double)));
     learn_moves = 0;
     if (s.has_det)
     {
-      invRow.alloc((size_t)nw * ld * vw());
-      phi.alloc((size_t)nw * 5 * ld * vw());
+      invRow.alloc((size_t)cap * ld * vw());
+      phi.alloc((size_t)cap * 5 * ld * vw());
       for (int g = 0; g < 2; g++)
       {
         const int n = s.n[g];
-        Ainv[g].alloc(det_words((size_t)nw * std::max(n, 1) * ld));
-        dpsi[g].alloc(det_words((size_t)nw * std::max(n, 1) * 4 * ld));
-        U[g].alloc(det_words((size_t)nw * ld * k));
-        V[g].alloc(det_words((size_t)nw * ld * k));
-        Binv[g].alloc(det_words((size_t)nw * k * k));
-        dlist[g].alloc((size_t)nw * k);
+        Ainv[g].alloc(det_words((size_t)cap * std::max(n, 1) * ld));
+        dpsi[g].alloc(det_words((size_t)cap * std::max(n, 1) * 4 * ld));
+        U[g].alloc(det_words((size_t)cap * ld * k));
+        V[g].alloc(det_words((size_t)cap * ld * k));
+        Binv[g].alloc(det_words((size_t)cap * k * k));
+        dlist[g].alloc((size_t)cap * k);
         HIPCHECK(hipMemset(U[g].p, 0, U[g].n * sizeof(double)));
         HIPCHECK(hipMemset(V[g].p, 0, V[g].n * sizeof(double)));
         HIPCHECK(hipMemset(Binv[g].p, 0, Binv[g].n * sizeof(double)));
       }
       if (det_fp32())
-        lu.alloc((size_t)nw * std::max(s.nup, s.ndn) * ld);
-      ipiv.alloc((size_t)nw * std::max(s.nup, s.ndn));
-      info.alloc(nw);
+        lu.alloc((size_t)cap * std::max(s.nup, s.ndn) * ld);
+      ipiv.alloc((size_t)cap * std::max(s.nup, s.ndn));
+      info.alloc(cap);
       if (k > 1)
-        flushT.alloc((size_t)nw * ld * k * vw()); // T (n x k) or BV_T (ld x k) per walker
+        flushT.alloc((size_t)cap * ld * k * vw()); // T (n x k) or BV_T (ld x k) per walker
     }
     st        = State{};
     st.R      = R.p;
@@ -524,10 +530,10 @@ struct qmi_ctx
       const int ld = sys.ld;
       for (int g = 0; g < 2; g++)
       {
-        U[g].alloc(det_words((size_t)nw * ld * k));
-        V[g].alloc(det_words((size_t)nw * ld * k));
-        Binv[g].alloc(det_words((size_t)nw * k * k));
-        dlist[g].alloc((size_t)nw * k);
+        U[g].alloc(det_words((size_t)cap * ld * k));
+        V[g].alloc(det_words((size_t)cap * ld * k));
+        Binv[g].alloc(det_words((size_t)cap * k * k));
+        dlist[g].alloc((size_t)cap * k);
         HIPCHECK(hipMemset(U[g].p, 0, U[g].n * sizeof(double)));
         HIPCHECK(hipMemset(V[g].p, 0, V[g].n * sizeof(double)));
         HIPCHECK(hipMemset(Binv[g].p, 0, Binv[g].n * sizeof(double)));
@@ -537,7 +543,7 @@ struct qmi_ctx
         st.dlist[g] = dlist[g].p;
       }
       if (k > 1)
-        flushT.alloc((size_t)nw * ld * k * vw());
+        flushT.alloc((size_t)cap * ld * k * vw());
     }
     invrow_id = -1;
     invalidate_graph();
@@ -610,7 +616,7 @@ struct qmi_ctx
   PinnedBuf<double> h_gl3;
   void eval_gl_enqueue()
   {
-    h_gl3.alloc((size_t)nw * 3);
+    h_gl3.alloc((size_t)cap * 3);
     {
       Scoped t(this, "eval_gl");
       HIPCHECK(cplx ? launch_eval_gl_c(sys, st, nw, gl3.p, nullptr, nullptr, stream)
@@ -888,6 +894,367 @@ struct qmi_ctx
   }
 
   // ---------------------------------------------------------------------------------------------
+  // DMC: DMCBatched::advanceWalkers / runDMCStep (QMCDrivers/DMC/DMCBatched.cpp:67-414) with the
+  // branch engine SFNBranch (SFNBranch.h:161-208, SFNBranch.cpp:93-245, 284-317) and the
+  // fluctuating-population walker control (WalkerControl.cpp:150-260, MCPopulation.cpp:34-66).
+  // One walker per crowd: each walker slot owns an RNG stream (seed + slot, created when the slot
+  // is first used); walker control draws from its own stream (seed - 1).  A branched copy takes
+  // the parent's whole state (positions, inverses, orbital derivatives, Jastrow accumulators,
+  // log det) where the reference reloads and recomputes it: the same values to rounding.
+  struct DmcState
+  {
+    bool on = false;
+    qmi_dmc_params p{};
+    long long iter = 0;
+    std::vector<double> weight, eloc; // per slot
+    std::mt19937 wc_rng;
+    double tau_eff = 0, etrial = 0, eref = 0, sigma2 = 0, enow = 0;
+    double cutoff = 0, branch_max = 0, filter = 0;
+    double r2acc = 0, r2prop = 0; // accumulator sums (SFNBranch R2Accepted / R2Proposed)
+    int warmup_todo = 0, etrial_todo = 1, target = 0;
+    bool main_stage = false;
+    std::deque<std::array<double, 3>> hist; // limited history: weight, energy, variance
+    size_t hist_limit = 1;
+    double ehist = 0, vhist = 0, nhist = 0; // unlimited history sums
+  } dmc;
+  DevBuf<double> dmc_rr;
+  DevBuf<int> dmc_pairs;
+  PinnedBuf<int> h_pairs;
+  PinnedBuf<double> h_rr, h_pot;
+
+  // SFNBranch::setBranchCutoff (SFNBranch.cpp:284-317)
+  void dmc_set_cutoff(double variance, double target_sigma, double max_sigma, int nelec)
+  {
+    const double tau = dmc.p.tau;
+    switch (dmc.p.cutoff_scheme)
+    {
+    case 1: dmc.cutoff = 2.0 / std::sqrt(tau); break;
+    case 2:
+      if (nelec > 0)
+        dmc.cutoff = 0.2 * std::sqrt(nelec / tau);
+      break;
+    case 3: dmc.cutoff = std::sqrt(variance) * std::min(target_sigma, std::sqrt(1.0 / tau)); break;
+    default: dmc.cutoff = std::min(std::max(variance * target_sigma, max_sigma), 2.5 / tau); break;
+    }
+    dmc.branch_max = dmc.cutoff * 1.5;
+    dmc.filter     = 1.0 / (dmc.branch_max - dmc.cutoff);
+  }
+
+  // SFNBranch::branchWeight (SFNBranch.h:199-208)
+  double dmc_branch_weight(double enew, double eold) const
+  {
+    double taueff  = dmc.tau_eff * 0.5;
+    const double x = std::max(dmc.eref - enew, dmc.eref - eold);
+    if (x > dmc.branch_max)
+      taueff = 0.0;
+    else if (x > dmc.cutoff)
+      taueff *= (1.0 - (x - dmc.cutoff) * dmc.filter);
+    return std::exp(taueff * (dmc.etrial * 2.0 - enew - eold));
+  }
+
+  // local energies of the walkers: kinetic (evaluateGL) + V_ee + V_eI + V_II, summed in that order
+  void dmc_local_energies(std::vector<double>& e, bool with_rr)
+  {
+    eval_gl_enqueue();
+    if (coul_out.n < (size_t)cap * 3)
+      coul_out.alloc((size_t)cap * 3);
+    h_pot.alloc((size_t)cap * 3);
+    {
+      Scoped t(this, "coulomb");
+      HIPCHECK(launch_coulomb(sys, st, coul, nw, coul_out.p, stream));
+    }
+    HIPCHECK(hipMemcpyAsync(h_pot.p, coul_out.p, (size_t)nw * 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    if (with_rr)
+      HIPCHECK(hipMemcpyAsync(h_rr.p, dmc_rr.p, (size_t)nw * 2 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    sync();
+    e.resize(nw);
+    eval_gl_finish(e.data());
+    for (int w = 0; w < nw; w++)
+    {
+      const double* v = h_pot.p + (size_t)w * 3;
+      e[w]            = e[w] + v[0] + v[1] + v[2];
+    }
+  }
+
+  void dmc_init(const qmi_dmc_params& p)
+  {
+    require_walkers();
+    if (wpc != 1)
+      throw QmiError(QMI_ERR_UNSUPPORTED, "DMC needs one walker per crowd (one RNG stream per walker)");
+    if (learned)
+      throw QmiError(QMI_ERR_UNSUPPORTED, "DMC with the learned proposal is not built");
+    if (!coul_ready)
+      throw QmiError(QMI_ERR_ARG, "DMC needs the Coulomb energy: call qmi_coulomb_setup first");
+    if (!(p.tau > 0) || !(p.feedback > 0) || p.warmup_steps < 0 || p.capacity < nw || p.recompute_period < 0 ||
+        p.cutoff_scheme < 0 || p.cutoff_scheme > 3 || p.target_walkers < 0)
+      throw QmiError(QMI_ERR_ARG, "invalid DMC parameters (tau, feedback > 0; capacity >= walkers)");
+    if (dcount[0] || dcount[1])
+      throw QmiError(QMI_ERR_ARG, "DMC start with pending delayed updates");
+    if (cap < p.capacity)
+    {
+      // re-allocate the walker slots for the capacity, keeping the walkers' positions
+      const int N = sys.N, Np = sys.Npad, n0 = nw;
+      std::vector<double> t((size_t)nw * 3 * Np), Rh((size_t)nw * N * 3);
+      HIPCHECK(hipMemcpy(t.data(), R.p, t.size() * sizeof(double), hipMemcpyDeviceToHost));
+      for (int w = 0; w < nw; w++)
+        for (int i = 0; i < N; i++)
+          for (int q = 0; q < 3; q++)
+            Rh[((size_t)w * N + i) * 3 + q] = t[((size_t)w * 3 + q) * Np + i];
+      walkers_load(n0, Rh.data(), 1, p.capacity);
+    }
+    invalidate_graph();
+    // initialLogEvaluation: recompute the wave function, local energies
+    {
+      Scoped t(this, "jastrow_recompute");
+      HIPCHECK(launch_jastrow_recompute(sys, st, nw, stream));
+    }
+    recompute_dets();
+    dmc_rr.alloc((size_t)cap * 2);
+    dmc_pairs.alloc((size_t)cap * 2);
+    h_pairs.alloc((size_t)cap * 2);
+    h_rr.alloc((size_t)cap * 2);
+    dmc = DmcState{};
+    dmc.p = p;
+    std::vector<double> e;
+    dmc_local_energies(e, false);
+    rng.clear();
+    for (int c = 0; c < nw; c++)
+      rng.emplace_back(p.seed + uint32_t(c));
+    rng_seed   = p.seed;
+    rng_seeded = true;
+    dmc.wc_rng.seed(p.seed - 1u);
+    dmc.weight.assign(cap, 1.0);
+    dmc.eloc.assign(cap, 0.0);
+    std::copy(e.begin(), e.end(), dmc.eloc.begin());
+    // MCPopulation::measureGlobalEnergyVariance + SFNBranch::initParam
+    double ws = 0, we = 0, wee = 0;
+    for (int w = 0; w < nw; w++)
+    {
+      ws += dmc.weight[w];
+      we += dmc.weight[w] * e[w];
+      wee += dmc.weight[w] * e[w] * e[w];
+    }
+    const double ene = we / ws, var = wee / ws - ene * ene;
+    dmc.r2acc   = 1.0e-10;
+    dmc.r2prop  = 1.0e-10;
+    dmc.etrial  = ene;
+    dmc.eref    = ene;
+    dmc.sigma2  = var;
+    dmc.tau_eff = p.tau * dmc.r2acc / dmc.r2prop;
+    dmc_set_cutoff(var, p.sigma_bound, 50, sys.N);
+    dmc.target      = p.target_walkers > 0 ? p.target_walkers : nw;
+    dmc.warmup_todo = p.warmup_steps;
+    dmc.main_stage  = p.warmup_steps == 0;
+    dmc.etrial_todo = 1;
+    dmc.hist_limit  = (size_t)std::max(1, (int)(1.0 / (p.feedback * p.tau)));
+    dmc.iter        = 0;
+    dmc.on          = true;
+  }
+
+  // one DMC step; out[QMI_DMC_STATS]
+  void dmc_step(double* out)
+  {
+    const SysC& s = sys;
+    const int N   = s.N;
+    const double tau = dmc.p.tau;
+    const int nw0 = nw;
+    // advanceWalkers: one sweep with the DMC acceptance
+    substep_buffers(false);
+    rng_generate();
+    HIPCHECK(hipMemsetAsync(dmc_rr.p, 0, (size_t)nw * 2 * sizeof(double), stream));
+    HIPCHECK(hipMemsetAsync(stats.p, 0, (size_t)nw * 2 * sizeof(double), stream));
+    enqueue_substep(tau, false, true);
+    HIPCHECK(hipEventSynchronize(ev_ubase));
+    finish_substep();
+    const bool recompute = dmc.p.recompute_period > 0 && ((step_count + 1) % dmc.p.recompute_period == 0);
+    if (recompute)
+      recompute_dets();
+    std::vector<double> hs((size_t)nw * 2);
+    HIPCHECK(hipMemcpyAsync(hs.data(), stats.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    std::vector<double> enew;
+    dmc_local_energies(enew, true);
+    step_count++;
+    double nacc = 0, nrej = 0;
+    for (int w = 0; w < nw; w++)
+    {
+      nacc += hs[2 * w];
+      nrej += hs[2 * w + 1];
+    }
+    // branch weights (DMCBatched.cpp:262-272)
+    for (int w = 0; w < nw; w++)
+    {
+      dmc.weight[w] *= dmc_branch_weight(enew[w], dmc.eloc[w]);
+      dmc.eloc[w] = enew[w];
+    }
+    // WalkerControl::branch: multiplicities, ensemble properties (computeCurData, writeDMCdat)
+    const bool no_branch = dmc.iter == 0;
+    std::vector<int> mult(nw);
+    for (int w = 0; w < nw; w++)
+      mult[w] = no_branch ? 1 : static_cast<int>(dmc.weight[w] + rng_uniform01(dmc.wc_rng));
+    double esum = 0, e2sum = 0, wsum = 0, r2a = 0, r2p = 0;
+    long long total = 0;
+    for (int w = 0; w < nw; w++)
+    {
+      total += mult[w];
+      r2a += h_rr.p[2 * w + 1];
+      r2p += h_rr.p[2 * w];
+      const double e = dmc.eloc[w], wg = dmc.weight[w];
+      esum += wg * e;
+      e2sum += wg * e * e;
+      wsum += wg;
+    }
+    const double winv = 1.0 / wsum;
+    const double eavg = esum * winv;
+    const double evar = e2sum * winv - eavg * eavg;
+    if (total < 1)
+      throw QmiError(QMI_ERR_NUMERIC, "the DMC population died out");
+    if (total > cap)
+      throw QmiError(QMI_ERR_NUMERIC, "DMC population " + std::to_string(total) + " exceeds the capacity " +
+                                          std::to_string(cap));
+    dmc_branch_slots(mult);
+    if (!no_branch)
+      for (int w = 0; w < nw; w++)
+        dmc.weight[w] = 1.0;
+    // SFNBranch::updateParamAfterPopControl
+    const double logN = std::log(static_cast<double>(dmc.target));
+    dmc.enow          = eavg;
+    dmc.r2acc += r2a;
+    dmc.r2prop += r2p;
+    dmc.tau_eff = tau * dmc.r2acc / dmc.r2prop;
+    if (dmc.main_stage)
+    {
+      if (dmc.p.refenergy_history == 1)
+      {
+        dmc.ehist += eavg;
+        dmc.vhist += evar;
+        dmc.nhist += 1.0;
+        dmc.eref = dmc.ehist / dmc.nhist;
+      }
+      else
+      {
+        if (dmc.hist.size() == dmc.hist_limit)
+          dmc.hist.pop_front();
+        dmc.hist.push_back({wsum, eavg, evar});
+        double sw = 0, se = 0;
+        for (auto& h : dmc.hist)
+        {
+          sw += h[0];
+          se += h[1] * h[0];
+        }
+        dmc.eref = se / sw;
+      }
+      if (--dmc.etrial_todo == 0)
+      {
+        dmc.etrial      = dmc.eref + dmc.p.feedback * (logN - std::log(wsum));
+        dmc.etrial_todo = 1;
+      }
+    }
+    else
+    {
+      dmc.eref   = eavg;
+      dmc.etrial = eavg + (logN - std::log(wsum)) / tau;
+      if (--dmc.warmup_todo == 0)
+      {
+        dmc.sigma2 = evar;
+        dmc_set_cutoff(dmc.sigma2, dmc.p.sigma_bound, 10, N);
+        dmc.main_stage = true;
+      }
+    }
+    dmc.iter++;
+    if (out)
+    {
+      out[0] = eavg;
+      out[1] = evar;
+      out[2] = wsum;
+      out[3] = nw0;
+      out[4] = nw;
+      out[5] = dmc.etrial;
+      out[6] = dmc.eref;
+      out[7] = dmc.tau_eff;
+      out[8] = nacc;
+      out[9] = nrej;
+    }
+  }
+
+  // Kill walkers of multiplicity 0 and copy those of multiplicity m > 1 (m - 1 copies): copies go
+  // to the dead slots in ascending order, then past the end; remaining holes are filled from the
+  // tail.  Two launches: copies (sources live, destinations dead or new), then tail moves.
+  void dmc_branch_slots(const std::vector<int>& mult)
+  {
+    std::vector<int> holes;
+    for (int w = 0; w < nw; w++)
+      if (mult[w] == 0)
+        holes.push_back(w);
+    std::vector<int> pairs;
+    size_t hi = 0;
+    int next  = nw;
+    for (int w = 0; w < nw; w++)
+      for (int c = 1; c < mult[w]; c++)
+      {
+        const int dst = hi < holes.size() ? holes[hi++] : next++;
+        pairs.push_back(w);
+        pairs.push_back(dst);
+        dmc.weight[dst] = dmc.weight[w];
+        dmc.eloc[dst]   = dmc.eloc[w];
+      }
+    const int n1 = next;
+    std::vector<char> live(n1, 1);
+    for (size_t h = hi; h < holes.size(); h++)
+      live[holes[h]] = 0;
+    const size_t ncopy = pairs.size() / 2;
+    int lo = 0, top = n1 - 1;
+    for (;;)
+    {
+      while (lo < n1 && live[lo])
+        lo++;
+      while (top >= 0 && !live[top])
+        top--;
+      if (lo >= top)
+        break;
+      pairs.push_back(top);
+      pairs.push_back(lo);
+      dmc.weight[lo] = dmc.weight[top];
+      dmc.eloc[lo]   = dmc.eloc[top];
+      live[lo]       = 1;
+      live[top]      = 0;
+    }
+    const int nw_new = top + 1;
+    if (!pairs.empty())
+    {
+      std::copy(pairs.begin(), pairs.end(), h_pairs.p);
+      HIPCHECK(hipMemcpyAsync(dmc_pairs.p, h_pairs.p, pairs.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+      WalkerArrays t{};
+      auto add = [&](void* base, size_t bytes) {
+        t.base[t.n]  = static_cast<char*>(base);
+        t.bytes[t.n] = bytes;
+        t.n++;
+      };
+      const int Np = sys.Npad;
+      add(R.p, (size_t)3 * Np * sizeof(double));
+      add(jas.p, (size_t)J_ROWS * Np * sizeof(double));
+      add(logdet.p, 4 * sizeof(double));
+      if (sys.has_det)
+        for (int g = 0; g < 2; g++)
+        {
+          const size_t n = std::max(sys.n[g], 1);
+          add(Ainv[g].p, det_words(n * sys.ld) * sizeof(double));
+          add(dpsi[g].p, det_words(n * 4 * sys.ld) * sizeof(double));
+        }
+      {
+        Scoped tm(this, "copy_walkers");
+        HIPCHECK(launch_copy_walkers(t, dmc_pairs.p, (int)ncopy, stream));
+        HIPCHECK(launch_copy_walkers(t, dmc_pairs.p + 2 * ncopy, (int)(pairs.size() / 2 - ncopy), stream));
+      }
+      // pinned pairs are reused by the next step: wait for the upload
+      sync();
+    }
+    nw     = nw_new;
+    ncrowd = nw;
+    while ((int)rng.size() < nw)
+      rng.emplace_back(rng_seed + uint32_t(rng.size()));
+  }
+
+  // ---------------------------------------------------------------------------------------------
   // RNG for one sub-step: Gaussians [iat][iw][3] per crowd (the reference layout of
   // walker_deltas, VMCBatched.cpp:109) scattered into the global [iat][w][3] array, then the crowd's
   // possible uniforms (N * crowd size), drawn speculatively and rewound after the sub-step.
@@ -1061,14 +1428,17 @@ struct qmi_ctx
   void substep_buffers(bool with_trace)
   {
     const int N = sys.N;
-    if (gauss.n != (size_t)N * nw * 3)
-      gauss.alloc((size_t)N * nw * 3);
-    if (unif.n != (size_t)ncrowd * N * wpc)
-      unif.alloc((size_t)ncrowd * N * wpc);
-    if (with_trace && trace.n != (size_t)N * nw * TRACE_FIELDS)
-      trace.alloc((size_t)N * nw * TRACE_FIELDS);
-    h_gauss.alloc((size_t)N * nw * 3);
-    h_unif.alloc((size_t)ncrowd * N * wpc);
+    // sized for the walker slots (a DMC population changes nw between sub-steps); reallocation
+    // only on growth, which invalidates captured graphs through walkers_load
+    if (gauss.n < (size_t)N * cap * 3)
+      gauss.alloc((size_t)N * cap * 3);
+    const size_t nunif = (size_t)((cap + wpc - 1) / wpc) * wpc * N; // whole crowds
+    if (unif.n < nunif)
+      unif.alloc(nunif);
+    if (with_trace && trace.n < (size_t)N * cap * TRACE_FIELDS)
+      trace.alloc((size_t)N * cap * TRACE_FIELDS);
+    h_gauss.alloc((size_t)N * cap * 3);
+    h_unif.alloc(nunif);
     h_ubase.alloc(ubase.n);
     if (!ev_ubase)
       HIPCHECK(hipEventCreateWithFlags(&ev_ubase, hipEventDisableTiming));
@@ -1105,6 +1475,8 @@ struct qmi_ctx
     a.unif  = unif.p;
     a.ubase = ubase.p;
     a.trace = with_trace ? trace.p : nullptr;
+    a.dmc   = dmc.on ? 1 : 0;
+    a.rr    = dmc.on ? dmc_rr.p : nullptr;
     for (int g = 0; g < 2; g++)
     {
       int iat = s.first[g];
@@ -1840,6 +2212,45 @@ extern "C"
   int qmi_coulomb_setup(qmi_ctx* c, double ion_charge, double dim_cutoff)
   {
     return guarded(c, [&] { c->coulomb_setup(ion_charge, dim_cutoff); });
+  }
+
+  int qmi_dmc_init(qmi_ctx* c, const qmi_dmc_params* p)
+  {
+    return guarded(c, [&] {
+      if (!p)
+        throw QmiError(QMI_ERR_ARG, "null parameters");
+      c->dmc_init(*p);
+    });
+  }
+
+  int qmi_dmc_run(qmi_ctx* c, int steps, double* stats)
+  {
+    return guarded(c, [&] {
+      c->require_walkers();
+      if (!c->dmc.on)
+        throw QmiError(QMI_ERR_ARG, "call qmi_dmc_init first");
+      if (steps < 0)
+        throw QmiError(QMI_ERR_ARG, "negative step count");
+      for (int i = 0; i < steps; i++)
+        c->dmc_step(stats ? stats + (size_t)i * QMI_DMC_STATS : nullptr);
+    });
+  }
+
+  int qmi_num_walkers(qmi_ctx* c) { return c ? c->nw : 0; }
+
+  int qmi_dmc_walker_data(qmi_ctx* c, double* weights, double* energies)
+  {
+    return guarded(c, [&] {
+      if (!c->dmc.on)
+        throw QmiError(QMI_ERR_ARG, "call qmi_dmc_init first");
+      for (int w = 0; w < c->nw; w++)
+      {
+        if (weights)
+          weights[w] = c->dmc.weight[w];
+        if (energies)
+          energies[w] = c->dmc.eloc[w];
+      }
+    });
   }
 
   int qmi_coulomb_nk(qmi_ctx* c) { return c && c->coul_ready ? c->coul.nk : 0; }

@@ -430,4 +430,30 @@ hipError_t launch_fill_i32(int* p, size_t n, int v, hipStream_t stream)
   return hipGetLastError();
 }
 
+// Walker copies of DMC branching: one workgroup per (pair, array), 16-byte copies
+__global__ __launch_bounds__(WG) void k_copy_walkers(WalkerArrays t, const int* __restrict__ pairs, int npairs)
+{
+  const int p = blockIdx.x, a = blockIdx.y;
+  if (p >= npairs || a >= t.n)
+    return;
+  const size_t nb = t.bytes[a] / 16;
+  const uint4* src = reinterpret_cast<const uint4*>(t.base[a] + (size_t)pairs[2 * p] * t.bytes[a]);
+  uint4* dst       = reinterpret_cast<uint4*>(t.base[a] + (size_t)pairs[2 * p + 1] * t.bytes[a]);
+  for (size_t i = threadIdx.x; i < nb; i += WG)
+    dst[i] = src[i];
+}
+
+hipError_t launch_copy_walkers(const WalkerArrays& t, const int* pairs, int npairs, hipStream_t stream)
+{
+  if (npairs <= 0)
+    return hipSuccess;
+  if (t.n < 1 || t.n > WA_MAX)
+    return hipErrorInvalidValue;
+  for (int a = 0; a < t.n; a++)
+    if (t.bytes[a] % 16 != 0 || reinterpret_cast<uintptr_t>(t.base[a]) % 16 != 0)
+      return hipErrorInvalidValue;
+  k_copy_walkers<<<dim3(npairs, t.n), WG, 0, stream>>>(t, pairs, npairs);
+  return hipGetLastError();
+}
+
 } // namespace qmi

@@ -35,6 +35,11 @@ struct MoveArgs
   const int* host_acc; // [nw] accept decisions (API accept)
   double* out;         // API outputs
   double* trace;       // [N][nw][TRACE_FIELDS] (nullable)
+  // DMC acceptance (DMCBatched.cpp:190-232): moves across a node of a real wave function are
+  // rejected without a draw, the uniform is compared only when prob_eff >= eps, and each walker
+  // accumulates tau |xi|^2 of its proposed and accepted moves in rr[nw][2] (one walker per crowd)
+  int dmc;
+  double* rr;
 };
 
 enum MoveKernel
@@ -94,6 +99,18 @@ hipError_t launch_api_spo_vgl_c(const SysC& s, const void* coefs, const double* 
                                 const double* pos, double* out, hipStream_t stream);
 hipError_t launch_api_det_ratios_c(const SysC& s, const void* coefs, const double* kpts, int nw, int nvp, int n,
                                    const double* pos, const double* inv, double* out, hipStream_t stream);
+// per-walker slices of the walker state (base of slot 0 and bytes per slot, a multiple of 16)
+constexpr int WA_MAX = 8;
+struct WalkerArrays
+{
+  char* base[WA_MAX];
+  size_t bytes[WA_MAX];
+  int n;
+};
+// slot dst <- slot src for each pair {src, dst} of pairs[npairs][2] (distinct sources and
+// destinations within one launch)
+hipError_t launch_copy_walkers(const WalkerArrays& t, const int* pairs, int npairs, hipStream_t stream);
+
 // Ewald tables of the periodic Coulomb energy (built on the host, device pointers)
 constexpr int COUL_MAX_NG = 1024;
 struct CoulombC

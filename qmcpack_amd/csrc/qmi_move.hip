@@ -456,7 +456,9 @@ __device__ inline void fused_move(const SysC& s, const State& st, const ST* __re
     double ratio;
     bool flag;
     const double pe = mh_prob(s, iat, a, v.jas, sh, ratio, flag);
-    bool acc        = false;
+    if (a.dmc) // a node crossing (ratio <= 0) is rejected before the draw; prob_eff against eps
+      flag = ratio > 0.0 && pe >= DBL_EPS;
+    bool acc = false;
     if (flag)
     {
       const double u = a.unif[(size_t)w * N + ucount];
@@ -464,6 +466,8 @@ __device__ inline void fused_move(const SysC& s, const State& st, const ST* __re
       acc = u < pe;
     }
     sh.bc[BC_ACC] = acc ? 1.0 : 0.0;
+    if (a.dmc)
+      dmc_record(a, iat, w, acc);
     if (acc)
       nacc += 1;
     else

@@ -282,5 +282,16 @@ __device__ inline void propose(const SysC& s, int iat, int w, const MoveArgs& a,
   sh.bc[BC_LOGGF] = -oneover2tau * (delta[0] * delta[0] + delta[1] * delta[1] + delta[2] * delta[2]);
 }
 
+// DMC bookkeeping of one move of walker w (DMCBatched.cpp:172-175, 225): tau |xi|^2 of the
+// unscaled normal xi, added to the proposed sum and, when accepted, to the accepted sum
+__device__ inline void dmc_record(const MoveArgs& a, int iat, int w, bool acc)
+{
+  const double* gz = a.gauss + ((size_t)iat * a.nw + w) * 3;
+  const double rr  = a.tau * (gz[0] * gz[0] + gz[1] * gz[1] + gz[2] * gz[2]);
+  a.rr[2 * (size_t)w] += rr;
+  if (acc)
+    a.rr[2 * (size_t)w + 1] += rr;
+}
+
 
 } // namespace qmi

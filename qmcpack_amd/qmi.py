@@ -20,6 +20,14 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 
 
+class DmcParams(C.Structure):
+    """qmi_dmc_params (include/qmcmi.h)."""
+    _fields_ = [("tau", C.c_double), ("feedback", C.c_double), ("sigma_bound", C.c_double),
+                ("warmup_steps", C.c_int), ("refenergy_history", C.c_int), ("cutoff_scheme", C.c_int),
+                ("capacity", C.c_int), ("target_walkers", C.c_int), ("recompute_period", C.c_int),
+                ("seed", C.c_uint32)]
+
+
 class QmiError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"qmcmi error {code}: {msg}")
@@ -236,6 +244,27 @@ class Context:
 
     def coulomb_setup(self, ion_charge, dim_cutoff=15.0):
         self._check(self.L.qmi_coulomb_setup(self.h, C.c_double(ion_charge), C.c_double(dim_cutoff)))
+
+    # ---------------------------------------------------------------- DMC
+    def dmc_init(self, tau, capacity, seed, feedback=1.0, sigma_bound=10.0, warmup_steps=200,
+                 refenergy_history=0, cutoff_scheme=0, target_walkers=0, recompute_period=0):
+        """DMC on the loaded walkers (one walker per crowd; call coulomb_setup first)."""
+        p = DmcParams(tau, feedback, sigma_bound, warmup_steps, refenergy_history, cutoff_scheme, capacity,
+                      target_walkers, recompute_period, seed)
+        self._check(self.L.qmi_dmc_init(self.h, C.byref(p)))
+
+    def dmc_run(self, steps):
+        """Per step [steps][10]: energy, variance, weight, walkers before / after branching, E_trial,
+        E_ref, tau_eff, accepted, rejected."""
+        out = np.empty((steps, 10))
+        self._check(self.L.qmi_dmc_run(self.h, int(steps), _p(out)))
+        self.nw = int(self.L.qmi_num_walkers(self.h))
+        return out
+
+    def dmc_walker_data(self):
+        w, e = np.empty(self.nw), np.empty(self.nw)
+        self._check(self.L.qmi_dmc_walker_data(self.h, _p(w), _p(e)))
+        return w, e
 
     def coulomb_nk(self):
         """Number of k vectors of the Ewald long-range sum (after coulomb_setup)."""

@@ -1,0 +1,90 @@
+"""DMC (DMCBatched + SFNBranch + WalkerControl, fluctuating population) on the HIP path against the
+CPU FP64 oracle's DMC driver (same walkers, same seeds).
+
+The oracle's DMC is pinned by tests/test_oracle_dmc.py (fixed-node sign rule, the branch weight of
+the first step, projection below the VMC energy, population feedback).  Both drivers draw the same
+random numbers (walker slot w from seed + w, walker control from seed - 1), so the populations
+must agree walker for walker: per step the walker counts and move counts exactly, the weighted
+energy, its variance, E_trial, E_ref and the effective time step within 1e-10 relative; at the end
+the positions (1e-10 absolute), weights and local energies (1e-10 relative).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from qmcpack_amd import qmi
+from qmcpack_amd.system import make_complex_system, make_system, make_walkers
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-10
+
+
+def rel(a, b, floor=1.0):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor))) if a.size else 0.0
+
+
+def dmc_pair(system, nw, steps, seed=21, **kw):
+    R = make_walkers(system, nw, 5)
+    zion = system.N / len(system.ions)
+    c = qmi.Context(0)
+    c.system_create(system)
+    c.walkers_load(R, 1)
+    c.coulomb_setup(zion)
+    c.dmc_init(seed=seed, **kw)
+    st = c.dmc_run(steps)
+    o = orc.VMC(system, R, 1, seed)
+    o.dmc_init(zion, seed=seed, **kw)
+    sto = o.dmc_run(steps)
+    return c, o, st, sto
+
+
+def check_dmc(c, o, st, sto):
+    print("walkers per step", st[:, 4].astype(int).tolist())
+    assert np.array_equal(st[:, [3, 4, 8, 9]], sto[:, [3, 4, 8, 9]])
+    for col in (0, 2, 5, 6, 7):
+        assert rel(st[:, col], sto[:, col]) < REL, col
+    assert rel(st[:, 1], sto[:, 1], floor=max(1.0, float(np.max(sto[:, 1])) * 1e-3)) < 1e-9
+    assert c.nw == o.nw
+    assert np.allclose(c.walkers_get(), o.get()["R"], rtol=0, atol=1e-10)
+    w, e = c.dmc_walker_data()
+    wo, eo = o.dmc_walker_data()
+    assert rel(w, wo) < REL and rel(e, eo) < REL
+
+
+@pytest.mark.parametrize("k,warmup,history", [(1, 4, 0), (4, 0, 1), (6, 3, 0)])
+def test_dmc_matches_oracle(k, warmup, history):
+    s = make_system(L=7.0, n_ion=3, n_up=7, n_dn=6, norb=8, M=12, delay_rank=k, seed=13)
+    c, o, st, sto = dmc_pair(s, 24, 12, tau=0.02, capacity=96, warmup_steps=warmup, refenergy_history=history,
+                             recompute_period=5)
+    check_dmc(c, o, st, sto)
+    assert st[-1, 4] != st[0, 3] or np.any(st[:, 3] != st[:, 4])  # the population changed
+
+
+def test_dmc_complex_orbitals():
+    s = make_complex_system(L=7.0, n_ion=3, n_up=6, n_dn=5, norb=8, M=12, delay_rank=2, seed=17)
+    c, o, st, sto = dmc_pair(s, 16, 10, tau=0.02, capacity=64, warmup_steps=2, cutoff_scheme=1)
+    check_dmc(c, o, st, sto)
+
+
+def test_dmc_production_shaped_system():
+    """128 electrons (64 per spin), delay rank 16 (flushes within the sweep), 64 walkers."""
+    s = make_system(L=10.0, n_ion=16, n_up=64, n_dn=64, norb=64, M=16, delay_rank=16, seed=19)
+    c, o, st, sto = dmc_pair(s, 64, 6, tau=0.005, capacity=256, warmup_steps=2)
+    check_dmc(c, o, st, sto)
+
+
+def test_dmc_capacity_exceeded_is_an_error():
+    s = make_system(L=7.0, n_ion=3, n_up=4, n_dn=4, norb=4, M=12, delay_rank=1, seed=23)
+    R = make_walkers(s, 8, 2)
+    c = qmi.Context(0)
+    c.system_create(s)
+    c.walkers_load(R, 1)
+    with pytest.raises(RuntimeError, match="qmi_coulomb_setup"):
+        c.dmc_init(tau=0.02, capacity=8, seed=3)
+    c.coulomb_setup(2.0)
+    # a target 100 times the population raises E_trial by log(100) / tau: the weights reach ~100
+    c.dmc_init(tau=0.02, capacity=16, seed=3, warmup_steps=5, target_walkers=800)
+    with pytest.raises(RuntimeError, match="exceeds the capacity"):
+        c.dmc_run(5)
