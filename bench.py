@@ -97,6 +97,66 @@ def coulomb_cost(N, n_ion, nk, pair_flops=40, k_flops=30):
     return dict(pairs=pairs, k_terms=nk * N, flops=pairs * pair_flops + nk * N * k_flops)
 
 
+def bench_nlpp(ctx, system, log, rcut=1.5, reps=3):
+    """Non-local pseudopotential energy of every walker (s, p channels, 12-point rule, random
+    rotations): device time per evaluation and virtual-move ratios per second."""
+    x = np.linspace(0.0, 2.0, 201)
+    v = np.array([-1.5 * np.exp(-1.2 * x * x), 0.8 * np.exp(-0.9 * x * x)])
+    # second derivatives of these Gaussians (the splines interpolate them)
+    v2 = np.array([-1.5 * np.exp(-1.2 * x * x) * (4 * 1.44 * x * x - 2.4),
+                   0.8 * np.exp(-0.9 * x * x) * (4 * 0.81 * x * x - 1.8)])
+    g = (1 + 5 ** 0.5) / 2
+    q = np.array([[0, 1, g], [0, -1, g], [0, 1, -g], [0, -1, -g], [1, g, 0], [-1, g, 0], [1, -g, 0], [-1, -g, 0],
+                  [g, 0, 1], [-g, 0, 1], [g, 0, -1], [-g, 0, -1]], dtype=float)
+    q /= np.linalg.norm(q, axis=1)[:, None]
+    ctx.nlpp_setup([0, 1], x[1] - x[0], v, v2, rcut, q, np.full(12, 1.0 / 12))
+    rng = np.random.default_rng(5)
+    rot = np.array([np.linalg.qr(rng.normal(size=(3, 3)))[0] for _ in range(ctx.nw)])
+    ctx.timing_enable(True)
+    ctx.timing_reset()
+    for _ in range(reps):
+        ctx.mw_evaluateNLPP(rot)
+    tm = ctx.timing().get("nlpp", dict(ms=0.0, calls=0))
+    ctx.timing_enable(False)
+    # electron-ion pairs within the cutoff (host count over the current positions)
+    R = ctx.walkers_get()
+    ions, L = system.ions, system.L
+    npairs = 0
+    for a in ions:
+        d = R - a
+        d -= L * np.round(d / L)
+        npairs += int(np.sum(np.sum(d * d, axis=2) < rcut * rcut))
+    ms = tm["ms"] / max(1, tm["calls"])
+    out = dict(ms_per_evaluation=ms, rcut=rcut, channels=[0, 1], points=12, pairs_per_walker=npairs / ctx.nw,
+               ratios_per_s=npairs * 12 / (ms * 1e-3) if ms > 0 else 0.0)
+    log("[bench] nlpp: " + json.dumps(out))
+    return out
+
+
+def bench_dmc(ctx, steps, log, tau=0.01):
+    """DMC on the same walkers (local energy kinetic + Ewald Coulomb, fluctuating population,
+    population feedback toward the starting count): electron moves per second over `steps` timed
+    steps after 5 untimed ones, with the per-step device time of each part."""
+    nw = ctx.nw
+    ctx.dmc_init(tau=tau, capacity=4 * nw, seed=2024, warmup_steps=200)
+    ctx.dmc_run(5)
+    ctx.timing_enable(True)
+    ctx.timing_reset()
+    t1 = time.perf_counter()
+    st = ctx.dmc_run(steps)
+    dt = time.perf_counter() - t1
+    tm = ctx.timing()
+    ctx.timing_enable(False)
+    moves = float(np.sum(st[:, 3])) * ctx.N
+    per = {k: v["ms"] / steps for k, v in tm.items()}
+    out = dict(moves_per_s=moves / dt, ms_per_step=dt * 1e3 / steps, tau=tau, steps=steps,
+               walkers_mean=float(np.mean(st[:, 3])), walkers_min=int(np.min(st[:, 4])),
+               walkers_max=int(np.max(st[:, 4])), energy_last=float(st[-1, 0]),
+               device_ms_per_step=per)
+    log("[bench] dmc: " + json.dumps(out))
+    return out
+
+
 def cpu_baseline(system, cfg, nwalk=16, nthreads=None):
     """The CPU FP64 oracle (a from-scratch restatement of the reference's CPU batched path) on a
     bounded sample: nwalk walkers of the same system, one crowd per thread, one sweep."""
@@ -265,6 +325,8 @@ def bench_uneven(args, log):
             "bytes_per_launch": bytes_launch,
             "avg_launch_ms": avg_ms,
         },
+        "dmc": dmc,
+        "nlpp": nlpp,
         "coulomb_roofline": {
             "kernel": "k_coulomb (Ewald: short-range pair sums from an LDS spline, long-range rho_k)",
             "avg_ms": tc["ms"] / max(1, tc["calls"]),
@@ -318,6 +380,8 @@ def main():
     ap.add_argument("--train-sweeps", type=int, default=2,
                     help="training sweeps of the learned proposal (2 x 1024 x 512 = 1.05e6 moves)")
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--no-nlpp", dest="nlpp", action="store_false", help="skip the pseudopotential energy leg")
+    ap.add_argument("--dmc-steps", type=int, default=20, help="timed DMC steps on the workload (0: skip)")
     ap.add_argument("--no-autotune", action="store_true", help="uneven workload: keep the default knobs")
     ap.add_argument("--eager", action="store_true",
                     help="launch the sweeps kernel by kernel instead of replaying a captured graph")
@@ -376,6 +440,8 @@ def main():
     tc = ctx.timing().get("coulomb", dict(ms=0.0, calls=0))
     coul = coulomb_cost(system.N, len(system.ions), ctx.coulomb_nk())
     ctx.timing_enable(False)
+    nlpp = bench_nlpp(ctx, system, log) if args.nlpp else None
+    dmc = bench_dmc(ctx, args.dmc_steps, log) if args.dmc_steps > 0 else None
     del ctx
     n, N, nion, k = system.n_up, system.N, len(system.ions), system.delay_rank
     # the fused move kernel runs either once per move (move_one) or once per delay window of k
@@ -461,6 +527,8 @@ def main():
             "achieved_tflops": (nw * flush_flops(n, k)) / (fl_ms * 1e-3) / 1e12 if fl_ms > 0 else 0.0,
             "mfma_peak_tflops": FP32_PEAK_TFLOPS if sd == 4 else FP64_PEAK_TFLOPS,
         },
+        "dmc": dmc,
+        "nlpp": nlpp,
         "coulomb_roofline": {
             "kernel": "k_coulomb (Ewald: short-range pair sums from an LDS spline, long-range rho_k)",
             "avg_ms": tc["ms"] / max(1, tc["calls"]),

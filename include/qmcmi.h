@@ -159,6 +159,19 @@ int qmi_mw_evaluate_ratios(qmi_ctx* ctx, int iat, int nvp, const double* pos, do
  * (the reference's LR_dim_cutoff, 15 by default).  Periodic orthorhombic cells; tables built once
  * per system. */
 int qmi_coulomb_setup(qmi_ctx* ctx, double ion_charge, double dim_cutoff);
+/* Non-local pseudopotential energy of one ion species: NonLocalECPotential::evaluate
+ * (QMCHamiltonians/NonLocalECPotential.cpp:200-228) with NonLocalECPComponent::evaluateOne
+ * (NonLocalECPComponent.cpp:109-166, 826-888).  Channel c has angular momentum l[c] and radial
+ * potential v_c(r) given as a cubic spline on the linear grid r_i = i dr, i < ngrid: values
+ * v[c][ngrid] and second derivatives v2[c][ngrid] (OneDimCubicSpline data); electron-ion pairs
+ * with r < rcut (<= grid end) contribute.  The quadrature rule (Quadrature3D): nknot unit vectors
+ * knots[nknot][3] and weights[nknot].  Real orbitals. */
+int qmi_nlpp_setup(qmi_ctx* ctx, int nchannel, const int* l, int ngrid, double dr, const double* v, const double* v2,
+                   double rcut, int nknot, const double* knots, const double* weights);
+/* v[nw]: non-local energy of every walker, the grid rotated by rot[w][3][3] (row-major, as
+ * rotateQuadratureGrid with generateRandomRotationMatrix); after mw_completeUpdates */
+int qmi_mw_evaluate_nlpp(qmi_ctx* ctx, const double* rot, double* v);
+
 /* DMC driver: DMCBatched::advanceWalkers / runDMCStep (QMCDrivers/DMC/DMCBatched.cpp:67-414),
  * SFNBranch (QMCDrivers/SFNBranch.{h,cpp}: branch weight, trial / reference energy, effective
  * time step, branching cutoff) and WalkerControl::branch with a fluctuating population

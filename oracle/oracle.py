@@ -295,6 +295,21 @@ class VMC:
             raise RuntimeError("oracle evaluate_log failed")
         return self._out(logpsi), self._out(G), self._out(L)
 
+    def nlpp_energy(self, ls, dr, v, v2, rcut, knots, weights, rot):
+        ls = np.ascontiguousarray(ls, dtype=np.int32)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        v2 = np.ascontiguousarray(v2, dtype=np.float64)
+        knots = np.ascontiguousarray(knots, dtype=np.float64)
+        weights = np.ascontiguousarray(weights, dtype=np.float64)
+        rot = np.ascontiguousarray(rot, dtype=np.float64)
+        out = np.empty(self.nw)
+        st = lib().orc_nlpp_energy(self.h, len(ls), ls.ctypes.data_as(C.POINTER(C.c_int)), v.shape[1], C.c_double(dr),
+                                   ptr(v), ptr(v2), C.c_double(rcut), len(weights), ptr(knots), ptr(weights), ptr(rot),
+                                   ptr(out))
+        if st != 0:
+            raise RuntimeError("oracle nlpp_energy failed")
+        return out
+
     def dmc_init(self, ion_charge, tau, capacity, seed, feedback=1.0, sigma_bound=10.0, warmup_steps=200,
                  refenergy_history=0, cutoff_scheme=0, target_walkers=0, recompute_period=0, dim_cutoff=15.0):
         p = OrcDmcParams(tau, feedback, sigma_bound, warmup_steps, refenergy_history, cutoff_scheme, capacity,

@@ -107,6 +107,9 @@ int orc_dmc_init(void* h, double ion_charge, double dim_cutoff, const orc_dmc_pa
 /* stats[steps][10]: as qmi_dmc_run */
 int orc_dmc_run(void* h, int steps, double* stats);
 int orc_num_walkers(void* h);
+/* non-local pseudopotential energy out[nw] (see qmi_nlpp_setup / qmi_mw_evaluate_nlpp) */
+int orc_nlpp_energy(void* h, int nch, const int* l, int ng, double dr, const double* v, const double* v2, double rcut,
+                    int nknot, const double* knots, const double* weights, const double* rot, double* out);
 int orc_dmc_walker_data(void* h, double* weights, double* energies);
 
 #ifdef __cplusplus

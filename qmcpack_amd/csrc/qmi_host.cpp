@@ -419,6 +419,7 @@ struct qmi_ctx
     nw          = 0;
     invalidate_graph();
     coul_ready = false;
+    nlpp_ready = false;
   }
 
   // ---------------------------------------------------------------------------------------------
@@ -891,6 +892,96 @@ struct qmi_ctx
     coul.const_ei = (double)N * zion * ni * vsk0;
     coul.vii      = vii;
     coul_ready    = true;
+  }
+
+  // ---------------------------------------------------------------------------------------------
+  // Non-local pseudopotential (one species): radial channels as cubic splines on a linear grid
+  // (values and second derivatives, the OneDimCubicSpline form), the quadrature grid and weights
+  bool nlpp_ready = false;
+  NlppC nlpp{};
+  DevBuf<double> nlpp_poly, nlpp_knots, nlpp_rot, nlpp_part, nlpp_out;
+
+  void nlpp_setup(int nch, const int* ls, int ng, double dr, const double* y, const double* y2, double rcut, int nknot,
+                  const double* knots, const double* weights)
+  {
+    if (!have_system)
+      throw QmiError(QMI_ERR_ARG, "system not created");
+    if (nch < 1 || nch > NLPP_MAX_CH || !ls || ng < 2 || !(dr > 0) || !y || !y2 || !(rcut > 0) ||
+        rcut > (ng - 1) * dr || nknot < 1 || nknot > NLPP_MAX_KNOTS || !knots || !weights)
+      throw QmiError(QMI_ERR_ARG, "invalid pseudopotential tables (1..4 channels, r_cut inside the grid, 1..64 points)");
+    if (sys.nion > NLPP_MAX_IONS)
+      throw QmiError(QMI_ERR_UNSUPPORTED, "more than 2048 ions");
+    NlppC c{};
+    c.nch  = nch;
+    c.lmax = 0;
+    for (int i = 0; i < nch; i++)
+    {
+      if (ls[i] < 0 || ls[i] > NLPP_MAX_L)
+        throw QmiError(QMI_ERR_ARG, "channel l must be in [0, 4]");
+      c.l[i]  = ls[i];
+      c.lmax  = std::max(c.lmax, ls[i]);
+    }
+    c.ng    = ng;
+    c.dr    = dr;
+    c.drinv = 1.0 / dr;
+    c.rcut  = rcut;
+    c.nknot = nknot;
+    // the spline's (1 - t) y_i + t y_{i+1} + h ((1 - t)^3 - (1 - t)) y2_i + h (t^3 - t) y2_{i+1},
+    // h = dr^2 / 6, as a cubic in t per interval
+    std::vector<double> poly((size_t)nch * (ng - 1) * 4);
+    const double hh = dr * dr / 6.0;
+    for (int ch = 0; ch < nch; ch++)
+      for (int i = 0; i < ng - 1; i++)
+      {
+        const double* yy  = y + (size_t)ch * ng;
+        const double* yy2 = y2 + (size_t)ch * ng;
+        double* q         = poly.data() + ((size_t)ch * (ng - 1) + i) * 4;
+        q[0]              = yy[i];
+        q[1]              = (yy[i + 1] - yy[i]) - hh * (2.0 * yy2[i] + yy2[i + 1]);
+        q[2]              = 3.0 * hh * yy2[i];
+        q[3]              = hh * (yy2[i + 1] - yy2[i]);
+      }
+    std::vector<double> kw((size_t)nknot * 4);
+    for (int j = 0; j < nknot; j++)
+    {
+      for (int a = 0; a < 3; a++)
+        kw[4 * j + a] = knots[3 * j + a];
+      kw[4 * j + 3] = weights[j];
+    }
+    nlpp_poly.alloc(poly.size());
+    nlpp_knots.alloc(kw.size());
+    HIPCHECK(hipMemcpy(nlpp_poly.p, poly.data(), poly.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(nlpp_knots.p, kw.data(), kw.size() * sizeof(double), hipMemcpyHostToDevice));
+    c.vpoly    = nlpp_poly.p;
+    c.knots    = nlpp_knots.p;
+    nlpp       = c;
+    nlpp_ready = true;
+  }
+
+  void nlpp_evaluate(const double* rot, double* v)
+  {
+    require_walkers();
+    if (!nlpp_ready)
+      throw QmiError(QMI_ERR_ARG, "call qmi_nlpp_setup first");
+    if (cplx)
+      throw QmiError(QMI_ERR_UNSUPPORTED, "the pseudopotential energy is built for real orbitals");
+    if (dcount[0] || dcount[1])
+      throw QmiError(QMI_ERR_ARG, "pseudopotential energy with pending delayed updates (call completeUpdates)");
+    if (!rot || !v)
+      throw QmiError(QMI_ERR_ARG, "null rotation or output");
+    if (nlpp_rot.n < (size_t)cap * 9)
+      nlpp_rot.alloc((size_t)cap * 9);
+    if (nlpp_part.n < (size_t)cap * sys.N)
+      nlpp_part.alloc((size_t)cap * sys.N);
+    if (nlpp_out.n < (size_t)cap)
+      nlpp_out.alloc(cap);
+    HIPCHECK(hipMemcpyAsync(nlpp_rot.p, rot, (size_t)nw * 9 * sizeof(double), hipMemcpyHostToDevice, stream));
+    {
+      Scoped t(this, "nlpp");
+      HIPCHECK(launch_nlpp(sys, st, coefs.p, nlpp, nw, nlpp_rot.p, nlpp_part.p, nlpp_out.p, stream));
+    }
+    HIPCHECK(hipMemcpyAsync(v, nlpp_out.p, (size_t)nw * sizeof(double), hipMemcpyDeviceToHost, stream));
+    sync();
   }
 
   // ---------------------------------------------------------------------------------------------
@@ -2212,6 +2303,17 @@ extern "C"
   int qmi_coulomb_setup(qmi_ctx* c, double ion_charge, double dim_cutoff)
   {
     return guarded(c, [&] { c->coulomb_setup(ion_charge, dim_cutoff); });
+  }
+
+  int qmi_nlpp_setup(qmi_ctx* c, int nchannel, const int* l, int ngrid, double dr, const double* v,
+                     const double* v2, double rcut, int nknot, const double* knots, const double* weights)
+  {
+    return guarded(c, [&] { c->nlpp_setup(nchannel, l, ngrid, dr, v, v2, rcut, nknot, knots, weights); });
+  }
+
+  int qmi_mw_evaluate_nlpp(qmi_ctx* c, const double* rot, double* v)
+  {
+    return guarded(c, [&] { c->nlpp_evaluate(rot, v); });
   }
 
   int qmi_dmc_init(qmi_ctx* c, const qmi_dmc_params* p)

@@ -430,16 +430,31 @@ hipError_t launch_fill_i32(int* p, size_t n, int v, hipStream_t stream)
   return hipGetLastError();
 }
 
-// Walker copies of DMC branching: one workgroup per (pair, array), 16-byte copies
+// Walker copies of DMC branching: workgroup (chunk, pair, array) copies 64 KB of one array's slice
+// with four independent 16-byte loads per thread in flight
+constexpr size_t COPY_CHUNK = 65536;
 __global__ __launch_bounds__(WG) void k_copy_walkers(WalkerArrays t, const int* __restrict__ pairs, int npairs)
 {
-  const int p = blockIdx.x, a = blockIdx.y;
+  const int p = blockIdx.y, a = blockIdx.z;
   if (p >= npairs || a >= t.n)
     return;
-  const size_t nb = t.bytes[a] / 16;
-  const uint4* src = reinterpret_cast<const uint4*>(t.base[a] + (size_t)pairs[2 * p] * t.bytes[a]);
-  uint4* dst       = reinterpret_cast<uint4*>(t.base[a] + (size_t)pairs[2 * p + 1] * t.bytes[a]);
-  for (size_t i = threadIdx.x; i < nb; i += WG)
+  const size_t bytes = t.bytes[a];
+  const size_t c0    = (size_t)blockIdx.x * COPY_CHUNK;
+  if (c0 >= bytes)
+    return;
+  const size_t n16 = (min(bytes - c0, COPY_CHUNK)) / 16;
+  const uint4* src = reinterpret_cast<const uint4*>(t.base[a] + (size_t)pairs[2 * p] * bytes + c0);
+  uint4* dst       = reinterpret_cast<uint4*>(t.base[a] + (size_t)pairs[2 * p + 1] * bytes + c0);
+  size_t i = threadIdx.x;
+  for (; i + 3 * WG < n16; i += 4 * WG)
+  {
+    const uint4 v0 = src[i], v1 = src[i + WG], v2 = src[i + 2 * WG], v3 = src[i + 3 * WG];
+    dst[i]          = v0;
+    dst[i + WG]     = v1;
+    dst[i + 2 * WG] = v2;
+    dst[i + 3 * WG] = v3;
+  }
+  for (; i < n16; i += WG)
     dst[i] = src[i];
 }
 
@@ -447,12 +462,17 @@ hipError_t launch_copy_walkers(const WalkerArrays& t, const int* pairs, int npai
 {
   if (npairs <= 0)
     return hipSuccess;
-  if (t.n < 1 || t.n > WA_MAX)
+  if (t.n < 1 || t.n > WA_MAX || npairs > 65535)
     return hipErrorInvalidValue;
+  size_t maxb = 0;
   for (int a = 0; a < t.n; a++)
+  {
     if (t.bytes[a] % 16 != 0 || reinterpret_cast<uintptr_t>(t.base[a]) % 16 != 0)
       return hipErrorInvalidValue;
-  k_copy_walkers<<<dim3(npairs, t.n), WG, 0, stream>>>(t, pairs, npairs);
+    maxb = std::max(maxb, t.bytes[a]);
+  }
+  const unsigned chunks = (unsigned)((maxb + COPY_CHUNK - 1) / COPY_CHUNK);
+  k_copy_walkers<<<dim3(chunks, npairs, t.n), WG, 0, stream>>>(t, pairs, npairs);
   return hipGetLastError();
 }
 

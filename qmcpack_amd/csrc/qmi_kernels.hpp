@@ -99,6 +99,22 @@ hipError_t launch_api_spo_vgl_c(const SysC& s, const void* coefs, const double* 
                                 const double* pos, double* out, hipStream_t stream);
 hipError_t launch_api_det_ratios_c(const SysC& s, const void* coefs, const double* kpts, int nw, int nvp, int n,
                                    const double* pos, const double* inv, double* out, hipStream_t stream);
+// Non-local pseudopotential of one ion species (host-built tables, device pointers)
+constexpr int NLPP_MAX_CH = 4, NLPP_MAX_L = 4, NLPP_MAX_IONS = 2048, NLPP_MAX_KNOTS = 64;
+struct NlppC
+{
+  int nch, lmax;          // channels, largest l among them
+  int l[NLPP_MAX_CH];     // angular momentum of each channel
+  int ng;                 // radial grid points on [0, (ng - 1) dr]
+  double dr, drinv, rcut; // grid spacing, cutoff radius (<= grid end)
+  const double* vpoly;    // [nch][ng - 1][4] v_l(r) = a + t (b + t (c + t d)) per interval
+  int nknot;
+  const double* knots;    // [nknot][4] unit vector, weight
+};
+// out[nw] = non-local energy of each walker; rot[nw][9] row-major rotation of the grid; part[nw][N]
+hipError_t launch_nlpp(const SysC& s, const State& st, const void* coefs, const NlppC& c, int nw, const double* rot,
+                       double* part, double* out, hipStream_t stream);
+
 // per-walker slices of the walker state (base of slot 0 and bytes per slot, a multiple of 16)
 constexpr int WA_MAX = 8;
 struct WalkerArrays

@@ -1,0 +1,195 @@
+// Non-local pseudopotential energy of every walker: NonLocalECPotential::evaluate over the
+// electron-ion pairs within the cutoff (QMCHamiltonians/NonLocalECPotential.cpp:200-228) with
+// NonLocalECPComponent::evaluateOne (NonLocalECPComponent.cpp:109-166, 826-888): for electron e at
+// distance r < r_cut from ion I (dr = r_e - r_I, minimum image), the quadrature points
+// r_e + (r u_j - dr) on the sphere of radius r around the ion (u_j the rotated grid) give the
+// wave-function ratios psi_j / psi (determinant from value-only orbitals and the stored inverse
+// row, J1, J2), and
+//   V_eI = sum_j w_j [sum_l (2 l + 1) v_l(r) P_l(dr . u_j / r)] psi_j / psi.
+// One workgroup per (electron, walker): the ions within the cutoff in ion order (wave ballots),
+// the inverse row in registers for every quadrature point.  Per-electron sums to part[w][N], then
+// one thread per walker adds them in electron order.
+#include "qmi_move_common.hpp"
+
+namespace qmi
+{
+
+struct NlLDS
+{
+  alignas(16) double red[9 * WG];
+  double bc[48];
+  double jc[3][MAX_JCOEF];
+  unsigned long long mask[NLPP_MAX_IONS / 64];
+  double vrad[NLPP_MAX_CH];
+};
+
+// v_l(r) from the per-interval cubic of channel ch (r inside the grid)
+__device__ inline double nlpp_vl(const NlppC& c, int ch, double r)
+{
+  const int loc   = min((int)(r * c.drinv), c.ng - 2);
+  const double t  = (r - loc * c.dr) * c.drinv;
+  const double* q = c.vpoly + ((size_t)ch * (c.ng - 1) + loc) * 4;
+  return q[0] + t * (q[1] + t * (q[2] + t * q[3]));
+}
+
+template<typename ST, typename DT, int NJ, int MAXM>
+__global__ __launch_bounds__(WG) void k_nlpp(SysC s, State st, const ST* __restrict__ coefs, NlppC c,
+                                             const double* __restrict__ rot, double* __restrict__ part)
+{
+  __shared__ NlLDS sh;
+  const int iel = blockIdx.x, w = blockIdx.y, t = threadIdx.x;
+  const int ni = s.nion, N = s.N, Np = s.Npad;
+  const double* R    = st.R + (size_t)w * 3 * Np;
+  const double pe[3] = {R[iel], R[Np + iel], R[2 * Np + iel]};
+  // ions within the cutoff, one bit per ion in ion order
+  const int wave = t >> 6, lane = t & 63;
+  for (int j0 = 0; j0 < ni; j0 += WG)
+  {
+    const int j = j0 + t;
+    bool in     = false;
+    if (j < ni)
+    {
+      double d[3];
+      in = min_image(s, st.ions[j], st.ions[ni + j], st.ions[2 * ni + j], pe, d) < c.rcut;
+    }
+    const unsigned long long b = __ballot(in);
+    if (lane == 0)
+      sh.mask[(j0 >> 6) + wave] = b;
+  }
+  __syncthreads();
+  const int nmask = (ni + 63) >> 6;
+  bool any        = false;
+  for (int m = 0; m < nmask; m++)
+    any |= sh.mask[m] != 0ull;
+  if (!any)
+  {
+    if (t == 0)
+      part[(size_t)w * N + iel] = 0.0;
+    return;
+  }
+  stage_jastrow(s, sh);
+  const int g = iel < s.nup ? 0 : 1, n = s.n[g], ld = s.ld, r0 = iel - s.first[g];
+  const double* jas = st.jas + (size_t)w * J_ROWS * Np;
+  double arow[MAXM];
+  if (s.has_det)
+  {
+    const DT* a = det_ptr<DT>(st.Ainv[g]) + ((size_t)w * n + r0) * ld;
+#pragma unroll
+    for (int m = 0; m < MAXM; m++)
+    {
+      const int o = t + m * WG;
+      arow[m]     = o < n ? (double)a[o] : 0.0;
+    }
+  }
+  const double* rw = rot + (size_t)w * 9;
+  const double rm[9] = {rw[0], rw[1], rw[2], rw[3], rw[4], rw[5], rw[6], rw[7], rw[8]};
+  double value = 0.0; // thread 0
+  __syncthreads();
+  for (int m = 0; m < nmask; m++)
+  {
+    unsigned long long bits = sh.mask[m];
+    while (bits)
+    {
+      const int ion = m * 64 + __ffsll((long long)bits) - 1;
+      bits &= bits - 1;
+      double dI[3];
+      const double r = min_image(s, st.ions[ion], st.ions[ni + ion], st.ions[2 * ni + ion], pe, dI);
+      const double dr[3] = {-dI[0], -dI[1], -dI[2]}; // r_e - r_I
+      const double rinv  = 1.0 / r;
+      if (t < c.nch)
+        sh.vrad[t] = nlpp_vl(c, t, r) * (double)(2 * c.l[t] + 1);
+      double pairpot = 0.0;
+      for (int j = 0; j < c.nknot; j++)
+      {
+        const double* k = c.knots + 4 * j;
+        double u[3], p[3];
+        for (int a = 0; a < 3; a++)
+          u[a] = rm[3 * a] * k[0] + rm[3 * a + 1] * k[1] + rm[3 * a + 2] * k[2];
+        for (int a = 0; a < 3; a++)
+          p[a] = pe[a] + (r * u[a] - dr[a]);
+        jastrow_ratio_sums<NJ>(s, g, iel, R, st.ions, p, sh);
+        double acc[1] = {0.0};
+        if (s.has_det)
+        {
+          SplineLoc L;
+          spline_locate(s, p, L);
+#pragma unroll
+          for (int mm = 0; mm < MAXM; mm++)
+          {
+            const int o = t + mm * WG;
+            if (o < n)
+              acc[0] += spline_v_orbital<ST>(s, coefs, L, o) * arow[mm];
+          }
+        }
+        block_sum<1>(acc, sh.red, sh.bc + 40);
+        if (t == 0)
+        {
+          const double* C = sh.bc + BC_C;
+          double ratio    = 1.0;
+          if (s.has_det)
+            ratio *= sh.bc[40];
+          if (s.has_j1)
+            ratio *= exp(jas[J_VAT * Np + iel] - C[0]);
+          if (s.has_j2)
+            ratio *= exp(jas[J_UAT * Np + iel] - C[5]);
+          // NonLocalECPComponent::calculateKnotPartialProduct: Legendre recursion in z
+          const double zz = (dr[0] * u[0] + dr[1] * u[1] + dr[2] * u[2]) * rinv;
+          double lpol[NLPP_MAX_L + 1];
+          lpol[0]         = 1.0;
+          double lpolprev = 0.0;
+          for (int l = 0; l < c.lmax; l++)
+          {
+            lpol[l + 1] = ((double)(2 * l + 1) * zz * lpol[l] - l * lpolprev) * (1.0 / (double)(l + 1));
+            lpolprev    = lpol[l];
+          }
+          double lsum = 0.0;
+          for (int ch = 0; ch < c.nch; ch++)
+            lsum += sh.vrad[ch] * lpol[c.l[ch]];
+          pairpot += (lsum * k[3]) * ratio;
+        }
+        __syncthreads(); // sh.bc reused by the next point
+      }
+      if (t == 0)
+        value += pairpot;
+    }
+  }
+  if (t == 0)
+    part[(size_t)w * N + iel] = value;
+}
+
+__global__ void k_nlpp_sum(int nw, int N, const double* __restrict__ part, double* __restrict__ out)
+{
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nw)
+    return;
+  double s = 0.0;
+  for (int i = 0; i < N; i++)
+    s += part[(size_t)w * N + i];
+  out[w] = s;
+}
+
+hipError_t launch_nlpp(const SysC& s, const State& st, const void* coefs, const NlppC& c, int nw, const double* rot,
+                       double* part, double* out, hipStream_t stream)
+{
+  if (s.nion > NLPP_MAX_IONS || c.nch < 1 || c.nch > NLPP_MAX_CH || c.lmax > NLPP_MAX_L || c.ng < 2 || nw < 1)
+    return hipErrorInvalidValue;
+  const int mm = std::max(move_maxm(s, 0), move_maxm(s, 1));
+  if (mm == 0)
+    return hipErrorInvalidValue;
+  const hipError_t e = dispatch_prec(s.prec, [&](auto stv, auto dtv) {
+    using ST = decltype(stv);
+    using DT = decltype(dtv);
+    const dim3 grid(s.N, nw);
+    if (mm == 1)
+      k_nlpp<ST, DT, 2, 1><<<grid, WG, 0, stream>>>(s, st, static_cast<const ST*>(coefs), c, rot, part);
+    else
+      k_nlpp<ST, DT, 4, 2><<<grid, WG, 0, stream>>>(s, st, static_cast<const ST*>(coefs), c, rot, part);
+    return hipGetLastError();
+  });
+  if (e != hipSuccess)
+    return e;
+  k_nlpp_sum<<<(nw + 63) / 64, 64, 0, stream>>>(nw, s.N, part, out);
+  return hipGetLastError();
+}
+
+} // namespace qmi

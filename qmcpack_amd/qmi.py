@@ -245,6 +245,26 @@ class Context:
     def coulomb_setup(self, ion_charge, dim_cutoff=15.0):
         self._check(self.L.qmi_coulomb_setup(self.h, C.c_double(ion_charge), C.c_double(dim_cutoff)))
 
+    # ---------------------------------------------------------------- pseudopotential
+    def nlpp_setup(self, ls, dr, v, v2, rcut, knots, weights):
+        """Channels with angular momenta ls; v, v2 [nch][ngrid] spline values / second derivatives on
+        r_i = i dr; quadrature knots [nknot][3] and weights [nknot]."""
+        ls = np.ascontiguousarray(ls, dtype=np.int32)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        v2 = np.ascontiguousarray(v2, dtype=np.float64)
+        knots = np.ascontiguousarray(knots, dtype=np.float64)
+        weights = np.ascontiguousarray(weights, dtype=np.float64)
+        self._check(self.L.qmi_nlpp_setup(self.h, len(ls), ls.ctypes.data_as(C.POINTER(C.c_int)), v.shape[1],
+                                          C.c_double(dr), _p(v), _p(v2), C.c_double(rcut), len(weights),
+                                          _p(knots), _p(weights)))
+
+    def mw_evaluateNLPP(self, rot):
+        """Non-local energy [nw]; rot [nw][3][3] rotations of the quadrature grid."""
+        rot = np.ascontiguousarray(rot, dtype=np.float64).reshape(self.nw, 9)
+        v = np.empty(self.nw)
+        self._check(self.L.qmi_mw_evaluate_nlpp(self.h, _p(rot), _p(v)))
+        return v
+
     # ---------------------------------------------------------------- DMC
     def dmc_init(self, tau, capacity, seed, feedback=1.0, sigma_bound=10.0, warmup_steps=200,
                  refenergy_history=0, cutoff_scheme=0, target_walkers=0, recompute_period=0):

@@ -1,0 +1,85 @@
+"""Non-local pseudopotential energy on the HIP path against the CPU FP64 oracle.
+
+The oracle's energy is pinned by tests/test_oracle_nlpp.py (constant psi gives the l = 0 channel
+sum, spherical psi around one ion, the angular assembly in numpy with scipy's Legendre
+polynomials).  Here the device evaluates the same walkers after VMC runs (completed delayed
+updates) with the same radial splines, quadrature rule and per-walker rotations: 1e-10 relative
+to max(|V|, 1).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from qmcpack_amd import qmi
+from qmcpack_amd.system import make_complex_system, make_system, make_walkers
+
+from test_oracle_nlpp import bare_system, channels, icosahedron, random_rotations
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-10
+
+
+def nlpp_pair(s, nw, ls, rcut, steps=1, seed=31, wpc=1):
+    R = make_walkers(s, nw, 3)
+    dr, v, v2, _ = channels(ls, seed=len(ls))
+    knots, w = icosahedron()
+    rot = random_rotations(nw, seed)
+    c = qmi.Context(0)
+    c.system_create(s)
+    c.walkers_load(R, wpc)
+    c.mw_evaluateLog()
+    if steps:
+        c.vmc_run(steps, 1, 0.4, 0, seed)
+    c.nlpp_setup(ls, dr, v, v2, rcut, knots, w)
+    E = c.mw_evaluateNLPP(rot)
+    o = orc.VMC(s, c.walkers_get(), 1, 1)
+    o.evaluate_log()
+    Eo = o.nlpp_energy(ls, dr, v, v2, rcut, knots, w, rot)
+    return c, E, Eo
+
+
+def worst(E, Eo):
+    return float(np.max(np.abs(E - Eo) / np.maximum(np.abs(Eo), 1.0)))
+
+
+@pytest.mark.parametrize("k,ls", [(1, [0, 1, 2]), (4, [0, 1]), (1, [0, 1, 2, 3])])
+def test_nlpp_matches_oracle(k, ls):
+    s = make_system(L=7.0, n_ion=6, n_up=8, n_dn=7, norb=9, M=12, delay_rank=k, seed=11)
+    c, E, Eo = nlpp_pair(s, 6, ls, 2.2, steps=2)
+    print("nlpp", E, Eo, worst(E, Eo))
+    assert np.any(np.abs(Eo) > 1e-3)
+    assert worst(E, Eo) < REL
+
+
+def test_nlpp_constant_psi():
+    rng = np.random.default_rng(1)
+    L = np.array([6.0, 6.5, 7.0])
+    s = bare_system(6, 5, rng.uniform(0, 1, size=(5, 3)) * L, L)
+    c, E, Eo = nlpp_pair(s, 4, [0, 1, 2, 3], 2.5, steps=0)
+    assert worst(E, Eo) < REL
+
+
+@pytest.mark.parametrize("n_up,n_dn", [(128, 128), (300, 270)])
+def test_nlpp_large_determinants(n_up, n_dn):
+    """256 orbitals (one per thread) and 300 (two per thread: the inverse row in registers)."""
+    s = make_system(L=14.0, n_ion=16, n_up=n_up, n_dn=n_dn, norb=max(n_up, n_dn), M=16, delay_rank=16, seed=5)
+    c, E, Eo = nlpp_pair(s, 2, [0, 1], 1.6, steps=0)
+    print("nlpp large", E, Eo, worst(E, Eo))
+    assert worst(E, Eo) < 1e-9
+
+
+def test_nlpp_refusals():
+    s = make_complex_system(L=7.0, n_ion=3, n_up=4, n_dn=4, norb=4, M=12, delay_rank=1, seed=2)
+    c = qmi.Context(0)
+    c.system_create(s)
+    c.walkers_load(make_walkers(s, 2, 1), 1)
+    with pytest.raises(RuntimeError, match="qmi_nlpp_setup"):
+        c.mw_evaluateNLPP(random_rotations(2, 1))
+    dr, v, v2, _ = channels([0])
+    knots, w = icosahedron()
+    with pytest.raises(RuntimeError, match="r_cut inside the grid"):
+        c.nlpp_setup([0], dr, v, v2, 100.0, knots, w)
+    c.nlpp_setup([0], dr, v, v2, 2.0, knots, w)
+    with pytest.raises(RuntimeError, match="real orbitals"):
+        c.mw_evaluateNLPP(random_rotations(2, 1))
