@@ -212,6 +212,20 @@ __device__ inline void functor_vgl_c(const double* c, double DeltaRInv, int max_
        s3 * (((1.0 / 6.0 * t + 0.0 / 6.0) * t + 0.0 / 6.0) * t + 0.0 / 6.0));
 }
 
+// value only of the same element (BsplineFunctor::evaluate, BsplineFunctor.h:117-133)
+__device__ inline double functor_v_c(const double* c, double DeltaRInv, int max_index, double r)
+{
+  r *= DeltaRInv;
+  int i;
+  double t;
+  spline_bound(r, max_index, i, t);
+  const double s0 = c[i], s1 = c[i + 1], s2 = c[i + 2], s3 = c[i + 3];
+  return s0 * (((-1.0 / 6.0 * t + 3.0 / 6.0) * t + -3.0 / 6.0) * t + 1.0 / 6.0) +
+      s1 * (((3.0 / 6.0 * t + -6.0 / 6.0) * t + 0.0 / 6.0) * t + 4.0 / 6.0) +
+      s2 * (((-3.0 / 6.0 * t + 3.0 / 6.0) * t + 3.0 / 6.0) * t + 1.0 / 6.0) +
+      s3 * (((1.0 / 6.0 * t + 0.0 / 6.0) * t + 0.0 / 6.0) * t + 0.0 / 6.0);
+}
+
 __device__ inline void functor_vgl(const FunctorC& f, double r, double& u, double& du, double& d2u)
 {
   functor_vgl_c(f.c, f.DeltaRInv, f.max_index, r, u, du, d2u);

@@ -14,13 +14,19 @@
 namespace qmi
 {
 
+constexpr int NL_KC = 12; // quadrature points evaluated together
+
 struct NlLDS
 {
-  alignas(16) double red[9 * WG];
-  double bc[48];
+  alignas(16) double red[4][3 * NL_KC]; // per-wave partials: det, J1, J2 of each point
   double jc[3][MAX_JCOEF];
   unsigned long long mask[NLPP_MAX_IONS / 64];
-  double vrad[NLPP_MAX_CH];
+  double vp[NL_KC][3];  // virtual positions
+  double ab[NL_KC][16]; // spline factors a_i b_j of each point
+  double cz[NL_KC][4];  // and c_k
+  long long off[NL_KC]; // table offset of each point's stencil corner
+  double kp[NL_KC];     // knot partial products w_j sum_l (2l+1) v_l(r) P_l(cos)
+  double sum[3 * NL_KC];
 };
 
 // v_l(r) from the per-interval cubic of channel ch (r inside the grid)
@@ -32,6 +38,9 @@ __device__ inline double nlpp_vl(const NlppC& c, int ch, double r)
   return q[0] + t * (q[1] + t * (q[2] + t * q[3]));
 }
 
+// Up to NL_KC quadrature points at once: each thread accumulates, for every point, its orbitals'
+// values dotted with the inverse row and its ions' / electrons' Jastrow values, so that the
+// stencil gathers of all points are in flight together and one reduction serves them all.
 template<typename ST, typename DT, int NJ, int MAXM>
 __global__ __launch_bounds__(WG) void k_nlpp(SysC s, State st, const ST* __restrict__ coefs, NlppC c,
                                              const double* __restrict__ rot, double* __restrict__ part)
@@ -71,6 +80,9 @@ __global__ __launch_bounds__(WG) void k_nlpp(SysC s, State st, const ST* __restr
   const int g = iel < s.nup ? 0 : 1, n = s.n[g], ld = s.ld, r0 = iel - s.first[g];
   const double* jas = st.jas + (size_t)w * J_ROWS * Np;
   double arow[MAXM];
+#pragma unroll
+  for (int m = 0; m < MAXM; m++)
+    arow[m] = 0.0;
   if (s.has_det)
   {
     const DT* a = det_ptr<DT>(st.Ainv[g]) + ((size_t)w * n + r0) * ld;
@@ -81,9 +93,19 @@ __global__ __launch_bounds__(WG) void k_nlpp(SysC s, State st, const ST* __restr
       arow[m]     = o < n ? (double)a[o] : 0.0;
     }
   }
-  const double* rw = rot + (size_t)w * 9;
-  const double rm[9] = {rw[0], rw[1], rw[2], rw[3], rw[4], rw[5], rw[6], rw[7], rw[8]};
-  double value = 0.0; // thread 0
+  // this thread's electrons for the J2 sums
+  double ex[NJ], ey[NJ], ez[NJ];
+#pragma unroll
+  for (int u = 0; u < NJ; u++)
+  {
+    const int j = t + u * WG;
+    const bool ok = j < N;
+    ex[u] = ok ? R[j] : 0.0;
+    ey[u] = ok ? R[Np + j] : 0.0;
+    ez[u] = ok ? R[2 * Np + j] : 0.0;
+  }
+  const double* rw   = rot + (size_t)w * 9;
+  double value       = 0.0; // thread 0
   __syncthreads();
   for (int m = 0; m < nmask; m++)
   {
@@ -95,45 +117,23 @@ __global__ __launch_bounds__(WG) void k_nlpp(SysC s, State st, const ST* __restr
       double dI[3];
       const double r = min_image(s, st.ions[ion], st.ions[ni + ion], st.ions[2 * ni + ion], pe, dI);
       const double dr[3] = {-dI[0], -dI[1], -dI[2]}; // r_e - r_I
-      const double rinv  = 1.0 / r;
-      if (t < c.nch)
-        sh.vrad[t] = nlpp_vl(c, t, r) * (double)(2 * c.l[t] + 1);
-      double pairpot = 0.0;
-      for (int j = 0; j < c.nknot; j++)
+      double pairpot     = 0.0;
+      for (int j0 = 0; j0 < c.nknot; j0 += NL_KC)
       {
-        const double* k = c.knots + 4 * j;
-        double u[3], p[3];
-        for (int a = 0; a < 3; a++)
-          u[a] = rm[3 * a] * k[0] + rm[3 * a + 1] * k[1] + rm[3 * a + 2] * k[2];
-        for (int a = 0; a < 3; a++)
-          p[a] = pe[a] + (r * u[a] - dr[a]);
-        jastrow_ratio_sums<NJ>(s, g, iel, R, st.ions, p, sh);
-        double acc[1] = {0.0};
-        if (s.has_det)
+        const int kc = min(NL_KC, c.nknot - j0);
+        if (t < kc)
         {
-          SplineLoc L;
-          spline_locate(s, p, L);
-#pragma unroll
-          for (int mm = 0; mm < MAXM; mm++)
+          // NonLocalECPComponent: rotated point, deltaV, knot partial product
+          const double* k = c.knots + 4 * (j0 + t);
+          double u[3], p[3];
+          for (int a = 0; a < 3; a++)
+            u[a] = rw[3 * a] * k[0] + rw[3 * a + 1] * k[1] + rw[3 * a + 2] * k[2];
+          for (int a = 0; a < 3; a++)
           {
-            const int o = t + mm * WG;
-            if (o < n)
-              acc[0] += spline_v_orbital<ST>(s, coefs, L, o) * arow[mm];
+            p[a]     = pe[a] + (r * u[a] - dr[a]);
+            sh.vp[t][a] = p[a];
           }
-        }
-        block_sum<1>(acc, sh.red, sh.bc + 40);
-        if (t == 0)
-        {
-          const double* C = sh.bc + BC_C;
-          double ratio    = 1.0;
-          if (s.has_det)
-            ratio *= sh.bc[40];
-          if (s.has_j1)
-            ratio *= exp(jas[J_VAT * Np + iel] - C[0]);
-          if (s.has_j2)
-            ratio *= exp(jas[J_UAT * Np + iel] - C[5]);
-          // NonLocalECPComponent::calculateKnotPartialProduct: Legendre recursion in z
-          const double zz = (dr[0] * u[0] + dr[1] * u[1] + dr[2] * u[2]) * rinv;
+          const double zz = (dr[0] * u[0] + dr[1] * u[1] + dr[2] * u[2]) * (1.0 / r);
           double lpol[NLPP_MAX_L + 1];
           lpol[0]         = 1.0;
           double lpolprev = 0.0;
@@ -144,10 +144,126 @@ __global__ __launch_bounds__(WG) void k_nlpp(SysC s, State st, const ST* __restr
           }
           double lsum = 0.0;
           for (int ch = 0; ch < c.nch; ch++)
-            lsum += sh.vrad[ch] * lpol[c.l[ch]];
-          pairpot += (lsum * k[3]) * ratio;
+            lsum += (nlpp_vl(c, ch, r) * (double)(2 * c.l[ch] + 1)) * lpol[c.l[ch]];
+          sh.kp[t] = lsum * k[3];
+          if (s.has_det)
+          {
+            SplineLoc L;
+            spline_locate(s, p, L);
+            sh.off[t] = L.off;
+            for (int i = 0; i < 4; i++)
+            {
+              sh.cz[t][i] = L.c[i];
+              for (int jj = 0; jj < 4; jj++)
+                sh.ab[t][i * 4 + jj] = L.a[i] * L.b[jj];
+            }
+          }
         }
-        __syncthreads(); // sh.bc reused by the next point
+        __syncthreads();
+        double adet[NL_KC], aj1[NL_KC], aj2[NL_KC];
+#pragma unroll
+        for (int q = 0; q < NL_KC; q++)
+          adet[q] = aj1[q] = aj2[q] = 0.0;
+        if (s.has_det)
+        {
+#pragma unroll
+          for (int mm = 0; mm < MAXM; mm++)
+          {
+            const int o = t + mm * WG;
+            if (o < n)
+            {
+#pragma unroll
+              for (int q = 0; q < NL_KC; q++)
+                if (q < kc)
+                {
+                  const ST* base = coefs + sh.off[q] + o;
+                  double val     = 0.0;
+#pragma unroll
+                  for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int jj = 0; jj < 4; jj++)
+                    {
+                      const ST* qq    = base + i * s.xs + jj * s.ys;
+                      const double s0 = sh.cz[q][0] * (double)qq[0] + sh.cz[q][1] * (double)qq[s.zs] +
+                          sh.cz[q][2] * (double)qq[2 * s.zs] + sh.cz[q][3] * (double)qq[3 * s.zs];
+                      val += sh.ab[q][i * 4 + jj] * s0;
+                    }
+                  adet[q] += val * arow[mm];
+                }
+            }
+          }
+        }
+        if (s.has_j1)
+          for (int j = t; j < ni; j += WG)
+          {
+            const double ix = st.ions[j], iy = st.ions[ni + j], iz = st.ions[2 * ni + j];
+#pragma unroll
+            for (int q = 0; q < NL_KC; q++)
+              if (q < kc)
+              {
+                double d[3];
+                const double rq = min_image(s, ix, iy, iz, sh.vp[q], d);
+                if (rq < s.j1.rcut)
+                  aj1[q] += functor_v_c(sh.jc[0], s.j1.DeltaRInv, s.j1.max_index, rq);
+              }
+          }
+        if (s.has_j2)
+        {
+#pragma unroll
+          for (int u = 0; u < NJ; u++)
+          {
+            const int j = t + u * WG;
+            if (j >= N || j == iel)
+              continue;
+            const int fi      = ((j < s.nup) ? 0 : 1) == g ? 0 : 1;
+            const FunctorC& f = s.j2[fi];
+#pragma unroll
+            for (int q = 0; q < NL_KC; q++)
+              if (q < kc)
+              {
+                double d[3];
+                const double rq = min_image(s, ex[u], ey[u], ez[u], sh.vp[q], d);
+                if (rq < f.rcut)
+                  aj2[q] += functor_v_c(sh.jc[1 + fi], f.DeltaRInv, f.max_index, rq);
+              }
+          }
+        }
+        // reduce the 3 kc sums: xor shuffles within each wave, then the four wave partials
+#pragma unroll
+        for (int q = 0; q < NL_KC; q++)
+        {
+          double v0 = adet[q], v1 = aj1[q], v2 = aj2[q];
+#pragma unroll
+          for (int off = 32; off >= 1; off >>= 1)
+          {
+            v0 += __shfl_xor(v0, off, 64);
+            v1 += __shfl_xor(v1, off, 64);
+            v2 += __shfl_xor(v2, off, 64);
+          }
+          if (lane == 0)
+          {
+            sh.red[wave][q]             = v0;
+            sh.red[wave][NL_KC + q]     = v1;
+            sh.red[wave][2 * NL_KC + q] = v2;
+          }
+        }
+        __syncthreads();
+        if (t < 3 * NL_KC)
+          sh.sum[t] = sh.red[0][t] + sh.red[1][t] + sh.red[2][t] + sh.red[3][t];
+        __syncthreads();
+        if (t == 0)
+          for (int q = 0; q < kc; q++)
+          {
+            double ratio = 1.0;
+            if (s.has_det)
+              ratio *= sh.sum[q];
+            if (s.has_j1)
+              ratio *= exp(jas[J_VAT * Np + iel] - sh.sum[NL_KC + q]);
+            if (s.has_j2)
+              ratio *= exp(jas[J_UAT * Np + iel] - sh.sum[2 * NL_KC + q]);
+            pairpot += sh.kp[q] * ratio;
+          }
+        __syncthreads(); // the point tables are rewritten by the next chunk
       }
       if (t == 0)
         value += pairpot;
