@@ -252,6 +252,13 @@ class Det:
         return ainv, complex(ld[0], ld[1])
 
 
+def rotation_matrix(u1, u2, u3):
+    """generateRotationMatrix(u1, u2, u3) as a 3x3 array."""
+    out = np.empty(9)
+    lib().orc_rotation_matrix(C.c_double(u1), C.c_double(u2), C.c_double(u3), ptr(out))
+    return out.reshape(3, 3)
+
+
 class OrcDmcParams(C.Structure):
     _fields_ = [("tau", C.c_double), ("feedback", C.c_double), ("sigma_bound", C.c_double),
                 ("warmup_steps", C.c_int), ("refenergy_history", C.c_int), ("cutoff_scheme", C.c_int),
@@ -309,6 +316,16 @@ class VMC:
         if st != 0:
             raise RuntimeError("oracle nlpp_energy failed")
         return out
+
+    def dmc_set_nlpp(self, ls, dr, v, v2, rcut, knots, weights):
+        """Pseudopotential in the DMC local energy (before dmc_init)."""
+        ls = np.ascontiguousarray(ls, dtype=np.int32)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        v2 = np.ascontiguousarray(v2, dtype=np.float64)
+        knots = np.ascontiguousarray(knots, dtype=np.float64)
+        weights = np.ascontiguousarray(weights, dtype=np.float64)
+        lib().orc_dmc_set_nlpp(self.h, len(ls), ls.ctypes.data_as(C.POINTER(C.c_int)), v.shape[1], C.c_double(dr),
+                               ptr(v), ptr(v2), C.c_double(rcut), len(weights), ptr(knots), ptr(weights))
 
     def dmc_init(self, ion_charge, tau, capacity, seed, feedback=1.0, sigma_bound=10.0, warmup_steps=200,
                  refenergy_history=0, cutoff_scheme=0, target_walkers=0, recompute_period=0, dim_cutoff=15.0):

@@ -107,6 +107,11 @@ int orc_dmc_init(void* h, double ion_charge, double dim_cutoff, const orc_dmc_pa
 /* stats[steps][10]: as qmi_dmc_run */
 int orc_dmc_run(void* h, int steps, double* stats);
 int orc_num_walkers(void* h);
+/* pseudopotential in the DMC local energy (before orc_dmc_init; real orbitals) */
+int orc_dmc_set_nlpp(void* h, int nch, const int* l, int ng, double dr, const double* v, const double* v2, double rcut,
+                     int nknot, const double* knots, const double* weights);
+/* generateRotationMatrix(u1, u2, u3), row-major */
+int orc_rotation_matrix(double u1, double u2, double u3, double* out9);
 /* non-local pseudopotential energy out[nw] (see qmi_nlpp_setup / qmi_mw_evaluate_nlpp) */
 int orc_nlpp_energy(void* h, int nch, const int* l, int ng, double dr, const double* v, const double* v2, double rcut,
                     int nknot, const double* knots, const double* weights, const double* rot, double* out);

@@ -994,7 +994,7 @@ struct qmi_ctx
   // log det) where the reference reloads and recomputes it: the same values to rounding.
   struct DmcState
   {
-    bool on = false;
+    bool on = false, nlpp = false;
     qmi_dmc_params p{};
     long long iter = 0;
     std::vector<double> weight, eloc; // per slot
@@ -1012,6 +1012,25 @@ struct qmi_ctx
   DevBuf<int> dmc_pairs;
   PinnedBuf<int> h_pairs;
   PinnedBuf<double> h_rr, h_pot;
+
+  // random rotation of the quadrature grid from three uniforms (RandomRotationMatrix.h:24-32 with
+  // Numerics/RotationMatrix3D.h: phi = 2 pi u1, psi = 2 pi u2, cos theta = 1 - 2 u3), row-major
+  static void rotation_from_uniforms(double u1, double u2, double u3, double* m)
+  {
+    const double phi = 2.0 * M_PI * u1, psi = 2.0 * M_PI * u2, cth = 1.0 - 2 * u3;
+    const double sph = std::sin(phi), cph = std::cos(phi);
+    const double sth = std::sqrt(1.0 - cth * cth);
+    const double sps = std::sin(psi), cps = std::cos(psi);
+    m[0] = cph * cth * cps - sph * sps;
+    m[1] = sph * cth * cps + cph * sps;
+    m[2] = -sth * cps;
+    m[3] = -cph * cth * sps - sph * cps;
+    m[4] = -sph * cth * sps + cph * cps;
+    m[5] = sth * sps;
+    m[6] = cph * sth;
+    m[7] = sph * sth;
+    m[8] = cth;
+  }
 
   // SFNBranch::setBranchCutoff (SFNBranch.cpp:284-317)
   void dmc_set_cutoff(double variance, double target_sigma, double max_sigma, int nelec)
@@ -1043,9 +1062,25 @@ struct qmi_ctx
     return std::exp(taueff * (dmc.etrial * 2.0 - enew - eold));
   }
 
-  // local energies of the walkers: kinetic (evaluateGL) + V_ee + V_eI + V_II, summed in that order
+  // local energies of the walkers: kinetic (evaluateGL) + V_ee + V_eI + V_II (+ V_NL when a
+  // pseudopotential is set: each walker's grid rotated by three draws from its slot stream, after
+  // the sweep's draws, as NonLocalECPotential draws from the crowd's generator), summed in that order
   void dmc_local_energies(std::vector<double>& e, bool with_rr)
   {
+    std::vector<double> vnl;
+    if (dmc.nlpp)
+    {
+      std::vector<double> rot((size_t)nw * 9);
+      for (int w = 0; w < nw; w++)
+      {
+        const double u1 = rng_uniform01(rng[w]);
+        const double u2 = rng_uniform01(rng[w]);
+        const double u3 = rng_uniform01(rng[w]);
+        rotation_from_uniforms(u1, u2, u3, rot.data() + (size_t)w * 9);
+      }
+      vnl.resize(nw);
+      nlpp_evaluate(rot.data(), vnl.data());
+    }
     eval_gl_enqueue();
     if (coul_out.n < (size_t)cap * 3)
       coul_out.alloc((size_t)cap * 3);
@@ -1064,6 +1099,8 @@ struct qmi_ctx
     {
       const double* v = h_pot.p + (size_t)w * 3;
       e[w]            = e[w] + v[0] + v[1] + v[2];
+      if (dmc.nlpp)
+        e[w] += vnl[w];
     }
   }
 
@@ -1104,15 +1141,18 @@ struct qmi_ctx
     dmc_pairs.alloc((size_t)cap * 2);
     h_pairs.alloc((size_t)cap * 2);
     h_rr.alloc((size_t)cap * 2);
-    dmc = DmcState{};
-    dmc.p = p;
-    std::vector<double> e;
-    dmc_local_energies(e, false);
+    if (nlpp_ready && cplx)
+      throw QmiError(QMI_ERR_UNSUPPORTED, "the pseudopotential energy is built for real orbitals");
+    dmc      = DmcState{};
+    dmc.p    = p;
+    dmc.nlpp = nlpp_ready;
     rng.clear();
     for (int c = 0; c < nw; c++)
       rng.emplace_back(p.seed + uint32_t(c));
     rng_seed   = p.seed;
     rng_seeded = true;
+    std::vector<double> e;
+    dmc_local_energies(e, false);
     dmc.wc_rng.seed(p.seed - 1u);
     dmc.weight.assign(cap, 1.0);
     dmc.eloc.assign(cap, 0.0);

@@ -88,3 +88,28 @@ def test_dmc_capacity_exceeded_is_an_error():
     c.dmc_init(tau=0.02, capacity=16, seed=3, warmup_steps=5, target_walkers=800)
     with pytest.raises(RuntimeError, match="exceeds the capacity"):
         c.dmc_run(5)
+
+
+def test_dmc_with_pseudopotential():
+    """The local energy with the non-local pseudopotential term, each walker's quadrature grid
+    rotated by three draws from its stream after the sweep (identical on both sides)."""
+    from test_oracle_nlpp import channels, icosahedron
+    s = make_system(L=7.0, n_ion=4, n_up=7, n_dn=6, norb=8, M=12, delay_rank=4, seed=29)
+    R = make_walkers(s, 20, 7)
+    zion = s.N / len(s.ions)
+    ls = [0, 1, 2]
+    dr, v, v2, _ = channels(ls, seed=4)
+    knots, w = icosahedron()
+    kw = dict(tau=0.02, capacity=80, warmup_steps=3, recompute_period=4)
+    c = qmi.Context(0)
+    c.system_create(s)
+    c.walkers_load(R, 1)
+    c.coulomb_setup(zion)
+    c.nlpp_setup(ls, dr, v, v2, 2.2, knots, w)
+    c.dmc_init(seed=51, **kw)
+    st = c.dmc_run(8)
+    o = orc.VMC(s, R, 1, 51)
+    o.dmc_set_nlpp(ls, dr, v, v2, 2.2, knots, w)
+    o.dmc_init(zion, seed=51, **kw)
+    sto = o.dmc_run(8)
+    check_dmc(c, o, st, sto)

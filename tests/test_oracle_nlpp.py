@@ -126,3 +126,40 @@ def test_full_wave_function_against_numpy_assembly():
             kp = sum((2 * l + 1) * float(sp[c](r)) * eval_legendre(l, cos) for c, l in enumerate(ls)) * w
             tot += float(np.sum(kp * ratio))
         assert abs(E[iw] - tot) < 1e-11 * max(1.0, abs(tot)), (E[iw], tot)
+
+
+def test_rotation_matrix_known_answers():
+    """generateRotationMatrix against the values of Numerics/tests/test_RotationMatrix3D.cpp:25-59."""
+    assert np.allclose(orc.rotation_matrix(0.0, 0.0, 0.0), np.eye(3), atol=1e-15)
+    r2 = np.array([[-0.459016994374947, 0.842075137094350, -0.283218753550188],
+                   [-0.489403985718866, 0.0263932022500210, 0.871657695220709],
+                   [0.741476323045772, 0.538714082201795, 0.400000000000000]])
+    r3 = np.array([[0.485410196624969, -0.352671151375484, 0.800000000000000],
+                   [-0.587785252292473, -0.809016994374947, 9.79717439317882e-17],
+                   [0.647213595499958, -0.470228201833979, -0.600000000000000]])
+    assert np.allclose(orc.rotation_matrix(0.1, 0.2, 0.3), r2, atol=1e-14)
+    assert np.allclose(orc.rotation_matrix(0.9, 0.5, 0.8), r3, atol=1e-14)
+    m = orc.rotation_matrix(0.37, 0.61, 0.83)
+    assert np.allclose(m @ m.T, np.eye(3), atol=1e-14) and np.isclose(np.linalg.det(m), 1.0)
+
+
+def test_dmc_local_energy_includes_pseudopotential():
+    """DMC's initial local energy = kinetic + Ewald + V_NL with the grid of walker w rotated by the
+    first three uniforms of its stream (seed + w)."""
+    s = make_system(L=6.0, n_ion=3, n_up=5, n_dn=4, norb=6, M=12, delay_rank=1, seed=7)
+    R = make_walkers(s, 4, 8)
+    zion = s.N / len(s.ions)
+    ls = [0, 1]
+    dr, v, v2, _ = channels(ls, seed=2)
+    knots, w = icosahedron()
+    ref = orc.VMC(s, R, 1, 1)
+    _, G, L = ref.evaluate_log()
+    base = -0.5 * (np.sum(G * G, axis=(1, 2)) + np.sum(L, axis=1)) + ref.potential(zion).sum(axis=1)
+    rot = np.array([orc.rotation_matrix(*orc.rng_uniform(40 + iw, 3)) for iw in range(4)])
+    vnl = ref.nlpp_energy(ls, dr, v, v2, 2.0, knots, w, rot)
+    assert np.any(np.abs(vnl) > 1e-3)
+    o = orc.VMC(s, R, 1, 1)
+    o.dmc_set_nlpp(ls, dr, v, v2, 2.0, knots, w)
+    o.dmc_init(zion, tau=0.02, capacity=16, seed=40, warmup_steps=2)
+    e0 = o.dmc_walker_data()[1]
+    assert np.allclose(e0, base + vnl, rtol=1e-12, atol=1e-12)
