@@ -133,10 +133,11 @@ def bench_nlpp(ctx, system, log, rcut=1.5, reps=3):
     return out
 
 
-def bench_dmc(ctx, steps, log, tau=0.01):
-    """DMC on the same walkers (local energy kinetic + Ewald Coulomb, fluctuating population,
-    population feedback toward the starting count): electron moves per second over `steps` timed
-    steps after 5 untimed ones, with the per-step device time of each part."""
+def bench_dmc(ctx, steps, log, tau=0.01, with_nlpp=False):
+    """DMC on the same walkers (local energy kinetic + Ewald Coulomb, plus the non-local
+    pseudopotential when one is set on the context; fluctuating population, feedback toward the
+    starting count): electron moves per second over `steps` timed steps after 5 untimed ones, with
+    the per-step device time of each part."""
     nw = ctx.nw
     ctx.dmc_init(tau=tau, capacity=4 * nw, seed=2024, warmup_steps=200)
     ctx.dmc_run(5)
@@ -150,6 +151,7 @@ def bench_dmc(ctx, steps, log, tau=0.01):
     moves = float(np.sum(st[:, 3])) * ctx.N
     per = {k: v["ms"] / steps for k, v in tm.items()}
     out = dict(moves_per_s=moves / dt, ms_per_step=dt * 1e3 / steps, tau=tau, steps=steps,
+               local_energy="kinetic + Ewald Coulomb" + (" + non-local pseudopotential" if with_nlpp else ""),
                walkers_mean=float(np.mean(st[:, 3])), walkers_min=int(np.min(st[:, 4])),
                walkers_max=int(np.max(st[:, 4])), energy_last=float(st[-1, 0]),
                device_ms_per_step=per)
@@ -441,7 +443,7 @@ def main():
     coul = coulomb_cost(system.N, len(system.ions), ctx.coulomb_nk())
     ctx.timing_enable(False)
     nlpp = bench_nlpp(ctx, system, log) if args.nlpp else None
-    dmc = bench_dmc(ctx, args.dmc_steps, log) if args.dmc_steps > 0 else None
+    dmc = bench_dmc(ctx, args.dmc_steps, log, with_nlpp=nlpp is not None) if args.dmc_steps > 0 else None
     del ctx
     n, N, nion, k = system.n_up, system.N, len(system.ions), system.delay_rank
     # the fused move kernel runs either once per move (move_one) or once per delay window of k
