@@ -113,3 +113,21 @@ def test_dmc_with_pseudopotential():
     o.dmc_init(zion, seed=51, **kw)
     sto = o.dmc_run(8)
     check_dmc(c, o, st, sto)
+
+
+def test_dmc_mixed_precision_runs():
+    """DMC on the FP32 determinant engine: the same driver, finite energies, a living population
+    and a weighted energy close to the FP64 run's at the first step."""
+    s = make_system(L=7.0, n_ion=3, n_up=7, n_dn=6, norb=8, M=12, delay_rank=4, seed=13)
+    R = make_walkers(s, 24, 5)
+    zion = s.N / len(s.ions)
+    out = []
+    for prec in ("fp64", "fp32"):
+        c = qmi.Context(0, precision=prec)
+        c.system_create(s)
+        c.walkers_load(R, 1)
+        c.coulomb_setup(zion)
+        c.dmc_init(tau=0.02, capacity=96, seed=21, warmup_steps=4)
+        out.append(c.dmc_run(6))
+    assert np.all(np.isfinite(out[1])) and np.all(out[1][:, 4] > 0)
+    assert abs(out[1][0, 0] - out[0][0, 0]) < 1e-3 * max(1.0, abs(out[0][0, 0]))
