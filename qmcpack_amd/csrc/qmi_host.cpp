@@ -926,6 +926,11 @@ struct qmi_ctx
     c.drinv = 1.0 / dr;
     c.rcut  = rcut;
     c.nknot = nknot;
+    {
+      // quadrature points per pass of the kernel: QMI_NLPP_POINTS = 6 (default) or 12
+      const char* e = std::getenv("QMI_NLPP_POINTS");
+      c.kc          = (e && std::atoi(e) == 12) ? 12 : 6;
+    }
     // the spline's (1 - t) y_i + t y_{i+1} + h ((1 - t)^3 - (1 - t)) y2_i + h (t^3 - t) y2_{i+1},
     // h = dr^2 / 6, as a cubic in t per interval
     std::vector<double> poly((size_t)nch * (ng - 1) * 4);

@@ -110,6 +110,7 @@ struct NlppC
   const double* vpoly;    // [nch][ng - 1][4] v_l(r) = a + t (b + t (c + t d)) per interval
   int nknot;
   const double* knots;    // [nknot][4] unit vector, weight
+  int kc;                 // quadrature points evaluated together by the kernel (6 or 12)
 };
 // out[nw] = non-local energy of each walker; rot[nw][9] row-major rotation of the grid; part[nw][N]
 hipError_t launch_nlpp(const SysC& s, const State& st, const void* coefs, const NlppC& c, int nw, const double* rot,

@@ -14,8 +14,7 @@
 namespace qmi
 {
 
-constexpr int NL_KC = 12; // quadrature points evaluated together
-
+template<int NL_KC>
 struct NlLDS
 {
   alignas(16) double red[4][3 * NL_KC]; // per-wave partials: det, J1, J2 of each point
@@ -38,14 +37,14 @@ __device__ inline double nlpp_vl(const NlppC& c, int ch, double r)
   return q[0] + t * (q[1] + t * (q[2] + t * q[3]));
 }
 
-// Up to NL_KC quadrature points at once: each thread accumulates, for every point, its orbitals'
+// Up to NL_KC (6 or 12) quadrature points at once (6 at occupancy 3 is the faster form): each thread accumulates, for every point, its orbitals'
 // values dotted with the inverse row and its ions' / electrons' Jastrow values, so that the
 // stencil gathers of all points are in flight together and one reduction serves them all.
-template<typename ST, typename DT, int NJ, int MAXM>
-__global__ __launch_bounds__(WG) void k_nlpp(SysC s, State st, const ST* __restrict__ coefs, NlppC c,
+template<typename ST, typename DT, int NJ, int MAXM, int NL_KC>
+__global__ __launch_bounds__(WG, NL_KC == 6 ? 3 : 1) void k_nlpp(SysC s, State st, const ST* __restrict__ coefs, NlppC c,
                                              const double* __restrict__ rot, double* __restrict__ part)
 {
-  __shared__ NlLDS sh;
+  __shared__ NlLDS<NL_KC> sh;
   const int iel = blockIdx.x, w = blockIdx.y, t = threadIdx.x;
   const int ni = s.nion, N = s.N, Np = s.Npad;
   const double* R    = st.R + (size_t)w * 3 * Np;
@@ -296,10 +295,15 @@ hipError_t launch_nlpp(const SysC& s, const State& st, const void* coefs, const 
     using ST = decltype(stv);
     using DT = decltype(dtv);
     const dim3 grid(s.N, nw);
-    if (mm == 1)
-      k_nlpp<ST, DT, 2, 1><<<grid, WG, 0, stream>>>(s, st, static_cast<const ST*>(coefs), c, rot, part);
+    const ST* cf = static_cast<const ST*>(coefs);
+    if (mm == 1 && c.kc == 6)
+      k_nlpp<ST, DT, 2, 1, 6><<<grid, WG, 0, stream>>>(s, st, cf, c, rot, part);
+    else if (mm == 1)
+      k_nlpp<ST, DT, 2, 1, 12><<<grid, WG, 0, stream>>>(s, st, cf, c, rot, part);
+    else if (c.kc == 6)
+      k_nlpp<ST, DT, 4, 2, 6><<<grid, WG, 0, stream>>>(s, st, cf, c, rot, part);
     else
-      k_nlpp<ST, DT, 4, 2><<<grid, WG, 0, stream>>>(s, st, static_cast<const ST*>(coefs), c, rot, part);
+      k_nlpp<ST, DT, 4, 2, 12><<<grid, WG, 0, stream>>>(s, st, cf, c, rot, part);
     return hipGetLastError();
   });
   if (e != hipSuccess)
