@@ -210,11 +210,11 @@ __device__ inline void phaseC(const SysC& s, int g, int iat, int n, const double
   block_sum<4>(acc, sh.red, sh.bc + BC_C + 9);
 }
 
-// Phase D helper (one thread): total ratio, gradient at the proposed position, log G_b.
-__device__ inline void combine_ratio(const SysC& s, int iat, const double* __restrict__ jas, const MoveLDS& sh,
-                                     double& ratio, double gnew[3])
+// Phase D helper (one thread): total ratio, gradient at the proposed position, log G_b; vat and
+// uat are the electron's current J1 and J2 values.
+__device__ inline void combine_ratio_v(const SysC& s, double vat, double uat, const MoveLDS& sh, double& ratio,
+                                       double gnew[3])
 {
-  const int Np = s.Npad;
   const double* C = sh.bc + BC_C;
   ratio           = 1.0;
   gnew[0] = gnew[1] = gnew[2] = 0.0;
@@ -230,15 +230,23 @@ __device__ inline void combine_ratio(const SysC& s, int iat, const double* __res
     gnew[0] += C[1];
     gnew[1] += C[2];
     gnew[2] += C[3];
-    ratio *= exp(jas[J_VAT * Np + iat] - C[0]);
+    ratio *= exp(vat - C[0]);
   }
   if (s.has_j2)
   {
     gnew[0] += C[6];
     gnew[1] += C[7];
     gnew[2] += C[8];
-    ratio *= exp(jas[J_UAT * Np + iat] - C[5]);
+    ratio *= exp(uat - C[5]);
   }
+}
+
+__device__ inline void combine_ratio(const SysC& s, int iat, const double* __restrict__ jas, const MoveLDS& sh,
+                                     double& ratio, double gnew[3])
+{
+  const int Np = s.Npad;
+  combine_ratio_v(s, s.has_j1 ? jas[J_VAT * Np + iat] : 0.0, s.has_j2 ? jas[J_UAT * Np + iat] : 0.0, sh, ratio,
+                  gnew);
 }
 
 // Phase E: accept (sh.bc[BC_ACC] != 0) or reject.  All threads.
@@ -407,6 +415,24 @@ __device__ inline void phaseE(const SysC& s, int g, int iat, int n, int ld, int 
 
 //------------------------------------------------------------------------------------------------
 // Phase D core (thread 0): returns prob_eff; flag = |ratio|^2 >= eps.
+__device__ inline double mh_prob_v(const SysC& s, double vat, double uat, const MoveArgs& a, MoveLDS& sh,
+                                   double& ratio, bool& flag)
+{
+  const double tauovermass = a.tau;
+  const double oneover2tau = 0.5 / tauovermass;
+  double gnew[3];
+  combine_ratio_v(s, vat, uat, sh, ratio, gnew);
+  double drev[3];
+  unr_drift(tauovermass, gnew, drev);
+  for (int q = 0; q < 3; q++)
+    drev[q] += sh.bc[BC_MOVE + q];
+  const double log_gb = -oneover2tau * (drev[0] * drev[0] + drev[1] * drev[1] + drev[2] * drev[2]);
+  const double prob   = ratio * ratio;
+  flag                = prob >= DBL_EPS;
+  sh.bc[BC_RATIO]     = ratio;
+  return prob * exp(log_gb - sh.bc[BC_LOGGF]);
+}
+
 __device__ inline double mh_prob(const SysC& s, int iat, const MoveArgs& a, const double* __restrict__ jas,
                                  MoveLDS& sh, double& ratio, bool& flag)
 {
@@ -442,26 +468,29 @@ __device__ inline void fused_move(const SysC& s, const State& st, const ST* __re
   {
     ainv_r[m] = inv[m] = 0.0;
   }
+  prefetch_move(s, iat, w, ucount, a, v.R, v.jas, sh);
   if (s.has_det)
   {
     phaseA<DT, MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, sB, ainv_r, inv);
     det_grad<DT, MAXM>(n, ld, r, v.dpsi, inv, sh);
   }
+  else
+    __syncthreads(); // publishes the prefetched scalars
   if (threadIdx.x == 0)
-    propose(s, iat, w, a, v.R, v.jas, sh);
+    propose_pf(s, a, sh);
   __syncthreads();
   phaseC<ST, MAXM>(s, g, iat, n, v.R, st.ions, coefs, inv, phv, sh);
   if (threadIdx.x == 0)
   {
     double ratio;
     bool flag;
-    const double pe = mh_prob(s, iat, a, v.jas, sh, ratio, flag);
+    const double pe = mh_prob_v(s, sh.bc[BC_PF + PF_VAT], sh.bc[BC_PF + PF_UAT], a, sh, ratio, flag);
     if (a.dmc) // a node crossing (ratio <= 0) is rejected before the draw; prob_eff against eps
       flag = ratio > 0.0 && pe >= DBL_EPS;
     bool acc = false;
     if (flag)
     {
-      const double u = a.unif[(size_t)w * N + ucount];
+      const double u = sh.bc[BC_PF + PF_UNIF]; // a.unif[w N + ucount]
       ucount++;
       acc = u < pe;
     }

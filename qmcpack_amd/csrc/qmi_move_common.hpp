@@ -249,6 +249,78 @@ __device__ inline WalkerView<DT> walker_view(const SysC& s, const State& st, int
 }
 
 // Phase B (thread 0): proposal from the gradient at the current position.
+// Scalars that thread 0 needs later in the move, loaded at its start by 15 threads into
+// sh.bc[BC_PF, BC_PF + 15) so that their latency overlaps the inverse-row phase: the walker's next
+// uniform, the normal deviate, the electron's position, its J1 and J2 gradients and values
+// (published by the caller's next barrier).
+constexpr int BC_PF = 33;
+enum PrefetchSlot
+{
+  PF_UNIF = 0,
+  PF_GAUSS = 1,
+  PF_R = 4,
+  PF_G1 = 7,
+  PF_G2 = 10,
+  PF_VAT = 13,
+  PF_UAT = 14
+};
+template<typename SH>
+__device__ inline void prefetch_move(const SysC& s, int iat, int w, int ucount, const MoveArgs& a,
+                                     const double* __restrict__ R, const double* __restrict__ jas, SH& sh)
+{
+  const int t = threadIdx.x, Np = s.Npad;
+  if (t >= 15)
+    return;
+  double v = 0.0;
+  if (t == PF_UNIF)
+    v = a.unif[(size_t)w * s.N + ucount];
+  else if (t < PF_R)
+    v = a.gauss[((size_t)iat * a.nw + w) * 3 + (t - PF_GAUSS)];
+  else if (t < PF_G1)
+    v = R[(t - PF_R) * Np + iat];
+  else if (t < PF_G2)
+    v = s.has_j1 ? jas[(J_G1X + t - PF_G1) * Np + iat] : 0.0;
+  else if (t < PF_VAT)
+    v = s.has_j2 ? jas[(J_DUX + t - PF_G2) * Np + iat] : 0.0;
+  else if (t == PF_VAT)
+    v = s.has_j1 ? jas[J_VAT * Np + iat] : 0.0;
+  else
+    v = s.has_j2 ? jas[J_UAT * Np + iat] : 0.0;
+  sh.bc[BC_PF + t] = v;
+}
+
+// propose() from the prefetched scalars (thread 0; the same arithmetic)
+template<typename SH>
+__device__ inline void propose_pf(const SysC& s, const MoveArgs& a, SH& sh)
+{
+  const double tauovermass = a.tau;
+  const double sqrttau     = sqrt(tauovermass);
+  const double oneover2tau = 0.5 / tauovermass;
+  const double* pf         = sh.bc + BC_PF;
+  double gnow[3]           = {0, 0, 0};
+  if (s.has_det)
+    for (int q = 0; q < 3; q++)
+      gnow[q] += sh.bc[BC_GRAD + q];
+  if (s.has_j1)
+    for (int q = 0; q < 3; q++)
+      gnow[q] += pf[PF_G1 + q];
+  if (s.has_j2)
+    for (int q = 0; q < 3; q++)
+      gnow[q] += pf[PF_G2 + q];
+  double delta[3], drift[3];
+  for (int q = 0; q < 3; q++)
+    delta[q] = pf[PF_GAUSS + q] * sqrttau;
+  unr_drift(tauovermass, gnow, drift);
+  for (int q = 0; q < 3; q++)
+  {
+    sh.bc[BC_GNOW + q] = gnow[q];
+    drift[q] += delta[q];
+    sh.bc[BC_MOVE + q] = drift[q];
+    sh.bc[BC_POS + q]  = pf[PF_R + q] + drift[q];
+  }
+  sh.bc[BC_LOGGF] = -oneover2tau * (delta[0] * delta[0] + delta[1] * delta[1] + delta[2] * delta[2]);
+}
+
 template<typename SH>
 __device__ inline void propose(const SysC& s, int iat, int w, const MoveArgs& a, const double* __restrict__ R,
                                const double* __restrict__ jas, SH& sh)
