@@ -145,6 +145,42 @@ __device__ inline void phaseA(const SysC& s, int n, int ld, int k, int d, int r,
   }
 }
 
+// the orbital gradients of row r, loaded before phase A so that their latency overlaps it
+template<typename DT, int MAXM>
+__device__ inline void load_dpsi_row(int n, int ld, int r, const DT* __restrict__ dpsi, double (&dg)[MAXM][3])
+{
+  const int t    = threadIdx.x;
+  const DT* drow = dpsi + (size_t)r * 4 * ld;
+#pragma unroll
+  for (int m = 0; m < MAXM; m++)
+  {
+    const int o = t + WG * m;
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+      dg[m][q] = o < n ? (double)drow[q * ld + o] : 0.0;
+  }
+}
+
+// det_grad from preloaded gradients (the same sums)
+template<int MAXM>
+__device__ inline void det_grad_pre(int n, const double (&dg)[MAXM][3], const double (&inv)[MAXM], MoveLDS& sh)
+{
+  const int t  = threadIdx.x;
+  double gv[3] = {0, 0, 0};
+#pragma unroll
+  for (int m = 0; m < MAXM; m++)
+  {
+    const int o = t + WG * m;
+    if (o < n)
+    {
+      gv[0] += inv[m] * dg[m][0];
+      gv[1] += inv[m] * dg[m][1];
+      gv[2] += inv[m] * dg[m][2];
+    }
+  }
+  block_sum<3>(gv, sh.red, sh.bc + BC_GRAD);
+}
+
 // determinant gradient at the current position: invRow . dpsiM[r]  -> sh.bc[BC_GRAD..+3)
 template<typename DT, int MAXM>
 __device__ inline void det_grad(int n, int ld, int r, const DT* __restrict__ dpsi, const double (&inv)[MAXM],
@@ -471,8 +507,10 @@ __device__ inline void fused_move(const SysC& s, const State& st, const ST* __re
   prefetch_move(s, iat, w, ucount, a, v.R, v.jas, sh);
   if (s.has_det)
   {
+    double dg[MAXM][3];
+    load_dpsi_row<DT, MAXM>(n, ld, r, v.dpsi, dg);
     phaseA<DT, MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, sB, ainv_r, inv);
-    det_grad<DT, MAXM>(n, ld, r, v.dpsi, inv, sh);
+    det_grad_pre<MAXM>(n, dg, inv, sh);
   }
   else
     __syncthreads(); // publishes the prefetched scalars
