@@ -327,18 +327,6 @@ def bench_uneven(args, log):
             "bytes_per_launch": bytes_launch,
             "avg_launch_ms": avg_ms,
         },
-        "dmc": dmc,
-        "nlpp": nlpp,
-        "coulomb_roofline": {
-            "kernel": "k_coulomb (Ewald: short-range pair sums from an LDS spline, long-range rho_k)",
-            "avg_ms": tc["ms"] / max(1, tc["calls"]),
-            "walkers": nw,
-            "k_vectors": coul["k_terms"] // N,
-            "flops_per_walker": coul["flops"],
-            "achieved_tflops": nw * coul["flops"] / (tc["ms"] / max(1, tc["calls"]) * 1e-3) / 1e12
-                               if tc["ms"] > 0 else 0.0,
-            "fp64_peak_tflops": FP64_PEAK_TFLOPS,
-        },
     }
     if not args.no_cpu:
         from oracle import oracle as orc

@@ -246,6 +246,49 @@ __device__ inline void phaseC(const SysC& s, int g, int iat, int n, const double
   block_sum<4>(acc, sh.red, sh.bc + BC_C + 9);
 }
 
+// phase C with this thread's electron positions loaded earlier (the fused move)
+template<typename ST, int MAXM>
+__device__ inline void phaseC_p(const SysC& s, int g, int iat, int n, const double (&px)[2 * MAXM],
+                                const double (&py)[2 * MAXM], const double (&pz)[2 * MAXM],
+                                const double* __restrict__ ions, const ST* __restrict__ coefs,
+                                const double (&inv)[MAXM], double (&phv)[MAXM][5], MoveLDS& sh)
+{
+  const int t = threadIdx.x;
+  const int N = s.N, Np = s.Npad;
+  const double pos[3] = {sh.bc[BC_POS], sh.bc[BC_POS + 1], sh.bc[BC_POS + 2]};
+  if (s.has_det && t == 64) // locate once; the Jastrow reduction's barriers publish it
+  {
+    SplineLoc L;
+    spline_locate(s, pos, L);
+    spline_stage(L, sh.spl);
+  }
+  jastrow_ratio_sums_p<2 * MAXM>(s, g, iat, px, py, pz, ions, pos, sh);
+  double acc[4] = {0, 0, 0, 0};
+  if (s.has_det)
+  {
+#pragma unroll
+    for (int m = 0; m < MAXM; m++)
+    {
+      const int o = t + WG * m;
+      if (o < n)
+      {
+        spline_vgl_orbital_lds<ST>(s, coefs, sh.spl, o, phv[m]);
+        acc[0] += inv[m] * phv[m][0];
+        acc[1] += inv[m] * phv[m][1];
+        acc[2] += inv[m] * phv[m][2];
+        acc[3] += inv[m] * phv[m][3];
+      }
+      else
+      {
+#pragma unroll
+        for (int q = 0; q < 5; q++)
+          phv[m][q] = 0.0;
+      }
+    }
+  }
+  block_sum<4>(acc, sh.red, sh.bc + BC_C + 9);
+}
+
 // Phase D helper (one thread): total ratio, gradient at the proposed position, log G_b; vat and
 // uat are the electron's current J1 and J2 values.
 __device__ inline void combine_ratio_v(const SysC& s, double vat, double uat, const MoveLDS& sh, double& ratio,
@@ -505,6 +548,9 @@ __device__ inline void fused_move(const SysC& s, const State& st, const ST* __re
     ainv_r[m] = inv[m] = 0.0;
   }
   prefetch_move(s, iat, w, ucount, a, v.R, v.jas, sh);
+  double ex[2 * MAXM], ey[2 * MAXM], ez[2 * MAXM];
+  if (s.has_j2)
+    load_electrons<2 * MAXM>(s, v.R, ex, ey, ez);
   if (s.has_det)
   {
     double dg[MAXM][3];
@@ -517,7 +563,7 @@ __device__ inline void fused_move(const SysC& s, const State& st, const ST* __re
   if (threadIdx.x == 0)
     propose_pf(s, a, sh);
   __syncthreads();
-  phaseC<ST, MAXM>(s, g, iat, n, v.R, st.ions, coefs, inv, phv, sh);
+  phaseC_p<ST, MAXM>(s, g, iat, n, ex, ey, ez, st.ions, coefs, inv, phv, sh);
   if (threadIdx.x == 0)
   {
     double ratio;
@@ -588,7 +634,7 @@ __global__ __launch_bounds__(WG, 4) void k_sweep_segment(SysC s, State st, const
 // The same fused move for a single electron a.iat0 (no move loop in the kernel: the register
 // allocation of the loop version is about twice that of one move, which halves occupancy).
 template<typename ST, typename DT, int MAXM>
-__global__ __launch_bounds__(WG) void k_move_one(SysC s, State st, const ST* __restrict__ coefs, MoveArgs a)
+__global__ __launch_bounds__(WG, 4) void k_move_one(SysC s, State st, const ST* __restrict__ coefs, MoveArgs a)
 {
   __shared__ MoveLDS sh;
   extern __shared__ double sB_raw[];

@@ -75,12 +75,46 @@ enum BcField
 
 // J1/J2 ratio sums at the proposed position pos -> sh.bc[BC_C..BC_C+9): J1 (u, g3, lap), J2 (u, g3)
 // (J1OrbitalSoA.h:136-186 computeU3; TwoBodyJastrow.cpp:420-446).  All threads; ends with a barrier.
+// positions of this thread's electrons t + u WG (zero past N)
+template<int NJ>
+__device__ inline void load_electrons(const SysC& s, const double* __restrict__ R, double (&px)[NJ], double (&py)[NJ],
+                                      double (&pz)[NJ])
+{
+  const int t = threadIdx.x, N = s.N, Np = s.Npad;
+#pragma unroll
+  for (int u = 0; u < NJ; u++)
+  {
+    const int j   = t + u * WG;
+    const bool ok = j < N;
+    px[u]         = ok ? R[j] : 0.0;
+    py[u]         = ok ? R[Np + j] : 0.0;
+    pz[u]         = ok ? R[2 * Np + j] : 0.0;
+  }
+}
+
+// the same sums with this thread's electron positions already in registers
+template<int NJ, typename SH>
+__device__ inline void jastrow_ratio_sums_p(const SysC& s, int g, int iat, const double (&px)[NJ],
+                                            const double (&py)[NJ], const double (&pz)[NJ],
+                                            const double* __restrict__ ions, const double pos[3], SH& sh);
+
 template<int NJ, typename SH>
 __device__ inline void jastrow_ratio_sums(const SysC& s, int g, int iat, const double* __restrict__ R,
                                           const double* __restrict__ ions, const double pos[3], SH& sh)
 {
+  double px[NJ], py[NJ], pz[NJ];
+  if (s.has_j2)
+    load_electrons<NJ>(s, R, px, py, pz);
+  jastrow_ratio_sums_p<NJ>(s, g, iat, px, py, pz, ions, pos, sh);
+}
+
+template<int NJ, typename SH>
+__device__ inline void jastrow_ratio_sums_p(const SysC& s, int g, int iat, const double (&px)[NJ],
+                                            const double (&py)[NJ], const double (&pz)[NJ],
+                                            const double* __restrict__ ions, const double pos[3], SH& sh)
+{
   const int t = threadIdx.x;
-  const int N = s.N, Np = s.Npad;
+  const int N = s.N;
   double acc[9];
 #pragma unroll
   for (int i = 0; i < 9; i++)
@@ -103,17 +137,6 @@ __device__ inline void jastrow_ratio_sums(const SysC& s, int g, int iat, const d
     }
   if (s.has_j2)
   {
-    // all positions first, so that the loads of every electron are in flight together
-    double px[NJ], py[NJ], pz[NJ];
-#pragma unroll
-    for (int u = 0; u < NJ; u++)
-    {
-      const int j   = t + u * WG;
-      const bool ok = j < N;
-      px[u]         = ok ? R[j] : 0.0;
-      py[u]         = ok ? R[Np + j] : 0.0;
-      pz[u]         = ok ? R[2 * Np + j] : 0.0;
-    }
 #pragma unroll
     for (int u = 0; u < NJ; u++)
     {
