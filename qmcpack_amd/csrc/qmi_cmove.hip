@@ -355,10 +355,10 @@ __device__ inline void phaseC_c(const SysC& s, int g, int iat, int n, const doub
 }
 
 // Total complex ratio and gradient at the proposed position from the phase-C sums (one thread).
-__device__ inline void combine_ratio_c(const SysC& s, int iat, const double* __restrict__ jas, const double* bc,
-                                       double2& ratio, double2 gnew[3])
+// vat, uat: the electron's current J1 and J2 values
+__device__ inline void combine_ratio_cv(const SysC& s, double vat, double uat, const double* bc, double2& ratio,
+                                        double2 gnew[3])
 {
-  const int Np    = s.Npad;
   const double* C = bc + BC_C;
   ratio           = zmk(1, 0);
   gnew[0] = gnew[1] = gnew[2] = zmk(0, 0);
@@ -374,17 +374,45 @@ __device__ inline void combine_ratio_c(const SysC& s, int iat, const double* __r
   {
     for (int q = 0; q < 3; q++)
       gnew[q].x += C[1 + q];
-    ratio = zscale(ratio, exp(jas[J_VAT * Np + iat] - C[0]));
+    ratio = zscale(ratio, exp(vat - C[0]));
   }
   if (s.has_j2)
   {
     for (int q = 0; q < 3; q++)
       gnew[q].x += C[6 + q];
-    ratio = zscale(ratio, exp(jas[J_UAT * Np + iat] - C[5]));
+    ratio = zscale(ratio, exp(uat - C[5]));
   }
 }
 
+__device__ inline void combine_ratio_c(const SysC& s, int iat, const double* __restrict__ jas, const double* bc,
+                                       double2& ratio, double2 gnew[3])
+{
+  const int Np = s.Npad;
+  combine_ratio_cv(s, s.has_j1 ? jas[J_VAT * Np + iat] : 0.0, s.has_j2 ? jas[J_UAT * Np + iat] : 0.0, bc, ratio,
+                   gnew);
+}
+
 // Phase D core (thread 0): returns prob_eff; flag = |ratio|^2 >= eps; |ratio| in ratio_abs.
+__device__ inline double mh_prob_cv(const SysC& s, double vat, double uat, const MoveArgs& a, CMoveLDS& sh,
+                                    double& ratio_abs, bool& flag)
+{
+  const double tauovermass = a.tau;
+  const double oneover2tau = 0.5 / tauovermass;
+  double2 ratio, gz[3];
+  combine_ratio_cv(s, vat, uat, sh.bc, ratio, gz);
+  const double gnew[3] = {gz[0].x, gz[1].x, gz[2].x};
+  double drev[3];
+  unr_drift(tauovermass, gnew, drev);
+  for (int q = 0; q < 3; q++)
+    drev[q] += sh.bc[BC_MOVE + q];
+  const double log_gb = -oneover2tau * (drev[0] * drev[0] + drev[1] * drev[1] + drev[2] * drev[2]);
+  const double prob   = znorm(ratio);
+  flag                = prob >= DBL_EPS;
+  ratio_abs           = sqrt(prob);
+  sh.bc[BC_RATIO]     = ratio_abs;
+  return prob * exp(log_gb - sh.bc[BC_LOGGF]);
+}
+
 __device__ inline double mh_prob_c(const SysC& s, int iat, const MoveArgs& a, const double* __restrict__ jas,
                                    CMoveLDS& sh, double& ratio_abs, bool& flag)
 {
@@ -581,26 +609,29 @@ __global__ __launch_bounds__(WG) void k_move_one_c(SysC s, State st, const doubl
 #pragma unroll
   for (int m = 0; m < MAXM; m++)
     ainv_r[m] = inv[m] = zmk(0, 0);
+  prefetch_move(s, iat, w, ucount, a, v.R, v.jas, sh);
   if (s.has_det)
   {
     phaseA_c<MAXM>(n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, sBz, ainv_r, inv);
     det_grad_c<MAXM>(n, ld, r, v.dpsi, inv, sh);
   }
+  else
+    __syncthreads(); // publishes the prefetched scalars
   if (threadIdx.x == 0)
-    propose(s, iat, w, a, v.R, v.jas, sh);
+    propose_pf(s, a, sh);
   __syncthreads();
   phaseC_c<MAXM>(s, g, iat, n, v.R, st.ions, coefs, st.kpts, inv, phv, sh);
   if (threadIdx.x == 0)
   {
     double ratio_abs;
     bool flag;
-    const double pe = mh_prob_c(s, iat, a, v.jas, sh, ratio_abs, flag);
+    const double pe = mh_prob_cv(s, sh.bc[BC_PF + PF_VAT], sh.bc[BC_PF + PF_UAT], a, sh, ratio_abs, flag);
     if (a.dmc) // complex wave function: no node check (SFNBranch.h:161-169), prob_eff against eps
       flag = pe >= DBL_EPS;
     bool acc = false;
     if (flag)
     {
-      const double u = a.unif[(size_t)w * N + ucount];
+      const double u = sh.bc[BC_PF + PF_UNIF]; // a.unif[w N + ucount]
       ucount++;
       acc = u < pe;
     }
