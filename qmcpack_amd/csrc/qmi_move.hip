@@ -246,48 +246,6 @@ __device__ inline void phaseC(const SysC& s, int g, int iat, int n, const double
   block_sum<4>(acc, sh.red, sh.bc + BC_C + 9);
 }
 
-// phase C with this thread's electron positions loaded earlier (the fused move)
-template<typename ST, int MAXM>
-__device__ inline void phaseC_p(const SysC& s, int g, int iat, int n, const double (&px)[2 * MAXM],
-                                const double (&py)[2 * MAXM], const double (&pz)[2 * MAXM],
-                                const double* __restrict__ ions, const ST* __restrict__ coefs,
-                                const double (&inv)[MAXM], double (&phv)[MAXM][5], MoveLDS& sh)
-{
-  const int t = threadIdx.x;
-  const int N = s.N, Np = s.Npad;
-  const double pos[3] = {sh.bc[BC_POS], sh.bc[BC_POS + 1], sh.bc[BC_POS + 2]};
-  if (s.has_det && t == 64) // locate once; the Jastrow reduction's barriers publish it
-  {
-    SplineLoc L;
-    spline_locate(s, pos, L);
-    spline_stage(L, sh.spl);
-  }
-  jastrow_ratio_sums_p<2 * MAXM>(s, g, iat, px, py, pz, ions, pos, sh);
-  double acc[4] = {0, 0, 0, 0};
-  if (s.has_det)
-  {
-#pragma unroll
-    for (int m = 0; m < MAXM; m++)
-    {
-      const int o = t + WG * m;
-      if (o < n)
-      {
-        spline_vgl_orbital_lds<ST>(s, coefs, sh.spl, o, phv[m]);
-        acc[0] += inv[m] * phv[m][0];
-        acc[1] += inv[m] * phv[m][1];
-        acc[2] += inv[m] * phv[m][2];
-        acc[3] += inv[m] * phv[m][3];
-      }
-      else
-      {
-#pragma unroll
-        for (int q = 0; q < 5; q++)
-          phv[m][q] = 0.0;
-      }
-    }
-  }
-  block_sum<4>(acc, sh.red, sh.bc + BC_C + 9);
-}
 
 // Phase D helper (one thread): total ratio, gradient at the proposed position, log G_b; vat and
 // uat are the electron's current J1 and J2 values.
@@ -548,9 +506,6 @@ __device__ inline void fused_move(const SysC& s, const State& st, const ST* __re
     ainv_r[m] = inv[m] = 0.0;
   }
   prefetch_move(s, iat, w, ucount, a, v.R, v.jas, sh);
-  double ex[2 * MAXM], ey[2 * MAXM], ez[2 * MAXM];
-  if (s.has_j2)
-    load_electrons<2 * MAXM>(s, v.R, ex, ey, ez);
   if (s.has_det)
   {
     double dg[MAXM][3];
@@ -563,7 +518,7 @@ __device__ inline void fused_move(const SysC& s, const State& st, const ST* __re
   if (threadIdx.x == 0)
     propose_pf(s, a, sh);
   __syncthreads();
-  phaseC_p<ST, MAXM>(s, g, iat, n, ex, ey, ez, st.ions, coefs, inv, phv, sh);
+  phaseC<ST, MAXM>(s, g, iat, n, v.R, st.ions, coefs, inv, phv, sh);
   if (threadIdx.x == 0)
   {
     double ratio;
@@ -634,7 +589,7 @@ __global__ __launch_bounds__(WG, 4) void k_sweep_segment(SysC s, State st, const
 // The same fused move for a single electron a.iat0 (no move loop in the kernel: the register
 // allocation of the loop version is about twice that of one move, which halves occupancy).
 template<typename ST, typename DT, int MAXM>
-__global__ __launch_bounds__(WG, 4) void k_move_one(SysC s, State st, const ST* __restrict__ coefs, MoveArgs a)
+__global__ __launch_bounds__(WG) void k_move_one(SysC s, State st, const ST* __restrict__ coefs, MoveArgs a)
 {
   __shared__ MoveLDS sh;
   extern __shared__ double sB_raw[];
@@ -1487,13 +1442,17 @@ __global__ __launch_bounds__(WG, 4) void k_learn_move(SysC s, State st, const ST
   const int g = a.g, n = s.n[g], ld = s.ld, k = s.k, N = s.N, Np = s.Npad;
   const int iat = a.iat0, r = iat - s.first[g], d = a.d0;
   // move scalars in sh.bc beyond the plain move's fields
-  constexpr int L_R = 32, L_V = 35, L_XI = 38, L_LQF = 41, L_GNEW = 42, L_VR = 45;
-  static_assert(L_VR + 3 <= 48, "broadcast slots");
+  constexpr int L_R = 32, L_V = 35, L_XI = 38, L_LQF = 41, L_GNEW = 42, L_VR = 45, L_PF = 48;
+  static_assert(L_VR + 3 <= L_PF && L_PF + 15 <= 64, "broadcast slots");
   WalkerView<DT> v = walker_view<DT>(s, st, g, w);
   double ainv_r[MAXM], inv[MAXM], phv[MAXM][5];
 #pragma unroll
   for (int m = 0; m < MAXM; m++)
     ainv_r[m] = inv[m] = 0.0;
+  // the scalars thread 0 needs later, loaded now (published by the barrier after phase A)
+  int ucount = a.ubase[(size_t)w * (N + 1) + iat];
+  prefetch_move(s, iat, w, ucount, a, v.R, v.jas, sh, L_PF);
+  const double* pf = sh.bc + L_PF;
   if (s.has_det)
   {
     phaseA<DT, MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, sB, ainv_r, inv);
@@ -1502,12 +1461,20 @@ __global__ __launch_bounds__(WG, 4) void k_learn_move(SysC s, State st, const ST
   __syncthreads();
   if (threadIdx.x == 0)
   {
-    double gn[3], dv[3];
-    grad_now(s, iat, v.jas, sh, gn);
+    double gn[3] = {0, 0, 0}, dv[3];
+    if (s.has_det) // grad_now from the prefetched Jastrow gradients
+      for (int q = 0; q < 3; q++)
+        gn[q] += sh.bc[BC_GRAD + q];
+    if (s.has_j1)
+      for (int q = 0; q < 3; q++)
+        gn[q] += pf[PF_G1 + q];
+    if (s.has_j2)
+      for (int q = 0; q < 3; q++)
+        gn[q] += pf[PF_G2 + q];
     unr_drift(a.tau, gn, dv);
     for (int q = 0; q < 3; q++)
     {
-      sh.bc[L_R + q]     = v.R[q * Np + iat];
+      sh.bc[L_R + q]     = pf[PF_R + q];
       sh.bc[L_V + q]     = dv[q];
       sh.bc[BC_GNOW + q] = gn[q];
     }
@@ -1523,7 +1490,7 @@ __global__ __launch_bounds__(WG, 4) void k_learn_move(SysC s, State st, const ST
   mlp_single(P, xf, sh, mo);
   if (threadIdx.x == 0)
   {
-    const double* gz = a.gauss + ((size_t)iat * a.nw + w) * 3;
+    const double* gz = pf + PF_GAUSS;
     const double sig = sqrt(a.tau) * exp((double)mo[3]);
     double xx        = 0;
     for (int q = 0; q < 3; q++)
@@ -1541,7 +1508,7 @@ __global__ __launch_bounds__(WG, 4) void k_learn_move(SysC s, State st, const ST
   if (threadIdx.x == 0)
   {
     double ratio, gnew[3], vr[3];
-    combine_ratio(s, iat, v.jas, sh, ratio, gnew);
+    combine_ratio_v(s, pf[PF_VAT], pf[PF_UAT], sh, ratio, gnew);
     unr_drift(a.tau, gnew, vr);
     sh.bc[BC_RATIO] = ratio;
     for (int q = 0; q < 3; q++)
@@ -1577,11 +1544,10 @@ __global__ __launch_bounds__(WG, 4) void k_learn_move(SysC s, State st, const ST
     const double pr2  = ratio * ratio;
     const bool flag   = pr2 >= DBL_EPS;
     const double prob = pr2 * exp(lqr - lqf);
-    int ucount        = a.ubase[(size_t)w * (N + 1) + iat];
     bool acc          = false;
     if (flag)
     {
-      const double u = a.unif[(size_t)w * N + ucount];
+      const double u = pf[PF_UNIF]; // a.unif[w N + ucount]
       ucount++;
       acc = u < prob;
     }

@@ -14,7 +14,7 @@ struct MoveLDS
   double y[MAX_K]; // row d of Binv (from phase A), zero beyond d
   double Y[MAX_K]; // new column d of Binv
   alignas(16) double red[9 * WG]; // block_sum / delayed_gemv scratch (up to 9 values per thread)
-  double bc[48];      // broadcast scalars
+  double bc[64];      // broadcast scalars
   double jc[3][MAX_JCOEF]; // Jastrow functor tables: J1, J2 like spins, J2 unlike spins
   SplineLDS spl;      // uniform spline factors of the proposed position
   int sel[N_EL_NB + N_ION_NB]; // learned proposal: selected neighbours (electrons, then ions)
@@ -289,7 +289,8 @@ enum PrefetchSlot
 };
 template<typename SH>
 __device__ inline void prefetch_move(const SysC& s, int iat, int w, int ucount, const MoveArgs& a,
-                                     const double* __restrict__ R, const double* __restrict__ jas, SH& sh)
+                                     const double* __restrict__ R, const double* __restrict__ jas, SH& sh,
+                                     int base = BC_PF)
 {
   const int t = threadIdx.x, Np = s.Npad;
   if (t >= 15)
@@ -309,17 +310,17 @@ __device__ inline void prefetch_move(const SysC& s, int iat, int w, int ucount, 
     v = s.has_j1 ? jas[J_VAT * Np + iat] : 0.0;
   else
     v = s.has_j2 ? jas[J_UAT * Np + iat] : 0.0;
-  sh.bc[BC_PF + t] = v;
+  sh.bc[base + t] = v;
 }
 
 // propose() from the prefetched scalars (thread 0; the same arithmetic)
 template<typename SH>
-__device__ inline void propose_pf(const SysC& s, const MoveArgs& a, SH& sh)
+__device__ inline void propose_pf(const SysC& s, const MoveArgs& a, SH& sh, int base = BC_PF)
 {
   const double tauovermass = a.tau;
   const double sqrttau     = sqrt(tauovermass);
   const double oneover2tau = 0.5 / tauovermass;
-  const double* pf         = sh.bc + BC_PF;
+  const double* pf         = sh.bc + base;
   double gnow[3]           = {0, 0, 0};
   if (s.has_det)
     for (int q = 0; q < 3; q++)

@@ -255,7 +255,9 @@ def test_training_improves_objective():
 def test_fused_learned_move_matches_five_launch_form(precision, monkeypatch):
     """The frozen-network learned move in one kernel (network evaluated per walker on row 0 of the
     matrix-core tile) reproduces the five-launch form (per-walker kernels around the batched
-    network): bit for bit in FP64 (energies, positions, acceptance counts, traces).  With the FP32
+    network).  FP64: identical decisions and acceptance counts; the two kernels are compiled
+    separately, so floating-point contraction may differ between them: energies within 1e-12
+    relative, positions and traces within 1e-11 (measured 6.6e-15, 6.9e-15 and 2.4e-13).  With the FP32
     determinant engine the fused kernel keeps row d of Binv from the inverse-row update in FP64
     while the five-launch form reloads it from the FP32-stored matrix, so there the decisions are
     identical and the values agree at the FP32 engine's accuracy on these random orbitals (ratios
@@ -273,8 +275,11 @@ def test_fused_learned_move_matches_five_launch_form(precision, monkeypatch):
         E, tr, lt = c.vmc_run_learned(2, 2, 0.4, 2, 55, trace=True)
         out.append((E, tr, lt, c.walkers_get(), c.vmc_stats()))
     if precision == "fp64":
-        for a, b in zip(out[0][:4], out[1][:4]):
-            assert np.array_equal(a, b)
+        (E1, tr1, lt1, R1, _), (E0, tr0, lt0, R0, _) = out
+        assert np.array_equal(tr1[..., 1], tr0[..., 1])
+        assert np.max(np.abs(E1 - E0) / np.abs(E0)) < 1e-12
+        assert np.max(np.abs(R1 - R0)) < 1e-11
+        assert np.max(np.abs(tr1 - tr0)) < 1e-11 and np.max(np.abs(lt1 - lt0)) < 1e-11
     else:
         (E1, tr1, lt1, R1, _), (E0, tr0, lt0, R0, _) = out
         assert np.array_equal(tr1[..., 1], tr0[..., 1])
