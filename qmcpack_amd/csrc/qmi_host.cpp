@@ -184,6 +184,7 @@ struct qmi_ctx
   // sub-step replayed from a captured HIP graph (no trace, no learned proposal)
   bool graph_on              = false;
   hipGraphExec_t sweep_exec  = nullptr;
+  hipGraphExec_t split_exec[2] = {nullptr, nullptr};
   double graph_tau           = 0;
   hipEvent_t ev_done         = nullptr;
   PinnedBuf<int> h_ubase;
@@ -248,6 +249,12 @@ struct qmi_ctx
     if (sweep_exec)
       (void)hipGraphExecDestroy(sweep_exec);
     sweep_exec = nullptr;
+    for (auto& x : split_exec)
+    {
+      if (x)
+        (void)hipGraphExecDestroy(x);
+      x = nullptr;
+    }
     graph_gen++;
   }
 
@@ -1429,6 +1436,62 @@ struct qmi_ctx
     }
   }
 
+  // run f(c0, c1) over the crowds on up to 16 host threads
+  template<typename F>
+  void over_crowds(F&& work)
+  {
+    const int nth = std::max(1, std::min<int>({ncrowd, 16, (int)std::thread::hardware_concurrency()}));
+    if (nth == 1)
+      work(0, ncrowd);
+    else
+    {
+      std::vector<std::thread> th;
+      const int per = (ncrowd + nth - 1) / nth;
+      for (int i = 0; i < nth; i++)
+        th.emplace_back(work, i * per, std::min(ncrowd, (i + 1) * per));
+      for (auto& x : th)
+        x.join();
+    }
+  }
+
+  // The same numbers in two parts (one walker per crowd), so that the Box-Muller work of the
+  // second half of the electrons overlaps the device's first half of the sweep.  Part 1: the
+  // uniforms (drawn from a copy of the stream advanced past the sub-step's Gaussians, as the
+  // sequential order puts them after all Gaussians) and the Gaussians of electrons [0, split_n);
+  // part 2: the Gaussians of electrons [split_n, N).  The Gaussians are drawn in pairs, part 1
+  // ending on an even count, so both parts together equal one sequential call.
+  int split_n = 0; // electrons of part 1 (0: sub-steps are not split)
+  int split_g1 = 0; // Gaussians drawn in part 1 (even)
+  void rng_generate_part(int part)
+  {
+    const int N = sys.N, n = 3 * N;
+    h_gauss.alloc((size_t)N * nw * 3);
+    h_unif.alloc((size_t)ncrowd * N * wpc);
+    rng_saved.resize(ncrowd);
+    split_g1       = std::min(n, (3 * split_n + 1) / 2 * 2);
+    const int g0   = part == 1 ? 0 : split_g1;
+    const int g1   = part == 1 ? split_g1 : n;
+    auto work = [&](int c0, int c1) {
+      std::vector<double> buf;
+      for (int c = c0; c < c1; c++)
+      {
+        if (part == 1)
+        {
+          std::mt19937 e = rng[c];
+          e.discard(2ull * (unsigned long long)((n + 1) / 2)); // the draws of all n Gaussians
+          rng_saved[c] = e;
+          rng_fill_uniform(h_unif.p + (size_t)c * N, (size_t)N, e);
+        }
+        buf.resize((size_t)(g1 - g0));
+        if (g1 > g0)
+          rng_fill_gauss(buf.data(), unsigned(g1 - g0), rng[c]);
+        for (int j = g0; j < g1; j++)
+          h_gauss.p[((size_t)(j / 3) * nw + c) * 3 + j % 3] = buf[j - g0];
+      }
+    };
+    over_crowds(work);
+  }
+
   void rng_rewind(const std::vector<int>& consumed)
   {
     for (int c = 0; c < ncrowd; c++)
@@ -1595,14 +1658,21 @@ struct qmi_ctx
   // One sub-step on the context's stream: RNG upload (from the pinned buffers filled by
   // rng_generate), every move of both spin groups with their flushes, and the per-crowd uniform
   // counts back to pinned memory.  Contains no memset and no pageable copy, so it can be captured.
-  void enqueue_substep(double tau, bool with_trace, bool mark_counts = false)
+  // part 0: the whole sub-step; 1: its moves of electrons [0, split_n); 2: the rest and the closing
+  // flushes (two launches around the second half of the host's random numbers)
+  void enqueue_substep(double tau, bool with_trace, bool mark_counts = false, int part = 0)
   {
     const SysC& s = sys;
     const int N   = s.N;
-    HIPCHECK(hipMemcpyAsync(gauss.p, h_gauss.p, (size_t)N * nw * 3 * sizeof(double), hipMemcpyHostToDevice, stream));
-    HIPCHECK(hipMemcpyAsync(unif.p, h_unif.p, (size_t)ncrowd * N * wpc * sizeof(double), hipMemcpyHostToDevice,
-                            stream));
-    HIPCHECK(launch_fill_i32(ubase.p, ubase.n, 0, stream));
+    const int e0 = part == 2 ? split_n : 0, e1 = part == 1 ? split_n : N;
+    if (part != 2)
+    {
+      HIPCHECK(hipMemcpyAsync(unif.p, h_unif.p, (size_t)ncrowd * N * wpc * sizeof(double), hipMemcpyHostToDevice,
+                              stream));
+      HIPCHECK(launch_fill_i32(ubase.p, ubase.n, 0, stream));
+    }
+    HIPCHECK(hipMemcpyAsync(gauss.p + (size_t)e0 * nw * 3, h_gauss.p + (size_t)e0 * nw * 3,
+                            (size_t)(e1 - e0) * nw * 3 * sizeof(double), hipMemcpyHostToDevice, stream));
     MoveArgs a{};
     a.nw    = nw;
     a.wpc   = wpc;
@@ -1615,8 +1685,8 @@ struct qmi_ctx
     a.rr    = dmc.on ? dmc_rr.p : nullptr;
     for (int g = 0; g < 2; g++)
     {
-      int iat = s.first[g];
-      const int last = s.first[g] + s.n[g];
+      int iat        = std::max(s.first[g], e0);
+      const int last = std::min(s.first[g] + s.n[g], e1);
       while (iat < last)
       {
         a.g  = g;
@@ -1675,6 +1745,8 @@ struct qmi_ctx
           flush(g);
       }
     }
+    if (part == 1)
+      return;
     // the uniform counts are final after the last move: copy them before the closing flushes, so
     // that the host can rewind and draw the next sub-step's numbers while the flushes run
     HIPCHECK(hipMemcpyAsync(h_ubase.p, ubase.p, ubase.n * sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -1706,6 +1778,23 @@ struct qmi_ctx
     const hipError_t e = hipGraphInstantiate(&sweep_exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     HIPCHECK(e);
+    graph_tau = tau;
+  }
+
+  // the two graphs of a split sub-step
+  void ensure_graph_split(double tau)
+  {
+    if (split_exec[0] && graph_tau == tau)
+      return;
+    invalidate_graph();
+    for (int part = 1; part <= 2; part++)
+    {
+      hipGraph_t graph = nullptr;
+      capture(stream, [&] { enqueue_substep(tau, false, false, part); }, &graph);
+      const hipError_t e = hipGraphInstantiate(&split_exec[part - 1], graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIPCHECK(e);
+    }
     graph_tau = tau;
   }
 
@@ -1760,6 +1849,16 @@ struct qmi_ctx
     }
     std::vector<double> ek(nw);
     const bool use_graph = graph_on && !htrace && !learned;
+    // one walker per crowd: split each sub-step at the spin-group boundary so that the second half
+    // of the host's Gaussians is drawn while the device runs the first half of the moves
+    split_n          = (wpc == 1 && !learned && s.nup > 0 && s.nup < N) ? s.nup : 0;
+    const bool split = split_n > 0;
+    auto gen1        = [&] {
+      if (split)
+        rng_generate_part(1);
+      else
+        rng_generate();
+    };
     // The host draws a sub-step's random numbers while the device finishes the previous one (its
     // closing flushes, evaluateGL): the draws depend only on the uniforms already consumed, whose
     // counts arrive before those kernels run.
@@ -1769,9 +1868,24 @@ struct qmi_ctx
       for (int sub = 0; sub < sub_steps; sub++)
       {
         if (!rng_ready)
-          rng_generate();
+          gen1();
         rng_ready = false;
-        if (use_graph)
+        if (split)
+        {
+          if (use_graph)
+          {
+            ensure_graph_split(tau);
+            HIPCHECK(hipGraphLaunch(split_exec[0], stream));
+          }
+          else
+            enqueue_substep(tau, htrace != nullptr, false, 1);
+          rng_generate_part(2);
+          if (use_graph)
+            HIPCHECK(hipGraphLaunch(split_exec[1], stream));
+          else
+            enqueue_substep(tau, htrace != nullptr, true, 2);
+        }
+        else if (use_graph)
         {
           ensure_graph(tau);
           HIPCHECK(hipGraphLaunch(sweep_exec, stream));
@@ -1793,7 +1907,7 @@ struct qmi_ctx
         finish_substep();
         if (sub + 1 < sub_steps)
         {
-          rng_generate();
+          gen1();
           rng_ready = true;
         }
       }
@@ -1803,7 +1917,7 @@ struct qmi_ctx
       eval_gl_enqueue();
       if (step + 1 < steps)
       {
-        rng_generate();
+        gen1();
         rng_ready = true;
       }
       sync();
