@@ -1659,7 +1659,7 @@ struct qmi_ctx
   // rng_generate), every move of both spin groups with their flushes, and the per-crowd uniform
   // counts back to pinned memory.  Contains no memset and no pageable copy, so it can be captured.
   // part 0: the whole sub-step; 1: its moves of electrons [0, split_n); 2: the rest and the closing
-  // flushes (two launches around the second half of the host's random numbers)
+  // flushes (two launches around the second part of the host's random numbers)
   void enqueue_substep(double tau, bool with_trace, bool mark_counts = false, int part = 0)
   {
     const SysC& s = sys;
@@ -1849,9 +1849,10 @@ struct qmi_ctx
     }
     std::vector<double> ek(nw);
     const bool use_graph = graph_on && !htrace && !learned;
-    // one walker per crowd: split each sub-step at the spin-group boundary so that the second half
-    // of the host's Gaussians is drawn while the device runs the first half of the moves
-    split_n          = (wpc == 1 && !learned && s.nup > 0 && s.nup < N) ? s.nup : 0;
+    // one walker per crowd: split each sub-step after its first N/8 electrons, so that the Gaussians
+    // of the rest are drawn while the device runs the first moves (the first part's share hides
+    // behind the previous step's evaluateGL)
+    split_n          = (wpc == 1 && !learned && N >= 2) ? std::min(N - 1, std::max(1, N / 8)) : 0;
     const bool split = split_n > 0;
     auto gen1        = [&] {
       if (split)
