@@ -1203,10 +1203,22 @@ struct qmi_ctx
     const int nw0 = nw;
     // advanceWalkers: one sweep with the DMC acceptance
     substep_buffers(false);
-    rng_generate();
+    // the sweep in two launches around the second part of the host's numbers (see vmc_run)
+    split_n = N >= 2 ? std::min(N - 1, std::max(1, N / 8)) : 0;
+    if (split_n > 0)
+      rng_generate_part(1);
+    else
+      rng_generate();
     HIPCHECK(hipMemsetAsync(dmc_rr.p, 0, (size_t)nw * 2 * sizeof(double), stream));
     HIPCHECK(hipMemsetAsync(stats.p, 0, (size_t)nw * 2 * sizeof(double), stream));
-    enqueue_substep(tau, false, true);
+    if (split_n > 0)
+    {
+      enqueue_substep(tau, false, false, 1);
+      rng_generate_part(2);
+      enqueue_substep(tau, false, true, 2);
+    }
+    else
+      enqueue_substep(tau, false, true);
     HIPCHECK(hipEventSynchronize(ev_ubase));
     finish_substep();
     const bool recompute = dmc.p.recompute_period > 0 && ((step_count + 1) % dmc.p.recompute_period == 0);
