@@ -2016,11 +2016,12 @@ struct qmi_batch
   }
 
   // one sub-step of every context: host RNG, one graph launch, rewind
-  void substep(double tau)
+  void substep(double tau, bool numbers_ready = false)
   {
     prepare(tau);
-    for (auto* c : ctxs)
-      c->rng_generate();
+    if (!numbers_ready)
+      for (auto* c : ctxs)
+        c->rng_generate();
     HIPCHECK(hipGraphLaunch(exec, origin));
     HIPCHECK(hipStreamSynchronize(origin));
     for (auto* c : ctxs)
@@ -2038,18 +2039,37 @@ struct qmi_batch
       ctxs[i]->seed_rng(seeds ? seeds[i] : ctxs[i]->rng_seed);
     }
     std::vector<std::vector<double>> ek(ctxs.size());
+    bool ready = false; // the next sub-step's numbers are drawn
     for (int step = 0; step < steps; step++)
     {
       for (int sub = 0; sub < sub_steps; sub++)
-        substep(tau);
+      {
+        substep(tau, ready);
+        ready = false;
+      }
       for (size_t i = 0; i < ctxs.size(); i++)
       {
         qmi_ctx* c           = ctxs[i];
         const bool recompute = recompute_period > 0 && ((c->step_count + 1) % recompute_period == 0);
         if (recompute)
           c->recompute_dets();
+      }
+      // evaluateGL of every context on its own stream; the host draws the next step's numbers
+      // meanwhile
+      for (auto* c : ctxs)
+        c->eval_gl_enqueue();
+      if (step + 1 < steps)
+      {
+        for (auto* c : ctxs)
+          c->rng_generate();
+        ready = true;
+      }
+      for (size_t i = 0; i < ctxs.size(); i++)
+      {
+        qmi_ctx* c = ctxs[i];
+        c->sync();
         ek[i].resize(c->nw);
-        c->eval_gl(nullptr, nullptr, nullptr, ek[i].data());
+        c->eval_gl_finish(ek[i].data());
         if (energies && energies[i])
           std::copy(ek[i].begin(), ek[i].end(), energies[i] + (size_t)step * c->nw);
         c->step_count++;
