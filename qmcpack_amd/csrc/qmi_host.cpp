@@ -1958,11 +1958,90 @@ struct qmi_batch
   std::vector<unsigned long long> exec_gen; // graph_gen of every context when exec was captured
   std::string err;
 
+  // the two graphs of a split batched sub-step (all contexts' first parts, then their second parts)
+  hipGraphExec_t split[2] = {nullptr, nullptr};
+  double split_tau        = 0;
+  std::vector<unsigned long long> split_gen;
+
   void invalidate()
   {
     if (exec)
       (void)hipGraphExecDestroy(exec);
     exec = nullptr;
+    for (auto& x : split)
+    {
+      if (x)
+        (void)hipGraphExecDestroy(x);
+      x = nullptr;
+    }
+  }
+
+  bool splittable() const
+  {
+    for (auto* c : ctxs)
+      if (c->wpc != 1 || c->sys.N < 2)
+        return false;
+    return true;
+  }
+
+  void prepare_split(double tau)
+  {
+    for (auto* c : ctxs)
+    {
+      c->require_walkers();
+      if (c->learned)
+        throw QmiError(QMI_ERR_UNSUPPORTED, "batched sweeps run the drift-diffusion proposal");
+      c->substep_buffers(false);
+      if (!c->ev_done)
+        HIPCHECK(hipEventCreateWithFlags(&c->ev_done, hipEventDisableTiming));
+      c->split_n = std::min(c->sys.N - 1, std::max(1, c->sys.N / 8));
+    }
+    bool stale = !split[0] || split_tau != tau || split_gen.size() != ctxs.size();
+    for (size_t i = 0; !stale && i < ctxs.size(); i++)
+      stale = split_gen[i] != ctxs[i]->graph_gen;
+    if (!stale)
+      return;
+    invalidate();
+    for (int part = 1; part <= 2; part++)
+    {
+      hipGraph_t graph = nullptr;
+      ctxs[0]->capture(origin, [&] {
+        HIPCHECK(hipEventRecord(ev_fork, origin));
+        hipEvent_t prev = ev_fork;
+        for (auto* c : ctxs)
+        {
+          HIPCHECK(hipStreamWaitEvent(c->stream, concurrent ? ev_fork : prev, 0));
+          c->enqueue_substep(tau, false, false, part);
+          HIPCHECK(hipEventRecord(c->ev_done, c->stream));
+          prev = c->ev_done;
+        }
+        for (auto* c : ctxs)
+          HIPCHECK(hipStreamWaitEvent(origin, c->ev_done, 0));
+      }, &graph);
+      const hipError_t e = hipGraphInstantiate(&split[part - 1], graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIPCHECK(e);
+    }
+    split_tau = tau;
+    split_gen.clear();
+    for (auto* c : ctxs)
+      split_gen.push_back(c->graph_gen);
+  }
+
+  // one batched sub-step in two graphs around the second part of every context's numbers
+  void substep_split(double tau, bool numbers_ready)
+  {
+    prepare_split(tau);
+    if (!numbers_ready)
+      for (auto* c : ctxs)
+        c->rng_generate_part(1);
+    HIPCHECK(hipGraphLaunch(split[0], origin));
+    for (auto* c : ctxs)
+      c->rng_generate_part(2);
+    HIPCHECK(hipGraphLaunch(split[1], origin));
+    HIPCHECK(hipStreamSynchronize(origin));
+    for (auto* c : ctxs)
+      c->finish_substep();
   }
 
   ~qmi_batch()
@@ -2039,13 +2118,34 @@ struct qmi_batch
       ctxs[i]->seed_rng(seeds ? seeds[i] : ctxs[i]->rng_seed);
     }
     std::vector<std::vector<double>> ek(ctxs.size());
-    bool ready = false; // the next sub-step's numbers are drawn
+    const bool two = splittable();
+    auto gen       = [&] {
+      for (auto* c : ctxs)
+      {
+        if (two)
+        {
+          c->split_n = std::min(c->sys.N - 1, std::max(1, c->sys.N / 8));
+          c->rng_generate_part(1);
+        }
+        else
+          c->rng_generate();
+      }
+    };
+    bool ready = false; // the next sub-step's numbers (or their first part) are drawn
     for (int step = 0; step < steps; step++)
     {
       for (int sub = 0; sub < sub_steps; sub++)
       {
-        substep(tau, ready);
+        if (two)
+          substep_split(tau, ready);
+        else
+          substep(tau, ready);
         ready = false;
+        if (sub + 1 < sub_steps)
+        {
+          gen();
+          ready = true;
+        }
       }
       for (size_t i = 0; i < ctxs.size(); i++)
       {
@@ -2060,8 +2160,7 @@ struct qmi_batch
         c->eval_gl_enqueue();
       if (step + 1 < steps)
       {
-        for (auto* c : ctxs)
-          c->rng_generate();
+        gen();
         ready = true;
       }
       for (size_t i = 0; i < ctxs.size(); i++)
