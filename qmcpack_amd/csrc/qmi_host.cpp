@@ -1864,7 +1864,7 @@ struct qmi_ctx
     // one walker per crowd: split each sub-step after its first N/8 electrons, so that the Gaussians
     // of the rest are drawn while the device runs the first moves (the first part's share hides
     // behind the previous step's evaluateGL)
-    split_n          = (wpc == 1 && !learned && N >= 2) ? std::min(N - 1, std::max(1, N / 8)) : 0;
+    split_n          = (wpc == 1 && N >= 2) ? std::min(N - 1, std::max(1, N / 8)) : 0;
     const bool split = split_n > 0;
     auto gen1        = [&] {
       if (split)
