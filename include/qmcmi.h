@@ -12,6 +12,7 @@
  */
 #ifndef QMCMI_H
 #define QMCMI_H
+#include <stddef.h>
 #include <stdint.h>
 #ifdef __cplusplus
 extern "C" {
@@ -201,6 +202,27 @@ int qmi_dmc_run(qmi_ctx* ctx, int steps, double* stats);
 int qmi_num_walkers(qmi_ctx* ctx);
 /* weights[n], local energies[n] of the current population (nullable outputs) */
 int qmi_dmc_walker_data(qmi_ctx* ctx, double* weights, double* energies);
+
+/* ---- restart files ---------------------------------------------------------------------------
+ * A checkpoint holds the walkers and the random streams (HDFWalkerOutput::dump,
+ * Particle/HDFWalkerOutput.cpp:81-160: state_0/{block, number_of_walkers, walker_partition,
+ * walkers [nw][N][3], walker_weights}; RandomNumberControl::write, Utilities/RandomNumberControl.cpp:
+ * 194-209 with StdRandom::save, Utilities/StdRandom.cpp: one std::mt19937 state per stream) and, for
+ * a DMC context, the branch state (BranchIO<SFNBranch>::write, QMCDrivers/BranchIO.cpp:109-160) with
+ * the walker-control stream and the walkers' local energies.  The datasets keep those names in a
+ * flat little-endian container ("QMICKPT1", u32 version 1, u32 dataset count; per dataset u32 name
+ * length, name, u32 type 1 f64 / 2 i64 / 3 u32, u32 rank, u64 dims, row-major payload). */
+/* writes the checkpoint into buf (capacity bytes); *bytes receives its size (buf NULL: size only).
+ * Needs no pending delayed updates. */
+int qmi_checkpoint_save(qmi_ctx* ctx, void* buf, size_t capacity, size_t* bytes);
+/* restart (HDFWalkerInput_0_4::read_hdf5, Particle/HDFWalkerInput_0_4.cpp:126-190;
+ * RandomNumberControl::read, :183-192; BranchIO::read, QMCDrivers/BranchIO.cpp:166-208): reloads the
+ * walkers and re-evaluates them from scratch (QMCDriverNew::initialLogEvaluation), restores the
+ * streams and the step count, and for a DMC checkpoint the branch state (qmi_dmc_run continues;
+ * qmi_coulomb_setup and, if the checkpoint has the pseudopotential term, qmi_nlpp_setup first).
+ * qmi_vmc_run continues the restored streams when given the checkpoint's seed.  The checkpoint is
+ * validated before the context changes. */
+int qmi_checkpoint_load(qmi_ctx* ctx, const void* buf, size_t bytes);
 /* number of k vectors of the long-range sum (0 before qmi_coulomb_setup) */
 int qmi_coulomb_nk(qmi_ctx* ctx);
 /* v[nw][3] = {electron-electron, electron-ion, ion-ion} energy of every walker */

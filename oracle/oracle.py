@@ -56,7 +56,7 @@ def lib():
                      "orc_det_complete", "orc_det_get", "orc_vmc_evaluate_log", "orc_wf_ratio_grad",
                      "orc_wf_accept_reject", "orc_wf_eval_grad", "orc_wf_evaluate_gl", "orc_vmc_run", "orc_vmc_replay",
                      "orc_vmc_get", "orc_vmc_is_complex", "orc_spo_vgl_c", "orc_invert_transpose_c",
-                     "orc_ewald_params", "orc_vmc_potential"):
+                     "orc_ewald_params", "orc_vmc_potential", "orc_vmc_set_rng"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -293,6 +293,13 @@ class VMC:
 
     def _out(self, a):
         return _cv(a) if self.cplx else a
+
+    def set_rng_states(self, states):
+        """Replace the crowds' streams by std::mt19937 states [crowds][625] (StdRandom::load)."""
+        st = np.ascontiguousarray(states, dtype=np.uint32)
+        rc = lib().orc_vmc_set_rng(self.h, st.shape[0], st.shape[1], st.ctypes.data_as(C.POINTER(C.c_uint32)))
+        if rc != 0:
+            raise RuntimeError(f"orc_vmc_set_rng failed ({rc})")
 
     def evaluate_log(self):
         """logpsi (log|psi|, or log|psi| + i phase for complex orbitals), G, L (value type)."""

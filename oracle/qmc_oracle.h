@@ -75,6 +75,8 @@ int orc_det_get(void* h, int iw, double* ainv, double* logdet2);
 /* wavefunction + VMC driver over a walker population split in crowds */
 void* orc_vmc_create(const orc_system_desc* d, int nw, const double* R, int walkers_per_crowd, uint32_t seed);
 void orc_vmc_destroy(void* h);
+/* load the crowds' std::mt19937 states (StdRandom::load): words[nstreams][state_size] */
+int orc_vmc_set_rng(void* h, int nstreams, int state_size, const uint32_t* words);
 /* initial log evaluation (recompute everything); logpsi[nw], G[nw][N][3], L[nw][N] (nullable) */
 int orc_vmc_evaluate_log(void* h, double* logpsi, double* G, double* L);
 /* single-walker protocol used by the known-answer tests */

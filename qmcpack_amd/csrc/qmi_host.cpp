@@ -1,6 +1,7 @@
 // Host side of the qmcmi C ABI: context, device memory, the multi-walker API and the
 // device-resident VMC driver.  All device work of a context runs on one HIP stream.
 #include "../../include/qmcmi.h"
+#include "qmi_checkpoint.hpp"
 #include "qmi_kernels.hpp"
 #include "qmi_rng.hpp"
 
@@ -1410,6 +1411,264 @@ struct qmi_ctx
   }
 
   // ---------------------------------------------------------------------------------------------
+  // Restart files.  Walkers (positions [nw][N][3], weights) as HDFWalkerOutput::write_configuration
+  // (Particle/HDFWalkerOutput.cpp:110-160), the random streams as RandomNumberControl::write_rank_0
+  // (Utilities/RandomNumberControl.cpp:382-) with StdRandom::save, and for a DMC context the branch
+  // state (BranchIO::write, QMCDrivers/BranchIO.cpp:109-160: parameters, energy history, R2
+  // accumulators) with the walker-control stream and the walkers' local energies.
+  enum
+  {
+    CK_V_TAU,
+    CK_V_TAUEFF,
+    CK_V_ETRIAL,
+    CK_V_EREF,
+    CK_V_BRANCHMAX,
+    CK_V_CUTOFF,
+    CK_V_FILTER,
+    CK_V_SIGMA2,
+    CK_V_ENOW,
+    CK_V_R2ACC,
+    CK_V_R2PROP,
+    CK_V_FEEDBACK,
+    CK_V_SIGMABOUND,
+    CK_V_EHIST,
+    CK_V_VHIST,
+    CK_V_NHIST,
+    CK_NV
+  };
+  enum
+  {
+    CK_I_WARMUP,
+    CK_I_COUNTER,
+    CK_I_TARGET,
+    CK_I_WARMUP_TODO,
+    CK_I_ETRIAL_TODO,
+    CK_I_MAIN_STAGE,
+    CK_I_HIST_LIMIT,
+    CK_I_CUTOFF_SCHEME,
+    CK_I_REFHIST,
+    CK_I_RECOMPUTE,
+    CK_I_CAPACITY,
+    CK_I_TARGET_IN,
+    CK_I_SEED,
+    CK_I_NLPP,
+    CK_NI
+  };
+
+  std::vector<char> checkpoint_save()
+  {
+    require_walkers();
+    if (dcount[0] || dcount[1])
+      throw QmiError(QMI_ERR_ARG, "checkpoint with pending delayed updates: call qmi_mw_complete_updates first");
+    const int N = sys.N, Np = sys.Npad;
+    std::vector<double> t((size_t)nw * 3 * Np), Rh((size_t)nw * N * 3);
+    HIPCHECK(hipMemcpyAsync(t.data(), R.p, t.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    sync();
+    for (int w = 0; w < nw; w++)
+      for (int i = 0; i < N; i++)
+        for (int q = 0; q < 3; q++)
+          Rh[((size_t)w * N + i) * 3 + q] = t[((size_t)w * 3 + q) * Np + i];
+    CkptWriter wr;
+    wr.add_i64("state_0/block", {step_count});
+    wr.add_i64("state_0/number_of_walkers", {nw});
+    wr.add_i64("state_0/walker_partition", {0, nw});
+    wr.add("state_0/walkers", CKPT_F64, {uint64_t(nw), uint64_t(N), 3}, Rh.data());
+    std::vector<double> wts(nw, 1.0);
+    if (dmc.on)
+      std::copy(dmc.weight.begin(), dmc.weight.begin() + nw, wts.begin());
+    wr.add_f64("state_0/walker_weights", wts);
+    wr.add_i64("crowd_size", {wpc});
+    const size_t ss = mt_state_size(), ns = rng_seeded ? rng.size() : 0;
+    std::vector<uint32_t> words;
+    words.reserve(ns * ss);
+    for (size_t c = 0; c < ns; c++)
+    {
+      const std::vector<uint32_t> v = mt_save(rng[c]);
+      words.insert(words.end(), v.begin(), v.end());
+    }
+    wr.add_i64("nprocs_nthreads_statesize", {1, (int64_t)ns, (int64_t)ss});
+    wr.add("random/std::mt19937", CKPT_U32, {uint64_t(ns), uint64_t(ss)}, words.data());
+    wr.add_i64("random_seed", {rng_seeded ? 1 : 0, (int64_t)rng_seed});
+    if (dmc.on)
+    {
+      std::vector<double> v(CK_NV);
+      v[CK_V_TAU]        = dmc.p.tau;
+      v[CK_V_TAUEFF]     = dmc.tau_eff;
+      v[CK_V_ETRIAL]     = dmc.etrial;
+      v[CK_V_EREF]       = dmc.eref;
+      v[CK_V_BRANCHMAX]  = dmc.branch_max;
+      v[CK_V_CUTOFF]     = dmc.cutoff;
+      v[CK_V_FILTER]     = dmc.filter;
+      v[CK_V_SIGMA2]     = dmc.sigma2;
+      v[CK_V_ENOW]       = dmc.enow;
+      v[CK_V_R2ACC]      = dmc.r2acc;
+      v[CK_V_R2PROP]     = dmc.r2prop;
+      v[CK_V_FEEDBACK]   = dmc.p.feedback;
+      v[CK_V_SIGMABOUND] = dmc.p.sigma_bound;
+      v[CK_V_EHIST]      = dmc.ehist;
+      v[CK_V_VHIST]      = dmc.vhist;
+      v[CK_V_NHIST]      = dmc.nhist;
+      wr.add_f64("branch/vparam", v);
+      std::vector<int64_t> iv(CK_NI);
+      iv[CK_I_WARMUP]        = dmc.p.warmup_steps;
+      iv[CK_I_COUNTER]       = dmc.iter;
+      iv[CK_I_TARGET]        = dmc.target;
+      iv[CK_I_WARMUP_TODO]   = dmc.warmup_todo;
+      iv[CK_I_ETRIAL_TODO]   = dmc.etrial_todo;
+      iv[CK_I_MAIN_STAGE]    = dmc.main_stage ? 1 : 0;
+      iv[CK_I_HIST_LIMIT]    = (int64_t)dmc.hist_limit;
+      iv[CK_I_CUTOFF_SCHEME] = dmc.p.cutoff_scheme;
+      iv[CK_I_REFHIST]       = dmc.p.refenergy_history;
+      iv[CK_I_RECOMPUTE]     = dmc.p.recompute_period;
+      iv[CK_I_CAPACITY]      = dmc.p.capacity;
+      iv[CK_I_TARGET_IN]     = dmc.p.target_walkers;
+      iv[CK_I_SEED]          = dmc.p.seed;
+      iv[CK_I_NLPP]          = dmc.nlpp ? 1 : 0;
+      wr.add_i64("branch/iparam", iv);
+      std::vector<double> h;
+      for (auto& e : dmc.hist)
+        h.insert(h.end(), e.begin(), e.end());
+      wr.add("branch/history", CKPT_F64, {uint64_t(dmc.hist.size()), 3}, h.data());
+      const std::vector<uint32_t> wc = mt_save(dmc.wc_rng);
+      wr.add("random_master/std::mt19937", CKPT_U32, {1, uint64_t(wc.size())}, wc.data());
+      wr.add_f64("dmc/local_energies", std::vector<double>(dmc.eloc.begin(), dmc.eloc.begin() + nw));
+    }
+    return wr.finish();
+  }
+
+  // Restart: the walkers are reloaded and re-evaluated from scratch (QMCDriverNew::
+  // initialLogEvaluation, QMCDriverNew.cpp:258-), the streams, the step count and the DMC state
+  // restored.  Everything is validated before the context changes.
+  void checkpoint_load(const void* buf, size_t bytes)
+  {
+    if (!have_system)
+      throw QmiError(QMI_ERR_ARG, "system not created");
+    const int N = sys.N;
+    try
+    {
+      CkptReader rd(buf, bytes);
+      const int64_t nwc = rd.i64("state_0/number_of_walkers", 1)[0];
+      if (nwc < 1 || nwc > (1 << 24))
+        throw QmiError(QMI_ERR_ARG, "checkpoint walker count out of range");
+      const CkptDataset& W = rd.need("state_0/walkers", CKPT_F64);
+      if (W.dims.size() != 3 || W.dims[1] != (uint64_t)N || W.dims[2] != 3)
+        throw QmiError(QMI_ERR_ARG, "checkpoint walkers do not have " + std::to_string(N) + " electrons");
+      if (W.dims[0] != (uint64_t)nwc)
+        throw QmiError(QMI_ERR_ARG, "checkpoint walkers and number_of_walkers disagree");
+      std::vector<double> Rh(W.size());
+      std::memcpy(Rh.data(), W.data, Rh.size() * sizeof(double));
+      const std::vector<double> wts = rd.f64("state_0/walker_weights", nwc);
+      const int64_t wpc_in          = rd.i64("crowd_size", 1)[0];
+      const int64_t block           = rd.i64("state_0/block", 1)[0];
+      const std::vector<int64_t> sh = rd.i64("nprocs_nthreads_statesize", 3);
+      const std::vector<int64_t> sd = rd.i64("random_seed", 2);
+      const size_t ss               = mt_state_size();
+      if (sh[0] != 1 || sh[1] < 0 || sh[2] != (int64_t)ss)
+        throw QmiError(QMI_ERR_ARG, "checkpoint random streams of another shape (processes, streams, state size)");
+      const CkptDataset& RS = rd.need("random/std::mt19937", CKPT_U32, sh[1] * (int64_t)ss);
+      std::vector<std::mt19937> streams(sh[1]);
+      for (int64_t c = 0; c < sh[1]; c++)
+        mt_load(streams[c], reinterpret_cast<const uint32_t*>(RS.data) + c * ss, ss);
+      if (wpc_in < 1 || block < 0)
+        throw QmiError(QMI_ERR_ARG, "checkpoint crowd size or block out of range");
+      const int64_t ncrowd_in = (nwc + std::min<int64_t>(wpc_in, nwc) - 1) / std::min<int64_t>(wpc_in, nwc);
+      const bool is_dmc = rd.find("branch/vparam") != nullptr;
+      if (!is_dmc && sd[0] && sh[1] != ncrowd_in)
+        throw QmiError(QMI_ERR_ARG, "checkpoint has one random stream per crowd: stream count does not match");
+      DmcState d{};
+      if (is_dmc)
+      {
+        const std::vector<double> v   = rd.f64("branch/vparam", CK_NV);
+        const std::vector<int64_t> iv = rd.i64("branch/iparam", CK_NI);
+        const CkptDataset& H          = rd.need("branch/history", CKPT_F64);
+        const CkptDataset& WC         = rd.need("random_master/std::mt19937", CKPT_U32, (int64_t)ss);
+        const std::vector<double> el  = rd.f64("dmc/local_energies", nwc);
+        if (wpc_in != 1 || !sd[0] || sh[1] < nwc)
+          throw QmiError(QMI_ERR_ARG, "DMC checkpoint needs one seeded random stream per walker slot");
+        if (!coul_ready)
+          throw QmiError(QMI_ERR_ARG, "DMC checkpoint: call qmi_coulomb_setup first");
+        if (iv[CK_I_NLPP] && !nlpp_ready)
+          throw QmiError(QMI_ERR_ARG, "DMC checkpoint with a pseudopotential term: call qmi_nlpp_setup first");
+        if (iv[CK_I_NLPP] && cplx)
+          throw QmiError(QMI_ERR_UNSUPPORTED, "the pseudopotential energy is built for real orbitals");
+        if (learned)
+          throw QmiError(QMI_ERR_UNSUPPORTED, "DMC with the learned proposal is not built");
+        if (iv[CK_I_CAPACITY] < nwc || iv[CK_I_HIST_LIMIT] < 1 || H.dims.size() != 2 || H.dims[1] != 3 ||
+            H.dims[0] > (uint64_t)iv[CK_I_HIST_LIMIT] || iv[CK_I_CUTOFF_SCHEME] < 0 || iv[CK_I_CUTOFF_SCHEME] > 3 ||
+            !(v[CK_V_TAU] > 0) || iv[CK_I_TARGET] < 1)
+          throw QmiError(QMI_ERR_ARG, "DMC checkpoint branch state out of range");
+        d.p.tau               = v[CK_V_TAU];
+        d.p.feedback          = v[CK_V_FEEDBACK];
+        d.p.sigma_bound       = v[CK_V_SIGMABOUND];
+        d.p.warmup_steps      = (int)iv[CK_I_WARMUP];
+        d.p.refenergy_history = (int)iv[CK_I_REFHIST];
+        d.p.cutoff_scheme     = (int)iv[CK_I_CUTOFF_SCHEME];
+        d.p.capacity          = (int)iv[CK_I_CAPACITY];
+        d.p.target_walkers    = (int)iv[CK_I_TARGET_IN];
+        d.p.recompute_period  = (int)iv[CK_I_RECOMPUTE];
+        d.p.seed              = (uint32_t)iv[CK_I_SEED];
+        d.nlpp                = iv[CK_I_NLPP] != 0;
+        d.iter                = iv[CK_I_COUNTER];
+        d.tau_eff             = v[CK_V_TAUEFF];
+        d.etrial              = v[CK_V_ETRIAL];
+        d.eref                = v[CK_V_EREF];
+        d.branch_max          = v[CK_V_BRANCHMAX];
+        d.cutoff              = v[CK_V_CUTOFF];
+        d.filter              = v[CK_V_FILTER];
+        d.sigma2              = v[CK_V_SIGMA2];
+        d.enow                = v[CK_V_ENOW];
+        d.r2acc               = v[CK_V_R2ACC];
+        d.r2prop              = v[CK_V_R2PROP];
+        d.ehist               = v[CK_V_EHIST];
+        d.vhist               = v[CK_V_VHIST];
+        d.nhist               = v[CK_V_NHIST];
+        d.target              = (int)iv[CK_I_TARGET];
+        d.warmup_todo         = (int)iv[CK_I_WARMUP_TODO];
+        d.etrial_todo         = (int)iv[CK_I_ETRIAL_TODO];
+        d.main_stage          = iv[CK_I_MAIN_STAGE] != 0;
+        d.hist_limit          = (size_t)iv[CK_I_HIST_LIMIT];
+        const double* h       = reinterpret_cast<const double*>(H.data);
+        for (uint64_t r = 0; r < H.dims[0]; r++)
+          d.hist.push_back({h[3 * r], h[3 * r + 1], h[3 * r + 2]});
+        mt_load(d.wc_rng, reinterpret_cast<const uint32_t*>(WC.data), ss);
+        d.weight.assign(iv[CK_I_CAPACITY], 1.0);
+        d.eloc.assign(iv[CK_I_CAPACITY], 0.0);
+        std::copy(wts.begin(), wts.end(), d.weight.begin());
+        std::copy(el.begin(), el.end(), d.eloc.begin());
+        d.on = true;
+      }
+      // the context changes from here on
+      walkers_load((int)nwc, Rh.data(), (int)wpc_in, is_dmc ? d.p.capacity : 0);
+      {
+        Scoped t(this, "jastrow_recompute");
+        HIPCHECK(launch_jastrow_recompute(sys, st, nw, stream));
+      }
+      recompute_dets();
+      sync();
+      step_count = block;
+      rng        = std::move(streams);
+      rng_seed   = (uint32_t)sd[1];
+      rng_seeded = sd[0] != 0;
+      if (is_dmc)
+      {
+        dmc_rr.alloc((size_t)cap * 2);
+        dmc_pairs.alloc((size_t)cap * 2);
+        h_pairs.alloc((size_t)cap * 2);
+        h_rr.alloc((size_t)cap * 2);
+        dmc = std::move(d);
+      }
+    }
+    catch (const QmiError&)
+    {
+      throw;
+    }
+    catch (const std::runtime_error& e)
+    {
+      throw QmiError(QMI_ERR_ARG, e.what());
+    }
+  }
+
+  // ---------------------------------------------------------------------------------------------
   // RNG for one sub-step: Gaussians [iat][iw][3] per crowd (the reference layout of
   // walker_deltas, VMCBatched.cpp:109) scattered into the global [iat][w][3] array, then the crowd's
   // possible uniforms (N * crowd size), drawn speculatively and rewound after the sub-step.
@@ -2630,6 +2889,28 @@ extern "C"
   }
 
   int qmi_num_walkers(qmi_ctx* c) { return c ? c->nw : 0; }
+
+  int qmi_checkpoint_save(qmi_ctx* c, void* buf, size_t capacity, size_t* bytes)
+  {
+    return guarded(c, [&] {
+      if (!bytes)
+        throw QmiError(QMI_ERR_ARG, "null size output");
+      const std::vector<char> v = c->checkpoint_save();
+      *bytes                    = v.size();
+      if (buf)
+      {
+        if (capacity < v.size())
+          throw QmiError(QMI_ERR_ARG, "checkpoint needs " + std::to_string(v.size()) + " bytes, buffer holds " +
+                                          std::to_string(capacity));
+        std::memcpy(buf, v.data(), v.size());
+      }
+    });
+  }
+
+  int qmi_checkpoint_load(qmi_ctx* c, const void* buf, size_t bytes)
+  {
+    return guarded(c, [&] { c->checkpoint_load(buf, bytes); });
+  }
 
   int qmi_dmc_walker_data(qmi_ctx* c, double* weights, double* energies)
   {

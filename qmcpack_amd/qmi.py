@@ -64,6 +64,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.qmi_last_error.argtypes = [C.c_void_p]
     L.qmi_batch_last_error.restype = C.c_char_p
     L.qmi_batch_last_error.argtypes = [C.c_void_p]
+    L.qmi_checkpoint_save.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.qmi_checkpoint_load.argtypes = [C.c_void_p, C.c_char_p, C.c_size_t]
     _lib = L
     return L
 
@@ -72,6 +74,54 @@ def exported_symbols_from_header(header: str = HEADER) -> list[str]:
     import re
     txt = open(header).read()
     return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(qmi_\w+)\s*\(", txt, flags=re.M)))
+
+
+# ---------------------------------------------------------------- restart files
+CKPT_MAGIC = b"QMICKPT1"
+_CKPT_TYPES = {1: np.dtype("<f8"), 2: np.dtype("<i8"), 3: np.dtype("<u4")}
+
+
+def read_checkpoint(data: bytes) -> dict:
+    """The datasets of a checkpoint (qmi_checkpoint_save, include/qmcmi.h) by name, as numpy arrays:
+    state_0/walkers [nw][N][3], state_0/walker_weights, random/std::mt19937 [streams][625], ..."""
+    import struct
+    mv = memoryview(data)
+    if len(mv) < 16 or bytes(mv[:8]) != CKPT_MAGIC:
+        raise ValueError("not a walker checkpoint")
+    version, count = struct.unpack_from("<II", mv, 8)
+    if version != 1:
+        raise ValueError(f"unsupported checkpoint format version {version}")
+    off, out = 16, {}
+    for _ in range(count):
+        (n,) = struct.unpack_from("<I", mv, off)
+        name = bytes(mv[off + 4:off + 4 + n]).decode()
+        off += 4 + n
+        t, rank = struct.unpack_from("<II", mv, off)
+        off += 8
+        dims = struct.unpack_from(f"<{rank}Q", mv, off)
+        off += 8 * rank
+        dt = _CKPT_TYPES[t]
+        size = int(np.prod(dims, dtype=np.int64)) * dt.itemsize
+        if off + size > len(mv):
+            raise ValueError("truncated checkpoint")
+        out[name] = np.frombuffer(mv[off:off + size], dtype=dt).reshape(dims).copy()
+        off += size
+    if off != len(mv):
+        raise ValueError("trailing bytes after the checkpoint's datasets")
+    return out
+
+
+def write_checkpoint(datasets: dict) -> bytes:
+    """Inverse of read_checkpoint (float64, int64 and uint32 arrays)."""
+    import struct
+    code = {np.dtype("float64"): 1, np.dtype("int64"): 2, np.dtype("uint32"): 3}
+    parts = [CKPT_MAGIC, struct.pack("<II", 1, len(datasets))]
+    for name, a in datasets.items():
+        a = np.ascontiguousarray(a)
+        nb = name.encode()
+        parts += [struct.pack("<I", len(nb)), nb, struct.pack("<II", code[a.dtype], a.ndim),
+                  struct.pack(f"<{a.ndim}Q", *a.shape), a.astype(a.dtype.newbyteorder("<")).tobytes()]
+    return b"".join(parts)
 
 
 def _p(a):
@@ -285,6 +335,20 @@ class Context:
         w, e = np.empty(self.nw), np.empty(self.nw)
         self._check(self.L.qmi_dmc_walker_data(self.h, _p(w), _p(e)))
         return w, e
+
+    # ---------------------------------------------------------------- restart files
+    def checkpoint_save(self) -> bytes:
+        """Walkers, random streams and (DMC) branch state; read_checkpoint() decodes it."""
+        n = C.c_size_t(0)
+        self._check(self.L.qmi_checkpoint_save(self.h, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        self._check(self.L.qmi_checkpoint_save(self.h, buf, n, C.byref(n)))
+        return buf.raw[:n.value]
+
+    def checkpoint_load(self, data: bytes):
+        """Restart from checkpoint_save's bytes (system, and for DMC coulomb_setup / nlpp_setup, first)."""
+        self._check(self.L.qmi_checkpoint_load(self.h, bytes(data), len(data)))
+        self.nw = int(self.L.qmi_num_walkers(self.h))
 
     def coulomb_nk(self):
         """Number of k vectors of the Ewald long-range sum (after coulomb_setup)."""
