@@ -190,13 +190,17 @@ typedef struct qmi_dmc_params
   int target_walkers;     /* 0: the population at qmi_dmc_init */
   int recompute_period;   /* full re-inversion every this many steps (0: never) */
   uint32_t seed;          /* walker slot w draws from seed + w, walker control from seed - 1 */
+  int nonlocal_moves;     /* DMC input "nonlocalmoves": 0 locality approximation, 1 T-moves v0
+                           * (NonLocalECPotential::makeNonLocalMovesPbyP, NonLocalECPotential.cpp:
+                           * 546-567; NonLocalTOperator::selectMove, NonLocalTOperator.cpp:79-106;
+                           * needs qmi_nlpp_setup) */
 } qmi_dmc_params;
-#define QMI_DMC_STATS 10
+#define QMI_DMC_STATS 11
 /* re-evaluates every walker (log psi, local energy), sets E_T = E_ref = mean energy */
 int qmi_dmc_init(qmi_ctx* ctx, const qmi_dmc_params* params);
 /* stats[steps][QMI_DMC_STATS] per step: weighted mean energy and variance before branching,
  * total weight, walkers before and after branching, E_trial and E_ref after the update,
- * effective time step, accepted and rejected moves */
+ * effective time step, accepted and rejected moves, accepted non-local moves */
 int qmi_dmc_run(qmi_ctx* ctx, int steps, double* stats);
 /* current population (walkers in slots 0 .. n-1) */
 int qmi_num_walkers(qmi_ctx* ctx);

@@ -252,6 +252,13 @@ class Det:
         return ainv, complex(ld[0], ld[1])
 
 
+def select_move(prob, weights, tau):
+    """NonLocalTOperator::selectMove (alpha = gamma = 0): selected candidate index or None."""
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    i = lib().orc_select_move(C.c_double(prob), len(w), ptr(w), C.c_double(tau))
+    return None if i < 0 else i
+
+
 def rotation_matrix(u1, u2, u3):
     """generateRotationMatrix(u1, u2, u3) as a 3x3 array."""
     out = np.empty(9)
@@ -263,7 +270,7 @@ class OrcDmcParams(C.Structure):
     _fields_ = [("tau", C.c_double), ("feedback", C.c_double), ("sigma_bound", C.c_double),
                 ("warmup_steps", C.c_int), ("refenergy_history", C.c_int), ("cutoff_scheme", C.c_int),
                 ("capacity", C.c_int), ("target_walkers", C.c_int), ("recompute_period", C.c_int),
-                ("seed", C.c_uint32)]
+                ("seed", C.c_uint32), ("nonlocal_moves", C.c_int)]
 
 
 class VMC:
@@ -335,14 +342,15 @@ class VMC:
                                ptr(v), ptr(v2), C.c_double(rcut), len(weights), ptr(knots), ptr(weights))
 
     def dmc_init(self, ion_charge, tau, capacity, seed, feedback=1.0, sigma_bound=10.0, warmup_steps=200,
-                 refenergy_history=0, cutoff_scheme=0, target_walkers=0, recompute_period=0, dim_cutoff=15.0):
+                 refenergy_history=0, cutoff_scheme=0, target_walkers=0, recompute_period=0, dim_cutoff=15.0,
+                 nonlocal_moves=0):
         p = OrcDmcParams(tau, feedback, sigma_bound, warmup_steps, refenergy_history, cutoff_scheme, capacity,
-                         target_walkers, recompute_period, seed)
+                         target_walkers, recompute_period, seed, nonlocal_moves)
         if lib().orc_dmc_init(self.h, C.c_double(ion_charge), C.c_double(dim_cutoff), C.byref(p)) != 0:
             raise RuntimeError("oracle dmc_init failed")
 
     def dmc_run(self, steps):
-        out = np.empty((steps, 10))
+        out = np.empty((steps, 11))
         st = lib().orc_dmc_run(self.h, int(steps), ptr(out))
         if st != 0:
             raise RuntimeError(f"oracle dmc_run failed ({st})")

@@ -104,9 +104,10 @@ typedef struct
   double tau, feedback, sigma_bound;
   int warmup_steps, refenergy_history, cutoff_scheme, capacity, target_walkers, recompute_period;
   uint32_t seed;
+  int nonlocal_moves; /* 0 locality approximation, 1 T-moves v0 */
 } orc_dmc_params;
 int orc_dmc_init(void* h, double ion_charge, double dim_cutoff, const orc_dmc_params* p);
-/* stats[steps][10]: as qmi_dmc_run */
+/* stats[steps][11]: as qmi_dmc_run */
 int orc_dmc_run(void* h, int steps, double* stats);
 int orc_num_walkers(void* h);
 /* pseudopotential in the DMC local energy (before orc_dmc_init; real orbitals) */
@@ -118,6 +119,8 @@ int orc_rotation_matrix(double u1, double u2, double u3, double* out9);
 int orc_nlpp_energy(void* h, int nch, const int* l, int ng, double dr, const double* v, const double* v2, double rcut,
                     int nknot, const double* knots, const double* weights, const double* rot, double* out);
 int orc_dmc_walker_data(void* h, double* weights, double* energies);
+/* NonLocalTOperator::selectMove (alpha = gamma = 0): selected candidate index or -1 */
+int orc_select_move(double prob, int n, const double* weights, double tau);
 
 #ifdef __cplusplus
 }

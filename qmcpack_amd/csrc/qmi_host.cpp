@@ -908,6 +908,10 @@ struct qmi_ctx
   bool nlpp_ready = false;
   NlppC nlpp{};
   DevBuf<double> nlpp_poly, nlpp_knots, nlpp_rot, nlpp_part, nlpp_out;
+  // non-local moves: per-electron scan weights of the last evaluation, selection in / out
+  DevBuf<double> tm_tsum, tm_base, tm_target, tm_out;
+  DevBuf<int> tm_iel;
+  std::vector<double> h_tsum;
 
   void nlpp_setup(int nch, const int* ls, int ng, double dr, const double* y, const double* y2, double rcut, int nknot,
                   const double* knots, const double* weights)
@@ -971,7 +975,8 @@ struct qmi_ctx
     nlpp_ready = true;
   }
 
-  void nlpp_evaluate(const double* rot, double* v)
+  // with_txy: also the per-electron scan weights of the non-local moves (into h_tsum)
+  void nlpp_evaluate(const double* rot, double* v, bool with_txy = false)
   {
     require_walkers();
     if (!nlpp_ready)
@@ -989,12 +994,130 @@ struct qmi_ctx
     if (nlpp_out.n < (size_t)cap)
       nlpp_out.alloc(cap);
     HIPCHECK(hipMemcpyAsync(nlpp_rot.p, rot, (size_t)nw * 9 * sizeof(double), hipMemcpyHostToDevice, stream));
+    TmoveArgs tm{};
+    if (with_txy)
+    {
+      if (tm_tsum.n < (size_t)cap * sys.N)
+        tm_tsum.alloc((size_t)cap * sys.N);
+      tm.mode  = 1;
+      tm.plus  = tmove_plus();
+      tm.minus = tmove_minus();
+      tm.tsum  = tm_tsum.p;
+    }
     {
       Scoped t(this, "nlpp");
-      HIPCHECK(launch_nlpp(sys, st, coefs.p, nlpp, nw, nlpp_rot.p, nlpp_part.p, nlpp_out.p, stream));
+      HIPCHECK(launch_nlpp(sys, st, coefs.p, nlpp, nw, nlpp_rot.p, nlpp_part.p, nlpp_out.p, tm, stream));
     }
     HIPCHECK(hipMemcpyAsync(v, nlpp_out.p, (size_t)nw * sizeof(double), hipMemcpyDeviceToHost, stream));
+    if (with_txy)
+    {
+      h_tsum.resize((size_t)nw * sys.N);
+      HIPCHECK(hipMemcpyAsync(h_tsum.data(), tm_tsum.p, h_tsum.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              stream));
+    }
     sync();
+  }
+
+  // NonLocalTOperator plusFactor = tau gamma, minusFactor = -tau (1 - alpha (1 + gamma)) with the
+  // defaults alpha = gamma = 0 (NonLocalTOperator.cpp:62-70)
+  double tmove_plus() const { return 0.0; }
+  double tmove_minus() const { return -dmc.p.tau; }
+
+  // T-moves v0 (NonLocalECPotential::makeNonLocalMovesPbyP, NonLocalECPotential.cpp:546-567): each
+  // walker draws one uniform u (after its grid rotation) and NonLocalTOperator::selectMove
+  // (NonLocalTOperator.cpp:79-106) picks the candidate with the last scan value <= u x total, the
+  // scan starting at 1 (no move) over the electron-major list of the last evaluation; the chosen
+  // electron is moved by deltaV and the move accepted when the ratio is non-zero.  The scan is
+  // summed per electron on the device, across electrons here; the selected electron's points are
+  // re-evaluated to find the point.  The moves go through the multi-walker protocol one electron
+  // at a time (walkers that do not move that electron take a zero displacement and are rejected).
+  int dmc_tmoves(const std::vector<double>& u)
+  {
+    const int N = sys.N;
+    std::vector<int> sel(nw, -1);
+    std::vector<double> base(nw, 0.0), target(nw, 0.0);
+    bool any = false;
+    for (int w = 0; w < nw; w++)
+    {
+      const double* T = h_tsum.data() + (size_t)w * N;
+      double total    = 1.0;
+      for (int e = 0; e < N; e++)
+        if (T[e] >= 0.0)
+          total += T[e];
+      const double tg = u[w] * total;
+      double acc      = 1.0;
+      for (int e = 0; e < N; e++)
+      {
+        if (T[e] < 0.0)
+          continue;
+        if (acc <= tg)
+        {
+          sel[w]  = e;
+          base[w] = acc;
+        }
+        acc += T[e];
+      }
+      target[w] = tg;
+      any |= sel[w] >= 0;
+    }
+    if (!any)
+      return 0;
+    if (tm_iel.n < (size_t)cap)
+    {
+      tm_iel.alloc(cap);
+      tm_base.alloc(cap);
+      tm_target.alloc(cap);
+      tm_out.alloc((size_t)cap * 4);
+    }
+    HIPCHECK(hipMemcpyAsync(tm_iel.p, sel.data(), (size_t)nw * sizeof(int), hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(tm_base.p, base.data(), (size_t)nw * sizeof(double), hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(tm_target.p, target.data(), (size_t)nw * sizeof(double), hipMemcpyHostToDevice, stream));
+    TmoveArgs tm{};
+    tm.mode       = 2;
+    tm.plus       = tmove_plus();
+    tm.minus      = tmove_minus();
+    tm.sel_iel    = tm_iel.p;
+    tm.sel_base   = tm_base.p;
+    tm.sel_target = tm_target.p;
+    tm.sel_out    = tm_out.p;
+    {
+      Scoped t(this, "tmove_select");
+      HIPCHECK(launch_nlpp(sys, st, coefs.p, nlpp, nw, nlpp_rot.p, nlpp_part.p, nlpp_out.p, tm, stream));
+    }
+    std::vector<double> so((size_t)nw * 4);
+    HIPCHECK(hipMemcpyAsync(so.data(), tm_out.p, so.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    sync();
+    auto check = [&](int rc) {
+      if (rc != QMI_OK)
+        throw QmiError(rc, err);
+    };
+    std::vector<double> disp((size_t)nw * 3), ratios(nw);
+    std::vector<int> acc(nw);
+    int accepted = 0;
+    for (int e = 0; e < N; e++)
+    {
+      bool here = false;
+      for (int w = 0; w < nw; w++)
+      {
+        const bool m = sel[w] == e && so[(size_t)w * 4] != 0.0;
+        for (int a = 0; a < 3; a++)
+          disp[(size_t)w * 3 + a] = m ? so[(size_t)w * 4 + 1 + a] : 0.0;
+        acc[w] = m ? 1 : 0;
+        here |= m;
+      }
+      if (!here)
+        continue;
+      check(qmi_mw_make_move(this, e, disp.data()));
+      check(qmi_mw_ratio_grad(this, e, ratios.data(), nullptr));
+      for (int w = 0; w < nw; w++)
+      {
+        acc[w] = acc[w] && ratios[w] != 0.0;
+        accepted += acc[w];
+      }
+      check(qmi_mw_accept_reject(this, e, acc.data(), 1));
+    }
+    check(qmi_mw_complete_updates(this));
+    return accepted;
   }
 
   // ---------------------------------------------------------------------------------------------
@@ -1092,7 +1215,7 @@ struct qmi_ctx
         rotation_from_uniforms(u1, u2, u3, rot.data() + (size_t)w * 9);
       }
       vnl.resize(nw);
-      nlpp_evaluate(rot.data(), vnl.data());
+      nlpp_evaluate(rot.data(), vnl.data(), with_rr && dmc.p.nonlocal_moves);
     }
     eval_gl_enqueue();
     if (coul_out.n < (size_t)cap * 3)
@@ -1156,6 +1279,8 @@ struct qmi_ctx
     h_rr.alloc((size_t)cap * 2);
     if (nlpp_ready && cplx)
       throw QmiError(QMI_ERR_UNSUPPORTED, "the pseudopotential energy is built for real orbitals");
+    if (p.nonlocal_moves != 0 && (p.nonlocal_moves != 1 || !nlpp_ready))
+      throw QmiError(QMI_ERR_ARG, "non-local moves (v0) need the pseudopotential: call qmi_nlpp_setup first");
     dmc      = DmcState{};
     dmc.p    = p;
     dmc.nlpp = nlpp_ready;
@@ -1229,6 +1354,16 @@ struct qmi_ctx
     HIPCHECK(hipMemcpyAsync(hs.data(), stats.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
     std::vector<double> enew;
     dmc_local_energies(enew, true);
+    // non-local moves after the local energy (DMCBatched.cpp:298-325): one uniform per walker from
+    // its stream, after the grid rotation's three
+    int ntmove = 0;
+    if (dmc.p.nonlocal_moves)
+    {
+      std::vector<double> u(nw);
+      for (int w = 0; w < nw; w++)
+        u[w] = rng_uniform01(rng[w]);
+      ntmove = dmc_tmoves(u);
+    }
     step_count++;
     double nacc = 0, nrej = 0;
     for (int w = 0; w < nw; w++)
@@ -1329,6 +1464,7 @@ struct qmi_ctx
       out[7] = dmc.tau_eff;
       out[8] = nacc;
       out[9] = nrej;
+      out[10] = ntmove;
     }
   }
 
@@ -1452,6 +1588,7 @@ struct qmi_ctx
     CK_I_TARGET_IN,
     CK_I_SEED,
     CK_I_NLPP,
+    CK_I_TMOVE,
     CK_NI
   };
 
@@ -1524,6 +1661,7 @@ struct qmi_ctx
       iv[CK_I_TARGET_IN]     = dmc.p.target_walkers;
       iv[CK_I_SEED]          = dmc.p.seed;
       iv[CK_I_NLPP]          = dmc.nlpp ? 1 : 0;
+      iv[CK_I_TMOVE]         = dmc.p.nonlocal_moves;
       wr.add_i64("branch/iparam", iv);
       std::vector<double> h;
       for (auto& e : dmc.hist)
@@ -1595,7 +1733,8 @@ struct qmi_ctx
           throw QmiError(QMI_ERR_UNSUPPORTED, "DMC with the learned proposal is not built");
         if (iv[CK_I_CAPACITY] < nwc || iv[CK_I_HIST_LIMIT] < 1 || H.dims.size() != 2 || H.dims[1] != 3 ||
             H.dims[0] > (uint64_t)iv[CK_I_HIST_LIMIT] || iv[CK_I_CUTOFF_SCHEME] < 0 || iv[CK_I_CUTOFF_SCHEME] > 3 ||
-            !(v[CK_V_TAU] > 0) || iv[CK_I_TARGET] < 1)
+            !(v[CK_V_TAU] > 0) || iv[CK_I_TARGET] < 1 || iv[CK_I_TMOVE] < 0 || iv[CK_I_TMOVE] > 1 ||
+            (iv[CK_I_TMOVE] && !iv[CK_I_NLPP]))
           throw QmiError(QMI_ERR_ARG, "DMC checkpoint branch state out of range");
         d.p.tau               = v[CK_V_TAU];
         d.p.feedback          = v[CK_V_FEEDBACK];
@@ -1607,6 +1746,7 @@ struct qmi_ctx
         d.p.target_walkers    = (int)iv[CK_I_TARGET_IN];
         d.p.recompute_period  = (int)iv[CK_I_RECOMPUTE];
         d.p.seed              = (uint32_t)iv[CK_I_SEED];
+        d.p.nonlocal_moves    = (int)iv[CK_I_TMOVE];
         d.nlpp                = iv[CK_I_NLPP] != 0;
         d.iter                = iv[CK_I_COUNTER];
         d.tau_eff             = v[CK_V_TAUEFF];

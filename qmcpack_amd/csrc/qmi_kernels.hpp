@@ -112,9 +112,26 @@ struct NlppC
   const double* knots;    // [nknot][4] unit vector, weight
   int kc;                 // quadrature points evaluated together by the kernel (6 or 12)
 };
+// Non-local moves (NonLocalECPotential::makeNonLocalMovesPbyP v0 with NonLocalTOperator::selectMove):
+// every quadrature point of every electron-ion pair is a candidate move r_e -> r_e + deltaV_j with
+// weight W_j = knot partial product x ratio, scanned with W plus (W > 0) or W minus (W <= 0).
+// mode 0: energy only; 1: energy and, per electron, the scan weight of its points (tsum[w][N],
+// -1 for an electron without pairs); 2: re-evaluate electron sel_iel[w] of each walker (grid of
+// one electron) and pick the last point whose scan value (from sel_base[w]) is <= sel_target[w]:
+// sel_out[w][4] = {found, deltaV}.
+struct TmoveArgs
+{
+  int mode;
+  double plus, minus;
+  double* tsum;
+  const int* sel_iel;
+  const double* sel_base;
+  const double* sel_target;
+  double* sel_out;
+};
 // out[nw] = non-local energy of each walker; rot[nw][9] row-major rotation of the grid; part[nw][N]
 hipError_t launch_nlpp(const SysC& s, const State& st, const void* coefs, const NlppC& c, int nw, const double* rot,
-                       double* part, double* out, hipStream_t stream);
+                       double* part, double* out, const TmoveArgs& tm, hipStream_t stream);
 
 // per-walker slices of the walker state (base of slot 0 and bytes per slot, a multiple of 16)
 constexpr int WA_MAX = 8;

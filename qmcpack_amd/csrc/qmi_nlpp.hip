@@ -25,6 +25,7 @@ struct NlLDS
   double cz[NL_KC][4];  // and c_k
   long long off[NL_KC]; // table offset of each point's stencil corner
   double kp[NL_KC];     // knot partial products w_j sum_l (2l+1) v_l(r) P_l(cos)
+  double dv[NL_KC][3];  // deltaV_j = r u_j - dr (the non-local move of each point)
   double sum[3 * NL_KC];
 };
 
@@ -42,10 +43,13 @@ __device__ inline double nlpp_vl(const NlppC& c, int ch, double r)
 // stencil gathers of all points are in flight together and one reduction serves them all.
 template<typename ST, typename DT, int NJ, int MAXM, int NL_KC>
 __global__ __launch_bounds__(WG, NL_KC == 6 ? 3 : 1) void k_nlpp(SysC s, State st, const ST* __restrict__ coefs, NlppC c,
-                                             const double* __restrict__ rot, double* __restrict__ part)
+                                             const double* __restrict__ rot, double* __restrict__ part, TmoveArgs tm)
 {
   __shared__ NlLDS<NL_KC> sh;
-  const int iel = blockIdx.x, w = blockIdx.y, t = threadIdx.x;
+  const int w = blockIdx.y, t = threadIdx.x;
+  const int iel = tm.mode == 2 ? tm.sel_iel[w] : (int)blockIdx.x;
+  if (iel < 0) // mode 2: no move selected for this walker
+    return;
   const int ni = s.nion, N = s.N, Np = s.Npad;
   const double* R    = st.R + (size_t)w * 3 * Np;
   const double pe[3] = {R[iel], R[Np + iel], R[2 * Np + iel]};
@@ -72,7 +76,14 @@ __global__ __launch_bounds__(WG, NL_KC == 6 ? 3 : 1) void k_nlpp(SysC s, State s
   if (!any)
   {
     if (t == 0)
-      part[(size_t)w * N + iel] = 0.0;
+    {
+      if (tm.mode == 2)
+        tm.sel_out[(size_t)w * 4] = 0.0;
+      else
+        part[(size_t)w * N + iel] = 0.0;
+      if (tm.mode == 1)
+        tm.tsum[(size_t)w * N + iel] = -1.0;
+    }
     return;
   }
   stage_jastrow(s, sh);
@@ -105,6 +116,8 @@ __global__ __launch_bounds__(WG, NL_KC == 6 ? 3 : 1) void k_nlpp(SysC s, State s
   }
   const double* rw   = rot + (size_t)w * 9;
   double value       = 0.0; // thread 0
+  double scan        = tm.mode == 2 ? tm.sel_base[w] : 0.0; // thread 0: scan weight
+  double sel[4]      = {0.0, 0.0, 0.0, 0.0};
   __syncthreads();
   for (int m = 0; m < nmask; m++)
   {
@@ -129,7 +142,8 @@ __global__ __launch_bounds__(WG, NL_KC == 6 ? 3 : 1) void k_nlpp(SysC s, State s
             u[a] = rw[3 * a] * k[0] + rw[3 * a + 1] * k[1] + rw[3 * a + 2] * k[2];
           for (int a = 0; a < 3; a++)
           {
-            p[a]     = pe[a] + (r * u[a] - dr[a]);
+            sh.dv[t][a] = r * u[a] - dr[a];
+            p[a]        = pe[a] + sh.dv[t][a];
             sh.vp[t][a] = p[a];
           }
           const double zz = (dr[0] * u[0] + dr[1] * u[1] + dr[2] * u[2]) * (1.0 / r);
@@ -260,7 +274,16 @@ __global__ __launch_bounds__(WG, NL_KC == 6 ? 3 : 1) void k_nlpp(SysC s, State s
               ratio *= exp(jas[J_VAT * Np + iel] - sh.sum[NL_KC + q]);
             if (s.has_j2)
               ratio *= exp(jas[J_UAT * Np + iel] - sh.sum[2 * NL_KC + q]);
-            pairpot += sh.kp[q] * ratio;
+            const double wgt = sh.kp[q] * ratio;
+            pairpot += wgt;
+            if (tm.mode == 2 && scan <= tm.sel_target[w])
+            {
+              sel[0] = 1.0;
+              sel[1] = sh.dv[q][0];
+              sel[2] = sh.dv[q][1];
+              sel[3] = sh.dv[q][2];
+            }
+            scan += wgt > 0.0 ? wgt * tm.plus : wgt * tm.minus;
           }
         __syncthreads(); // the point tables are rewritten by the next chunk
       }
@@ -269,7 +292,15 @@ __global__ __launch_bounds__(WG, NL_KC == 6 ? 3 : 1) void k_nlpp(SysC s, State s
     }
   }
   if (t == 0)
-    part[(size_t)w * N + iel] = value;
+  {
+    if (tm.mode == 2)
+      for (int a = 0; a < 4; a++)
+        tm.sel_out[(size_t)w * 4 + a] = sel[a];
+    else
+      part[(size_t)w * N + iel] = value;
+    if (tm.mode == 1)
+      tm.tsum[(size_t)w * N + iel] = scan;
+  }
 }
 
 __global__ void k_nlpp_sum(int nw, int N, const double* __restrict__ part, double* __restrict__ out)
@@ -284,7 +315,7 @@ __global__ void k_nlpp_sum(int nw, int N, const double* __restrict__ part, doubl
 }
 
 hipError_t launch_nlpp(const SysC& s, const State& st, const void* coefs, const NlppC& c, int nw, const double* rot,
-                       double* part, double* out, hipStream_t stream)
+                       double* part, double* out, const TmoveArgs& tm, hipStream_t stream)
 {
   if (s.nion > NLPP_MAX_IONS || c.nch < 1 || c.nch > NLPP_MAX_CH || c.lmax > NLPP_MAX_L || c.ng < 2 || nw < 1)
     return hipErrorInvalidValue;
@@ -294,19 +325,19 @@ hipError_t launch_nlpp(const SysC& s, const State& st, const void* coefs, const 
   const hipError_t e = dispatch_prec(s.prec, [&](auto stv, auto dtv) {
     using ST = decltype(stv);
     using DT = decltype(dtv);
-    const dim3 grid(s.N, nw);
+    const dim3 grid(tm.mode == 2 ? 1 : s.N, nw);
     const ST* cf = static_cast<const ST*>(coefs);
     if (mm == 1 && c.kc == 6)
-      k_nlpp<ST, DT, 2, 1, 6><<<grid, WG, 0, stream>>>(s, st, cf, c, rot, part);
+      k_nlpp<ST, DT, 2, 1, 6><<<grid, WG, 0, stream>>>(s, st, cf, c, rot, part, tm);
     else if (mm == 1)
-      k_nlpp<ST, DT, 2, 1, 12><<<grid, WG, 0, stream>>>(s, st, cf, c, rot, part);
+      k_nlpp<ST, DT, 2, 1, 12><<<grid, WG, 0, stream>>>(s, st, cf, c, rot, part, tm);
     else if (c.kc == 6)
-      k_nlpp<ST, DT, 4, 2, 6><<<grid, WG, 0, stream>>>(s, st, cf, c, rot, part);
+      k_nlpp<ST, DT, 4, 2, 6><<<grid, WG, 0, stream>>>(s, st, cf, c, rot, part, tm);
     else
-      k_nlpp<ST, DT, 4, 2, 12><<<grid, WG, 0, stream>>>(s, st, cf, c, rot, part);
+      k_nlpp<ST, DT, 4, 2, 12><<<grid, WG, 0, stream>>>(s, st, cf, c, rot, part, tm);
     return hipGetLastError();
   });
-  if (e != hipSuccess)
+  if (e != hipSuccess || tm.mode == 2)
     return e;
   k_nlpp_sum<<<(nw + 63) / 64, 64, 0, stream>>>(nw, s.N, part, out);
   return hipGetLastError();

@@ -25,7 +25,7 @@ class DmcParams(C.Structure):
     _fields_ = [("tau", C.c_double), ("feedback", C.c_double), ("sigma_bound", C.c_double),
                 ("warmup_steps", C.c_int), ("refenergy_history", C.c_int), ("cutoff_scheme", C.c_int),
                 ("capacity", C.c_int), ("target_walkers", C.c_int), ("recompute_period", C.c_int),
-                ("seed", C.c_uint32)]
+                ("seed", C.c_uint32), ("nonlocal_moves", C.c_int)]
 
 
 class QmiError(RuntimeError):
@@ -317,16 +317,17 @@ class Context:
 
     # ---------------------------------------------------------------- DMC
     def dmc_init(self, tau, capacity, seed, feedback=1.0, sigma_bound=10.0, warmup_steps=200,
-                 refenergy_history=0, cutoff_scheme=0, target_walkers=0, recompute_period=0):
-        """DMC on the loaded walkers (one walker per crowd; call coulomb_setup first)."""
+                 refenergy_history=0, cutoff_scheme=0, target_walkers=0, recompute_period=0, nonlocal_moves=0):
+        """DMC on the loaded walkers (one walker per crowd; call coulomb_setup first; nonlocal_moves=1:
+        T-moves v0, after nlpp_setup)."""
         p = DmcParams(tau, feedback, sigma_bound, warmup_steps, refenergy_history, cutoff_scheme, capacity,
-                      target_walkers, recompute_period, seed)
+                      target_walkers, recompute_period, seed, nonlocal_moves)
         self._check(self.L.qmi_dmc_init(self.h, C.byref(p)))
 
     def dmc_run(self, steps):
-        """Per step [steps][10]: energy, variance, weight, walkers before / after branching, E_trial,
-        E_ref, tau_eff, accepted, rejected."""
-        out = np.empty((steps, 10))
+        """Per step [steps][11]: energy, variance, weight, walkers before / after branching, E_trial,
+        E_ref, tau_eff, accepted, rejected, accepted non-local moves."""
+        out = np.empty((steps, 11))
         self._check(self.L.qmi_dmc_run(self.h, int(steps), _p(out)))
         self.nw = int(self.L.qmi_num_walkers(self.h))
         return out

@@ -42,7 +42,7 @@ def dmc_pair(system, nw, steps, seed=21, **kw):
 
 def check_dmc(c, o, st, sto):
     print("walkers per step", st[:, 4].astype(int).tolist())
-    assert np.array_equal(st[:, [3, 4, 8, 9]], sto[:, [3, 4, 8, 9]])
+    assert np.array_equal(st[:, [3, 4, 8, 9, 10]], sto[:, [3, 4, 8, 9, 10]])
     for col in (0, 2, 5, 6, 7):
         assert rel(st[:, col], sto[:, col]) < REL, col
     assert rel(st[:, 1], sto[:, 1], floor=max(1.0, float(np.max(sto[:, 1])) * 1e-3)) < 1e-9
@@ -88,6 +88,8 @@ def test_dmc_capacity_exceeded_is_an_error():
     c.dmc_init(tau=0.02, capacity=16, seed=3, warmup_steps=5, target_walkers=800)
     with pytest.raises(RuntimeError, match="exceeds the capacity"):
         c.dmc_run(5)
+    with pytest.raises(RuntimeError, match="qmi_nlpp_setup"):
+        c.dmc_init(tau=0.02, capacity=16, seed=3, nonlocal_moves=1)
 
 
 def test_dmc_with_pseudopotential():
@@ -113,6 +115,36 @@ def test_dmc_with_pseudopotential():
     o.dmc_init(zion, seed=51, **kw)
     sto = o.dmc_run(8)
     check_dmc(c, o, st, sto)
+
+
+@pytest.mark.parametrize("k", [1, 4])
+def test_dmc_with_tmoves(k):
+    """T-moves v0 after the local energy: each walker draws one more number from its stream,
+    selects a candidate among the quadrature points of its electron-ion pairs (NonLocalTOperator::
+    selectMove) and moves that electron when the ratio is non-zero; the accepted-move counts must
+    agree step by step with the oracle's, and the populations walker for walker."""
+    from test_oracle_nlpp import channels, icosahedron
+    s = make_system(L=7.0, n_ion=4, n_up=7, n_dn=6, norb=8, M=12, delay_rank=k, seed=29)
+    R = make_walkers(s, 24, 7)
+    zion = s.N / len(s.ions)
+    ls = [0, 1, 2]
+    dr, v, v2, _ = channels(ls, seed=4)
+    knots, w = icosahedron()
+    kw = dict(tau=0.05, capacity=96, warmup_steps=3, recompute_period=4, nonlocal_moves=1)
+    c = qmi.Context(0)
+    c.system_create(s)
+    c.walkers_load(R, 1)
+    c.coulomb_setup(zion)
+    c.nlpp_setup(ls, dr, v, v2, 2.2, knots, w)
+    c.dmc_init(seed=57, **kw)
+    st = c.dmc_run(8)
+    o = orc.VMC(s, R, 1, 57)
+    o.dmc_set_nlpp(ls, dr, v, v2, 2.2, knots, w)
+    o.dmc_init(zion, seed=57, **kw)
+    sto = o.dmc_run(8)
+    print("T-moves per step", st[:, 10].astype(int).tolist())
+    check_dmc(c, o, st, sto)
+    assert st[:, 10].sum() >= 5
 
 
 def test_dmc_mixed_precision_runs():
