@@ -133,14 +133,14 @@ def bench_nlpp(ctx, system, log, rcut=1.5, reps=3):
     return out
 
 
-def bench_dmc(ctx, steps, log, tau=0.01, with_nlpp=False):
-    """DMC on the same walkers (local energy kinetic + Ewald Coulomb, plus the non-local
-    pseudopotential when one is set on the context; fluctuating population, feedback toward the
-    starting count): electron moves per second over `steps` timed steps after 5 untimed ones, with
-    the per-step device time of each part."""
+def bench_dmc(ctx, steps, log, tau=0.01, with_nlpp=False, nonlocal_moves=0, untimed=5):
+    """DMC on the context's walkers (local energy kinetic + Ewald Coulomb, plus the non-local
+    pseudopotential when one is set on the context, optionally with T-moves; fluctuating
+    population, feedback toward the starting count): electron moves per second over `steps` timed
+    steps after `untimed` ones, with the per-step device time of each part."""
     nw = ctx.nw
-    ctx.dmc_init(tau=tau, capacity=4 * nw, seed=2024, warmup_steps=200)
-    ctx.dmc_run(5)
+    ctx.dmc_init(tau=tau, capacity=4 * nw, seed=2024, warmup_steps=200, nonlocal_moves=nonlocal_moves)
+    ctx.dmc_run(untimed)
     ctx.timing_enable(True)
     ctx.timing_reset()
     t1 = time.perf_counter()
@@ -155,6 +155,9 @@ def bench_dmc(ctx, steps, log, tau=0.01, with_nlpp=False):
                walkers_mean=float(np.mean(st[:, 3])), walkers_min=int(np.min(st[:, 4])),
                walkers_max=int(np.max(st[:, 4])), energy_last=float(st[-1, 0]),
                device_ms_per_step=per)
+    if nonlocal_moves:
+        out["nonlocal_moves"] = "T-moves v0"
+        out["tmoves_accepted_per_step"] = float(np.mean(st[:, 10]))
     log("[bench] dmc: " + json.dumps(out))
     return out
 
@@ -372,6 +375,8 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--no-nlpp", dest="nlpp", action="store_false", help="skip the pseudopotential energy leg")
     ap.add_argument("--dmc-steps", type=int, default=20, help="timed DMC steps on the workload (0: skip)")
+    ap.add_argument("--dmc-tmove-steps", type=int, default=5,
+                    help="timed DMC steps with T-moves after the DMC leg (needs the pseudopotential leg; 0: skip)")
     ap.add_argument("--no-autotune", action="store_true", help="uneven workload: keep the default knobs")
     ap.add_argument("--eager", action="store_true",
                     help="launch the sweeps kernel by kernel instead of replaying a captured graph")
@@ -432,6 +437,8 @@ def main():
     ctx.timing_enable(False)
     nlpp = bench_nlpp(ctx, system, log) if args.nlpp else None
     dmc = bench_dmc(ctx, args.dmc_steps, log, with_nlpp=nlpp is not None) if args.dmc_steps > 0 else None
+    dmc_tm = (bench_dmc(ctx, args.dmc_tmove_steps, log, with_nlpp=True, nonlocal_moves=1, untimed=2)
+              if nlpp is not None and args.dmc_tmove_steps > 0 else None)
     del ctx
     n, N, nion, k = system.n_up, system.N, len(system.ions), system.delay_rank
     # the fused move kernel runs either once per move (move_one) or once per delay window of k
@@ -518,6 +525,7 @@ def main():
             "mfma_peak_tflops": FP32_PEAK_TFLOPS if sd == 4 else FP64_PEAK_TFLOPS,
         },
         "dmc": dmc,
+        "dmc_tmoves": dmc_tm,
         "nlpp": nlpp,
         "coulomb_roofline": {
             "kernel": "k_coulomb (Ewald: short-range pair sums from an LDS spline, long-range rho_k)",

@@ -14,7 +14,7 @@
 namespace qmi
 {
 
-template<int NL_KC>
+template<int NL_KC, int TM>
 struct NlLDS
 {
   alignas(16) double red[4][3 * NL_KC]; // per-wave partials: det, J1, J2 of each point
@@ -25,8 +25,8 @@ struct NlLDS
   double cz[NL_KC][4];  // and c_k
   long long off[NL_KC]; // table offset of each point's stencil corner
   double kp[NL_KC];     // knot partial products w_j sum_l (2l+1) v_l(r) P_l(cos)
-  double dv[NL_KC][3];  // deltaV_j = r u_j - dr (the non-local move of each point)
   double sum[3 * NL_KC];
+  double dv[TM ? NL_KC : 1][3]; // deltaV_j = r u_j - dr (the non-local move of each point)
 };
 
 // v_l(r) from the per-interval cubic of channel ch (r inside the grid)
@@ -41,15 +41,17 @@ __device__ inline double nlpp_vl(const NlppC& c, int ch, double r)
 // Up to NL_KC (6 or 12) quadrature points at once (6 at occupancy 3 is the faster form): each thread accumulates, for every point, its orbitals'
 // values dotted with the inverse row and its ions' / electrons' Jastrow values, so that the
 // stencil gathers of all points are in flight together and one reduction serves them all.
-template<typename ST, typename DT, int NJ, int MAXM, int NL_KC>
-__global__ __launch_bounds__(WG, NL_KC == 6 ? 3 : 1) void k_nlpp(SysC s, State st, const ST* __restrict__ coefs, NlppC c,
-                                             const double* __restrict__ rot, double* __restrict__ part, TmoveArgs tm)
+// TM: 0 energy only, 1 energy and the per-electron scan weights, 2 selection (TmoveArgs)
+template<typename ST, typename DT, int NJ, int MAXM, int NL_KC, int TM>
+__device__ __forceinline__ void nlpp_body(SysC s, State st, const ST* __restrict__ coefs, NlppC c,
+                                          const double* __restrict__ rot, double* __restrict__ part, TmoveArgs tm)
 {
-  __shared__ NlLDS<NL_KC> sh;
+  __shared__ NlLDS<NL_KC, TM> sh;
   const int w = blockIdx.y, t = threadIdx.x;
-  const int iel = tm.mode == 2 ? tm.sel_iel[w] : (int)blockIdx.x;
-  if (iel < 0) // mode 2: no move selected for this walker
-    return;
+  const int iel = TM == 2 ? tm.sel_iel[w] : (int)blockIdx.x;
+  if constexpr (TM == 2)
+    if (iel < 0) // no move selected for this walker
+      return;
   const int ni = s.nion, N = s.N, Np = s.Npad;
   const double* R    = st.R + (size_t)w * 3 * Np;
   const double pe[3] = {R[iel], R[Np + iel], R[2 * Np + iel]};
@@ -77,11 +79,11 @@ __global__ __launch_bounds__(WG, NL_KC == 6 ? 3 : 1) void k_nlpp(SysC s, State s
   {
     if (t == 0)
     {
-      if (tm.mode == 2)
+      if constexpr (TM == 2)
         tm.sel_out[(size_t)w * 4] = 0.0;
       else
         part[(size_t)w * N + iel] = 0.0;
-      if (tm.mode == 1)
+      if constexpr (TM == 1)
         tm.tsum[(size_t)w * N + iel] = -1.0;
     }
     return;
@@ -116,7 +118,7 @@ __global__ __launch_bounds__(WG, NL_KC == 6 ? 3 : 1) void k_nlpp(SysC s, State s
   }
   const double* rw   = rot + (size_t)w * 9;
   double value       = 0.0; // thread 0
-  double scan        = tm.mode == 2 ? tm.sel_base[w] : 0.0; // thread 0: scan weight
+  double scan        = TM == 2 ? tm.sel_base[w] : 0.0; // thread 0: scan weight
   double sel[4]      = {0.0, 0.0, 0.0, 0.0};
   __syncthreads();
   for (int m = 0; m < nmask; m++)
@@ -142,8 +144,10 @@ __global__ __launch_bounds__(WG, NL_KC == 6 ? 3 : 1) void k_nlpp(SysC s, State s
             u[a] = rw[3 * a] * k[0] + rw[3 * a + 1] * k[1] + rw[3 * a + 2] * k[2];
           for (int a = 0; a < 3; a++)
           {
-            sh.dv[t][a] = r * u[a] - dr[a];
-            p[a]        = pe[a] + sh.dv[t][a];
+            const double dv = r * u[a] - dr[a];
+            if constexpr (TM != 0)
+              sh.dv[t][a] = dv;
+            p[a]        = pe[a] + dv;
             sh.vp[t][a] = p[a];
           }
           const double zz = (dr[0] * u[0] + dr[1] * u[1] + dr[2] * u[2]) * (1.0 / r);
@@ -276,14 +280,17 @@ __global__ __launch_bounds__(WG, NL_KC == 6 ? 3 : 1) void k_nlpp(SysC s, State s
               ratio *= exp(jas[J_UAT * Np + iel] - sh.sum[2 * NL_KC + q]);
             const double wgt = sh.kp[q] * ratio;
             pairpot += wgt;
-            if (tm.mode == 2 && scan <= tm.sel_target[w])
+            if constexpr (TM != 0)
             {
-              sel[0] = 1.0;
-              sel[1] = sh.dv[q][0];
-              sel[2] = sh.dv[q][1];
-              sel[3] = sh.dv[q][2];
+              if (TM == 2 && scan <= tm.sel_target[w])
+              {
+                sel[0] = 1.0;
+                sel[1] = sh.dv[q][0];
+                sel[2] = sh.dv[q][1];
+                sel[3] = sh.dv[q][2];
+              }
+              scan += wgt > 0.0 ? wgt * tm.plus : wgt * tm.minus;
             }
-            scan += wgt > 0.0 ? wgt * tm.plus : wgt * tm.minus;
           }
         __syncthreads(); // the point tables are rewritten by the next chunk
       }
@@ -293,14 +300,31 @@ __global__ __launch_bounds__(WG, NL_KC == 6 ? 3 : 1) void k_nlpp(SysC s, State s
   }
   if (t == 0)
   {
-    if (tm.mode == 2)
+    if constexpr (TM == 2)
       for (int a = 0; a < 4; a++)
         tm.sel_out[(size_t)w * 4 + a] = sel[a];
     else
       part[(size_t)w * N + iel] = value;
-    if (tm.mode == 1)
+    if constexpr (TM == 1)
       tm.tsum[(size_t)w * N + iel] = scan;
   }
+}
+
+// the energy alone keeps the plain signature; the T-move modes take the move arguments
+template<typename ST, typename DT, int NJ, int MAXM, int NL_KC>
+__global__ __launch_bounds__(WG, NL_KC == 6 ? 3 : 1) void k_nlpp(SysC s, State st, const ST* __restrict__ coefs, NlppC c,
+                                                               const double* __restrict__ rot,
+                                                               double* __restrict__ part)
+{
+  nlpp_body<ST, DT, NJ, MAXM, NL_KC, 0>(s, st, coefs, c, rot, part, TmoveArgs{});
+}
+
+template<typename ST, typename DT, int NJ, int MAXM, int NL_KC, int TM>
+__global__ __launch_bounds__(WG, NL_KC == 6 ? 3 : 1) void k_nlpp_tm(SysC s, State st, const ST* __restrict__ coefs,
+                                                                  NlppC c, const double* __restrict__ rot,
+                                                                  double* __restrict__ part, TmoveArgs tm)
+{
+  nlpp_body<ST, DT, NJ, MAXM, NL_KC, TM>(s, st, coefs, c, rot, part, tm);
 }
 
 __global__ void k_nlpp_sum(int nw, int N, const double* __restrict__ part, double* __restrict__ out)
@@ -322,21 +346,41 @@ hipError_t launch_nlpp(const SysC& s, const State& st, const void* coefs, const 
   const int mm = std::max(move_maxm(s, 0), move_maxm(s, 1));
   if (mm == 0)
     return hipErrorInvalidValue;
-  const hipError_t e = dispatch_prec(s.prec, [&](auto stv, auto dtv) {
-    using ST = decltype(stv);
-    using DT = decltype(dtv);
-    const dim3 grid(tm.mode == 2 ? 1 : s.N, nw);
-    const ST* cf = static_cast<const ST*>(coefs);
-    if (mm == 1 && c.kc == 6)
-      k_nlpp<ST, DT, 2, 1, 6><<<grid, WG, 0, stream>>>(s, st, cf, c, rot, part, tm);
-    else if (mm == 1)
-      k_nlpp<ST, DT, 2, 1, 12><<<grid, WG, 0, stream>>>(s, st, cf, c, rot, part, tm);
-    else if (c.kc == 6)
-      k_nlpp<ST, DT, 4, 2, 6><<<grid, WG, 0, stream>>>(s, st, cf, c, rot, part, tm);
-    else
-      k_nlpp<ST, DT, 4, 2, 12><<<grid, WG, 0, stream>>>(s, st, cf, c, rot, part, tm);
-    return hipGetLastError();
-  });
+  auto launch = [&](auto tmv) {
+    constexpr int TM = decltype(tmv)::value;
+    return dispatch_prec(s.prec, [&](auto stv, auto dtv) {
+      using ST = decltype(stv);
+      using DT = decltype(dtv);
+      const dim3 grid(TM == 2 ? 1 : s.N, nw);
+      const ST* cf = static_cast<const ST*>(coefs);
+      if constexpr (TM == 0)
+      {
+        if (mm == 1 && c.kc == 6)
+          k_nlpp<ST, DT, 2, 1, 6><<<grid, WG, 0, stream>>>(s, st, cf, c, rot, part);
+        else if (mm == 1)
+          k_nlpp<ST, DT, 2, 1, 12><<<grid, WG, 0, stream>>>(s, st, cf, c, rot, part);
+        else if (c.kc == 6)
+          k_nlpp<ST, DT, 4, 2, 6><<<grid, WG, 0, stream>>>(s, st, cf, c, rot, part);
+        else
+          k_nlpp<ST, DT, 4, 2, 12><<<grid, WG, 0, stream>>>(s, st, cf, c, rot, part);
+      }
+      else
+      {
+        if (mm == 1 && c.kc == 6)
+          k_nlpp_tm<ST, DT, 2, 1, 6, TM><<<grid, WG, 0, stream>>>(s, st, cf, c, rot, part, tm);
+        else if (mm == 1)
+          k_nlpp_tm<ST, DT, 2, 1, 12, TM><<<grid, WG, 0, stream>>>(s, st, cf, c, rot, part, tm);
+        else if (c.kc == 6)
+          k_nlpp_tm<ST, DT, 4, 2, 6, TM><<<grid, WG, 0, stream>>>(s, st, cf, c, rot, part, tm);
+        else
+          k_nlpp_tm<ST, DT, 4, 2, 12, TM><<<grid, WG, 0, stream>>>(s, st, cf, c, rot, part, tm);
+      }
+      return hipGetLastError();
+    });
+  };
+  const hipError_t e = tm.mode == 2   ? launch(std::integral_constant<int, 2>{})
+                       : tm.mode == 1 ? launch(std::integral_constant<int, 1>{})
+                                      : launch(std::integral_constant<int, 0>{});
   if (e != hipSuccess || tm.mode == 2)
     return e;
   k_nlpp_sum<<<(nw + 63) / 64, 64, 0, stream>>>(nw, s.N, part, out);
