@@ -124,11 +124,12 @@ def test_loaded_streams_drive_the_run():
     assert np.array_equal(c.walkers_get(), ref.walkers_get()) and np.array_equal(E, Er)
 
 
-@pytest.mark.parametrize("nlpp,warmup,history", [(False, 3, 0), (False, 7, 0), (True, 2, 1)])
-def test_dmc_restart_follows_the_uninterrupted_run(nlpp, warmup, history):
-    """Checkpoint after 5 steps (in the main stage with a limited energy history, during warmup, or
-    with the pseudopotential term and the unlimited history), restart in a fresh context, 5 more
-    steps against the oracle's 10 uninterrupted ones."""
+@pytest.mark.parametrize("nlpp,warmup,history,tmoves", [(False, 3, 0, 0), (False, 7, 0, 0), (True, 2, 1, 0),
+                                                        (True, 3, 0, 1)])
+def test_dmc_restart_follows_the_uninterrupted_run(nlpp, warmup, history, tmoves):
+    """Checkpoint after 5 steps (in the main stage with a limited energy history, during warmup,
+    with the pseudopotential term and the unlimited history, or with T-moves), restart in a fresh
+    context, 5 more steps against the oracle's 10 uninterrupted ones."""
     from test_oracle_nlpp import channels, icosahedron
     s = system(k=4, seed=29, n_ion=4)
     R = make_walkers(s, 20, 7)
@@ -136,7 +137,8 @@ def test_dmc_restart_follows_the_uninterrupted_run(nlpp, warmup, history):
     ls = [0, 1, 2]
     dr, v, v2, _ = channels(ls, seed=4)
     knots, w = icosahedron()
-    kw = dict(tau=0.02, capacity=96, warmup_steps=warmup, refenergy_history=history, recompute_period=4)
+    kw = dict(tau=0.02, capacity=96, warmup_steps=warmup, refenergy_history=history, recompute_period=4,
+              nonlocal_moves=tmoves)
 
     def ctx():
         c = qmi.Context(0)
@@ -167,7 +169,7 @@ def test_dmc_restart_follows_the_uninterrupted_run(nlpp, warmup, history):
     o.dmc_run(5)
     sto = o.dmc_run(5)
     print("walkers per step", st[:, 4].astype(int).tolist())
-    assert np.array_equal(st[:, [3, 4, 8, 9]], sto[:, [3, 4, 8, 9]])
+    assert np.array_equal(st[:, [3, 4, 8, 9, 10]], sto[:, [3, 4, 8, 9, 10]])
     for col in (0, 2, 5, 6, 7):
         assert rel(st[:, col], sto[:, col]) < REL, col
     assert rel(st[:, 1], sto[:, 1], floor=max(1.0, float(np.max(sto[:, 1])) * 1e-3)) < 1e-9
