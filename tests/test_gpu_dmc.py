@@ -147,6 +147,33 @@ def test_dmc_with_tmoves(k):
     assert st[:, 10].sum() >= 5
 
 
+def test_dmc_tmoves_production_shaped():
+    """T-moves on 128 electrons (64 per spin), delay rank 16, 96 walkers: more electrons move per
+    step than the delay rank, so the protocol moves flush the delayed block inside the loop."""
+    from test_oracle_nlpp import channels, icosahedron
+    s = make_system(L=10.0, n_ion=16, n_up=64, n_dn=64, norb=64, M=16, delay_rank=16, seed=19)
+    R = make_walkers(s, 96, 5)
+    zion = s.N / len(s.ions)
+    ls = [0, 1]
+    dr, v, v2, _ = channels(ls, seed=6)
+    knots, w = icosahedron()
+    kw = dict(tau=0.1, capacity=384, warmup_steps=2, nonlocal_moves=1)
+    c = qmi.Context(0)
+    c.system_create(s)
+    c.walkers_load(R, 1)
+    c.coulomb_setup(zion)
+    c.nlpp_setup(ls, dr, v, v2, 2.0, knots, w)
+    c.dmc_init(seed=73, **kw)
+    st = c.dmc_run(3)
+    o = orc.VMC(s, R, 1, 73)
+    o.dmc_set_nlpp(ls, dr, v, v2, 2.0, knots, w)
+    o.dmc_init(zion, seed=73, **kw)
+    sto = o.dmc_run(3)
+    print("T-moves per step", st[:, 10].astype(int).tolist())
+    check_dmc(c, o, st, sto)
+    assert st[:, 10].max() > 16
+
+
 def test_dmc_mixed_precision_runs():
     """DMC on the FP32 determinant engine: the same driver, finite energies, a living population
     and a weighted energy close to the FP64 run's at the first step."""
