@@ -909,8 +909,8 @@ struct qmi_ctx
   NlppC nlpp{};
   DevBuf<double> nlpp_poly, nlpp_knots, nlpp_rot, nlpp_part, nlpp_out;
   // non-local moves: per-electron scan weights of the last evaluation, selection in / out
-  DevBuf<double> tm_tsum, tm_base, tm_target, tm_out;
-  DevBuf<int> tm_iel;
+  DevBuf<double> tm_tsum, tm_base, tm_target, tm_out, tm_disp;
+  DevBuf<int> tm_iel, tm_chosen, tm_acc, tm_count;
   std::vector<double> h_tsum;
 
   void nlpp_setup(int nch, const int* ls, int ng, double dr, const double* y, const double* y2, double rcut, int nknot,
@@ -1029,8 +1029,9 @@ struct qmi_ctx
   // scan starting at 1 (no move) over the electron-major list of the last evaluation; the chosen
   // electron is moved by deltaV and the move accepted when the ratio is non-zero.  The scan is
   // summed per electron on the device, across electrons here; the selected electron's points are
-  // re-evaluated to find the point.  The moves go through the multi-walker protocol one electron
-  // at a time (walkers that do not move that electron take a zero displacement and are rejected).
+  // re-evaluated to find the point.  The moves go through the multi-walker protocol kernels one
+  // electron at a time (walkers that do not move that electron take a zero displacement and are
+  // rejected).
   int dmc_tmoves(const std::vector<double>& u)
   {
     const int N = sys.N;
@@ -1087,36 +1088,68 @@ struct qmi_ctx
     std::vector<double> so((size_t)nw * 4);
     HIPCHECK(hipMemcpyAsync(so.data(), tm_out.p, so.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
     sync();
-    auto check = [&](int rc) {
-      if (rc != QMI_OK)
-        throw QmiError(rc, err);
-    };
-    std::vector<double> disp((size_t)nw * 3), ratios(nw);
-    std::vector<int> acc(nw);
-    int accepted = 0;
+    // the moves of each electron that some walker moves, enqueued without host round trips:
+    // displacements (zero for the walkers that do not move it), ratio, the acceptance mask on the
+    // device (moved and a non-zero ratio), accept; the delayed block flushed when full
+    std::vector<int> elist;
+    std::vector<double> disp;
+    std::vector<int> chosen;
     for (int e = 0; e < N; e++)
     {
       bool here = false;
+      for (int w = 0; w < nw && !here; w++)
+        here = sel[w] == e && so[(size_t)w * 4] != 0.0;
+      if (!here)
+        continue;
+      elist.push_back(e);
       for (int w = 0; w < nw; w++)
       {
         const bool m = sel[w] == e && so[(size_t)w * 4] != 0.0;
         for (int a = 0; a < 3; a++)
-          disp[(size_t)w * 3 + a] = m ? so[(size_t)w * 4 + 1 + a] : 0.0;
-        acc[w] = m ? 1 : 0;
-        here |= m;
+          disp.push_back(m ? so[(size_t)w * 4 + 1 + a] : 0.0);
+        chosen.push_back(m ? 1 : 0);
       }
-      if (!here)
-        continue;
-      check(qmi_mw_make_move(this, e, disp.data()));
-      check(qmi_mw_ratio_grad(this, e, ratios.data(), nullptr));
-      for (int w = 0; w < nw; w++)
-      {
-        acc[w] = acc[w] && ratios[w] != 0.0;
-        accepted += acc[w];
-      }
-      check(qmi_mw_accept_reject(this, e, acc.data(), 1));
     }
-    check(qmi_mw_complete_updates(this));
+    if (elist.empty())
+      return 0;
+    if (tm_disp.n < disp.size())
+      tm_disp.alloc(disp.size());
+    if (tm_chosen.n < chosen.size())
+      tm_chosen.alloc(chosen.size());
+    if (tm_acc.n < (size_t)cap)
+      tm_acc.alloc(cap);
+    if (tm_count.n < 1)
+      tm_count.alloc(1);
+    HIPCHECK(hipMemcpyAsync(tm_disp.p, disp.data(), disp.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(tm_chosen.p, chosen.data(), chosen.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+    HIPCHECK(launch_fill_i32(tm_count.p, 1, 0, stream));
+    for (size_t i = 0; i < elist.size(); i++)
+    {
+      const int e = elist[i], g = group(e);
+      HIPCHECK(launch_make_move(sys, st, e, nw, tm_disp.p + i * (size_t)nw * 3, stream));
+      MoveArgs a = move_args(e);
+      a.out      = out.p;
+      a.prepare  = (sys.k == 1 || invrow_id != e) ? 1 : 0;
+      {
+        Scoped t(this, "api_ratio_grad");
+        HIPCHECK(move_kernel(MK_API_RATIO, a));
+      }
+      HIPCHECK(launch_tmove_accept_mask(nw, out.p, tm_chosen.p + i * (size_t)nw, tm_acc.p, tm_count.p, stream));
+      MoveArgs b = move_args(e);
+      b.host_acc = tm_acc.p;
+      {
+        Scoped t(this, "api_accept");
+        HIPCHECK(move_kernel(MK_API_ACCEPT, b));
+      }
+      invrow_id = -1;
+      if (sys.k > 1 && ++dcount[g] == sys.k)
+        flush(g);
+    }
+    flush(0);
+    flush(1);
+    int accepted = 0;
+    HIPCHECK(hipMemcpyAsync(&accepted, tm_count.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+    sync();
     return accepted;
   }
 

@@ -422,6 +422,26 @@ __global__ void k_fill_i32(int* __restrict__ p, size_t n, int v)
     p[i] = v;
 }
 
+// Non-local moves: acc[w] = chosen[w] and a non-zero ratio; count[0] += the accepted moves
+__global__ void k_tmove_accept_mask(int nw, const double* __restrict__ ratio, const int* __restrict__ chosen,
+                                    int* __restrict__ acc, int* __restrict__ count)
+{
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nw)
+    return;
+  const int a = chosen[w] != 0 && ratio[w] != 0.0;
+  acc[w]      = a;
+  if (a)
+    atomicAdd(count, 1);
+}
+
+hipError_t launch_tmove_accept_mask(int nw, const double* ratio, const int* chosen, int* acc, int* count,
+                                    hipStream_t stream)
+{
+  k_tmove_accept_mask<<<(nw + 255) / 256, 256, 0, stream>>>(nw, ratio, chosen, acc, count);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill_i32(int* p, size_t n, int v, hipStream_t stream)
 {
   if (n == 0)
