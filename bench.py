@@ -392,7 +392,7 @@ def main():
         sys.exit(2)
 
     from qmcpack_amd import qmi
-    from qmcpack_amd.system import config_system, make_walkers
+    from qmcpack_amd.system import workload_system, make_walkers
 
     def log(*a):
         if not args.quiet:
@@ -407,7 +407,7 @@ def main():
         return
 
     t0 = time.perf_counter()
-    system, cfg = config_system(1)
+    system, cfg = workload_system("production")
     nw = args.walkers or cfg["nw"]
     R = make_walkers(system, nw, cfg["seed"])
     log(f"[bench] system built in {time.perf_counter() - t0:.1f}s: N={system.N} n={system.n_up} "

@@ -22,12 +22,12 @@ import dataclasses
 import numpy as np
 
 # Workload definitions.  Sizes are the ones the benchmark and parity tests run.
-CONFIGS = {
-    # plain method, small: CPU-runnable
-    0: dict(L=10.0, n_ion=16, n_up=64, n_dn=64, norb=64, M=48, nw=16, walkers_per_crowd=16,
+WORKLOADS = {
+    # plain method, small: CPU-runnable (16 walkers x 128 electrons, one crowd, delay rank 1)
+    "small": dict(L=10.0, n_ion=16, n_up=64, n_dn=64, norb=64, M=48, nw=16, walkers_per_crowd=16,
             delay_rank=1, tau=0.3, sweeps=10, seed=20240),
-    # plain method at production size
-    1: dict(L=18.0, n_ion=128, n_up=256, n_dn=256, norb=256, M=64, nw=1024, walkers_per_crowd=1,
+    # plain method at production size (1024 walkers x 512 electrons, delay rank 32)
+    "production": dict(L=18.0, n_ion=128, n_up=256, n_dn=256, norb=256, M=64, nw=1024, walkers_per_crowd=1,
             delay_rank=32, tau=0.3, sweeps=50, recompute_period=64, seed=20241),
 }
 
@@ -145,7 +145,8 @@ def make_walkers(system: System, nw: int, seed: int) -> np.ndarray:
     return rng.uniform(0.0, 1.0, size=(nw, system.N, 3)) * system.L[None, None, :]
 
 
-def config_system(cfg: int, **overrides) -> tuple[System, dict]:
-    c = dict(CONFIGS[cfg])
+def workload_system(name: str, **overrides) -> tuple[System, dict]:
+    """The System of a named workload and its parameters (walkers, tau, sweeps, seed, ...)."""
+    c = dict(WORKLOADS[name])
     c.update(overrides)
     return make_system(**c), c

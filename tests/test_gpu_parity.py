@@ -16,7 +16,7 @@ import pytest
 
 from oracle import oracle as orc
 from qmcpack_amd import qmi
-from qmcpack_amd.system import System, config_system, jastrow_params, make_system, make_walkers
+from qmcpack_amd.system import System, workload_system, jastrow_params, make_system, make_walkers
 
 pytestmark = pytest.mark.gpu
 
@@ -158,7 +158,7 @@ def test_vmc_config0_one_crowd():
     """Workload: 16 walkers x 128 electrons, 64 orbitals per spin on a 48^3 grid, 16 ions, one crowd
     (shared RNG stream, device Metropolis-Hastings with the reference's uniform order), delay rank 1,
     tau 0.3, 10 sweeps."""
-    s, cfg = config_system(0)
+    s, cfg = workload_system("small")
     R = make_walkers(s, cfg["nw"], cfg["seed"])
     res = vmc_parity(s, R, cfg["walkers_per_crowd"], steps=cfg["sweeps"], sub_steps=1, tau=cfg["tau"],
                      recompute_period=0, seed=cfg["seed"])

@@ -188,8 +188,8 @@ def test_vmc_chain_is_chaotic():
     """Drift-diffusion VMC amplifies rounding exponentially: a 1e-15 relative change of the start
     moves the local energy by far more than 1e-10 within two sweeps.  Parity of two FP64
     implementations is therefore checked on a shared trajectory (replay), not free-running."""
-    from qmcpack_amd.system import config_system, make_walkers
-    s, cfg = config_system(0)
+    from qmcpack_amd.system import workload_system, make_walkers
+    s, cfg = workload_system("small")
     R = make_walkers(s, 4, cfg["seed"])
     E = []
     for RR in (R, R * (1 + 1e-15)):
