@@ -1118,11 +1118,11 @@ struct qmi_ctx
       tm_chosen.alloc(chosen.size());
     if (tm_acc.n < (size_t)cap)
       tm_acc.alloc(cap);
-    if (tm_count.n < 1)
-      tm_count.alloc(1);
+    if (tm_count.n < 2)
+      tm_count.alloc(2);
     HIPCHECK(hipMemcpyAsync(tm_disp.p, disp.data(), disp.size() * sizeof(double), hipMemcpyHostToDevice, stream));
     HIPCHECK(hipMemcpyAsync(tm_chosen.p, chosen.data(), chosen.size() * sizeof(int), hipMemcpyHostToDevice, stream));
-    HIPCHECK(launch_fill_i32(tm_count.p, 1, 0, stream));
+    HIPCHECK(launch_fill_i32(tm_count.p, 2, 0, stream));
     for (size_t i = 0; i < elist.size(); i++)
     {
       const int e = elist[i], g = group(e);
@@ -1147,10 +1147,12 @@ struct qmi_ctx
     }
     flush(0);
     flush(1);
-    int accepted = 0;
-    HIPCHECK(hipMemcpyAsync(&accepted, tm_count.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+    int counts[2] = {0, 0};
+    HIPCHECK(hipMemcpyAsync(counts, tm_count.p, sizeof(counts), hipMemcpyDeviceToHost, stream));
     sync();
-    return accepted;
+    if (counts[1])
+      throw QmiError(QMI_ERR_NUMERIC, "NaN ratio in a non-local move");
+    return counts[0];
   }
 
   // ---------------------------------------------------------------------------------------------

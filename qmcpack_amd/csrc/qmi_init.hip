@@ -422,17 +422,22 @@ __global__ void k_fill_i32(int* __restrict__ p, size_t n, int v)
     p[i] = v;
 }
 
-// Non-local moves: acc[w] = chosen[w] and a non-zero ratio; count[0] += the accepted moves
+// Non-local moves: acc[w] = chosen[w] and a non-zero ratio; count[0] += the accepted moves,
+// count[1] += the chosen moves whose ratio is NaN (not accepted)
 __global__ void k_tmove_accept_mask(int nw, const double* __restrict__ ratio, const int* __restrict__ chosen,
                                     int* __restrict__ acc, int* __restrict__ count)
 {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nw)
     return;
-  const int a = chosen[w] != 0 && ratio[w] != 0.0;
-  acc[w]      = a;
+  const double r = ratio[w];
+  const bool nan = r != r;
+  const int a    = chosen[w] != 0 && r != 0.0 && !nan;
+  acc[w]         = a;
   if (a)
     atomicAdd(count, 1);
+  if (chosen[w] != 0 && nan)
+    atomicAdd(count + 1, 1);
 }
 
 hipError_t launch_tmove_accept_mask(int nw, const double* ratio, const int* chosen, int* acc, int* count,

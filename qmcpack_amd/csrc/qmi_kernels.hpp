@@ -80,7 +80,7 @@ hipError_t launch_flush(const SysC& s, const State& st, int g, int d, int nw, do
 // p[0, n) = v (a kernel, so that it can be captured in a graph)
 hipError_t launch_fill_i32(int* p, size_t n, int v, hipStream_t stream);
 // non-local moves: acc[w] = chosen[w] && ratio[w] != 0 (ratio[nw] as written by the ratio kernel),
-// count[0] += accepted
+// count[0] += accepted, count[1] += chosen moves with a NaN ratio
 hipError_t launch_tmove_accept_mask(int nw, const double* ratio, const int* chosen, int* acc, int* count,
                                     hipStream_t stream);
 
