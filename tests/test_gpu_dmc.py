@@ -190,3 +190,27 @@ def test_dmc_mixed_precision_runs():
         out.append(c.dmc_run(6))
     assert np.all(np.isfinite(out[1])) and np.all(out[1][:, 4] > 0)
     assert abs(out[1][0, 0] - out[0][0, 0]) < 1e-3 * max(1.0, abs(out[0][0, 0]))
+
+
+def test_dmc_tmoves_mixed_precision_runs():
+    """T-moves on the FP32 determinant engine and FP32 table: the same driver, finite statistics,
+    accepted non-local moves, the first step's energy close to the FP64 run's."""
+    from test_oracle_nlpp import channels, icosahedron
+    s = make_system(L=7.0, n_ion=4, n_up=7, n_dn=6, norb=8, M=12, delay_rank=4, seed=29)
+    R = make_walkers(s, 32, 7)
+    zion = s.N / len(s.ions)
+    ls = [0, 1, 2]
+    dr, v, v2, _ = channels(ls, seed=4)
+    knots, w = icosahedron()
+    out = []
+    for prec in ("fp64", "fp32"):
+        c = qmi.Context(0, precision=prec)
+        c.system_create(s)
+        c.walkers_load(R, 1)
+        c.coulomb_setup(zion)
+        c.nlpp_setup(ls, dr, v, v2, 2.2, knots, w)
+        c.dmc_init(tau=0.1, capacity=128, seed=81, warmup_steps=3, nonlocal_moves=1)
+        out.append(c.dmc_run(4))
+    print("T-moves per step", out[1][:, 10].astype(int).tolist())
+    assert np.all(np.isfinite(out[1])) and np.all(out[1][:, 4] > 0) and out[1][:, 10].sum() >= 2
+    assert abs(out[1][0, 0] - out[0][0, 0]) < 1e-3 * max(1.0, abs(out[0][0, 0]))
