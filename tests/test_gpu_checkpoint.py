@@ -34,8 +34,8 @@ def system(k=4, seed=13, n_ion=3):
     return make_system(L=7.0, n_ion=n_ion, n_up=7, n_dn=6, norb=8, M=12, delay_rank=k, seed=seed)
 
 
-def vmc_ctx(s, R, wpc):
-    c = qmi.Context(0)
+def vmc_ctx(s, R, wpc, prec="fp64"):
+    c = qmi.Context(0, precision=prec)
     c.system_create(s)
     c.walkers_load(R, wpc)
     c.mw_evaluateLog()
@@ -103,23 +103,24 @@ def test_saved_streams_are_mt19937_states():
         assert len(found) == 1, w
 
 
-def test_loaded_streams_drive_the_run():
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_loaded_streams_drive_the_run(prec):
     """Streams written by numpy (fresh seeds 999 + w) replace the saved ones: the restarted run equals
-    a run seeded with 999 on the same walkers, bit for bit."""
+    a run seeded with 999 on the same walkers, bit for bit (FP64 and the FP32 engine)."""
     s = system(k=2)
     R = make_walkers(s, 4, 3)
-    b = vmc_ctx(s, R, 1)
+    b = vmc_ctx(s, R, 1, prec)
     b.vmc_run(2, seed=5)
     d = qmi.read_checkpoint(b.checkpoint_save())
     rs = np.zeros((4, 625), dtype=np.uint32)
     for w in range(4):
         rs[w, :624], rs[w, 624] = np_state(999 + w)
     d["random/std::mt19937"] = rs
-    c = qmi.Context(0)
+    c = qmi.Context(0, precision=prec)
     c.system_create(s)
     c.checkpoint_load(qmi.write_checkpoint(d))
     E, _ = c.vmc_run(2, seed=5)  # the checkpoint's seed: continue the loaded streams
-    ref = vmc_ctx(s, d["state_0/walkers"], 1)
+    ref = vmc_ctx(s, d["state_0/walkers"], 1, prec)
     Er, _ = ref.vmc_run(2, seed=999)
     assert np.array_equal(c.walkers_get(), ref.walkers_get()) and np.array_equal(E, Er)
 
