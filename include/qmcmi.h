@@ -143,6 +143,20 @@ int qmi_mw_spo_vgl(qmi_ctx* ctx, const double* pos, int n, double* out);
  * ratios[nw][nvp]. */
 int qmi_mw_det_ratios(qmi_ctx* ctx, int nvp, const double* pos, int n, const double* inv_row, double* ratios);
 
+/* DiracMatrixInverter::mw_invert_transpose (Fermion/DiracMatrixInverter.hpp:41-44; CUDA form
+ * DiracMatrixInverterCUDA.hpp:305-360 with getrf/getri and computeLogDet_kernel,
+ * detail/CUDA/cuBLAS_LU.cu:62-108; per matrix DiracMatrix::invert_transpose, DiracMatrix.h:189-220):
+ * nmat matrices a[nmat][n][n] (row-major; (re, im) pairs when complex_values) ->
+ * inv_a[nmat][n][n] = (a^-1)^T and log_values[nmat][2] = (log|det a|, phase) (nullable).  The batched
+ * LU with partial pivoting of the determinant recompute, on the context's device and stream (no system
+ * needed).  A singular matrix returns QMI_ERR_NUMERIC. */
+int qmi_mw_invert_transpose(qmi_ctx* ctx, int complex_values, int n, int nmat, const double* a, double* inv_a,
+                            double* log_values);
+/* DelayedUpdateBatched::mw_transferAinv_D2H (DelayedUpdateBatched.h:813-824, guard_no_delay) /
+ * DiracDeterminantBatched::get_psiMinv (DiracDeterminantBatched.h:276): the inverse Slater matrix of
+ * spin group g of every walker, ainv[nw][n][n] (row i = inverse row of electron i of the group; FP32
+ * storage widened), after the delayed updates are completed. */
+int qmi_mw_get_inverse(qmi_ctx* ctx, int g, double* ainv);
 /* DistanceTable::mw_move (DistanceTable.h:153-161) with SoaDistanceTableAA/AB rows
  * (SoaDistanceTableAA.h:58-75, SoaDistanceTableAB.h:59-78): distances and displacements
  * (source - target, minimum image) from pos[nw][3] to all electrons and all ions.

@@ -970,7 +970,7 @@ hipError_t launch_move_c(int which, const SysC& s, const State& st, const void* 
 
 //------------------------------------------------------------------------------------------------
 // Recompute: C2C orbitals at every electron of determinant g, values transposed into the complex
-// LU workspace lu ([nw][n][ld] complex = Ainv), gradients/Laplacians into dpsi.
+// LU workspace lu ([nw][n][ld] complex), gradients/Laplacians into dpsi.
 __global__ __launch_bounds__(WG) void k_spo_rows_c(SysC s, State st, const double* __restrict__ coefs, int g,
                                                    double2* __restrict__ lu)
 {
@@ -1006,34 +1006,6 @@ hipError_t launch_spo_rows_c(const SysC& s, const State& st, const void* coefs, 
 {
   k_spo_rows_c<<<dim3(s.n[g], nw), WG, 0, stream>>>(s, st, static_cast<const double*>(coefs), g,
                                                     reinterpret_cast<double2*>(lu));
-  return hipGetLastError();
-}
-
-// log det = sum log(+-U_ii) of the complex LU (DiracMatrix.h:101-106): log|v| and arg(v) per factor
-__global__ void k_logdet_c(SysC s, State st, int g, int nw, const int* __restrict__ ipiv, const double2* __restrict__ lu)
-{
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= nw)
-    return;
-  const int n = s.n[g], ld = s.ld;
-  const double2* A = lu + (size_t)w * n * ld;
-  const int* p     = ipiv + (size_t)w * n;
-  double re = 0, im = 0;
-  for (int i = 0; i < n; i++)
-  {
-    const double2 dg = A[(size_t)i * ld + i];
-    const double2 v  = (p[i] == i + 1) ? dg : zmk(-dg.x, -dg.y);
-    re += 0.5 * log(znorm(v));
-    im += atan2(v.y, v.x);
-  }
-  st.logdet[(size_t)w * 4 + g * 2 + 0] = re;
-  st.logdet[(size_t)w * 4 + g * 2 + 1] = im;
-}
-
-hipError_t launch_logdet_c(const SysC& s, const State& st, int g, int nw, const int* ipiv, const double* lu,
-                           hipStream_t stream)
-{
-  k_logdet_c<<<(nw + 63) / 64, 64, 0, stream>>>(s, st, g, nw, ipiv, reinterpret_cast<const double2*>(lu));
   return hipGetLastError();
 }
 

@@ -5,9 +5,6 @@
 #include "qmi_kernels.hpp"
 #include "qmi_rng.hpp"
 
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
-
 #include <algorithm>
 #include <array>
 #include <deque>
@@ -160,7 +157,6 @@ struct qmi_ctx
   int device     = 0;
   int precision  = QMI_PREC_FP64;
   hipStream_t stream = nullptr;
-  rocblas_handle blas = nullptr;
   std::string err;
   bool have_system = false;
   bool cplx        = false; // complex (C2C) orbitals
@@ -172,8 +168,9 @@ struct qmi_ctx
   DevBuf<double> R, jas, logdet, stats, invRow, phi, mv, flushT, out, gl3, Gbuf, Lbuf, hostin;
   // determinant engine, in elements of the determinant storage type (double or float)
   DevBuf<double> Ainv[2], dpsi[2], U[2], V[2], Binv[2];
-  DevBuf<double> lu; // FP64 LU workspace of the mixed precisions ([nw][n][ld])
-  DevBuf<int> dlist[2], ipiv, info, ubase, acc;
+  // recompute: FP64 LU factors and the solve's work array ([nw][n][ld] values), unit rows of P e_k
+  DevBuf<double> lu, luw;
+  DevBuf<int> dlist[2], ipiv, lperm, info, ubase, acc;
   DevBuf<double> gauss, unif, trace;
   State st{};
   int dcount[2] = {0, 0};
@@ -488,10 +485,11 @@ struct qmi_ctx
         HIPCHECK(hipMemset(V[g].p, 0, V[g].n * sizeof(double)));
         HIPCHECK(hipMemset(Binv[g].p, 0, Binv[g].n * sizeof(double)));
       }
-      if (det_fp32())
-        lu.alloc((size_t)cap * std::max(s.nup, s.ndn) * ld);
+      lu.alloc((size_t)cap * std::max(s.nup, s.ndn) * ld * vw());
+      luw.alloc((size_t)cap * std::max(s.nup, s.ndn) * ld * vw());
       ipiv.alloc((size_t)cap * std::max(s.nup, s.ndn));
-      info.alloc(cap);
+      lperm.alloc((size_t)cap * std::max(s.nup, s.ndn));
+      info.alloc((size_t)2 * cap);
       if (k > 1)
         flushT.alloc((size_t)cap * ld * k * vw()); // T (n x k) or BV_T (ld x k) per walker
     }
@@ -559,7 +557,8 @@ struct qmi_ctx
   }
 
   // ---------------------------------------------------------------------------------------------
-  // determinant recompute: orbital rows, LU (rocSOLVER), log det, inverse
+  // determinant recompute: orbital rows, batched LU, log det, inverse (FP64 in every precision;
+  // the FP32 engine stores the inverse rounded to FP32, DiracMatrix.h:204-220)
   void recompute_dets()
   {
     const SysC& s = sys;
@@ -567,57 +566,45 @@ struct qmi_ctx
       return;
     if (dcount[0] || dcount[1])
       throw QmiError(QMI_ERR_ARG, "recompute with pending delayed updates");
+    const int out_kind = cplx ? LU_OUT_C128 : det_fp32() ? LU_OUT_F32 : LU_OUT_F64;
     for (int g = 0; g < 2; g++)
     {
       const int n = s.n[g];
       if (n == 0)
         continue;
-      // FP64 LU workspace: Ainv itself in FP64 contexts
-      double* L64 = det_fp32() ? lu.p : Ainv[g].p;
-      const rocblas_stride stride = (rocblas_stride)n * s.ld;
-      auto* Lz = reinterpret_cast<rocblas_double_complex*>(L64);
       {
         Scoped t(this, "spo_rows");
-        HIPCHECK(cplx ? launch_spo_rows_c(s, st, coefs.p, g, nw, L64, stream)
-                      : launch_spo_rows(s, st, coefs.p, g, nw, L64, stream));
+        HIPCHECK(cplx ? launch_spo_rows_c(s, st, coefs.p, g, nw, lu.p, stream)
+                      : launch_spo_rows(s, st, coefs.p, g, nw, lu.p, stream));
       }
       {
-        Scoped t(this, "lu_getrf");
-        const rocblas_status rs = cplx ? rocsolver_zgetrf_strided_batched(blas, n, n, Lz, s.ld, stride, ipiv.p, n,
-                                                                          info.p, nw)
-                                       : rocsolver_dgetrf_strided_batched(blas, n, n, L64, s.ld, stride, ipiv.p, n,
-                                                                          info.p, nw);
-        if (rs != rocblas_status_success)
-          throw QmiError(QMI_ERR_DEVICE, "rocsolver getrf_strided_batched failed");
+        Scoped t(this, "lu_factor");
+        HIPCHECK(launch_lu_factor(cplx, n, s.ld, nw, lu.p, ipiv.p, lperm.p, info.p + (size_t)g * cap, stream));
       }
-      check_info("LU factorisation: singular Slater matrix");
       {
         Scoped t(this, "logdet");
-        HIPCHECK(cplx ? launch_logdet_c(s, st, g, nw, ipiv.p, L64, stream)
-                      : launch_logdet(s, st, g, nw, ipiv.p, L64, stream));
+        HIPCHECK(launch_lu_logdet(cplx, n, s.ld, nw, lu.p, ipiv.p, logdet.p + 2 * g, 4, stream));
       }
       {
-        Scoped t(this, "lu_getri");
-        const rocblas_status rs =
-            cplx ? rocsolver_zgetri_strided_batched(blas, n, Lz, s.ld, stride, ipiv.p, n, info.p, nw)
-                 : rocsolver_dgetri_strided_batched(blas, n, L64, s.ld, stride, ipiv.p, n, info.p, nw);
-        if (rs != rocblas_status_success)
-          throw QmiError(QMI_ERR_DEVICE, "rocsolver getri_strided_batched failed");
+        Scoped t(this, "lu_inverse");
+        HIPCHECK(launch_lu_inverse(out_kind, n, s.ld, nw, lu.p, lperm.p, luw.p, Ainv[g].p, stream));
       }
-      check_info("matrix inverse: singular Slater matrix");
-      HIPCHECK(launch_store_inverse(s, st, g, nw, L64, stream));
     }
+    check_info("LU factorisation: singular Slater matrix");
     invrow_id = -1;
   }
 
+  // info of both determinants ([2][cap]): the first zero pivot of any walker is an error
   void check_info(const char* what)
   {
-    std::vector<int> h(nw);
-    HIPCHECK(hipMemcpyAsync(h.data(), info.p, nw * sizeof(int), hipMemcpyDeviceToHost, stream));
+    std::vector<int> h((size_t)2 * cap);
+    HIPCHECK(hipMemcpyAsync(h.data(), info.p, h.size() * sizeof(int), hipMemcpyDeviceToHost, stream));
     sync();
-    for (int w = 0; w < nw; w++)
-      if (h[w] != 0)
-        throw QmiError(QMI_ERR_NUMERIC, std::string(what) + " (walker " + std::to_string(w) + ")");
+    for (int g = 0; g < 2; g++)
+      for (int w = 0; w < nw; w++)
+        if (sys.n[g] > 0 && h[(size_t)g * cap + w] != 0)
+          throw QmiError(QMI_ERR_NUMERIC, std::string(what) + " (walker " + std::to_string(w) + ", determinant " +
+                                              std::to_string(g) + ")");
   }
 
   // mw_evaluateGL in two halves: the kernel and the copies (per-walker {ekin, log psi, phase} to
@@ -2782,13 +2769,6 @@ extern "C"
       c->move_one = std::string(mk) != "segment";
     if (const char* lf = std::getenv("QMI_LEARN_FUSED"))
       c->fused_learn = std::string(lf) != "0";
-    if (rocblas_create_handle(&c->blas) != rocblas_status_success ||
-        rocblas_set_stream(c->blas, c->stream) != rocblas_status_success)
-    {
-      (void)hipStreamDestroy(c->stream);
-      delete c;
-      return QMI_ERR_DEVICE;
-    }
     *out = c;
     return QMI_OK;
   }
@@ -2799,8 +2779,6 @@ extern "C"
       return QMI_ERR_ARG;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    if (c->blas)
-      rocblas_destroy_handle(c->blas);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return QMI_OK;
@@ -2997,6 +2975,77 @@ extern "C"
                        : launch_api_det_ratios(c->sys, c->coefs.p, c->nw, nvp, n, dpos.p, dinv.p, dout.p, c->stream));
       HIPCHECK(hipMemcpyAsync(ratios, dout.p, dout.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
       c->sync();
+    });
+  }
+
+  int qmi_mw_invert_transpose(qmi_ctx* c, int complex_values, int n, int nmat, const double* a, double* inv_a,
+                              double* log_values)
+  {
+    return guarded(c, [&] {
+      if (n <= 0 || n > MAX_N || nmat <= 0 || !a || !inv_a)
+        throw QmiError(QMI_ERR_ARG, "matrix size must be in [1, 512], at least one matrix, non-null arrays");
+      const int vw = complex_values ? 2 : 1;
+      const size_t per = (size_t)n * n * vw;
+      // the factorisation works on A(i, c) = a[i][c] stored column-major
+      std::vector<double> t(per * nmat);
+      for (int m = 0; m < nmat; m++)
+        for (int i = 0; i < n; i++)
+          for (int j = 0; j < n; j++)
+            for (int q = 0; q < vw; q++)
+              t[m * per + ((size_t)j * n + i) * vw + q] = a[m * per + ((size_t)i * n + j) * vw + q];
+      DevBuf<double> da, dw, dout, dlog;
+      DevBuf<int> dpiv, dperm, dinfo;
+      da.alloc(t.size());
+      dw.alloc(t.size());
+      dout.alloc(t.size());
+      dlog.alloc((size_t)nmat * 2);
+      dpiv.alloc((size_t)nmat * n);
+      dperm.alloc((size_t)nmat * n);
+      dinfo.alloc(nmat);
+      HIPCHECK(hipMemcpyAsync(da.p, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      HIPCHECK(launch_lu_factor(complex_values != 0, n, n, nmat, da.p, dpiv.p, dperm.p, dinfo.p, c->stream));
+      HIPCHECK(launch_lu_logdet(complex_values != 0, n, n, nmat, da.p, dpiv.p, dlog.p, 2, c->stream));
+      HIPCHECK(launch_lu_inverse(complex_values ? LU_OUT_C128 : LU_OUT_F64, n, n, nmat, da.p, dperm.p, dw.p, dout.p,
+                                 c->stream));
+      std::vector<int> info(nmat);
+      HIPCHECK(hipMemcpyAsync(info.data(), dinfo.p, nmat * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIPCHECK(hipMemcpyAsync(inv_a, dout.p, dout.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      if (log_values)
+        HIPCHECK(hipMemcpyAsync(log_values, dlog.p, dlog.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      c->sync();
+      for (int m = 0; m < nmat; m++)
+        if (info[m] != 0)
+          throw QmiError(QMI_ERR_NUMERIC, "singular matrix " + std::to_string(m) + " (zero pivot in column " +
+                                              std::to_string(info[m] - 1) + ")");
+    });
+  }
+
+  int qmi_mw_get_inverse(qmi_ctx* c, int g, double* ainv)
+  {
+    return guarded(c, [&] {
+      c->require_walkers();
+      if (!c->sys.has_det || g < 0 || g > 1 || !ainv)
+        throw QmiError(QMI_ERR_ARG, "no determinants or bad spin group");
+      if (c->dcount[g] != 0)
+        throw QmiError(QMI_ERR_ARG, "delayed updates pending: call qmi_mw_complete_updates first");
+      const int n = c->sys.n[g], ld = c->sys.ld;
+      const size_t es = c->det_esize();
+      std::vector<unsigned char> h((size_t)c->nw * n * ld * es);
+      HIPCHECK(hipMemcpyAsync(h.data(), c->Ainv[g].p, h.size(), hipMemcpyDeviceToHost, c->stream));
+      c->sync();
+      const int vw = c->vw();
+      for (int w = 0; w < c->nw; w++)
+        for (int i = 0; i < n; i++)
+          for (int j = 0; j < n * vw; j++)
+          {
+            const size_t src = ((size_t)w * n + i) * ld * vw + j;
+            double v;
+            if (es == 4)
+              v = reinterpret_cast<const float*>(h.data())[src];
+            else
+              v = reinterpret_cast<const double*>(h.data())[src];
+            ainv[((size_t)w * n + i) * n * vw + j] = v;
+          }
     });
   }
 

@@ -73,58 +73,6 @@ hipError_t launch_spo_rows(const SysC& s, const State& st, const void* coefs, in
   });
 }
 
-// Ainv (determinant storage type) from the FP64 inverse in the LU workspace.
-template<typename DT>
-__global__ void k_store_inverse(SysC s, State st, int g, const double* __restrict__ lu)
-{
-  const int n = s.n[g], ld = s.ld;
-  const size_t per = (size_t)n * ld;
-  const size_t w   = blockIdx.y;
-  DT* A            = det_ptr<DT>(st.Ainv[g]) + w * per;
-  const double* L  = lu + w * per;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < per; e += (size_t)gridDim.x * blockDim.x)
-    A[e] = (DT)L[e];
-}
-
-hipError_t launch_store_inverse(const SysC& s, const State& st, int g, int nw, const double* lu, hipStream_t stream)
-{
-  if (s.prec == PREC_FP64 || s.prec == PREC_T32 || s.prec == PREC_TBF16)
-    return hipSuccess; // FP64 determinant engine: the LU workspace is Ainv itself
-  const size_t per = (size_t)s.n[g] * s.ld;
-  k_store_inverse<float><<<dim3((unsigned)std::min<size_t>(64, (per + 255) / 256), nw), 256, 0, stream>>>(s, st, g, lu);
-  return hipGetLastError();
-}
-
-// log det from the LU diagonal and pivots (DiracMatrix.h:101-106): sum log(+-U_ii), sign flipped
-// where ipiv[i] != i+1; the phase counts pi per negative factor.
-__global__ void k_logdet(SysC s, State st, int g, int nw, const int* __restrict__ ipiv, const double* __restrict__ lu)
-{
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= nw)
-    return;
-  const int n = s.n[g], ld = s.ld;
-  const double* A = lu + (size_t)w * n * ld;
-  const int* p    = ipiv + (size_t)w * n;
-  double re = 0, im = 0;
-  for (int i = 0; i < n; i++)
-  {
-    const double dg = A[(size_t)i * ld + i];
-    const double v  = (p[i] == i + 1) ? dg : -dg;
-    re += log(fabs(v));
-    if (v < 0)
-      im += M_PI;
-  }
-  st.logdet[(size_t)w * 4 + g * 2 + 0] = re;
-  st.logdet[(size_t)w * 4 + g * 2 + 1] = im;
-}
-
-hipError_t launch_logdet(const SysC& s, const State& st, int g, int nw, const int* ipiv, const double* lu,
-                         hipStream_t stream)
-{
-  k_logdet<<<(nw + 63) / 64, 64, 0, stream>>>(s, st, g, nw, ipiv, lu);
-  return hipGetLastError();
-}
-
 // J1 and J2 accumulators from scratch (J1OrbitalSoA.h:240-249; TwoBodyJastrow.cpp:667-713, summed
 // over the full row instead of the lower triangle).  One thread per electron.
 __global__ __launch_bounds__(WG) void k_jastrow_recompute(SysC s, State st)

@@ -60,14 +60,28 @@ hipError_t launch_make_move(const SysC& s, const State& st, int iat, int nw, con
 
 // initialisation / recompute / GL
 hipError_t launch_wrap_coefs(const SysC& s, const double* unique, int norb, void* table, hipStream_t stream);
-// orbital rows of determinant g: values transposed into the FP64 LU workspace lu ([nw][n][ld]; Ainv
-// itself in FP64 contexts), gradients/Laplacians into dpsi
+// orbital rows of determinant g: values transposed into the FP64 LU workspace lu ([nw][n][ld]),
+// gradients/Laplacians into dpsi
 hipError_t launch_spo_rows(const SysC& s, const State& st, const void* coefs, int g, int nw, double* lu,
                            hipStream_t stream);
-hipError_t launch_logdet(const SysC& s, const State& st, int g, int nw, const int* ipiv, const double* lu,
-                         hipStream_t stream);
-// FP64 inverse in lu -> Ainv in the determinant storage type (mixed precisions)
-hipError_t launch_store_inverse(const SysC& s, const State& st, int g, int nw, const double* lu, hipStream_t stream);
+// Batched LU with partial pivoting of nmat matrices A(i, c) = a[m][c * ld + i] (FP64 real, or complex
+// (re, im) pairs): factors in place, ipiv[m][n] (1-based), perm[m][k] = row of the unit of P e_k,
+// info[m] = first zero pivot (1-based) or 0
+hipError_t launch_lu_factor(bool complex_values, int n, int ld, int nmat, double* a, int* piv, int* perm, int* info,
+                            hipStream_t stream);
+// log det of the factored matrices: out[m * ostride + {0, 1}] = {log|det|, phase}
+hipError_t launch_lu_logdet(bool complex_values, int n, int ld, int nmat, const double* a, const int* piv, double* out,
+                            int ostride, hipStream_t stream);
+// inverse from the factors, written transposed (out[m][k * ld + i] = (A^-1)(i, k)) in the storage type
+// out_kind; work: FP64 [nmat][n][ld] (complex: pairs)
+enum LuOut
+{
+  LU_OUT_F64 = 0,
+  LU_OUT_F32,
+  LU_OUT_C128
+};
+hipError_t launch_lu_inverse(int out_kind, int n, int ld, int nmat, const double* lu, const int* perm, double* work,
+                             void* out, hipStream_t stream);
 hipError_t launch_jastrow_recompute(const SysC& s, const State& st, int nw, hipStream_t stream);
 // per-walker outputs: out3[nw][3] = {ekin, logpsi, phase}; G[nw][N][3], L[nw][N] (nullable)
 hipError_t launch_eval_gl(const SysC& s, const State& st, int nw, double* out3, double* G, double* L,
@@ -94,8 +108,6 @@ hipError_t launch_move_c(int which, const SysC& s, const State& st, const void* 
                          hipStream_t stream);
 hipError_t launch_spo_rows_c(const SysC& s, const State& st, const void* coefs, int g, int nw, double* lu,
                              hipStream_t stream);
-hipError_t launch_logdet_c(const SysC& s, const State& st, int g, int nw, const int* ipiv, const double* lu,
-                           hipStream_t stream);
 hipError_t launch_eval_gl_c(const SysC& s, const State& st, int nw, double* out3, double* G, double* L,
                             hipStream_t stream);
 hipError_t launch_flush_c(const SysC& s, const State& st, int g, int d, int nw, double* scratchT, hipStream_t stream);

@@ -292,6 +292,27 @@ class Context:
         self._check(self.L.qmi_mw_evaluate_ratios(self.h, int(iat), nvp, _p(pos), _p(out)))
         return self._out(out)
 
+    def mw_invertTranspose(self, a):
+        """DiracMatrixInverter::mw_invert_transpose: a [nmat][n][n] (real or complex) -> ((a^-1)^T,
+        log det per matrix as log|det| + i phase)."""
+        a = np.asarray(a)
+        cplx = np.iscomplexobj(a)
+        nmat, n = a.shape[0], a.shape[1]
+        src = np.ascontiguousarray(np.stack([a.real, a.imag], axis=-1) if cplx else a, dtype=np.float64)
+        out = np.empty_like(src)
+        lv = np.empty((nmat, 2))
+        self._check(self.L.qmi_mw_invert_transpose(self.h, int(cplx), n, nmat, _p(src), _p(out), _p(lv)))
+        inv = out[..., 0] + 1j * out[..., 1] if cplx else out
+        return inv, lv[:, 0] + 1j * lv[:, 1]
+
+    def mw_getInverse(self, g):
+        """DelayedUpdateBatched::mw_transferAinv_D2H: inverse Slater matrix of spin group g of every
+        walker, [nw][n][n] (row i = inverse row of electron i of the group)."""
+        n = self.n_up if g == 0 else self.N - self.n_up
+        out = self._vals(self.nw, n, n)
+        self._check(self.L.qmi_mw_get_inverse(self.h, int(g), _p(out)))
+        return self._out(out)
+
     def coulomb_setup(self, ion_charge, dim_cutoff=15.0):
         self._check(self.L.qmi_coulomb_setup(self.h, C.c_double(ion_charge), C.c_double(dim_cutoff)))
 

@@ -222,6 +222,37 @@ def test_ddb_4x4_sequences(kats, k_delay):
 
 
 # ---------------------------------------------------------------------------------------- distance
+# ---------------------------------------------------------------------------------------- LU inverse
+def _wrap_phase(x):
+    return (x + np.pi) % (2 * np.pi) - np.pi
+
+
+@pytest.mark.parametrize("name", ["dirac_matrix_real", "dirac_matrix_testmatrix1"])
+def test_dirac_matrix_real(kats, name):
+    """DiracMatrix::invert_transpose known answers: inverse and log det of 3x3 and 4x4 matrices that
+    need row interchanges (the 4x4 determinant is negative twice over: phase 2 pi)."""
+    k = kats[name]
+    a = np.array(k["a"], dtype=float)
+    inv, logdet, info = orc.invert_transpose(np.ascontiguousarray(a.T))
+    assert info == 0
+    assert close(inv, k["inv"], rel=1e-9, abs_=1e-10)
+    lv = np.atleast_1d(k["log_value"])
+    assert abs(logdet.real - lv[0]) < 1e-5 * abs(lv[0])
+    if lv.size > 1:
+        assert abs(_wrap_phase(logdet.imag - lv[1])) < 1e-5
+
+
+def test_dirac_matrix_complex(kats):
+    k = kats["dirac_matrix_complex"]
+    a = np.array(k["a_re"]) + 1j * np.array(k["a_im"])
+    inv, logdet, info = orc.invert_transpose_c(a)
+    assert info == 0
+    ref = np.array(k["inv_re"]) + 1j * np.array(k["inv_im"])
+    assert np.max(np.abs(inv - ref)) < 1e-14
+    assert abs(logdet.real - k["log_value"][0]) < 1e-13
+    assert abs(_wrap_phase(logdet.imag - k["log_value"][1])) < 1e-13
+
+
 def test_distance_pbc(kats):
     k = kats["distance_pbc"]
     e = np.array(k["elec"])
