@@ -55,6 +55,8 @@ def lib():
         for name in ("orc_det_recompute", "orc_det_eval_grad", "orc_det_ratio_grad", "orc_det_accept_reject",
                      "orc_det_complete", "orc_det_get", "orc_vmc_evaluate_log", "orc_wf_ratio_grad",
                      "orc_wf_accept_reject", "orc_wf_eval_grad", "orc_wf_evaluate_gl", "orc_vmc_run", "orc_vmc_replay",
+                     "orc_vmc_replay_exact", "orc_vmc_fake_rng", "orc_vmc_test_orbital", "orc_vmc_sweep",
+                     "orc_get_drift", "orc_tau_params", "orc_rng_gauss_const",
                      "orc_vmc_get", "orc_vmc_is_complex", "orc_spo_vgl_c", "orc_invert_transpose_c",
                      "orc_ewald_params", "orc_vmc_potential", "orc_vmc_set_rng"):
             getattr(L, name).restype = C.c_int
@@ -84,6 +86,28 @@ def rng_uniform(seed, n):
 def rng_gauss(seed, n):
     out = np.empty(n)
     lib().orc_rng_gauss(C.c_uint32(seed), C.c_int(n), ptr(out))
+    return out
+
+
+def rng_gauss_const(value, n):
+    """Box-Muller deviates (assignGaussRand) of a generator returning `value` on every call; one
+    extra slot after the n outputs stays zero (the reference checks that nothing is written there)."""
+    out = np.zeros(n + 1)
+    lib().orc_rng_gauss_const(C.c_double(value), C.c_int(n), ptr(out))
+    return out
+
+
+def get_drift(tau, grad):
+    g = np.ascontiguousarray(grad, dtype=np.float64)
+    out = np.empty(3)
+    lib().orc_get_drift(C.c_double(tau), ptr(g), ptr(out))
+    return out
+
+
+def tau_params(tau, inv_mass):
+    """(tauovermass, oneover2tau, sqrttau) of TauParams<POS>."""
+    out = np.empty(3)
+    lib().orc_tau_params(C.c_double(tau), C.c_double(inv_mass), ptr(out))
     return out
 
 
@@ -301,6 +325,24 @@ class VMC:
     def _out(self, a):
         return _cv(a) if self.cplx else a
 
+    def fake_rng(self, value=0.5):
+        """Every crowd's generator returns `value` on every call (the reference's FakeRandom)."""
+        lib().orc_vmc_fake_rng(self.h, C.c_double(value))
+
+    def test_orbital(self, mode, fake_ratio=1.0):
+        """Test-only wave-function component: 'constant' (ConstantOrbital, ratio fake_ratio) or
+        'linear' (LinearOrbital, gradient (1, 2, 3))."""
+        m = {"none": 0, "constant": 1, "linear": 2}[mode]
+        if lib().orc_vmc_test_orbital(self.h, m, C.c_double(fake_ratio)) != 0:
+            raise ValueError(mode)
+
+    def sweep(self, tau, dmc=False):
+        """One sweep with the VMC acceptance, or the DMC one (node rejection before the draw)."""
+        E = np.empty(self.nw)
+        if lib().orc_vmc_sweep(self.h, C.c_double(tau), int(bool(dmc)), ptr(E)) != 0:
+            raise RuntimeError("oracle sweep failed")
+        return E
+
     def set_rng_states(self, states):
         """Replace the crowds' streams by std::mt19937 states [crowds][625] (StdRandom::load)."""
         st = np.ascontiguousarray(states, dtype=np.uint32)
@@ -393,18 +435,20 @@ class VMC:
             raise RuntimeError("oracle VMC run failed")
         return E, tr
 
-    def replay(self, forced, sub_steps=1, tau=0.3, recompute_period=0, nthreads=0):
+    def replay(self, forced, sub_steps=1, tau=0.3, recompute_period=0, nthreads=0, exact=False):
         """Follow the trajectory of `forced` ([steps][sub][N][nw][8] trace of another implementation);
-        returns own energies and own trace."""
+        returns own energies and own trace, and with exact=True also the trace of the long double
+        determinant shadow (field 0: ratio, 5-7: drift gradient)."""
         forced = np.ascontiguousarray(forced, dtype=np.float64)
         steps = forced.shape[0]
         E = np.empty((steps, self.nw))
         tr = np.empty_like(forced)
-        st = lib().orc_vmc_replay(self.h, steps, sub_steps, C.c_double(tau), recompute_period, nthreads,
-                                  ptr(E), ptr(tr), ptr(forced))
+        ex = np.zeros_like(forced) if exact else None
+        st = lib().orc_vmc_replay_exact(self.h, steps, sub_steps, C.c_double(tau), recompute_period, nthreads,
+                                        ptr(E), ptr(tr), ptr(forced), ptr(ex))
         if st != 0:
             raise RuntimeError("oracle VMC replay failed")
-        return E, tr
+        return (E, tr, ex) if exact else (E, tr)
 
     def potential(self, ion_charge):
         """[nw][3]: electron-electron, electron-ion, ion-ion Coulomb energies (Ewald)."""

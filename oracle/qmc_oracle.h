@@ -94,6 +94,20 @@ int orc_vmc_run(void* h, int steps, int sub_steps, double tau, int recompute_per
  * `trace` (own proposed position in fields 2..4, own decision in field 1). */
 int orc_vmc_replay(void* h, int steps, int sub_steps, double tau, int recompute_period, int nthreads,
                    double* energies, double* trace, const double* forced);
+/* replay with the long double determinant shadow: exact[step][sub][iat][iw][8], field 0 the ratio,
+ * 5-7 the drift gradient of every traced move */
+int orc_vmc_replay_exact(void* h, int steps, int sub_steps, double tau, int recompute_period, int nthreads,
+                         double* energies, double* trace, const double* forced, double* exact);
+/* Driver known answers of the reference's tests: a generator returning `value` on every call
+ * (FakeRandom), test-only wave-function components (mode 1: ConstantOrbital with ratio fake_ratio,
+ * mode 2: LinearOrbital), one sweep with the VMC or DMC acceptance, the drift, the tau parameters
+ * and the Box-Muller deviates of a constant generator. */
+int orc_vmc_fake_rng(void* h, double value);
+int orc_vmc_test_orbital(void* h, int mode, double fake_ratio);
+int orc_vmc_sweep(void* h, double tau, int dmc, double* energies);
+int orc_get_drift(double tau, const double* g, double* drift);
+int orc_tau_params(double tau, double inv_mass, double* out);
+int orc_rng_gauss_const(double value, int n, double* out);
 /* R[nw][N][3], logpsi[nw], G, L, stats[2] = {accepted, rejected} (nullable each) */
 int orc_vmc_get(void* h, double* R, double* logpsi, double* G, double* L, double* stats);
 

@@ -132,12 +132,23 @@ def test_c2c_vmc_delayed(k, wpc):
 def test_c2c_vmc_large(n_up, n_dn, k):
     """Twisted complex orbitals at production determinant sizes (a quarter-sized cell of the
     1024-electron density), including determinants beyond 256 orbitals (two orbitals per thread).
-    Ratios relative to max(|ratio|, 0.1) at 1e-9, as for the real determinants of this size."""
+    Ratio moduli (relative to max(|ratio|, 1e-2)) and gradients are within 1e-10 of the oracle and
+    of its long double determinant shadow (the exact reference), decisions identical, proposals
+    and energies within 1e-10."""
     s = make_complex_system(L=14.0, n_ion=16, n_up=n_up, n_dn=n_dn, norb=max(n_up, n_dn), M=12,
                             delay_rank=k, seed=41)
     R = make_walkers(s, 2, 42)
-    res = vmc_parity_c(s, R, 1, steps=1, sub_steps=1, tau=0.3, recompute_period=0, seed=80)
-    check_vmc(*res, ratio_floor=0.1, ratio_tol=1e-9)
+    c = ctx(s, R, 1)
+    c.mw_evaluateLog()
+    E, tr = c.vmc_run(1, 1, 0.3, 0, 80, trace=True)
+    o = orc.VMC(s, R, 1, 80)
+    o.evaluate_log()
+    Eo, tro, ex = o.replay(tr, 1, 0.3, 0, exact=True)
+    dev, own = relerr(tr[..., 0], ex[..., 0], floor=RATIO_FLOOR), relerr(tro[..., 0], ex[..., 0], floor=RATIO_FLOOR)
+    gdev, gown = relerr(tr[..., 5:8], ex[..., 5:8], floor=1.0), relerr(tro[..., 5:8], ex[..., 5:8], floor=1.0)
+    print(f"vs exact: |ratio| device {dev:.3e}, oracle {own:.3e}; gradient device {gdev:.3e}, oracle {gown:.3e}")
+    assert dev < REL and gdev < REL
+    check_vmc(c, o, E, tr, Eo, tro, ratio_floor=RATIO_FLOOR)
 
 
 def test_c2c_evaluate_ratios_virtual_moves():

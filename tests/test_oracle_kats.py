@@ -32,21 +32,124 @@ def test_mt19937_standard_answer(kats):
     assert int(round(u[-1] * 2.0 ** 32)) == k["expected"]
 
 
-def test_box_muller_from_uniforms():
-    # RandomSeqGenerator.h:31-50: pairs (t1 cos t2, t1 sin t2) from two uniforms; odd tail cos only
-    for n in (6, 7):
-        u = orc.rng_uniform(99, 2 * ((n + 1) // 2))
-        g = orc.rng_gauss(99, n)
-        eps = np.finfo(float).eps
-        ref = []
-        for i in range(0, n - 1, 2):
-            t1 = np.sqrt(-2.0 * np.log(1.0 - (1.0 - eps) * u[i]))
-            t2 = 2.0 * np.pi * u[i + 1]
-            ref += [t1 * np.cos(t2), t1 * np.sin(t2)]
-        if n % 2:
-            t1 = np.sqrt(-2.0 * np.log(1.0 - (1.0 - eps) * u[n - 1]))
-            ref.append(t1 * np.cos(2.0 * np.pi * u[n]))
-        assert close(g, ref, rel=1e-14, abs_=1e-15)
+@pytest.mark.parametrize("case", range(5))
+def test_box_muller_fake_random(kats, case):
+    """assignGaussRand over the reference's constant generator: pairs (r cos t, r sin t), the odd
+    tail's cosine, the guard against u = 1, and nothing written past n."""
+    c = kats["gauss_fake_random"]["cases"][case]
+    g = orc.rng_gauss_const(c["value"], c["n"])  # n outputs and one slot that must stay zero
+    e = c["expected"]  # the reference's array: n outputs, then (for n < len) the untouched slot
+    # the reference checks these with Catch's default Approx (1.19e-5 relative); its listed values
+    # carry ~1e-10 relative noise around the closed form sqrt(2 ln 2) (cos pi, sin pi) at u = 0.5;
+    # sin(2 pi) of the u = 1 case is -2.4e-16 in FP64, hence the absolute floor
+    assert close(g[:len(e)], e, rel=1.2e-5, abs_=1e-14)
+    assert g[c["n"]] == 0.0
+    if c["value"] == 0.5:
+        assert abs(g[0] + np.sqrt(2 * np.log(2))) < 1e-15
+
+
+def test_box_muller_distribution():
+    """The Gaussians of the mt19937 stream are standard normal (moments and the tail fraction of
+    2e5 draws), so no term of the transform is dropped for general uniforms."""
+    g = orc.rng_gauss(2024, 200001)
+    assert abs(g.mean()) < 0.01 and abs(g.var() - 1) < 0.01
+    assert abs(np.mean(np.abs(g) > 1.959964) - 0.05) < 0.003
+    assert abs(np.mean(g ** 3)) < 0.02 and abs(np.mean(g ** 4) - 3) < 0.05
+
+
+# ---------------------------------------------------------------------------------------- driver
+def test_drift_unr(kats):
+    k = kats["drift_unr"]
+    tom = k["tau"] / k["mass"]
+    for i in range(k["nx"]):
+        gx = -k["xtot"] / 2 + i * k["xtot"] / k["nx"]
+        d = orc.get_drift(tom, [gx, 0.0, 0.0])
+        f = gx * gx * tom
+        ref = (-1 + np.sqrt(1 + 2 * f)) / f * gx * tom if f > 0 else tom * gx
+        assert abs(d[0] - ref) <= 1e-12 * max(1, abs(ref)) and d[1] == 0 and d[2] == 0
+
+
+def test_tau_params(kats):
+    k = kats["tau_params"]
+    t = orc.tau_params(k["tau"], k["invmass"])
+    assert close(t, [k["tauovermass"], k["oneover2tau"], k["sqrttau"]], rel=1e-15)
+
+
+# The reference checks driver positions with Catch's default Approx (1.19e-5 relative); the digits
+# it lists differ between its own tests by ~3e-11 (test_vmc.cpp vs test_dmc.cpp for the same move),
+# so the oracle is held to that tolerance and, beyond it, to the closed form of the move:
+# R1 = R0 + drift + sqrt(tau) xi, xi the Box-Muller pairs of u = 0.5 filled electron by electron.
+APPROX = 1.2e-5
+
+
+def _closed_form_move(R0, tau, grad):
+    r = np.sqrt(2 * np.log(2))
+    xi = np.array([r * np.cos(np.pi), r * np.sin(np.pi)] * 3).reshape(2, 3)
+    g = np.asarray(grad, dtype=float)
+    v2 = g @ g
+    drift = g * ((-1 + np.sqrt(1 + 2 * tau * v2)) / v2 if v2 > 0 else tau)
+    return np.asarray(R0) + drift + np.sqrt(tau) * xi
+
+
+def _two_electron_open_cell(R0):
+    """The reference driver tests' setup: one species of two electrons, one ion at the origin,
+    an open cell, and no wave-function components besides the test-only one."""
+    return System(L=np.zeros(3), ions=np.zeros((1, 3)), n_up=2, n_dn=0, M=(0, 0, 0), norb=0, coefs=None,
+                  j1_params=None, j1_rcut=1.0, j1_cusp=0.0, j2_uu=None, j2_ud=None, j2_rcut=1.0,
+                  j2_cusp_uu=0.0, j2_cusp_ud=0.0, delay_rank=1), np.array([R0], dtype=float)
+
+
+def test_vmc_pbyp_constant_orbital(kats):
+    """VMC drift-diffusion step of test_vmc.cpp: ConstantOrbital (no drift), FakeRandom 0.5."""
+    k = kats["vmc_pbyp_constant_orbital"]
+    s, R = _two_electron_open_cell(k["R0"])
+    o = orc.VMC(s, R, 1, 1)
+    o.test_orbital("constant")
+    o.fake_rng(0.5)
+    o.evaluate_log()
+    o.sweep(k["tau"])
+    st = o.get()
+    assert np.allclose(st["R"][0], k["R1"], rtol=APPROX, atol=1e-14)
+    assert np.allclose(st["R"][0], _closed_form_move(k["R0"], k["tau"], [0, 0, 0]), rtol=1e-15, atol=1e-15)
+    assert (st["accepted"], st["rejected"]) == (k["accepted"], k["rejected"])
+
+
+def test_dmc_pbyp_constant_orbital(kats):
+    """DMC step of test_dmc.cpp: accepted moves, then a negative ratio (a node crossing of a real
+    wave function) rejects both moves before the draw."""
+    k = kats["dmc_pbyp_constant_orbital"]
+    s, R = _two_electron_open_cell(k["R0"])
+    o = orc.VMC(s, R, 1, 1)
+    o.test_orbital("constant")
+    o.fake_rng(0.5)
+    o.evaluate_log()
+    o.sweep(k["tau"], dmc=True)
+    st = o.get()
+    assert np.allclose(st["R"][0], k["R1"], rtol=APPROX, atol=1e-14)
+    assert np.allclose(st["R"][0], _closed_form_move(k["R0"], k["tau"], [0, 0, 0]), rtol=1e-15, atol=1e-15)
+    assert (st["accepted"], st["rejected"]) == (k["after_first"]["accepted"], k["after_first"]["rejected"])
+    o.test_orbital("constant", fake_ratio=-1.0)
+    o.sweep(k["tau"], dmc=True)
+    st = o.get()
+    assert np.allclose(st["R"][0], k["R1"], rtol=APPROX, atol=1e-14)
+    assert (st["accepted"], st["rejected"]) == (k["after_second_with_ratio_m1"]["accepted"],
+                                                k["after_second_with_ratio_m1"]["rejected"])
+
+
+def test_dmc_pbyp_linear_orbital(kats):
+    """DMC step of test_dmc.cpp with LinearOrbital: the UNR drift of the gradient (1, 2, 3), the
+    Green's function ratio with no gradient at the proposed position, FakeRandom 0.5."""
+    k = kats["dmc_pbyp_linear_orbital"]
+    s, R = _two_electron_open_cell(k["R0"])
+    o = orc.VMC(s, R, 1, 1)
+    o.test_orbital("linear")
+    o.fake_rng(0.5)
+    o.evaluate_log()
+    o.sweep(k["tau"], dmc=True)
+    st = o.get()
+    assert np.allclose(st["R"][0], k["R1"], rtol=APPROX, atol=1e-14)
+    assert np.allclose(st["R"][0], _closed_form_move(k["R0"], k["tau"], [1, 2, 3]), rtol=1e-14, atol=1e-15)
+    assert (st["accepted"], st["rejected"]) == (k["accepted"], k["rejected"])
 
 
 # ---------------------------------------------------------------------------------------- spline
