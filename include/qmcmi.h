@@ -119,6 +119,52 @@ int qmi_mw_make_move(qmi_ctx* ctx, int iat, const double* displ);
  * ratios[nw]; grads[nw][3] (total gradient of log psi at the proposed position). */
 int qmi_mw_ratio_grad(qmi_ctx* ctx, int iat, double* ratios, double* grads);
 
+/* ---- per-component form ----------------------------------------------------------------------
+ * A context built with only some of the components (qmi_system_desc without coefs: Jastrows only;
+ * without j1_params / j2 arrays: no J1 / J2) is one WaveFunctionComponent of the reference, or a
+ * subset of them; its qmi_mw_eval_grad is the component's mw_evalGrad (WaveFunctionComponent.h:
+ * 234-237, grad_now assigned), qmi_mw_evaluate_log returns the component's log value, G and L, and
+ * qmi_mw_accept_reject / qmi_mw_complete_updates are its mw_accept_rejectMove / mw_completeUpdates.
+ * Several such contexts loaded with the same walkers and driven by the same calls (make_move with
+ * the same displacements, the same acceptance masks) keep identical positions, as the components of
+ * one TrialWaveFunction share one ParticleSet.
+ *
+ * WaveFunctionComponent::mw_ratioGrad (WaveFunctionComponent.h:305-309): ratios[nw] = the ratio of
+ * this context's components, grad_new[nw][3] += their gradient at the proposed position (the
+ * accumulation of DiracDeterminantBatched.cpp:352 and TwoBodyJastrow.cpp:576). */
+int qmi_mw_component_ratio_grad(qmi_ctx* ctx, int iat, double* ratios, double* grad_new);
+
+/* SPOSet::mw_evaluateVGLandDetRatioGrads (SPOSet.h:346-352; SplineR2R.cpp:414-580) with the caller's
+ * inverse rows (invRow_ptr_list): orbital value, gradient and Laplacian of the first n orbitals at
+ * pos[nw][3] into phi_vgl[5][nw][n] (the reference's phi_vgl_v layout; nullable), the determinant
+ * ratio ratios[nw] = sum_j inv_rows[w][j] phi_j and grads[nw][3] = sum_j inv_rows[w][j] dphi_j / ratio. */
+int qmi_mw_spo_vgl_det_ratio_grads(qmi_ctx* ctx, const double* pos, int n, const double* inv_rows, double* phi_vgl,
+                                   double* ratios, double* grads);
+
+/* Stateful distance tables over the walkers of a context (DistanceTable, Particle/DistanceTable.h:
+ * 104-183), dense SoA semantics: kind 0 = electron pairs (SoaDistanceTableAA.h:45-127; row i holds
+ * electrons j < i, the rest is not maintained), kind 1 = electrons to ions (SoaDistanceTableAB.h:
+ * 41-86).  Minimum-image distances, displacements source - target.  The table is sized for the
+ * context's walkers at creation; it reads the context's current positions. */
+typedef struct qmi_dtable qmi_dtable;
+int qmi_dt_create(qmi_ctx* ctx, int kind, qmi_dtable** dt);
+int qmi_dt_destroy(qmi_dtable* dt);
+/* DistanceTable::mw_evaluate (DistanceTable.h:104-121): the full table from the current positions */
+int qmi_dt_mw_evaluate(qmi_dtable* dt);
+/* DistanceTable::mw_move (DistanceTable.h:153-161; SoaDistanceTableAA.h:57-75, SoaDistanceTableAB.h:
+ * 60-71): proposal rows from rnew[nw][3] for particle iat; prepare_old = 1 also computes the rows of
+ * its current position (AA: old rows with the largest double at iat; AB: written into row iat). */
+int qmi_dt_mw_move(qmi_dtable* dt, int iat, const double* rnew, int prepare_old);
+/* partial = 0: DistanceTable::update (SoaDistanceTableAA.h:80-98, row [0, iat) and the column below
+ * it from the proposal rows) for the walkers with accepted[w] != 0; partial = 1:
+ * DistanceTable::mw_updatePartial (DistanceTable.h:166-183, SoaDistanceTableAA.h:100-120): row
+ * [0, iat) from the proposal where accepted[w], else from the old rows (AB: from the proposal where
+ * accepted, else unchanged) */
+int qmi_dt_mw_update(qmi_dtable* dt, int iat, const int* accepted, int partial);
+/* copies (nullable): r[nw][N][ns], dr[nw][N][3][ns], temp_r[nw][ns], temp_dr[nw][3][ns], old_r[nw][ns],
+ * old_dr[nw][3][ns] (AA only); ns = N (AA) or the ion count (AB) */
+int qmi_dt_get(qmi_dtable* dt, double* r, double* dr, double* temp_r, double* temp_dr, double* old_r, double* old_dr);
+
 /* TrialWaveFunction::mw_accept_rejectMove (TrialWaveFunction.cpp:790-821) ->
  * DelayedUpdateBatched::mw_accept_rejectRow (DelayedUpdateBatched.h:542-670) and
  * ParticleSet::mw_accept_rejectMove forward mode (ParticleSet.cpp:717-758).

@@ -179,6 +179,76 @@ def min_image(L, src, pos):
     return r, dr
 
 
+def round_table(coefs, prec):
+    """Spline coefficients as the device stores them in precision mode prec ("fp64"; "fp32" / "t32":
+    FP32; "bf16" / "tbf16": BF16 by round to nearest even of the FP32 value), returned in FP64, so
+    that the FP64 oracle on this table sees the device's table."""
+    if prec == "fp64":
+        return np.asarray(coefs, dtype=np.float64)
+    f = np.asarray(coefs, dtype=np.float64).astype(np.float32)
+    if prec in ("fp32", "t32"):
+        return f.astype(np.float64)
+    u = f.view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    return u.view(np.float32).astype(np.float64)
+
+
+class DistanceTable:
+    """Restatement of the dense SoA distance tables of one walker: SoaDistanceTableAA
+    (src/Particle/SoaDistanceTableAA.h:45-120; row i holds electrons j < i) and SoaDistanceTableAB
+    (src/Particle/SoaDistanceTableAB.h:41-80; row i holds every source), over min_image.  Rows of
+    the table that the reference does not maintain are NaN here, so a test sees only the maintained
+    part.  Displacements are source - target, [row][j][3]."""
+
+    def __init__(self, L, kind, N, sources=None):
+        self.L, self.kind, self.N = np.asarray(L, dtype=np.float64), kind, N
+        self.sources = None if sources is None else np.asarray(sources, dtype=np.float64)
+        ns = N if kind == "AA" else self.sources.shape[0]
+        self.ns = ns
+        self.r = np.full((N, ns), np.nan)
+        self.dr = np.full((N, ns, 3), np.nan)
+        self.temp_r = self.temp_dr = self.old_r = self.old_dr = None
+
+    def _src(self, R):
+        return R if self.kind == "AA" else self.sources
+
+    def evaluate(self, R):
+        """SoaDistanceTableAA.h:45-55 (rows from 1, columns [0, i)) / SoaDistanceTableAB.h:41-58."""
+        for i in range(self.N):
+            r, dr = min_image(self.L, self._src(R), R[i])
+            last = i if self.kind == "AA" else self.ns
+            self.r[i, :last], self.dr[i, :last] = r[:last], dr[:last]
+
+    def move(self, R, rnew, iat, prepare_old=True):
+        """SoaDistanceTableAA.h:57-75 / SoaDistanceTableAB.h:60-71."""
+        self.temp_r, self.temp_dr = min_image(self.L, self._src(R), rnew)
+        if prepare_old:
+            r, dr = min_image(self.L, self._src(R), R[iat])
+            if self.kind == "AA":
+                r[iat] = np.finfo(np.float64).max
+                self.old_r, self.old_dr = r, dr
+            else:
+                self.r[iat], self.dr[iat] = r, dr
+
+    def update(self, iat):
+        """SoaDistanceTableAA.h:80-98 (row [0, iat) and the column below) / SoaDistanceTableAB.h:74-80."""
+        n = iat if self.kind == "AA" else self.ns
+        self.r[iat, :n], self.dr[iat, :n] = self.temp_r[:n], self.temp_dr[:n]
+        if self.kind == "AA":
+            for i in range(iat + 1, self.N):
+                self.r[i, iat] = self.temp_r[i]
+                self.dr[i, iat] = -self.temp_dr[i]
+
+    def update_partial(self, iat, from_temp):
+        """SoaDistanceTableAA.h:100-120; DistanceTable.h:166-172 (AB: update when from_temp)."""
+        if self.kind == "AB":
+            if from_temp:
+                self.update(iat)
+            return
+        src_r, src_dr = (self.temp_r, self.temp_dr) if from_temp else (self.old_r, self.old_dr)
+        self.r[iat, :iat], self.dr[iat, :iat] = src_r[:iat], src_dr[:iat]
+
+
 def invert_transpose(a):
     a = np.ascontiguousarray(a, dtype=np.float64)
     n = a.shape[0]

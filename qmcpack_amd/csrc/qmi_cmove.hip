@@ -394,12 +394,13 @@ __device__ inline void combine_ratio_c(const SysC& s, int iat, const double* __r
 
 // Phase D core (thread 0): returns prob_eff; flag = |ratio|^2 >= eps; |ratio| in ratio_abs.
 __device__ inline double mh_prob_cv(const SysC& s, double vat, double uat, const MoveArgs& a, CMoveLDS& sh,
-                                    double& ratio_abs, bool& flag)
+                                    double& ratio_abs, bool& flag, double& gn2)
 {
   const double tauovermass = a.tau;
   const double oneover2tau = 0.5 / tauovermass;
   double2 ratio, gz[3];
   combine_ratio_cv(s, vat, uat, sh.bc, ratio, gz);
+  gn2 = znorm(gz[0]) + znorm(gz[1]) + znorm(gz[2]);
   const double gnew[3] = {gz[0].x, gz[1].x, gz[2].x};
   double drev[3];
   unr_drift(tauovermass, gnew, drev);
@@ -414,12 +415,13 @@ __device__ inline double mh_prob_cv(const SysC& s, double vat, double uat, const
 }
 
 __device__ inline double mh_prob_c(const SysC& s, int iat, const MoveArgs& a, const double* __restrict__ jas,
-                                   CMoveLDS& sh, double& ratio_abs, bool& flag)
+                                   CMoveLDS& sh, double& ratio_abs, bool& flag, double& gn2)
 {
   const double tauovermass = a.tau;
   const double oneover2tau = 0.5 / tauovermass;
   double2 ratio, gz[3];
   combine_ratio_c(s, iat, jas, sh.bc, ratio, gz);
+  gn2 = znorm(gz[0]) + znorm(gz[1]) + znorm(gz[2]);
   const double gnew[3] = {gz[0].x, gz[1].x, gz[2].x};
   double drev[3];
   unr_drift(tauovermass, gnew, drev);
@@ -625,7 +627,9 @@ __global__ __launch_bounds__(WG) void k_move_one_c(SysC s, State st, const doubl
   {
     double ratio_abs;
     bool flag;
-    const double pe = mh_prob_cv(s, sh.bc[BC_PF + PF_VAT], sh.bc[BC_PF + PF_UAT], a, sh, ratio_abs, flag);
+    double gn2;
+    const double pe = mh_prob_cv(s, sh.bc[BC_PF + PF_VAT], sh.bc[BC_PF + PF_UAT], a, sh, ratio_abs, flag, gn2);
+    nan_guard(a, w, iat, ratio_abs * ratio_abs, gn2, gnow_norm2(sh.bc));
     if (a.dmc) // complex wave function: no node check (SFNBranch.h:161-169), prob_eff against eps
       flag = pe >= DBL_EPS;
     bool acc = false;
@@ -688,7 +692,9 @@ __global__ __launch_bounds__(WG) void k_propose_c(SysC s, State st, const double
   {
     double ratio_abs;
     bool flag;
-    const double pe = mh_prob_c(s, iat, a, v.jas, sh, ratio_abs, flag);
+    double gn2;
+    const double pe = mh_prob_c(s, iat, a, v.jas, sh, ratio_abs, flag, gn2);
+    nan_guard(a, w, iat, ratio_abs * ratio_abs, gn2, gnow_norm2(sh.bc));
     mv[MV_RATIO]    = ratio_abs;
     mv[MV_PROB]     = pe;
     mv[MV_FLAG]     = flag ? 1.0 : 0.0;
@@ -1449,6 +1455,58 @@ hipError_t launch_api_det_ratios_c(const SysC& s, const void* coefs, const doubl
   k_api_det_ratios_c<<<dim3(nvp, nw), WG, 0, stream>>>(s, static_cast<const double*>(coefs), kpts, nvp, n, pos,
                                                         reinterpret_cast<const double2*>(inv),
                                                         reinterpret_cast<double2*>(out));
+  return hipGetLastError();
+}
+
+// SPOSet::mw_evaluateVGLandDetRatioGrads with complex orbitals (SplineC2C.cpp:140-279 for the orbital
+// VGL): phi[5][nw][n] complex, rg[w][4] complex = ratio, gradient / ratio
+__global__ __launch_bounds__(WG) void k_api_vgl_ratio_grads_c(SysC s, const double* __restrict__ coefs,
+                                                              const double* __restrict__ kpts, int nw, int n,
+                                                              const double* __restrict__ pos,
+                                                              const double2* __restrict__ inv,
+                                                              double2* __restrict__ phi, double2* __restrict__ rg)
+{
+  __shared__ SplineLDS spl;
+  __shared__ double red[8 * WG];
+  __shared__ double bc[8];
+  const int w       = blockIdx.x;
+  const double p[3] = {pos[w * 3], pos[w * 3 + 1], pos[w * 3 + 2]};
+  if (threadIdx.x == 0)
+  {
+    SplineLoc L;
+    spline_locate(s, p, L);
+    spline_stage(L, spl);
+  }
+  __syncthreads();
+  double2 acc[4] = {zmk(0, 0), zmk(0, 0), zmk(0, 0), zmk(0, 0)};
+  for (int o = threadIdx.x; o < n; o += WG)
+  {
+    double2 v[5];
+    spline_vgl_orbital_c(s, coefs, spl, o, kpts, p, v);
+    if (phi)
+      for (int q = 0; q < 5; q++)
+        phi[((size_t)q * nw + w) * n + o] = v[q];
+    const double2 a = inv[(size_t)w * n + o];
+    for (int q = 0; q < 4; q++)
+      acc[q] = zfma(v[q], a, acc[q]);
+  }
+  block_sum_z<4>(acc, red, bc);
+  if (threadIdx.x == 0)
+  {
+    const double2 r = bcz(bc, 0);
+    rg[w * 4]       = r;
+    for (int q = 1; q < 4; q++)
+      rg[w * 4 + q] = zdiv(bcz(bc, 2 * q), r);
+  }
+}
+
+hipError_t launch_api_vgl_ratio_grads_c(const SysC& s, const void* coefs, const double* kpts, int nw, int n,
+                                        const double* pos, const double* inv, double* phi, double* rg,
+                                        hipStream_t stream)
+{
+  k_api_vgl_ratio_grads_c<<<nw, WG, 0, stream>>>(s, static_cast<const double*>(coefs), kpts, nw, n, pos,
+                                                  reinterpret_cast<const double2*>(inv),
+                                                  reinterpret_cast<double2*>(phi), reinterpret_cast<double2*>(rg));
   return hipGetLastError();
 }
 

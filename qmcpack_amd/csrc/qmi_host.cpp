@@ -15,6 +15,7 @@
 #include <functional>
 #include <limits>
 #include <map>
+#include <memory>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -171,6 +172,8 @@ struct qmi_ctx
   // recompute: FP64 LU factors and the solve's work array ([nw][n][ld] values), unit rows of P e_k
   DevBuf<double> lu, luw;
   DevBuf<int> dlist[2], ipiv, lperm, info, ubase, acc;
+  DevBuf<int> nanflag; // [cap] device NaN guard: 1 + the first electron with a NaN ratio or gradient
+  PinnedBuf<int> h_nan;
   DevBuf<double> gauss, unif, trace;
   State st{};
   int dcount[2] = {0, 0};
@@ -242,8 +245,12 @@ struct qmi_ctx
     timing.resolve();
   }
 
+  // an executable graph is destroyed only after every launch of it has completed: its launches run on
+  // this context's stream (alone, or as a branch forked from a batch's origin stream, which joins it)
   void invalidate_graph()
   {
+    if (sweep_exec || split_exec[0] || split_exec[1])
+      (void)hipStreamSynchronize(stream);
     if (sweep_exec)
       (void)hipGraphExecDestroy(sweep_exec);
     sweep_exec = nullptr;
@@ -255,6 +262,11 @@ struct qmi_ctx
     }
     graph_gen++;
   }
+
+  // distance tables over this context's walkers: detached (their context pointer cleared) when the
+  // context is destroyed first, so that destroying them later touches no freed context
+  std::vector<qmi_dtable*> dtables;
+  void detach_dtables();
 
   ~qmi_ctx()
   {
@@ -337,6 +349,8 @@ struct qmi_ctx
     std::copy(symGG, symGG + 6, s.symGG);
     s.prec    = precision;
     s.has_det = d->norb > 0 && d->coefs != nullptr;
+    if (!s.has_det)
+      s.k = 1; // the delay rank is the determinants' (no delayed-update buffers without them)
     const bool zc = s.has_det && d->complex_orbitals;
     if (zc && precision != QMI_PREC_FP64)
       throw QmiError(QMI_ERR_UNSUPPORTED, "complex orbitals need the FP64 precision mode");
@@ -457,6 +471,9 @@ struct qmi_ctx
     HIPCHECK(hipMemset(logdet.p, 0, logdet.n * sizeof(double)));
     stats.alloc((size_t)cap * 2);
     HIPCHECK(hipMemset(stats.p, 0, stats.n * sizeof(double)));
+    nanflag.alloc(cap);
+    HIPCHECK(hipMemset(nanflag.p, 0, nanflag.n * sizeof(int)));
+    h_nan.alloc(cap);
     mv.alloc((size_t)cap * MV_SIZE);
     HIPCHECK(hipMemset(mv.p, 0, mv.n * sizeof(double)));
     out.alloc((size_t)cap * 8 + 16);
@@ -529,7 +546,7 @@ struct qmi_ctx
       throw QmiError(QMI_ERR_ARG, "delay rank must be in [1, 64] and not exceed the determinant size");
     if (dcount[0] || dcount[1])
       throw QmiError(QMI_ERR_ARG, "delay rank change with pending delayed updates");
-    if (k == sys.k)
+    if (k == sys.k || !sys.has_det)
       return;
     sys.k = k;
     if (sys.has_det)
@@ -619,10 +636,32 @@ struct qmi_ctx
                     : launch_eval_gl(sys, st, nw, gl3.p, nullptr, nullptr, stream));
     }
     HIPCHECK(hipMemcpyAsync(h_gl3.p, gl3.p, (size_t)nw * 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    nan_enqueue();
+  }
+
+  // the device NaN guard of the moves since the last check, read once per step (NaNguard.cpp:18-31,
+  // TrialWaveFunction.cpp:592, 736-739): copied to pinned memory on the stream, checked after a
+  // synchronisation; a flagged walker clears the flags and raises QMI_ERR_NUMERIC
+  void nan_enqueue()
+  {
+    HIPCHECK(hipMemcpyAsync(h_nan.p, nanflag.p, (size_t)nw * sizeof(int), hipMemcpyDeviceToHost, stream));
+  }
+  void nan_check()
+  {
+    for (int w = 0; w < nw; w++)
+      if (h_nan.p[w] != 0)
+      {
+        const int iat = h_nan.p[w] - 1;
+        HIPCHECK(hipMemsetAsync(nanflag.p, 0, nanflag.n * sizeof(int), stream));
+        HIPCHECK(hipStreamSynchronize(stream));
+        throw QmiError(QMI_ERR_NUMERIC, "NaN ratio or gradient of walker " + std::to_string(w) + " at particle " +
+                                            std::to_string(iat) + " (TWF::mw_calcRatioGrad / mw_evalGrad)");
+      }
   }
 
   void eval_gl_finish(double* ekin)
   {
+    nan_check();
     for (int w = 0; w < nw; w++)
     {
       const double* h = h_gl3.p + (size_t)w * 3;
@@ -659,7 +698,9 @@ struct qmi_ctx
       HIPCHECK(hipMemcpyAsync(G, dG, nG * sizeof(double), hipMemcpyDeviceToHost, stream));
     if (L)
       HIPCHECK(hipMemcpyAsync(L, dL, nL * sizeof(double), hipMemcpyDeviceToHost, stream));
+    nan_enqueue();
     sync();
+    nan_check();
     for (int w = 0; w < nw; w++)
     {
       if (ekin)
@@ -698,6 +739,7 @@ struct qmi_ctx
     a.d0   = sys.k > 1 ? dcount[a.g] : 0;
     a.nw   = nw;
     a.wpc  = wpc;
+    a.nanflag = nanflag.p;
     return a;
   }
 
@@ -1493,6 +1535,16 @@ struct qmi_ctx
   // Kill walkers of multiplicity 0 and copy those of multiplicity m > 1 (m - 1 copies): copies go
   // to the dead slots in ascending order, then past the end; remaining holes are filled from the
   // tail.  Two launches: copies (sources live, destinations dead or new), then tail moves.
+  // a VMC section after DMC (as consecutive driver sections of one input) samples |psi|^2 again: the
+  // DMC acceptance (node rejection, rr sums) is switched off and graphs captured with it are dropped
+  void end_dmc()
+  {
+    if (!dmc.on)
+      return;
+    dmc.on = false;
+    invalidate_graph();
+  }
+
   void dmc_branch_slots(const std::vector<int>& mult)
   {
     std::vector<int> holes;
@@ -1562,6 +1614,8 @@ struct qmi_ctx
       // pinned pairs are reused by the next step: wait for the upload
       sync();
     }
+    if (nw_new != nw)
+      invalidate_graph(); // captured sweeps are bound to the walker count
     nw     = nw_new;
     ncrowd = nw;
     while ((int)rng.size() < nw)
@@ -2116,6 +2170,7 @@ struct qmi_ctx
     a.trace = with_trace ? trace.p : nullptr;
     a.dmc   = dmc.on ? 1 : 0;
     a.rr    = dmc.on ? dmc_rr.p : nullptr;
+    a.nanflag = nanflag.p;
     for (int g = 0; g < 2; g++)
     {
       int iat        = std::max(s.first[g], e0);
@@ -2263,6 +2318,7 @@ struct qmi_ctx
     const int N   = s.N;
     if (steps < 0 || sub_steps < 1 || !(tau > 0))
       throw QmiError(QMI_ERR_ARG, "invalid steps / sub_steps / tau");
+    end_dmc();
     seed_rng(seed);
     substep_buffers(htrace != nullptr);
     if (learned && cplx)
@@ -2364,6 +2420,45 @@ struct qmi_ctx
 };
 
 //--------------------------------------------------------------------------------------------------
+// A stateful distance table over the walkers of one context (DistanceTable, Particle/DistanceTable.h):
+// the electron positions are the context's; the table, the proposal rows and (AA) the rows of the
+// current position live in device memory sized for the context's walkers at creation.
+struct qmi_dtable
+{
+  qmi_ctx* c = nullptr; // null once the context is destroyed
+  int device = 0;
+  int kind = DT_AA, nw = 0, N = 0, ns = 0;
+  DevBuf<double> r, dr, temp_r, temp_dr, old_r, old_dr, rnew;
+  DevBuf<int> acc;
+
+  DTableArrays arrays()
+  {
+    DTableArrays t;
+    t.kind    = kind;
+    t.ns      = ns;
+    t.r       = r.p;
+    t.dr      = dr.p;
+    t.temp_r  = temp_r.p;
+    t.temp_dr = temp_dr.p;
+    t.old_r   = old_r.p;
+    t.old_dr  = old_dr.p;
+    return t;
+  }
+  void check() const
+  {
+    if (c->nw != nw || c->sys.N != N || c->device != device)
+      throw QmiError(QMI_ERR_ARG, "the context's walkers changed since the distance table was created");
+  }
+};
+
+void qmi_ctx::detach_dtables()
+{
+  for (qmi_dtable* t : dtables)
+    t->c = nullptr;
+  dtables.clear();
+}
+
+//--------------------------------------------------------------------------------------------------
 // Several contexts (walker populations of different systems: electron counts, cells, tables, real
 // or complex orbitals) advanced together: one sub-step of every context is captured into one HIP
 // graph, each context on its own stream (concurrent branches) or chained one after the other.
@@ -2386,6 +2481,8 @@ struct qmi_batch
 
   void invalidate()
   {
+    if (exec || split[0] || split[1])
+      (void)hipStreamSynchronize(origin);
     if (exec)
       (void)hipGraphExecDestroy(exec);
     exec = nullptr;
@@ -2536,6 +2633,7 @@ struct qmi_batch
     for (size_t i = 0; i < ctxs.size(); i++)
     {
       ctxs[i]->require_walkers();
+      ctxs[i]->end_dmc();
       ctxs[i]->seed_rng(seeds ? seeds[i] : ctxs[i]->rng_seed);
     }
     std::vector<std::vector<double>> ek(ctxs.size());
@@ -2646,6 +2744,7 @@ struct qmi_batch
     {
       qmi_ctx* c = ctxs[i];
       c->require_walkers();
+      c->end_dmc();
       if (c->learned)
         throw QmiError(QMI_ERR_UNSUPPORTED, "batched sweeps run the drift-diffusion proposal");
       if (!c->rng_seeded)
@@ -2780,6 +2879,7 @@ extern "C"
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     (void)hipStreamDestroy(c->stream);
+    c->detach_dtables();
     delete c;
     return QMI_OK;
   }
@@ -2895,6 +2995,18 @@ extern "C"
     });
   }
 
+  int qmi_mw_component_ratio_grad(qmi_ctx* c, int iat, double* ratios, double* grad_new)
+  {
+    if (c && !grad_new)
+      return guarded(c, [] { throw QmiError(QMI_ERR_ARG, "grad_new must not be null"); });
+    std::vector<double> g(c ? (size_t)c->nw * 3 * c->vw() : 0);
+    const int rc = qmi_mw_ratio_grad(c, iat, ratios, g.data());
+    if (rc == QMI_OK)
+      for (size_t i = 0; i < g.size(); i++)
+        grad_new[i] += g[i];
+    return rc;
+  }
+
   int qmi_mw_accept_reject(qmi_ctx* c, int iat, const int* accepted, int safe_to_delay)
   {
     return guarded(c, [&] {
@@ -2955,6 +3067,44 @@ extern "C"
       }
       HIPCHECK(hipMemcpyAsync(outv, dout.p, dout.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
       c->sync();
+    });
+  }
+
+  int qmi_mw_spo_vgl_det_ratio_grads(qmi_ctx* c, const double* pos, int n, const double* inv_rows, double* phi_vgl,
+                                     double* ratios, double* grads)
+  {
+    return guarded(c, [&] {
+      c->require_walkers();
+      if (!c->sys.has_det || n <= 0 || n > c->sys.npad / c->vw() || !pos || !inv_rows || !ratios || !grads)
+        throw QmiError(QMI_ERR_ARG, "no orbital table, bad orbital count or null array");
+      const int vw = c->vw();
+      DevBuf<double> dpos, dinv, dphi, drg;
+      dpos.alloc((size_t)c->nw * 3);
+      dinv.alloc((size_t)c->nw * n * vw);
+      if (phi_vgl)
+        dphi.alloc((size_t)5 * c->nw * n * vw);
+      drg.alloc((size_t)c->nw * 4 * vw);
+      HIPCHECK(hipMemcpyAsync(dpos.p, pos, dpos.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      HIPCHECK(hipMemcpyAsync(dinv.p, inv_rows, dinv.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      {
+        qmi_ctx::Scoped t(c, "api_vgl_ratio_grads");
+        HIPCHECK(c->cplx ? launch_api_vgl_ratio_grads_c(c->sys, c->coefs.p, c->kpts.p, c->nw, n, dpos.p, dinv.p,
+                                                        dphi.p, drg.p, c->stream)
+                         : launch_api_vgl_ratio_grads(c->sys, c->coefs.p, c->nw, n, dpos.p, dinv.p, dphi.p, drg.p,
+                                                      c->stream));
+      }
+      std::vector<double> h(drg.n);
+      HIPCHECK(hipMemcpyAsync(h.data(), drg.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      if (phi_vgl)
+        HIPCHECK(hipMemcpyAsync(phi_vgl, dphi.p, dphi.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      c->sync();
+      for (int w = 0; w < c->nw; w++)
+        for (int e = 0; e < vw; e++)
+        {
+          ratios[w * vw + e] = h[(w * 4) * vw + e];
+          for (int q = 0; q < 3; q++)
+            grads[(w * 3 + q) * vw + e] = h[(w * 4 + 1 + q) * vw + e];
+        }
     });
   }
 
@@ -3193,6 +3343,116 @@ extern "C"
       if (ei_dr && ni)
         HIPCHECK(hipMemcpyAsync(ei_dr, dd.p, (size_t)nw * ni * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
       c->sync();
+    });
+  }
+
+  int qmi_dt_create(qmi_ctx* c, int kind, qmi_dtable** out)
+  {
+    return guarded(c, [&] {
+      c->require_walkers();
+      if (!out || (kind != DT_AA && kind != DT_AB))
+        throw QmiError(QMI_ERR_ARG, "kind must be 0 (electron pairs) or 1 (electron-ion), out non-null");
+      if (kind == DT_AB && c->sys.nion <= 0)
+        throw QmiError(QMI_ERR_ARG, "no ions for an electron-ion table");
+      auto t    = std::make_unique<qmi_dtable>();
+      t->c      = c;
+      t->device = c->device;
+      t->kind   = kind;
+      t->nw   = c->nw;
+      t->N    = c->sys.N;
+      t->ns   = kind == DT_AA ? c->sys.N : c->sys.nion;
+      const size_t nw = t->nw, N = t->N, ns = t->ns;
+      t->r.alloc(nw * N * ns);
+      t->dr.alloc(nw * N * 3 * ns);
+      t->temp_r.alloc(nw * ns);
+      t->temp_dr.alloc(nw * 3 * ns);
+      t->old_r.alloc(kind == DT_AA ? nw * ns : 1);
+      t->old_dr.alloc(kind == DT_AA ? nw * 3 * ns : 1);
+      t->rnew.alloc(nw * 3);
+      t->acc.alloc(nw);
+      HIPCHECK(hipMemsetAsync(t->r.p, 0, t->r.n * sizeof(double), c->stream));
+      HIPCHECK(hipMemsetAsync(t->dr.p, 0, t->dr.n * sizeof(double), c->stream));
+      c->sync();
+      c->dtables.push_back(t.get());
+      *out = t.release();
+    });
+  }
+
+  int qmi_dt_destroy(qmi_dtable* t)
+  {
+    if (!t)
+      return QMI_ERR_ARG;
+    (void)hipSetDevice(t->device);
+    if (t->c)
+    {
+      (void)hipStreamSynchronize(t->c->stream);
+      auto& v = t->c->dtables;
+      v.erase(std::remove(v.begin(), v.end(), t), v.end());
+    }
+    delete t; // device buffers are freed with the device current
+    return QMI_OK;
+  }
+
+  int qmi_dt_mw_evaluate(qmi_dtable* t)
+  {
+    if (!t || !t->c)
+      return QMI_ERR_ARG; // null, or its context was destroyed
+    return guarded(t->c, [&] {
+      t->check();
+      HIPCHECK(launch_dt_evaluate(t->c->sys, t->c->st, t->arrays(), t->nw, t->c->stream));
+      t->c->sync();
+    });
+  }
+
+  int qmi_dt_mw_move(qmi_dtable* t, int iat, const double* rnew, int prepare_old)
+  {
+    if (!t || !t->c)
+      return QMI_ERR_ARG; // null, or its context was destroyed
+    return guarded(t->c, [&] {
+      t->check();
+      if (iat < 0 || iat >= t->N || !rnew)
+        throw QmiError(QMI_ERR_ARG, "bad particle index or null positions");
+      HIPCHECK(hipMemcpyAsync(t->rnew.p, rnew, t->rnew.n * sizeof(double), hipMemcpyHostToDevice, t->c->stream));
+      HIPCHECK(launch_dt_move(t->c->sys, t->c->st, t->arrays(), t->nw, iat, t->rnew.p, prepare_old ? 1 : 0,
+                              t->c->stream));
+      t->c->sync();
+    });
+  }
+
+  int qmi_dt_mw_update(qmi_dtable* t, int iat, const int* accepted, int partial)
+  {
+    if (!t || !t->c)
+      return QMI_ERR_ARG; // null, or its context was destroyed
+    return guarded(t->c, [&] {
+      t->check();
+      if (iat < 0 || iat >= t->N || !accepted)
+        throw QmiError(QMI_ERR_ARG, "bad particle index or null mask");
+      HIPCHECK(hipMemcpyAsync(t->acc.p, accepted, t->acc.n * sizeof(int), hipMemcpyHostToDevice, t->c->stream));
+      HIPCHECK(launch_dt_update(t->c->sys, t->arrays(), t->nw, iat, t->acc.p, partial ? 1 : 0, t->c->stream));
+      t->c->sync();
+    });
+  }
+
+  int qmi_dt_get(qmi_dtable* t, double* r, double* dr, double* temp_r, double* temp_dr, double* old_r, double* old_dr)
+  {
+    if (!t || !t->c)
+      return QMI_ERR_ARG; // null, or its context was destroyed
+    return guarded(t->c, [&] {
+      t->check();
+      auto get = [&](double* dst, const DevBuf<double>& src) {
+        if (dst)
+          HIPCHECK(hipMemcpyAsync(dst, src.p, src.n * sizeof(double), hipMemcpyDeviceToHost, t->c->stream));
+      };
+      get(r, t->r);
+      get(dr, t->dr);
+      get(temp_r, t->temp_r);
+      get(temp_dr, t->temp_dr);
+      if (t->kind == DT_AA)
+      {
+        get(old_r, t->old_r);
+        get(old_dr, t->old_dr);
+      }
+      t->c->sync();
     });
   }
 

@@ -327,6 +327,52 @@ hipError_t launch_api_det_ratios(const SysC& s, const void* coefs, int nw, int n
   });
 }
 
+// SPOSet::mw_evaluateVGLandDetRatioGrads (SplineR2R.cpp:414-580): orbital VGL at pos[w] written as
+// phi[5][nw][n] (value, gradient x/y/z, Laplacian), the determinant ratio sum_o inv[w][o] phi_o and
+// the gradient sum_o inv[w][o] dphi_o / ratio.  Grid nw.
+template<typename ST>
+__global__ __launch_bounds__(WG) void k_api_vgl_ratio_grads(SysC s, const ST* __restrict__ coefs, int nw, int n,
+                                                            const double* __restrict__ pos,
+                                                            const double* __restrict__ inv, double* __restrict__ phi,
+                                                            double* __restrict__ rg)
+{
+  __shared__ double red[4 * WG];
+  __shared__ double bc[4];
+  const int w       = blockIdx.x;
+  const double p[3] = {pos[w * 3], pos[w * 3 + 1], pos[w * 3 + 2]};
+  SplineLoc L;
+  spline_locate(s, p, L);
+  double acc[4] = {0, 0, 0, 0};
+  for (int o = threadIdx.x; o < n; o += WG)
+  {
+    double v[5];
+    spline_vgl_orbital<ST>(s, coefs, L, o, v);
+    if (phi)
+      for (int q = 0; q < 5; q++)
+        phi[((size_t)q * nw + w) * n + o] = v[q];
+    const double a = inv[(size_t)w * n + o];
+    for (int q = 0; q < 4; q++)
+      acc[q] = fma(v[q], a, acc[q]);
+  }
+  block_sum<4>(acc, red, bc);
+  if (threadIdx.x == 0)
+  {
+    rg[w * 4] = bc[0];
+    for (int q = 1; q < 4; q++)
+      rg[w * 4 + q] = bc[q] / bc[0];
+  }
+}
+
+hipError_t launch_api_vgl_ratio_grads(const SysC& s, const void* coefs, int nw, int n, const double* pos,
+                                      const double* inv, double* phi, double* rg, hipStream_t stream)
+{
+  return dispatch_prec(s.prec, [&](auto stv, auto) {
+    using ST = decltype(stv);
+    k_api_vgl_ratio_grads<ST><<<nw, WG, 0, stream>>>(s, static_cast<const ST*>(coefs), nw, n, pos, inv, phi, rg);
+    return hipGetLastError();
+  });
+}
+
 // DistanceTable::mw_move rows: electrons and ions relative to pos[w] (source - target)
 __global__ void k_api_distances(SysC s, State st, const double* __restrict__ pos, double* __restrict__ ee_r,
                                 double* __restrict__ ee_dr, double* __restrict__ ei_r, double* __restrict__ ei_dr)

@@ -40,7 +40,19 @@ struct MoveArgs
   // accumulates tau |xi|^2 of its proposed and accepted moves in rr[nw][2] (one walker per crowd)
   int dmc;
   double* rr;
+  // NaNguard (NaNguard.cpp:18-31) on the device: nanflag[w] = 1 + the first electron of walker w
+  // whose ratio, or whose gradient, is NaN (0: none); read by the host once per step
+  int* nanflag;
 };
+
+// the check of TrialWaveFunction::mw_evalGrad / mw_calcRatioGrad (TrialWaveFunction.cpp:592, 736-739):
+// squared norms of the ratio, of the gradient at the proposed position (checked only for a non-zero
+// ratio) and of the gradient at the current position
+__device__ inline void nan_guard(const MoveArgs& a, int w, int iat, double ratio2, double gnew2, double gnow2)
+{
+  if (a.nanflag && (isnan(ratio2) || (ratio2 != 0.0 && isnan(gnew2)) || isnan(gnow2)) && a.nanflag[w] == 0)
+    a.nanflag[w] = iat + 1;
+}
 
 enum MoveKernel
 {
@@ -103,6 +115,30 @@ hipError_t launch_api_spo_vgl(const SysC& s, const void* coefs, int nw, int n, c
                               hipStream_t stream);
 hipError_t launch_api_det_ratios(const SysC& s, const void* coefs, int nw, int nvp, int n, const double* pos,
                                  const double* inv, double* out, hipStream_t stream);
+// SPOSet::mw_evaluateVGLandDetRatioGrads: phi[5][nw][n] (nullable), rg[nw][4] = ratio, gradient / ratio
+hipError_t launch_api_vgl_ratio_grads(const SysC& s, const void* coefs, int nw, int n, const double* pos,
+                                      const double* inv, double* phi, double* rg, hipStream_t stream);
+// stateful distance tables (AA: electron pairs, lower triangle; AB: electron-ion)
+enum DTableKind
+{
+  DT_AA = 0,
+  DT_AB = 1
+};
+struct DTableArrays
+{
+  int kind, ns;          // ns sources per row: N (AA) or the ion count (AB)
+  double* r;             // [nw][N][ns]
+  double* dr;            // [nw][N][3][ns]
+  double* temp_r;        // [nw][ns]
+  double* temp_dr;       // [nw][3][ns]
+  double* old_r;         // [nw][ns] (AA)
+  double* old_dr;        // [nw][3][ns] (AA)
+};
+hipError_t launch_dt_evaluate(const SysC& s, const State& st, const DTableArrays& t, int nw, hipStream_t stream);
+hipError_t launch_dt_move(const SysC& s, const State& st, const DTableArrays& t, int nw, int iat, const double* rnew,
+                          int prepare_old, hipStream_t stream);
+hipError_t launch_dt_update(const SysC& s, const DTableArrays& t, int nw, int iat, const int* acc, int partial,
+                            hipStream_t stream);
 // complex (C2C) orbitals: complex FP64 determinant engine; value arrays hold (re, im) pairs
 hipError_t launch_move_c(int which, const SysC& s, const State& st, const void* coefs, const MoveArgs& a,
                          hipStream_t stream);
@@ -115,6 +151,9 @@ hipError_t launch_api_spo_vgl_c(const SysC& s, const void* coefs, const double* 
                                 const double* pos, double* out, hipStream_t stream);
 hipError_t launch_api_det_ratios_c(const SysC& s, const void* coefs, const double* kpts, int nw, int nvp, int n,
                                    const double* pos, const double* inv, double* out, hipStream_t stream);
+hipError_t launch_api_vgl_ratio_grads_c(const SysC& s, const void* coefs, const double* kpts, int nw, int n,
+                                        const double* pos, const double* inv, double* phi, double* rg,
+                                        hipStream_t stream);
 // Non-local pseudopotential of one ion species (host-built tables, device pointers)
 constexpr int NLPP_MAX_CH = 4, NLPP_MAX_L = 4, NLPP_MAX_IONS = 2048, NLPP_MAX_KNOTS = 64;
 struct NlppC

@@ -453,12 +453,13 @@ __device__ inline void phaseE(const SysC& s, int g, int iat, int n, int ld, int 
 //------------------------------------------------------------------------------------------------
 // Phase D core (thread 0): returns prob_eff; flag = |ratio|^2 >= eps.
 __device__ inline double mh_prob_v(const SysC& s, double vat, double uat, const MoveArgs& a, MoveLDS& sh,
-                                   double& ratio, bool& flag)
+                                   double& ratio, bool& flag, double& gn2)
 {
   const double tauovermass = a.tau;
   const double oneover2tau = 0.5 / tauovermass;
   double gnew[3];
   combine_ratio_v(s, vat, uat, sh, ratio, gnew);
+  gn2 = gnew[0] * gnew[0] + gnew[1] * gnew[1] + gnew[2] * gnew[2];
   double drev[3];
   unr_drift(tauovermass, gnew, drev);
   for (int q = 0; q < 3; q++)
@@ -471,12 +472,13 @@ __device__ inline double mh_prob_v(const SysC& s, double vat, double uat, const 
 }
 
 __device__ inline double mh_prob(const SysC& s, int iat, const MoveArgs& a, const double* __restrict__ jas,
-                                 MoveLDS& sh, double& ratio, bool& flag)
+                                 MoveLDS& sh, double& ratio, bool& flag, double& gn2)
 {
   const double tauovermass = a.tau;
   const double oneover2tau = 0.5 / tauovermass;
   double gnew[3];
   combine_ratio(s, iat, jas, sh, ratio, gnew);
+  gn2 = gnew[0] * gnew[0] + gnew[1] * gnew[1] + gnew[2] * gnew[2];
   double drev[3];
   unr_drift(tauovermass, gnew, drev);
   for (int q = 0; q < 3; q++)
@@ -523,7 +525,9 @@ __device__ inline void fused_move(const SysC& s, const State& st, const ST* __re
   {
     double ratio;
     bool flag;
-    const double pe = mh_prob_v(s, sh.bc[BC_PF + PF_VAT], sh.bc[BC_PF + PF_UAT], a, sh, ratio, flag);
+    double gn2;
+    const double pe = mh_prob_v(s, sh.bc[BC_PF + PF_VAT], sh.bc[BC_PF + PF_UAT], a, sh, ratio, flag, gn2);
+    nan_guard(a, w, iat, ratio * ratio, gn2, gnow_norm2(sh.bc));
     if (a.dmc) // a node crossing (ratio <= 0) is rejected before the draw; prob_eff against eps
       flag = ratio > 0.0 && pe >= DBL_EPS;
     bool acc = false;
@@ -640,7 +644,9 @@ __global__ __launch_bounds__(WG) void k_propose(SysC s, State st, const ST* __re
   {
     double ratio;
     bool flag;
-    const double pe = mh_prob(s, iat, a, v.jas, sh, ratio, flag);
+    double gn2;
+    const double pe = mh_prob(s, iat, a, v.jas, sh, ratio, flag, gn2);
+    nan_guard(a, w, iat, ratio * ratio, gn2, gnow_norm2(sh.bc));
     mv[MV_RATIO]    = ratio;
     mv[MV_PROB]     = pe;
     mv[MV_FLAG]     = flag ? 1.0 : 0.0;
@@ -652,14 +658,17 @@ __global__ __launch_bounds__(WG) void k_propose(SysC s, State st, const ST* __re
     for (int q = 0; q < 13; q++)
       mv[MV_C + q] = sh.bc[BC_C + q];
   }
-  double* phi = st.phi + (size_t)w * 5 * ld;
-#pragma unroll
-  for (int m = 0; m < MAXM; m++)
+  if (s.has_det) // the proposed orbitals (no orbital buffers without determinants)
   {
-    const int o = threadIdx.x + WG * m;
-    if (o < n)
-      for (int q = 0; q < 5; q++)
-        phi[q * ld + o] = phv[m][q];
+    double* phi = st.phi + (size_t)w * 5 * ld;
+#pragma unroll
+    for (int m = 0; m < MAXM; m++)
+    {
+      const int o = threadIdx.x + WG * m;
+      if (o < n)
+        for (int q = 0; q < 5; q++)
+          phi[q * ld + o] = phv[m][q];
+    }
   }
 }
 
@@ -719,18 +728,19 @@ __global__ __launch_bounds__(WG) void k_accept_crowd(SysC s, State st, MoveArgs 
     for (int q = 0; q < 13; q++)
       sh.bc[BC_C + q] = mv[MV_C + q];
   }
-  if (k > 1 && threadIdx.x < d)
+  if (s.has_det && k > 1 && threadIdx.x < d)
     sh.y[threadIdx.x] = v.Binv[d * k + threadIdx.x];
   __syncthreads();
   double ainv_r[MAXM], phv[MAXM][5];
   const double* phi = st.phi + (size_t)w * 5 * ld;
+  const int nd      = s.has_det ? n : 0; // no determinant buffers without determinants
 #pragma unroll
   for (int m = 0; m < MAXM; m++)
   {
     const int o = threadIdx.x + WG * m;
-    ainv_r[m]   = o < n ? v.Ainv[(size_t)r * ld + o] : 0.0;
+    ainv_r[m]   = o < nd ? v.Ainv[(size_t)r * ld + o] : 0.0;
     for (int q = 0; q < 5; q++)
-      phv[m][q] = o < n ? phi[q * ld + o] : 0.0;
+      phv[m][q] = o < nd ? phi[q * ld + o] : 0.0;
   }
   __syncthreads();
   phaseE<DT, MAXM>(s, g, iat, n, ld, k, d, r, v.R, v.Ainv, v.dpsi, v.U, v.V, v.Binv, v.dl, v.jas, v.logdet, ainv_r, phv,
@@ -841,14 +851,17 @@ __global__ __launch_bounds__(WG) void k_api_ratio_grad(SysC s, State st, const S
     for (int q = 0; q < 3; q++)
       a.out[a.nw + (size_t)w * 3 + q] = gnew[q];
   }
-  double* phi = st.phi + (size_t)w * 5 * ld;
-#pragma unroll
-  for (int m = 0; m < MAXM; m++)
+  if (s.has_det) // the proposed orbitals (no orbital buffers without determinants)
   {
-    const int o = threadIdx.x + WG * m;
-    if (o < n)
-      for (int q = 0; q < 5; q++)
-        phi[q * ld + o] = phv[m][q];
+    double* phi = st.phi + (size_t)w * 5 * ld;
+#pragma unroll
+    for (int m = 0; m < MAXM; m++)
+    {
+      const int o = threadIdx.x + WG * m;
+      if (o < n)
+        for (int q = 0; q < 5; q++)
+          phi[q * ld + o] = phv[m][q];
+    }
   }
 }
 
@@ -877,18 +890,19 @@ __global__ __launch_bounds__(WG) void k_api_accept(SysC s, State st, MoveArgs a)
     for (int q = 0; q < 13; q++)
       sh.bc[BC_C + q] = mv[MV_C + q];
   }
-  if (k > 1 && threadIdx.x < d)
+  if (s.has_det && k > 1 && threadIdx.x < d)
     sh.y[threadIdx.x] = v.Binv[d * k + threadIdx.x];
   __syncthreads();
   double ainv_r[MAXM], phv[MAXM][5];
   const double* phi = st.phi + (size_t)w * 5 * ld;
+  const int nd      = s.has_det ? n : 0; // no determinant buffers without determinants
 #pragma unroll
   for (int m = 0; m < MAXM; m++)
   {
     const int o = threadIdx.x + WG * m;
-    ainv_r[m]   = o < n ? v.Ainv[(size_t)r * ld + o] : 0.0;
+    ainv_r[m]   = o < nd ? v.Ainv[(size_t)r * ld + o] : 0.0;
     for (int q = 0; q < 5; q++)
-      phv[m][q] = o < n ? phi[q * ld + o] : 0.0;
+      phv[m][q] = o < nd ? phi[q * ld + o] : 0.0;
   }
   __syncthreads();
   phaseE<DT, MAXM>(s, g, iat, n, ld, k, d, r, v.R, v.Ainv, v.dpsi, v.U, v.V, v.Binv, v.dl, v.jas, v.logdet, ainv_r, phv,
@@ -1248,6 +1262,8 @@ __global__ __launch_bounds__(WG) void k_learn_prop(SysC s, State st, const ST* _
   {
     double ratio, gnew[3], vr[3];
     combine_ratio(s, iat, v.jas, sh, ratio, gnew);
+    nan_guard(a, w, iat, ratio * ratio, gnew[0] * gnew[0] + gnew[1] * gnew[1] + gnew[2] * gnew[2],
+              ls[LS_GNOW] * ls[LS_GNOW] + ls[LS_GNOW + 1] * ls[LS_GNOW + 1] + ls[LS_GNOW + 2] * ls[LS_GNOW + 2]);
     unr_drift(a.tau, gnew, vr);
     mv[MV_RATIO] = ratio;
     for (int q = 0; q < 3; q++)
@@ -1509,6 +1525,8 @@ __global__ __launch_bounds__(WG, 4) void k_learn_move(SysC s, State st, const ST
   {
     double ratio, gnew[3], vr[3];
     combine_ratio_v(s, pf[PF_VAT], pf[PF_UAT], sh, ratio, gnew);
+    nan_guard(a, w, iat, ratio * ratio, gnew[0] * gnew[0] + gnew[1] * gnew[1] + gnew[2] * gnew[2],
+              gnow_norm2(sh.bc));
     unr_drift(a.tau, gnew, vr);
     sh.bc[BC_RATIO] = ratio;
     for (int q = 0; q < 3; q++)

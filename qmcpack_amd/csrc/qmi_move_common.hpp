@@ -72,6 +72,12 @@ enum BcField
   BC_GNOW = 29  // 3 gradient at the current position
 };
 
+// squared norm of the gradient at the current position (NaN guard)
+__device__ inline double gnow_norm2(const double* bc)
+{
+  return bc[BC_GNOW] * bc[BC_GNOW] + bc[BC_GNOW + 1] * bc[BC_GNOW + 1] + bc[BC_GNOW + 2] * bc[BC_GNOW + 2];
+}
+
 
 // J1/J2 ratio sums at the proposed position pos -> sh.bc[BC_C..BC_C+9): J1 (u, g3, lap), J2 (u, g3)
 // (J1OrbitalSoA.h:136-186 computeU3; TwoBodyJastrow.cpp:420-446).  All threads; ends with a barrier.

@@ -256,6 +256,31 @@ class Context:
         self._check(self.L.qmi_mw_ratio_grad(self.h, int(iat), _p(r), _p(g)))
         return self._out(r), self._out(g)
 
+    def mw_ratioGrad(self, iat, grad_new):
+        """WaveFunctionComponent::mw_ratioGrad of this context's components: returns the ratios and
+        adds their gradient at the proposed position to grad_new [nw][3] in place."""
+        r = self._vals(self.nw)
+        g = self._vin(grad_new)
+        self._check(self.L.qmi_mw_component_ratio_grad(self.h, int(iat), _p(r), _p(g)))
+        grad_new[...] = self._out(g)
+        return self._out(r)
+
+    def mw_evaluateVGLandDetRatioGrads(self, pos, inv_rows, with_vgl=True):
+        """SPOSet::mw_evaluateVGLandDetRatioGrads with caller inverse rows inv_rows [nw][n]:
+        (phi_vgl [5][nw][n] or None, ratios [nw], grads [nw][3])."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.nw, 3)
+        n = np.asarray(inv_rows).shape[1]
+        inv = self._vin(inv_rows)
+        phi = self._vals(5, self.nw, n) if with_vgl else None
+        r, g = self._vals(self.nw), self._vals(self.nw, 3)
+        self._check(self.L.qmi_mw_spo_vgl_det_ratio_grads(self.h, _p(pos), int(n), _p(inv), _p(phi), _p(r), _p(g)))
+        return (self._out(phi) if with_vgl else None), self._out(r), self._out(g)
+
+    def distance_table(self, kind):
+        """A stateful DistanceTable over this context's walkers: kind "AA" (electron pairs) or
+        "AB" (electrons to ions)."""
+        return DistanceTable(self, kind)
+
     def mw_accept_rejectMove(self, iat, accepted, safe_to_delay=True):
         acc = np.ascontiguousarray(np.asarray(accepted, dtype=np.int32).reshape(self.nw))
         self._check(self.L.qmi_mw_accept_reject(self.h, int(iat), acc.ctypes.data_as(_ip), int(bool(safe_to_delay))))
@@ -486,6 +511,62 @@ class Context:
         self._check(self.L.qmi_timing_get(self.h, buf, 4096, _p(ms), _p(calls), 64, C.byref(nk)))
         names = buf.value.decode().split("\n")[: nk.value]
         return {n: dict(ms=float(ms[i]), calls=int(calls[i])) for i, n in enumerate(names)}
+
+
+class DistanceTable:
+    """qmi_dtable: DistanceTable (src/Particle/DistanceTable.h) of one context's walkers.  AA rows
+    hold electrons j < i (SoaDistanceTableAA), AB rows every ion (SoaDistanceTableAB);
+    displacements are source - target."""
+
+    def __init__(self, ctx: Context, kind: str):
+        self.ctx = ctx
+        self.kind = {"AA": 0, "AB": 1}[kind]
+        self.h = C.c_void_p()
+        ctx._check(ctx.L.qmi_dt_create(ctx.h, self.kind, C.byref(self.h)))
+        self.nw, self.N = ctx.nw, ctx.N
+        self.ns = ctx.N if self.kind == 0 else None
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.ctx.L.qmi_dt_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def mw_evaluate(self):
+        self.ctx._check(self.ctx.L.qmi_dt_mw_evaluate(self.h))
+
+    def mw_move(self, iat, rnew, prepare_old=True):
+        rnew = np.ascontiguousarray(rnew, dtype=np.float64).reshape(self.nw, 3)
+        self.ctx._check(self.ctx.L.qmi_dt_mw_move(self.h, int(iat), _p(rnew), int(bool(prepare_old))))
+
+    def _mask(self, accepted):
+        return np.ascontiguousarray(np.asarray(accepted, dtype=np.int32).reshape(self.nw))
+
+    def update(self, iat, accepted):
+        """DistanceTable::update for the walkers whose move is accepted."""
+        acc = self._mask(accepted)
+        self.ctx._check(self.ctx.L.qmi_dt_mw_update(self.h, int(iat), acc.ctypes.data_as(_ip), 0))
+
+    def mw_updatePartial(self, iat, from_temp):
+        acc = self._mask(from_temp)
+        self.ctx._check(self.ctx.L.qmi_dt_mw_update(self.h, int(iat), acc.ctypes.data_as(_ip), 1))
+
+    def get(self, ns):
+        """(r [nw][N][ns], dr [nw][N][ns][3], temp_r [nw][ns], temp_dr [nw][ns][3], old_r, old_dr)
+        with displacements moved to the last axis (AB: old rows are None)."""
+        nw, N = self.nw, self.N
+        r, dr = np.empty((nw, N, ns)), np.empty((nw, N, 3, ns))
+        tr, tdr = np.empty((nw, ns)), np.empty((nw, 3, ns))
+        aa = self.kind == 0
+        orr, odr = (np.empty((nw, ns)), np.empty((nw, 3, ns))) if aa else (None, None)
+        self.ctx._check(self.ctx.L.qmi_dt_get(self.h, _p(r), _p(dr), _p(tr), _p(tdr), _p(orr), _p(odr)))
+        mv = lambda a: None if a is None else np.moveaxis(a, -2, -1)
+        return r, mv(dr), tr, mv(tdr), orr, mv(odr)
 
 
 class Batch:

@@ -34,14 +34,7 @@ def relerr(a, b, floor=0.0):
     return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)))
 
 
-def round_table(coefs, prec):
-    """The coefficients as the device stores them, returned in FP64."""
-    f = np.asarray(coefs, dtype=np.float64).astype(np.float32)
-    if prec in ("fp32", "t32"):
-        return f.astype(np.float64)
-    u = f.view(np.uint32).astype(np.uint64)
-    u = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
-    return u.view(np.float32).astype(np.float64)
+round_table = orc.round_table  # the coefficients as the device stores them, in FP64
 
 
 def plane_wave_system(n_up, n_dn, M=16, L=10.0, seed=1, k=8):
