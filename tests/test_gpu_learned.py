@@ -107,13 +107,20 @@ def replay_positions(R0, tr):
     return r
 
 
-def test_zero_head_is_plain_drift_diffusion():
+# FP64 engine with the full table and with the table-only narrowings (FP32, BF16 storage): the oracle
+# runs on the table as the device stores it, so FP64 parity holds in all three
+ENGINE_FP64 = ["fp64", "t32", "tbf16"]
+
+
+@pytest.mark.parametrize("precision", ENGINE_FP64)
+def test_zero_head_is_plain_drift_diffusion(precision):
     s = small_system()
     R = np.random.default_rng(6).uniform(0, 1, size=(6, s.N, 3)) * s.L
-    c = ctx_for(s, R)
+    c = ctx_for(s, R, precision)
     c.proposal_config(learned=True, train=False, init_seed=99)
     E, tr, lt = c.vmc_run_learned(3, 2, 0.4, 2, 77, trace=True)
     assert np.all(lt[..., 0:4] == 0) and np.all(lt[..., 4:8] == 0)
+    s = dataclasses.replace(s, coefs=orc.round_table(s.coefs, precision))
     o = orc.VMC(s, R, 1, 77)
     o.evaluate_log()
     Eo, tro = o.replay(tr, 2, 0.4, 2)
@@ -125,16 +132,18 @@ def test_zero_head_is_plain_drift_diffusion():
     assert np.array_equal(c.walkers_get(), o.get()["R"])
 
 
-def test_random_network_acceptance_and_objective():
+@pytest.mark.parametrize("precision", ENGINE_FP64)
+def test_random_network_acceptance_and_objective(precision):
     rng = np.random.default_rng(9)
     s = small_system()
     R = rng.uniform(0, 1, size=(6, s.N, 3)) * s.L
-    c = ctx_for(s, R)
+    c = ctx_for(s, R, precision)
     P = lp.init_params(rng, zero_head=False, scale=0.5)
     c.mlp_params(flat(P))
     c.proposal_config(learned=True, train=False)
     tau = 0.4
     E, tr, lt = c.vmc_run_learned(2, 1, tau, 0, 78, trace=True)
+    s = dataclasses.replace(s, coefs=orc.round_table(s.coefs, precision))
     o = orc.VMC(s, R, 1, 78)
     o.evaluate_log()
     Eo, tro = o.replay(tr, 1, tau, 0)
@@ -229,12 +238,14 @@ def test_exact_sampling_one_electron():
     assert chi2 < 4.0
 
 
-def test_training_improves_objective():
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "t32", "tbf16"])
+def test_training_improves_objective(precision):
     """Training (Adam on -mean acceptance-weighted squared jump) for 20 sweeps of 256 walkers of the
-    22-electron system; the trained network's objective exceeds the untrained (plain) proposal's."""
+    22-electron system; the trained network's objective exceeds the untrained (plain) proposal's, in
+    every storage mode of the table and engine."""
     s = small_system()
     R = np.random.default_rng(13).uniform(0, 1, size=(256, s.N, 3)) * s.L
-    c = ctx_for(s, R)
+    c = ctx_for(s, R, precision)
     c.proposal_config(learned=True, train=False, init_seed=5)
     c.vmc_run(3, 1, 0.3, 0, 21, energies=False)
     c.proposal_stats()
