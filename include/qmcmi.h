@@ -198,6 +198,18 @@ int qmi_mw_det_ratios(qmi_ctx* ctx, int nvp, const double* pos, int n, const dou
  * needed).  A singular matrix returns QMI_ERR_NUMERIC. */
 int qmi_mw_invert_transpose(qmi_ctx* ctx, int complex_values, int n, int nmat, const double* a, double* inv_a,
                             double* log_values);
+/* No counterpart in the reference.  qmi_mw_invert_transpose for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): a,
+ * inv_a, log_values (nullable) and info are DEVICE arrays (a[nmat][n][n], inv_a[nmat][n][n] with
+ * (re, im) pairs when complex_values, log_values[nmat][2], info[nmat] = 0 or the 1-based column of the
+ * first zero pivot).  The LU factors, the solve work array and the pivots live in a fixed device
+ * scratch of QMI_NOALLOC_SCRATCH_BYTES held by the library on each GPU it runs on; a call needing
+ * more (16 n^2 nmat bytes real, 32 n^2 nmat complex, plus 8 n nmat) returns QMI_ERR_ARG.  The work is
+ * enqueued on the context's stream without synchronisation or allocation; singular matrices are
+ * reported only through info.  Calls that overlap on one GPU (any contexts) race on the scratch. */
+#define QMI_NOALLOC_SCRATCH_BYTES (64u << 20)
+int qmi_mw_invert_transpose_noalloc(qmi_ctx* ctx, int complex_values, int n, int nmat, const double* a,
+                                    double* inv_a, double* log_values, int* info);
 /* DelayedUpdateBatched::mw_transferAinv_D2H (DelayedUpdateBatched.h:813-824, guard_no_delay) /
  * DiracDeterminantBatched::get_psiMinv (DiracDeterminantBatched.h:276): the inverse Slater matrix of
  * spin group g of every walker, ainv[nw][n][n] (row i = inverse row of electron i of the group; FP32

@@ -3170,6 +3170,36 @@ extern "C"
     });
   }
 
+  // The same factorisation, inverse and log det as qmi_mw_invert_transpose (identical kernels and
+  // arithmetic), on device arrays, with every intermediate in the library's static device scratch:
+  // no allocation, no synchronisation.
+  int qmi_mw_invert_transpose_noalloc(qmi_ctx* c, int complex_values, int n, int nmat, const double* a,
+                                      double* inv_a, double* log_values, int* info)
+  {
+    return guarded(c, [&] {
+      if (n <= 0 || n > MAX_N || nmat <= 0 || !a || !inv_a || !info)
+        throw QmiError(QMI_ERR_ARG, "matrix size must be in [1, 512], at least one matrix, non-null arrays");
+      const int vw      = complex_values ? 2 : 1;
+      const size_t per  = (size_t)n * n * vw;                       // doubles per matrix
+      const size_t need = 2 * per * nmat * sizeof(double) + 2 * (size_t)n * nmat * sizeof(int);
+      if (need > (size_t)QMI_NOALLOC_SCRATCH_BYTES)
+        throw QmiError(QMI_ERR_ARG, "the call needs " + std::to_string(need) + " bytes of scratch, more than " +
+                                        std::to_string((size_t)QMI_NOALLOC_SCRATCH_BYTES) + " (fewer matrices per call)");
+      void* base = nullptr;
+      HIPCHECK(lu_noalloc_scratch(c->device, &base));
+      double* lu   = static_cast<double*>(base);        // [nmat][n][n] factors, column-major
+      double* work = lu + per * nmat;                    // [nmat][n][n] solve work array
+      int* piv     = reinterpret_cast<int*>(work + per * nmat);
+      int* perm    = piv + (size_t)n * nmat;
+      HIPCHECK(launch_lu_transpose_in(n, nmat, vw, a, lu, c->stream));
+      HIPCHECK(launch_lu_factor(complex_values != 0, n, n, nmat, lu, piv, perm, info, c->stream));
+      if (log_values)
+        HIPCHECK(launch_lu_logdet(complex_values != 0, n, n, nmat, lu, piv, log_values, 2, c->stream));
+      HIPCHECK(launch_lu_inverse(complex_values ? LU_OUT_C128 : LU_OUT_F64, n, n, nmat, lu, perm, work, inv_a,
+                                 c->stream));
+    });
+  }
+
   int qmi_mw_get_inverse(qmi_ctx* c, int g, double* ainv)
   {
     return guarded(c, [&] {

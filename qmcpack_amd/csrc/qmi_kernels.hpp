@@ -94,6 +94,11 @@ enum LuOut
 };
 hipError_t launch_lu_inverse(int out_kind, int n, int ld, int nmat, const double* lu, const int* perm, double* work,
                              void* out, hipStream_t stream);
+// allocation-free inverter: the address of this device's scratch (resolved once per device, with that
+// device current), and the transposing copy of row-major matrices into the factorisation's layout
+constexpr int QMI_MAX_DEVICES = 64;
+hipError_t lu_noalloc_scratch(int device, void** out);
+hipError_t launch_lu_transpose_in(int n, int nmat, int vw, const double* a, double* t, hipStream_t stream);
 hipError_t launch_jastrow_recompute(const SysC& s, const State& st, int nw, hipStream_t stream);
 // per-walker outputs: out3[nw][3] = {ekin, logpsi, phase}; G[nw][N][3], L[nw][N] (nullable)
 hipError_t launch_eval_gl(const SysC& s, const State& st, int nw, double* out3, double* G, double* L,

@@ -22,7 +22,10 @@
 // trailing update are blocked for the LDS, the triangular solves by row blocks of BR rows with one lane
 // per right-hand side. The results are therefore bit-identical to a sequential restatement of the same
 // arithmetic (complex products and quotients written out with the fused forms below).
+#include "../../include/qmcmi.h"
 #include "qmi_kernels.hpp"
+
+#include <mutex>
 
 namespace qmi
 {
@@ -532,7 +535,58 @@ hipError_t launch_factor_v(int n, int ld, int nmat, V* a, int* piv, int* perm, i
     return launch_factor_nb<V, 8>(n, ld, nmat, a, piv, perm, info, stream);
   return launch_factor_nb<V, 4>(n, ld, nmat, a, piv, perm, info, stream);
 }
+// row-major a[m][i][j] (vw doubles per value) -> column-major t[m][j][i]: the storage of A(i, c) =
+// a[i][c] that the factorisation works on.  Grid (ceil(n / 64), n, nmat), 64 threads: row i of the
+// output column j.
+__global__ __launch_bounds__(64) void k_lu_transpose_in(int n, int vw, const double* __restrict__ a,
+                                                       double* __restrict__ t)
+{
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y;
+  const size_t per = (size_t)n * n * vw, m = blockIdx.z;
+  if (i >= n)
+    return;
+  for (int q = 0; q < vw; q++)
+    t[m * per + ((size_t)j * n + i) * vw + q] = a[m * per + ((size_t)i * n + j) * vw + q];
+}
+
+// The allocation-free inverter's device scratch: the LU factors, the solve work array, pivots and
+// permutations of one call, carved by the host.  One copy per GPU, held until the process ends;
+// host code reaches it only through the address hipGetSymbolAddress returns for that GPU.
+static __device__ __align__(256) unsigned char scratch[QMI_NOALLOC_SCRATCH_BYTES];
 } // namespace
+
+hipError_t lu_noalloc_scratch(int device, void** out)
+{
+  static std::mutex mu;
+  static void* addr[QMI_MAX_DEVICES] = {};
+  if (device < 0 || device >= QMI_MAX_DEVICES || !out)
+    return hipErrorInvalidValue;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device])
+  {
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess)
+      return e;
+    if (cur != device) // resolved with that device current: each GPU holds its own copy
+      return hipErrorInvalidDevice;
+    void* p = nullptr;
+    e       = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess)
+      return e;
+    addr[device] = p;
+  }
+  *out = addr[device];
+  return hipSuccess;
+}
+
+hipError_t launch_lu_transpose_in(int n, int nmat, int vw, const double* a, double* t, hipStream_t stream)
+{
+  if (n <= 0 || nmat <= 0 || (vw != 1 && vw != 2))
+    return hipErrorInvalidValue;
+  k_lu_transpose_in<<<dim3((n + 63) / 64, n, nmat), 64, 0, stream>>>(n, vw, a, t);
+  return hipGetLastError();
+}
 
 hipError_t launch_lu_factor(bool complex_values, int n, int ld, int nmat, double* a, int* piv, int* perm, int* info,
                             hipStream_t stream)

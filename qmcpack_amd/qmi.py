@@ -330,6 +330,14 @@ class Context:
         inv = out[..., 0] + 1j * out[..., 1] if cplx else out
         return inv, lv[:, 0] + 1j * lv[:, 1]
 
+    def mw_invertTranspose_noalloc(self, cplx, n, nmat, a_dev, inv_dev, log_dev, info_dev):
+        """qmi_mw_invert_transpose_noalloc on device addresses (integers): enqueued on this context's
+        stream, no allocation, no synchronisation; info_dev receives the zero-pivot column per matrix."""
+        vp = C.c_void_p
+        self._check(self.L.qmi_mw_invert_transpose_noalloc(
+            self.h, int(bool(cplx)), int(n), int(nmat), vp(a_dev), vp(inv_dev), vp(log_dev) if log_dev else None,
+            vp(info_dev)))
+
     def mw_getInverse(self, g):
         """DelayedUpdateBatched::mw_transferAinv_D2H: inverse Slater matrix of spin group g of every
         walker, [nw][n][n] (row i = inverse row of electron i of the group)."""
