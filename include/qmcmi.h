@@ -362,6 +362,11 @@ int qmi_vmc_run_ex(qmi_ctx* ctx, int steps, int sub_steps, double tau, int recom
  * flush_variant (0: matrix-core flush for n <= 512, 1: three LDS-tiled passes).  Between sweeps. */
 int qmi_tune_set(qmi_ctx* ctx, int delay_rank, int move_kernel, int flush_variant);
 int qmi_tune_get(qmi_ctx* ctx, int* out3);
+/* Walker partitions of the device-resident sweep (no reference counterpart): partitions > 0 sets P
+ * (1..8); the fused single-move path then runs the moves of P walker ranges on P streams forked from
+ * the context's stream (at least 64 walkers per range), so that launches of different ranges overlap
+ * on the compute units.  Results are unchanged.  *current (nullable) receives P. */
+int qmi_tune_partitions(qmi_ctx* ctx, int partitions, int* current);
 /* on = 1: qmi_vmc_run replays each sub-step from a captured graph (runs without trace) */
 int qmi_graph_enable(qmi_ctx* ctx, int on);
 typedef struct qmi_batch qmi_batch;

@@ -600,7 +600,7 @@ __global__ __launch_bounds__(WG) void k_move_one_c(SysC s, State st, const doubl
   __shared__ CMoveLDS sh;
   extern __shared__ double2 sBz[];
   stage_jastrow(s, sh);
-  const int w = blockIdx.x;
+  const int w = a.w0 + blockIdx.x;
   if (w >= a.nw)
     return;
   const int g = a.g, n = s.n[g], ld = s.ld, k = s.k, N = s.N;
@@ -950,7 +950,11 @@ hipError_t launch_move_c(int which, const SysC& s, const State& st, const void* 
   const int maxn   = move_maxm(s, a.g) * WG;
   if (maxn == 0)
     return hipErrorInvalidValue;
-  const dim3 grid(a.nw), block(WG);
+  if ((a.w0 != 0 || a.wn != 0) && which != MK_MOVE_ONE)
+    return hipErrorInvalidValue; // only the fused kernel takes a walker partition
+  if (a.w0 < 0 || a.wn < 0 || a.w0 + a.wn > a.nw)
+    return hipErrorInvalidValue;
+  const dim3 grid(a.wn > 0 ? a.wn : a.nw), block(WG);
   const size_t dyn = binv_lds_bytes(s.k, sizeof(double2));
   const double* c  = static_cast<const double*>(coefs);
   auto run         = [&](auto mtag) -> hipError_t {

@@ -20,6 +20,7 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 using namespace qmi;
@@ -193,6 +194,15 @@ struct qmi_ctx
   // invalidates a graph (reallocation, tuning, new walkers) increments it
   unsigned long long graph_gen = 0;
   hipEvent_t ev_ubase = nullptr; // recorded when a sub-step's uniform counts have reached the host
+  // Walker partitions of the fused sweep: the moves of walkers [p nw / P, (p + 1) nw / P) run on
+  // pstream[p], forked from and joined to the context stream, so that the single-move launches of
+  // different partitions overlap on the compute units instead of all walkers passing through the
+  // same latency-bound phase at once.  Results are those of one partition (each walker's arithmetic
+  // is unchanged).
+  int npart = 1;
+  std::vector<hipStream_t> pstream;
+  std::vector<hipEvent_t> pev_moves, pev_done;
+  hipEvent_t pev_fork = nullptr;
   long long step_count = 0;
   // RNG: one stream per crowd
   std::vector<std::mt19937> rng;
@@ -268,8 +278,96 @@ struct qmi_ctx
   std::vector<qmi_dtable*> dtables;
   void detach_dtables();
 
+  void free_partitions()
+  {
+    for (auto s_ : pstream)
+    {
+      (void)hipStreamSynchronize(s_);
+      (void)hipStreamDestroy(s_);
+    }
+    for (auto e : pev_moves)
+      (void)hipEventDestroy(e);
+    for (auto e : pev_done)
+      (void)hipEventDestroy(e);
+    if (pev_fork)
+      (void)hipEventDestroy(pev_fork);
+    pstream.clear();
+    pev_moves.clear();
+    pev_done.clear();
+    pev_fork = nullptr;
+  }
+
+  void ensure_partitions(int P)
+  {
+    if ((int)pstream.size() >= P)
+      return;
+    if (!pev_fork)
+      HIPCHECK(hipEventCreateWithFlags(&pev_fork, hipEventDisableTiming));
+    while ((int)pstream.size() < P)
+    {
+      hipStream_t s_ = nullptr;
+      hipEvent_t e1 = nullptr, e2 = nullptr;
+      HIPCHECK(hipStreamCreate(&s_));
+      pstream.push_back(s_);
+      HIPCHECK(hipEventCreateWithFlags(&e1, hipEventDisableTiming));
+      pev_moves.push_back(e1);
+      HIPCHECK(hipEventCreateWithFlags(&e2, hipEventDisableTiming));
+      pev_done.push_back(e2);
+    }
+  }
+
+  // the partitions of the next fused sub-step: 1 unless one walker per crowd runs the single-move
+  // kernel without the learned proposal and every partition keeps at least 64 walkers
+  int partitions() const
+  {
+    if (npart <= 1 || wpc != 1 || learned || !(move_one || cplx))
+      return 1;
+    return std::max(1, std::min(npart, nw / 64));
+  }
+
+  // walkers [w0, ...) as a population of their own: every per-walker array advanced by w0 walkers
+  State state_from(int w0) const
+  {
+    State t  = st;
+    auto adv = [&](auto*& p, size_t bytes) {
+      if (p)
+        p = reinterpret_cast<std::remove_reference_t<decltype(p)>>(reinterpret_cast<char*>(p) + bytes * w0);
+    };
+    const size_t es = det_esize(), ld = sys.ld, k = sys.k, Np = sys.Npad, v = vw();
+    adv(t.R, 3 * Np * sizeof(double));
+    adv(t.jas, J_ROWS * Np * sizeof(double));
+    adv(t.logdet, 4 * sizeof(double));
+    adv(t.stats, 2 * sizeof(double));
+    adv(t.invRow, ld * v * sizeof(double));
+    adv(t.phi, 5 * ld * v * sizeof(double));
+    adv(t.mv, MV_SIZE * sizeof(double));
+    for (int g = 0; g < 2; g++)
+    {
+      const size_t n = sys.n[g];
+      adv(t.Ainv[g], n * ld * es);
+      adv(t.dpsi[g], n * 4 * ld * es);
+      adv(t.U[g], ld * k * es);
+      adv(t.V[g], ld * k * es);
+      adv(t.Binv[g], k * k * es);
+      adv(t.dlist[g], k * sizeof(int));
+    }
+    return t;
+  }
+
+  // flush of spin group g at delay d for walkers [w0, w0 + wn) on stream s_
+  void flush_part(int g, int d, int w0, int wn, hipStream_t s_)
+  {
+    if (sys.k == 1 || d == 0)
+      return;
+    const State t    = state_from(w0);
+    double* scratchT = flushT.p ? flushT.p + (size_t)w0 * sys.ld * sys.k * vw() : nullptr;
+    HIPCHECK(cplx ? launch_flush_c(sys, t, g, d, wn, scratchT, s_)
+                  : launch_flush(sys, t, g, d, wn, scratchT, s_, flush_variant));
+  }
+
   ~qmi_ctx()
   {
+    free_partitions();
     invalidate_graph();
     if (ev_done)
       (void)hipEventDestroy(ev_done);
@@ -2171,6 +2269,12 @@ struct qmi_ctx
     a.dmc   = dmc.on ? 1 : 0;
     a.rr    = dmc.on ? dmc_rr.p : nullptr;
     a.nanflag = nanflag.p;
+    const int P = partitions();
+    if (P > 1)
+    {
+      enqueue_partitioned(a, P, e0, e1, part, mark_counts);
+      return;
+    }
     for (int g = 0; g < 2; g++)
     {
       int iat        = std::max(s.first[g], e0);
@@ -2242,6 +2346,68 @@ struct qmi_ctx
       HIPCHECK(hipEventRecord(ev_ubase, stream));
     flush(0);
     flush(1);
+  }
+
+  // moves [e0, e1) of the sub-step (and, unless part == 1, the closing flushes) for P walker
+  // partitions, each on its own stream forked from the context stream; the uniform counts reach the
+  // host once every partition's moves are done, before the closing flushes
+  void enqueue_partitioned(const MoveArgs& a, int P, int e0, int e1, int part, bool mark_counts)
+  {
+    const SysC& s = sys;
+    ensure_partitions(P);
+    HIPCHECK(hipEventRecord(pev_fork, stream));
+    int dc[2] = {dcount[0], dcount[1]};
+    for (int p = 0; p < P; p++)
+    {
+      hipStream_t ps = pstream[p];
+      const int w0 = (int)((long long)p * nw / P), w1 = (int)((long long)(p + 1) * nw / P);
+      HIPCHECK(hipStreamWaitEvent(ps, pev_fork, 0));
+      MoveArgs b = a;
+      b.w0       = w0;
+      b.wn       = w1 - w0;
+      dc[0]      = dcount[0];
+      dc[1]      = dcount[1];
+      for (int g = 0; g < 2; g++)
+      {
+        const int last = std::min(s.first[g] + s.n[g], e1);
+        for (int iat = std::max(s.first[g], e0); iat < last; iat++)
+        {
+          b.g    = g;
+          b.d0   = s.k > 1 ? dc[g] : 0;
+          b.iat0 = iat;
+          b.iat1 = iat + 1;
+          HIPCHECK(cplx ? launch_move_c(MK_MOVE_ONE, s, st, coefs.p, b, ps)
+                        : launch_move(MK_MOVE_ONE, s, st, coefs.p, b, ps));
+          if (s.k > 1 && ++dc[g] == s.k)
+          {
+            flush_part(g, dc[g], w0, w1 - w0, ps);
+            dc[g] = 0;
+          }
+        }
+      }
+      HIPCHECK(hipEventRecord(pev_moves[p], ps));
+    }
+    dcount[0] = dc[0];
+    dcount[1] = dc[1];
+    invrow_id = -1;
+    if (part != 1)
+    {
+      for (int p = 0; p < P; p++)
+        HIPCHECK(hipStreamWaitEvent(stream, pev_moves[p], 0));
+      HIPCHECK(hipMemcpyAsync(h_ubase.p, ubase.p, ubase.n * sizeof(int), hipMemcpyDeviceToHost, stream));
+      if (mark_counts)
+        HIPCHECK(hipEventRecord(ev_ubase, stream));
+      for (int p = 0; p < P; p++)
+      {
+        const int w0 = (int)((long long)p * nw / P), w1 = (int)((long long)(p + 1) * nw / P);
+        flush_part(0, dcount[0], w0, w1 - w0, pstream[p]);
+        flush_part(1, dcount[1], w0, w1 - w0, pstream[p]);
+        HIPCHECK(hipEventRecord(pev_done[p], pstream[p]));
+      }
+      dcount[0] = dcount[1] = 0;
+    }
+    for (int p = 0; p < P; p++)
+      HIPCHECK(hipStreamWaitEvent(stream, part != 1 ? pev_done[p] : pev_moves[p], 0));
   }
 
   // after the sub-step's work has completed: rewind each crowd's generator to the uniforms used
@@ -2868,6 +3034,8 @@ extern "C"
       c->move_one = std::string(mk) != "segment";
     if (const char* lf = std::getenv("QMI_LEARN_FUSED"))
       c->fused_learn = std::string(lf) != "0";
+    if (const char* np = std::getenv("QMI_PARTITIONS"))
+      c->npart = std::max(1, std::min(8, std::atoi(np)));
     *out = c;
     return QMI_OK;
   }
@@ -3688,6 +3856,21 @@ extern "C"
         c->flush_variant = flush_variant;
         c->invalidate_graph();
       }
+    });
+  }
+
+  int qmi_tune_partitions(qmi_ctx* c, int partitions, int* current)
+  {
+    return guarded(c, [&] {
+      if (partitions > 8)
+        throw QmiError(QMI_ERR_ARG, "at most 8 walker partitions");
+      if (partitions > 0 && partitions != c->npart)
+      {
+        c->npart = partitions;
+        c->invalidate_graph();
+      }
+      if (current)
+        *current = c->npart;
     });
   }
 

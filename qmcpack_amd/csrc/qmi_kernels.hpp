@@ -27,6 +27,9 @@ struct MoveArgs
   int g;          // spin group of the moves
   int d0;         // delay count at iat0
   int nw, wpc;    // walkers, walkers per crowd
+  // walkers [w0, w0 + wn) of this launch (wn = 0: all nw); only the fused move kernels (k_move_one,
+  // k_sweep_segment, k_move_one_c) take a partition, the others run with w0 = 0
+  int w0, wn;
   int prepare;    // API ratio kernel: prepare the inverse row first
   double tau;
   const double* gauss; // [N][nw][3] standard normals of the sub-step

@@ -540,10 +540,9 @@ __device__ inline void fused_move(const SysC& s, const State& st, const ST* __re
     sh.bc[BC_ACC] = acc ? 1.0 : 0.0;
     if (a.dmc)
       dmc_record(a, iat, w, acc);
-    if (acc)
-      nacc += 1;
-    else
-      nrej += 1;
+    // arithmetic selects (a branch on acc made the pair a dynamically indexed private array)
+    nacc += acc ? 1.0 : 0.0;
+    nrej += acc ? 0.0 : 1.0;
     if (a.trace)
     {
       double* tr = a.trace + ((size_t)iat * a.nw + w) * TRACE_FIELDS;
@@ -570,7 +569,7 @@ __global__ __launch_bounds__(WG, 4) void k_sweep_segment(SysC s, State st, const
   extern __shared__ double sB_raw[];
   DT* sB = reinterpret_cast<DT*>(sB_raw);
   stage_jastrow(s, sh);
-  const int w = blockIdx.x;
+  const int w = a.w0 + blockIdx.x;
   if (w >= a.nw)
     return;
   const int N = s.N;
@@ -599,7 +598,7 @@ __global__ __launch_bounds__(WG) void k_move_one(SysC s, State st, const ST* __r
   extern __shared__ double sB_raw[];
   DT* sB = reinterpret_cast<DT*>(sB_raw);
   stage_jastrow(s, sh);
-  const int w = blockIdx.x;
+  const int w = a.w0 + blockIdx.x;
   if (w >= a.nw)
     return;
   const int N = s.N;
@@ -1686,7 +1685,11 @@ static hipError_t launch_move_t(int which, const SysC& s, const State& st, const
   const int maxn = move_maxm(s, a.g) * WG;
   if (maxn == 0)
     return hipErrorInvalidValue;
-  dim3 grid(a.nw), block(WG);
+  if ((a.w0 != 0 || a.wn != 0) && which != MK_SEGMENT && which != MK_MOVE_ONE)
+    return hipErrorInvalidValue; // only the fused kernels take a walker partition
+  if (a.w0 < 0 || a.wn < 0 || a.w0 + a.wn > a.nw)
+    return hipErrorInvalidValue;
+  dim3 grid(a.wn > 0 ? a.wn : a.nw), block(WG);
   const size_t dyn = binv_lds_bytes(s.k, sizeof(DT));
   if (maxn <= WG)
   {

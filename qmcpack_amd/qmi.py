@@ -502,6 +502,12 @@ class Context:
         self._check(self.L.qmi_tune_get(self.h, out.ctypes.data_as(_ip)))
         return dict(delay_rank=int(out[0]), move_kernel=int(out[1]), flush_variant=int(out[2]))
 
+    def tune_partitions(self, partitions=0):
+        """Walker partitions of the fused sweep (partitions > 0 sets them); returns the current count."""
+        cur = C.c_int()
+        self._check(self.L.qmi_tune_partitions(self.h, int(partitions), C.byref(cur)))
+        return cur.value
+
     def graph_enable(self, on=True):
         self._check(self.L.qmi_graph_enable(self.h, int(bool(on))))
 
