@@ -1,0 +1,164 @@
+"""The benchmark's configurations under the oracle, at their real sizes (3 walkers each, so that the
+CPU oracle finishes in seconds):
+
+* production, FP64 (1024 x 512-electron workload's system: 256 orbitals per spin on the 64^3
+  production table, 128 ions, L = 18, delay rank 32): trajectory replay against the FP64 oracle and
+  its long double determinant shadow;
+* mixed precision on that system: delay rank 64 (the FP32 matrix-core flush over four 16-column
+  tiles), FP32 / BF16 tables with the FP32 determinant engine and the FP32 table with the FP64 engine,
+  full FP64 re-inversion every 32 / 64 / 128 sweeps; per-quantity errors (ratio, gradient at the
+  current position, local energy, and log psi / G / L after the run) against the FP64 oracle on the
+  identical trajectory, both on the original FP64 table and on the table as the device stores it;
+* the learned proposal on that system in the FP32 mode: a zero output layer against the plain
+  oracle, and with a trained-like random network the acceptance probability and objective recomputed
+  from the traced ratios and network outputs.
+
+Errors are relative to max(|x|, floor): ratios floor 1e-2, everything else 1.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from oracle import learned as lp
+from oracle import oracle as orc
+from qmcpack_amd import qmi
+from qmcpack_amd.system import make_walkers, workload_system
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-10
+
+
+def relerr(a, b, floor=0.0):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)))
+
+
+@pytest.fixture(scope="module")
+def production():
+    s, cfg = workload_system("production")
+    return s, make_walkers(s, 3, cfg["seed"]), cfg
+
+
+def device_run(s, R, prec, steps, recompute_period, seed, learned=None):
+    c = qmi.Context(0, precision=prec)
+    c.system_create(s)
+    c.walkers_load(R, 1)
+    c.mw_evaluateLog()
+    if learned is not None:
+        c.mlp_params(learned)
+        c.proposal_config(learned=True, train=False)
+        E, tr, lt = c.vmc_run_learned(steps, 1, 0.3, recompute_period, seed, trace=True)
+    else:
+        E, tr = c.vmc_run(steps, 1, 0.3, recompute_period, seed, trace=True)
+        lt = None
+    lpsi, G, L, _ = c.mw_evaluateGL(False)
+    return c, E, tr, lt, (lpsi, G, L)
+
+
+def oracle_replay(s, R, table, tr, recompute_period, seed, exact=False):
+    o = orc.VMC(dataclasses.replace(s, coefs=table), R, 1, seed)
+    o.evaluate_log()
+    out = o.replay(tr, 1, 0.3, recompute_period, exact=exact)
+    return (o,) + tuple(out)
+
+
+def test_production_table_fp64_replay(production):
+    s, R, cfg = production
+    c, E, tr, _, _ = device_run(s, R, "fp64", 1, 0, 83)
+    o, Eo, tro, ex = oracle_replay(s, R, s.coefs, tr, 0, 83, exact=True)
+    err = dict(decisions=int(np.sum(tr[..., 1] != tro[..., 1])), ratio=relerr(tr[..., 0], tro[..., 0], 1e-2),
+               ratio_exact=relerr(tr[..., 0], ex[..., 0], 1e-2), oracle_exact=relerr(tro[..., 0], ex[..., 0], 1e-2),
+               grad=relerr(tr[..., 5:8], tro[..., 5:8], 1.0), energy=relerr(E, Eo, 1.0))
+    print("production table, FP64:", err)
+    # On this table no FP64 implementation stays within 1e-10 of the exact ratio near small ratios
+    # (the FP64 oracle is 8e-10 away): the device is held to the oracle's own distance from the
+    # exact reference, and to the oracle within 2e-9
+    assert err["decisions"] == 0
+    assert err["ratio_exact"] <= max(REL, 2 * err["oracle_exact"])
+    assert err["ratio"] < 2e-9 and err["grad"] < 1e-9 and err["energy"] < REL
+    assert np.array_equal(c.walkers_get(), o.get()["R"])
+
+
+@pytest.mark.parametrize("prec,period", [("fp32", 32), ("bf16", 64), ("t32", 128)])
+def test_mixed_precision_per_quantity(production, prec, period):
+    s, R, cfg = production
+    s = dataclasses.replace(s, delay_rank=64)
+    steps = period + 2
+    c, E, tr, _, (lpsi, G, L) = device_run(s, R, prec, steps, period, 84)
+    res = {}
+    for name, table in (("device table", orc.round_table(s.coefs, prec)), ("FP64 table", s.coefs)):
+        o, Eo, tro = oracle_replay(s, R, table, tr, period, 84)
+        lpo, Go, Lo = o.evaluate_log()
+        per_sweep = np.max(np.abs(tr[..., 0] - tro[..., 0]) / np.maximum(np.abs(tro[..., 0]), 1e-2),
+                           axis=(1, 2, 3))
+        res[name] = dict(ratio=relerr(tr[..., 0], tro[..., 0], 1e-2),
+                         ratio_median=float(np.median(np.abs(tr[..., 0] - tro[..., 0]) /
+                                                      np.maximum(np.abs(tro[..., 0]), 1e-2))),
+                         grad=relerr(tr[..., 5:8], tro[..., 5:8], 1.0), energy=relerr(E, Eo, 1.0),
+                         logpsi=relerr(lpsi, lpo, 1.0), G=relerr(G, Go, 1.0), L=relerr(L, Lo, 1.0),
+                         decisions=int(np.sum(tr[..., 1] != tro[..., 1])), moves=int(tr[..., 1].size),
+                         ratio_first_sweep=float(per_sweep[0]), ratio_before_reinversion=float(per_sweep[period - 1]),
+                         ratio_after_reinversion=float(per_sweep[period]))
+    print(f"{prec}, delay rank 64, re-inversion every {period} sweeps:")
+    for name, e in res.items():
+        print(f"  vs FP64 oracle on the {name}: " + ", ".join(
+            f"{k} {v:.2e}" if isinstance(v, float) else f"{k} {v}" for k, v in e.items()))
+    same, full = res["device table"], res["FP64 table"]
+    if prec == "t32":
+        # FP64 engine: FP64 parity on the device's table (the largest ratio difference over the 2e5
+        # moves is FP64 rounding near small ratios at these condition numbers, median 5e-13); the
+        # FP32 table itself moves ratios by a median 2e-7 relative
+        assert same["decisions"] == 0 and same["ratio"] < 2e-8 and same["grad"] < 1e-9
+        assert same["energy"] < REL
+        assert full["ratio"] < 1e-3 and full["ratio_median"] < 1e-5 and full["energy"] < 1e-6
+    else:
+        # FP32 engine: the FP32 inverse carries cond(A) x 6e-8 per ratio; the re-inversion keeps it
+        # from growing over the run
+        assert same["decisions"] <= same["moves"] // 200
+        assert same["energy"] < 1e-3 and same["logpsi"] < 1e-4
+        assert same["ratio_after_reinversion"] <= max(10 * same["ratio_first_sweep"], 1e-3)
+
+
+def test_learned_fp32_production(production):
+    s, R, cfg = production
+    s = dataclasses.replace(s, delay_rank=64)
+    rng = np.random.default_rng(85)
+    zero = lp.init_params(rng, zero_head=True)
+    flatp = lambda P: np.concatenate([P[k].ravel() for k in ("W1", "b1", "W2", "b2", "W3", "b3", "W4", "b4")]).astype(np.float32)
+    # zero output layer: plain drift-diffusion in the FP32 mode, against the plain FP64 oracle on the
+    # FP32 table following the same trajectory (the FP32 engine's accuracy)
+    _, E0, tr0, lt0, _ = device_run(s, R, "fp32", 1, 0, 86, learned=flatp(zero))
+    assert np.all(lt0[..., 0:8] == 0)
+    _, Eo, tro = oracle_replay(s, R, orc.round_table(s.coefs, "fp32"), tr0, 0, 86)
+    dec = int(np.sum(tr0[..., 1] != tro[..., 1]))
+    rerr = relerr(tr0[..., 0], tro[..., 0], 1e-2)
+    print(f"learned zero head, FP32 engine vs FP64 oracle: decisions differing {dec} of {tr0[..., 1].size}, "
+          f"ratio {rerr:.2e}, energy {relerr(E0, Eo, 1.0):.2e}")
+    assert dec <= tr0[..., 1].size // 200
+    # a random network: acceptance probability and objective from the traced quantities
+    P = lp.init_params(rng, zero_head=False, scale=0.5)
+    _, E, tr, lt, _ = device_run(s, R, "fp32", 1, 0, 87, learned=flatp(P))
+    tau = 0.3
+    r0 = R.copy()
+    r = np.empty(tr.shape[:-1] + (3,))
+    for i in range(s.N):  # positions before each move
+        r[0, 0, i] = r0[:, i]
+        acc = tr[0, 0, i, :, 1] != 0
+        r0[acc, i] = tr[0, 0, i, acc, 2:5]
+    rp = tr[..., 2:5]
+    v = np.apply_along_axis(lambda g: lp.unr_drift(tau, g), -1, tr[..., 5:8])
+    mu, s_f, mu_r, s_r, vr = lt[..., 0:3], lt[..., 3], lt[..., 4:7], lt[..., 7], lt[..., 12:15]
+    xi = (rp - r - v - mu) / (np.sqrt(tau) * np.exp(s_f))[..., None]
+    lqf = -0.5 * np.sum(xi * xi, -1) - 3 * s_f
+    e = r - (rp + vr + mu_r)
+    lqr = -np.sum(e * e, -1) / (2 * tau * np.exp(2 * s_r)) - 3 * s_r
+    prob = tr[..., 0] ** 2 * np.exp(lqr - lqf)
+    perr = np.max(np.abs(prob - lt[..., 10]) / np.maximum(prob, 1e-3))
+    f = np.minimum(1.0, prob) * np.sum((rp - r) ** 2, -1)
+    ferr = np.max(np.abs(f - lt[..., 11]) / np.maximum(np.abs(f), 1e-3))
+    print(f"learned random network, FP32: acceptance probability {perr:.2e}, objective {ferr:.2e}, "
+          f"mean acceptance {np.mean(tr[..., 1]):.3f}")
+    assert perr < 1e-6 and ferr < 1e-6
+    assert np.all(tr[..., 1][lt[..., 10] >= 1.0] == 1)
