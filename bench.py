@@ -77,7 +77,7 @@ def flush_flops(n, k):
 
 def measured_traffic(kernel_prefix="qmi::k_sweep_segment"):
     """HBM bytes per dispatch of the dominant kernel from the committed rocprofv3 counter summary
-    (FETCH_SIZE and WRITE_SIZE runs, calibrated against a copy kernel of known size)."""
+    (read and write requests by size, TCC_EA0_RDREQ_{32,64,128}B and TCC_EA0_WRREQ{,_64B})."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_config1_fp64.json")
     if not os.path.exists(path):
         return None, None
