@@ -232,6 +232,20 @@ int qmi_mw_evaluate_ratios(qmi_ctx* ctx, int iat, int nvp, const double* pos, do
  * (the reference's LR_dim_cutoff, 15 by default).  Periodic orthorhombic cells; tables built once
  * per system. */
 int qmi_coulomb_setup(qmi_ctx* ctx, double ion_charge, double dim_cutoff);
+/* Long-range handlers of the periodic Coulomb sum (the lattice input's LR_handler,
+ * Particle/ParticleIO/LatticeIO.cpp:123-143; LRCoulombSingleton::getHandler,
+ * Particle/LongRange/LRCoulombSingleton.cpp:58-97) */
+#define QMI_LR_EWALD 0       /* "ewald": EwaldHandler3D */
+#define QMI_LR_OPT_BREAKUP 1 /* "opt_breakup", the reference's default: LRHandlerTemp<CoulombFunctor,
+                              * LPQHIBasis>, the optimized breakup fitted on 15 knots */
+/* qmi_coulomb_setup with the long-range handler chosen (QMI_LR_*); qmi_coulomb_setup is
+ * qmi_coulomb_setup_handler(..., QMI_LR_EWALD). */
+int qmi_coulomb_setup_handler(qmi_ctx* ctx, double ion_charge, double dim_cutoff, int lr_handler);
+/* The optimized breakup of an orthorhombic cell with sides L[3] (host computation, no device
+ * work and no context): the 45 basis coefficients (LRHandlerTemp::coefs after InitBreakup,
+ * LRHandlerTemp.h:195-227, 3 per knot: value, slope, curvature) into coefs[ncoefs >= 45] and
+ * info[4] = {r_c, v_l(r = 0) (evaluateLR_r0), v_s(k = 0) (evaluateSR_k0), chi^2 of the fit}. */
+int qmi_lr_breakup(const double* L, double dim_cutoff, double* coefs, int ncoefs, double* info);
 /* Non-local pseudopotential energy of one ion species: NonLocalECPotential::evaluate
  * (QMCHamiltonians/NonLocalECPotential.cpp:200-228) with NonLocalECPComponent::evaluateOne
  * (NonLocalECPComponent.cpp:109-166, 826-888).  Channel c has angular momentum l[c] and radial

@@ -1,6 +1,7 @@
 // Host side of the qmcmi C ABI: context, device memory, the multi-walker API and the
 // device-resident VMC driver.  All device work of a context runs on one HIP stream.
 #include "../../include/qmcmi.h"
+#include "qmi_breakup.hpp"
 #include "qmi_checkpoint.hpp"
 #include "qmi_kernels.hpp"
 #include "qmi_rng.hpp"
@@ -871,20 +872,31 @@ struct qmi_ctx
       y2[i] = d[i] - c[i] * y2[i + 1];
   }
 
-  void coulomb_setup(double zion, double dim_cutoff)
+  // handler QMI_LR_EWALD: erfc / Gaussian split (EwaldHandler3D); QMI_LR_OPT_BREAKUP: the fitted
+  // LPQHI split (LRHandlerTemp<CoulombFunctor, LPQHIBasis>, built by lr_breakup). Both give the
+  // device the same tables: F_k per k vector, r v_sr(r) on the spline grid, the constants.
+  void coulomb_setup(double zion, double dim_cutoff, int handler = QMI_LR_EWALD)
   {
     if (!have_system)
       throw QmiError(QMI_ERR_ARG, "system not created");
     const SysC& s = sys;
     if (!s.periodic[0] || !s.periodic[1] || !s.periodic[2])
-      throw QmiError(QMI_ERR_UNSUPPORTED, "the Ewald energy needs a periodic cell");
+      throw QmiError(QMI_ERR_UNSUPPORTED, "the periodic Coulomb energy needs a periodic cell");
     if (!(dim_cutoff > 0))
       throw QmiError(QMI_ERR_ARG, "dim_cutoff must be positive");
+    if (handler != QMI_LR_EWALD && handler != QMI_LR_OPT_BREAKUP)
+      throw QmiError(QMI_ERR_ARG, "unknown long-range handler");
+    const bool opt   = handler == QMI_LR_OPT_BREAKUP;
     const double* L  = s.L;
     const double rc  = 0.5 * std::min(L[0], std::min(L[1], L[2]));
     const double kc  = dim_cutoff / rc;
     const double sig = std::sqrt(kc / (2.0 * rc));
     const double vol = L[0] * L[1] * L[2];
+    LrBreakup brk;
+    if (opt)
+      brk = lr_breakup(L, dim_cutoff);
+    // |k|^2 bound of the long-range sum (the fitted split keeps at least the first shell)
+    const double kc2 = opt ? brk.kc2_lr : kc * kc;
     struct K
     {
       double x, y, z, k2;
@@ -893,7 +905,7 @@ struct qmi_ctx
     std::vector<K> ks;
     int mmax[3];
     for (int d = 0; d < 3; d++)
-      mmax[d] = (int)std::floor(L[d] * kc / (2 * M_PI)) + 1;
+      mmax[d] = (int)std::floor(L[d] * std::sqrt(kc2) / (2 * M_PI)) + 1;
     for (int i = -mmax[0]; i <= mmax[0]; i++)
       for (int j = -mmax[1]; j <= mmax[1]; j++)
         for (int k = -mmax[2]; k <= mmax[2]; k++)
@@ -902,7 +914,7 @@ struct qmi_ctx
             continue;
           const double x = 2 * M_PI * i / L[0], y = 2 * M_PI * j / L[1], z = 2 * M_PI * k / L[2];
           const double k2 = x * x + y * y + z * z;
-          if (k2 <= kc * kc)
+          if (k2 <= kc2)
             ks.push_back({x, y, z, k2, i, j, k});
         }
     std::stable_sort(ks.begin(), ks.end(),
@@ -920,7 +932,7 @@ struct qmi_ctx
         if (kk != key)
         {
           key    = kk;
-          fshell = knorm * std::exp(-ks[k].k2 * kgauss) / ks[k].k2;
+          fshell = opt ? brk.fk(std::sqrt(ks[k].k2)) : knorm * std::exp(-ks[k].k2 * kgauss) / ks[k].k2;
         }
         fall[k] = fshell;
       }
@@ -957,7 +969,7 @@ struct qmi_ctx
     for (int i = 0; i < ng; i++)
       gx[i] = i * dr;
     for (int i = 1; i < ng - 1; i++)
-      gy[i] = gx[i] * (std::erfc(sig * gx[i]) / gx[i]);
+      gy[i] = opt ? brk.rvsr(gx[i]) : gx[i] * (std::erfc(sig * gx[i]) / gx[i]);
     gy[0]      = 2.0 * gy[1] - gy[2];
     gy[ng - 1] = 0.0;
     clamped_spline(gx, gy, (gy[1] - gy[0]) / dr, 0.0, gy2);
@@ -970,7 +982,8 @@ struct qmi_ctx
       const double q1 = cr * (cr * cr - 1.0) * h6 * dr, q2 = cl * (cl * cl - 1.0) * h6 * dr;
       return (cr * gy[loc] + cl * gy[loc + 1] + q1 * gy2[loc] + q2 * gy2[loc + 1]) / r;
     };
-    const double vlr0 = 2.0 * sig / std::sqrt(M_PI), vsk0 = M_PI / (sig * sig * vol);
+    const double vlr0 = opt ? brk.vlr_r0() : 2.0 * sig / std::sqrt(M_PI);
+    const double vsk0 = opt ? brk.vsr_k0() : M_PI / (sig * sig * vol);
     const int N = s.N, ni = s.nion;
     // ion-ion energy (CoulombPBCAA on the ions)
     double vii = 0;
@@ -3425,6 +3438,31 @@ extern "C"
   int qmi_coulomb_setup(qmi_ctx* c, double ion_charge, double dim_cutoff)
   {
     return guarded(c, [&] { c->coulomb_setup(ion_charge, dim_cutoff); });
+  }
+
+  int qmi_coulomb_setup_handler(qmi_ctx* c, double ion_charge, double dim_cutoff, int lr_handler)
+  {
+    return guarded(c, [&] { c->coulomb_setup(ion_charge, dim_cutoff, lr_handler); });
+  }
+
+  int qmi_lr_breakup(const double* L, double dim_cutoff, double* coefs, int ncoefs, double* info)
+  {
+    if (!L || !coefs || !info || ncoefs < 45)
+      return QMI_ERR_ARG;
+    try
+    {
+      const qmi::LrBreakup b = qmi::lr_breakup(L, dim_cutoff);
+      std::copy(b.t.begin(), b.t.end(), coefs);
+      info[0] = b.rc;
+      info[1] = b.vlr_r0();
+      info[2] = b.vsr_k0();
+      info[3] = b.chi2;
+      return QMI_OK;
+    }
+    catch (const std::exception&)
+    {
+      return QMI_ERR_ARG;
+    }
   }
 
   int qmi_nlpp_setup(qmi_ctx* c, int nchannel, const int* l, int ngrid, double dr, const double* v,

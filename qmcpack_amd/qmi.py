@@ -70,6 +70,22 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     return L
 
 
+LR_HANDLERS = {"ewald": 0, "opt_breakup": 1}
+
+
+def lr_breakup(L, dim_cutoff=15.0):
+    """The optimized breakup of an orthorhombic cell (host computation, no device): the 45 basis
+    coefficients and {r_c, v_l(r=0), v_s(k=0), chi^2} (LRHandlerTemp::InitBreakup)."""
+    lib = load_library()
+    Lv = np.ascontiguousarray(L, dtype=np.float64)
+    t = np.zeros(45)
+    info = np.zeros(4)
+    code = lib.qmi_lr_breakup(_p(Lv), C.c_double(dim_cutoff), _p(t), C.c_int(45), _p(info))
+    if code != 0:
+        raise QmiError(code, "qmi_lr_breakup: invalid cell or cutoff")
+    return t, dict(rc=info[0], vlr_r0=info[1], vsr_k0=info[2], chi2=info[3])
+
+
 def exported_symbols_from_header(header: str = HEADER) -> list[str]:
     import re
     txt = open(header).read()
@@ -346,8 +362,12 @@ class Context:
         self._check(self.L.qmi_mw_get_inverse(self.h, int(g), _p(out)))
         return self._out(out)
 
-    def coulomb_setup(self, ion_charge, dim_cutoff=15.0):
-        self._check(self.L.qmi_coulomb_setup(self.h, C.c_double(ion_charge), C.c_double(dim_cutoff)))
+    def coulomb_setup(self, ion_charge, dim_cutoff=15.0, handler="ewald"):
+        """Periodic Coulomb tables; handler is the lattice input's LR_handler: "ewald"
+        (EwaldHandler3D) or "opt_breakup" (the optimized LPQHI breakup, the reference's default)."""
+        code = LR_HANDLERS[handler]
+        self._check(self.L.qmi_coulomb_setup_handler(self.h, C.c_double(ion_charge), C.c_double(dim_cutoff),
+                                                     C.c_int(code)))
 
     # ---------------------------------------------------------------- pseudopotential
     def nlpp_setup(self, ls, dr, v, v2, rcut, knots, weights):
@@ -410,7 +430,7 @@ class Context:
         return int(self.L.qmi_coulomb_nk(self.h))
 
     def mw_evaluatePotential(self):
-        """[nw][3]: electron-electron, electron-ion, ion-ion Coulomb energy (Ewald)."""
+        """[nw][3]: electron-electron, electron-ion, ion-ion Coulomb energy (the handler of coulomb_setup)."""
         v = np.empty((self.nw, 3))
         self._check(self.L.qmi_mw_evaluate_potential(self.h, _p(v)))
         return v
