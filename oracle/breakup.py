@@ -16,7 +16,9 @@ Functions and the reference code they follow:
                                    SVD with singular values below eps * S_max dropped)
   fk, vlr_r0, vsr_k0, rvsr         LRHandlerTemp.h:98-108, 153-167, 172-178 (evaluate,
                                    evaluateSR_k0, evaluateLR_r0, evalFk), LRCoulombSingleton.cpp
-                                   :32-56 (CoulombFunctor: Fk, Xk, integrate_r2)
+                                   :32-56 (CoulombFunctor: Fk, Xk, integrate_r2); F_k below
+                                   k delta = 2.5 by quadrature (lpqhi_c_quadrature), where the
+                                   reference's recursion loses up to 1e-9
 
 Pins (tests/test_oracle_breakup.py): the reference's known answers for CoulombPBCAA with this
 breakup (test_coulomb_pbcAA.cpp:52-163: constants and energies of one charge in a unit cube and two
@@ -119,6 +121,27 @@ def lpqhi_c(b: Basis, k):
     return out
 
 
+_GL_X, _GL_W = np.polynomial.legendre.leggauss(40)
+
+
+def lpqhi_c_quadrature(b: Basis, k):
+    """c[n, k] by 40-point Gauss-Legendre on each polynomial piece, for k delta below ~2.5 (the
+    long-range sum's k <= k_c): there each step of the E+ / E- recursion divides by k delta and the
+    first shells lose up to 1e-9 relative; the integrand r sin(k r) h_n(r) is analytic on a piece
+    spanning under 2.5 radians, where 40 nodes are exact to rounding."""
+    k = np.atleast_1d(np.asarray(k, float))
+    out = np.zeros((b.size, k.size))
+    for n in range(b.size):
+        i = n // 3
+        for lo, hi in ((max(0.0, b.delta * (i - 1)), b.delta * i), (b.delta * i, min(b.rc, b.delta * (i + 1)))):
+            if hi <= lo:
+                continue
+            r = 0.5 * (hi - lo) * _GL_X + 0.5 * (hi + lo)
+            f = r * lpqhi_h(b, n, r) * (0.5 * (hi - lo) * _GL_W)
+            out[n] += np.sin(np.outer(k, r)) @ f
+    return out * (4.0 * np.pi / (k * b.volume))
+
+
 def lattice_kpoints(L, kcut):
     """Every nonzero k = 2 pi m / L with |k| <= kcut, grouped into shells of equal |k|^2 (shell key
     floor(|k|^2 * 1e7), KContainer.cpp:110-117, 150-176, 226-265): (vectors [nk, 3] in shell order,
@@ -195,9 +218,15 @@ class Breakup:
 
     def fk(self, k):
         """F_k of the long-range sum: the transform of V_l (LRHandlerTemp.h:253-259 evalFk)."""
-        k = np.asarray(k, float)
+        k = np.atleast_1d(np.asarray(k, float))
         b = self.basis
-        return 4 * np.pi / (b.volume * k * k) * np.cos(k * self.rc) + self.t @ lpqhi_c(b, k)
+        small = k * b.delta < 2.5
+        c = np.empty((b.size, k.size))
+        if np.any(small):
+            c[:, small] = lpqhi_c_quadrature(b, k[small])
+        if np.any(~small):
+            c[:, ~small] = lpqhi_c(b, k[~small])
+        return 4 * np.pi / (b.volume * k * k) * np.cos(k * self.rc) + self.t @ c
 
     def vlr_r0(self):
         return float(sum(self.t[n] * lpqhi_h(self.basis, n, 0.0) for n in range(self.basis.size)))

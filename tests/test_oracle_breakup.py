@@ -98,6 +98,12 @@ def test_fourier_coefficients_and_r2_integrals(unit_cube):
                       for a0, a1 in ((lo, b.delta * i), (b.delta * i, hi)) if a1 > a0)
             ref = 4 * np.pi / (k * b.volume) * val
             assert abs(c[n, q] - ref) <= 1e-9 * np.max(np.abs(c[:, q])), (n, k)
+        cq = bk.lpqhi_c_quadrature(b, ks[:2])
+        for q, k in enumerate(ks[:2]):
+            f = lambda r: r * np.sin(k * r) * bk.lpqhi_h(b, n, r)
+            val = sum(integrate.quad(f, a0, a1, limit=400, epsabs=1e-16, epsrel=1e-13)[0]
+                      for a0, a1 in ((lo, b.delta * i), (b.delta * i, hi)) if a1 > a0)
+            assert abs(cq[n, q] - 4 * np.pi / (k * b.volume) * val) <= 1e-13 * np.max(np.abs(cq[:, q]))
         r2 = sum(integrate.quad(lambda r: r * r * bk.lpqhi_h(b, n, r), a0, a1, epsrel=1e-14)[0]
                  for a0, a1 in ((lo, b.delta * i), (b.delta * i, hi)) if a1 > a0)
         assert abs(bk.lpqhi_hintr2(b, n) - r2) < 1e-14
