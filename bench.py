@@ -192,6 +192,28 @@ def chain_summary(E, dt, nw, N, steps):
                 effective_samples_per_s=nw * steps / kappa / dt, energy=emean, energy_err=eerr)
 
 
+def pick_launch(ctx, tau, seed):
+    """Launch form of the plain sweep, timed over 3 untimed sweeps each: one captured graph per
+    sub-step with all walkers in one launch per move, or eager launches with the walkers in two
+    partitions on two streams (their launches overlap, which hides the tail of each move's launch:
+    the workgroups dispatched last on a compute unit finish last).  Same arithmetic either way."""
+    modes = ([("graph", 1)] if GRAPH else [("eager", 1)]) + [("eager", 2)]
+    best = None
+    for mode, P in modes:
+        ctx.tune_partitions(P)
+        ctx.graph_enable(mode == "graph")
+        ctx.vmc_run(1, 1, tau, 0, seed + 3)
+        t1 = time.perf_counter()
+        ctx.vmc_run(3, 1, tau, 0, seed + 4)
+        dt = time.perf_counter() - t1
+        if best is None or dt < best[0]:
+            best = (dt, mode, P)
+    _, mode, P = best
+    ctx.tune_partitions(P)
+    ctx.graph_enable(mode == "graph")
+    return mode if P == 1 else f"{mode}, {P} walker partitions on {P} streams"
+
+
 def run_variant(qmi, system, R, cfg, precision, learned, steps, warmup, train_sweeps, lr, log):
     """One context: optional training of the learned proposal, warmup, then `steps` timed sweeps."""
     nw, N, tau, seed = R.shape[0], system.N, cfg["tau"], cfg["seed"]
@@ -199,8 +221,9 @@ def run_variant(qmi, system, R, cfg, precision, learned, steps, warmup, train_sw
     ctx.system_create(system)
     ctx.walkers_load(R, 1)
     ctx.mw_evaluateLog()
-    if GRAPH and not learned:
-        ctx.graph_enable(True)
+    launch = "graph" if GRAPH else "eager"
+    if not learned:
+        launch = pick_launch(ctx, tau, seed)
     extra = {}
     if learned:
         ctx.proposal_config(learned=True, train=True, lr=lr, init_seed=7)
@@ -224,6 +247,7 @@ def run_variant(qmi, system, R, cfg, precision, learned, steps, warmup, train_sw
     else:
         acc = ctx.vmc_stats()
         out.update(mean_acceptance=acc["accepted"] / max(1, acc["accepted"] + acc["rejected"]))
+    out["launch"] = launch if not learned else "eager"
     log(f"[bench] {precision}{' + learned' if learned else ''}: " + json.dumps(out))
     return ctx, out
 
@@ -368,7 +392,7 @@ def main():
                     help="storage precision of the headline run: fp64 (the plain form), fp32 (FP32 table "
                          "and determinant engine), bf16 (BF16 table, FP32 determinant engine), t32 / tbf16 "
                          "(FP32 / BF16 table, FP64 determinant engine)")
-    ap.add_argument("--variants", default="fp32,t32,learned:fp32,learned:t32",
+    ap.add_argument("--variants", default="fp32,bf16,t32,learned:fp32,learned:bf16,learned:t32",
                     help="comma-separated extra runs (precision, or learned:<precision>); empty for none")
     ap.add_argument("--train-sweeps", type=int, default=2,
                     help="training sweeps of the learned proposal (2 x 1024 x 512 = 1.05e6 moves)")
@@ -418,8 +442,10 @@ def main():
     value = head["moves_per_s"]
     tau, seed = cfg["tau"], cfg["seed"]
 
-    # per-kernel device time (HIP events on the library stream; eager launches), a separate short pass
+    # per-kernel device time (HIP events on the library stream; eager launches of all walkers), a
+    # separate short pass
     ctx.graph_enable(False)
+    ctx.tune_partitions(1)
     ctx.timing_enable(True)
     ctx.timing_reset()
     nprof = 2
@@ -495,7 +521,7 @@ def main():
                         "on a 64^3 periodic B-spline table, 128 ions, L=18 bohr, J1+J2 B-spline Jastrows, "
                         "delay rank 32, tau 0.3, one RNG stream per walker",
             "walkers": nw, "electrons": N, "orbitals_per_spin": n, "grid": 64, "ions": nion,
-            "delay_rank": k, "precision": args.precision,
+            "delay_rank": k, "precision": args.precision, "launch": head["launch"],
         },
         "effective_samples_per_s": head["effective_samples_per_s"],
         "kappa": head["kappa"],
