@@ -232,6 +232,15 @@ int qmi_mw_evaluate_ratios(qmi_ctx* ctx, int iat, int nvp, const double* pos, do
  * (the reference's LR_dim_cutoff, 15 by default).  Periodic orthorhombic cells; tables built once
  * per system. */
 int qmi_coulomb_setup(qmi_ctx* ctx, double ion_charge, double dim_cutoff);
+/* Cell vectors of the orbitals' lattice, R[3][3] row-major (rows are the vectors; the
+ * reference's CrystalLattice R, Lattice/CrystalLattice.h, with G = R^-1 used by SplineR2R /
+ * SplineC2C to reduce positions and to map orbital gradients and Hessians,
+ * SplineR2R.cpp:338-374).  Replaces the diagonal cell of the system description.  A cell with
+ * off-diagonal vectors serves the orbital paths only (qmi_mw_spo_*, determinant ratios and
+ * moves without Jastrow factors): with Jastrow functors in the system, or after
+ * qmi_coulomb_setup / qmi_nlpp_setup / qmi_dt_create, it returns QMI_ERR_UNSUPPORTED, and those
+ * calls refuse it afterwards (their minimum images are orthorhombic). */
+int qmi_system_set_lattice(qmi_ctx* ctx, const double* R);
 /* Long-range handlers of the periodic Coulomb sum (the lattice input's LR_handler,
  * Particle/ParticleIO/LatticeIO.cpp:123-143; LRCoulombSingleton::getHandler,
  * Particle/LongRange/LRCoulombSingleton.cpp:58-97) */

@@ -162,6 +162,7 @@ struct qmi_ctx
   hipStream_t stream = nullptr;
   std::string err;
   bool have_system = false;
+  bool skew_cell   = false; // qmi_system_set_lattice gave a cell with off-diagonal vectors
   bool cplx        = false; // complex (C2C) orbitals
   SysC sys{};
   DevBuf<double> coefs, ions, kpts;
@@ -408,10 +409,66 @@ struct qmi_ctx
   }
 
   // ---------------------------------------------------------------------------------------------
+  // Cell vectors R (rows) of the orbitals' lattice: reduced coordinates u = r R^-1, orbital
+  // gradients G g and Laplacians sum_ij H_ij (G^T G)_ij with G = R^-1 (CrystalLattice::toUnit,
+  // SplineR2R.cpp:338-374 assign_vgl).  A cell with off-diagonal vectors serves the orbitals only:
+  // the Jastrow, Coulomb, pseudopotential and distance-table paths take minimum images in an
+  // orthorhombic cell and refuse it.
+  void set_lattice(const double* R)
+  {
+    if (!have_system)
+      throw QmiError(QMI_ERR_ARG, "system not created");
+    if (!R)
+      throw QmiError(QMI_ERR_ARG, "null lattice");
+    const double det = R[0] * (R[4] * R[8] - R[5] * R[7]) - R[1] * (R[3] * R[8] - R[5] * R[6]) +
+        R[2] * (R[3] * R[7] - R[4] * R[6]);
+    if (!(std::fabs(det) > 0) || !std::isfinite(det))
+      throw QmiError(QMI_ERR_ARG, "singular lattice");
+    bool diag = true;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++)
+        if (i != j && R[i * 3 + j] != 0.0)
+          diag = false;
+    if (!diag && (sys.has_j1 || sys.has_j2))
+      throw QmiError(QMI_ERR_UNSUPPORTED, "Jastrow distances are built for orthorhombic cells");
+    if (!diag && (coul_ready || nlpp_ready || !dtables.empty()))
+      throw QmiError(QMI_ERR_UNSUPPORTED, "Coulomb, pseudopotential and distance tables need an orthorhombic cell");
+    if (diag && !(R[0] > 0 && R[4] > 0 && R[8] > 0))
+      throw QmiError(QMI_ERR_ARG, "diagonal lattice needs positive sides");
+    // G = R^-1 by cofactors
+    double G[9];
+    G[0] = (R[4] * R[8] - R[5] * R[7]) / det;
+    G[1] = (R[2] * R[7] - R[1] * R[8]) / det;
+    G[2] = (R[1] * R[5] - R[2] * R[4]) / det;
+    G[3] = (R[5] * R[6] - R[3] * R[8]) / det;
+    G[4] = (R[0] * R[8] - R[2] * R[6]) / det;
+    G[5] = (R[2] * R[3] - R[0] * R[5]) / det;
+    G[6] = (R[3] * R[7] - R[4] * R[6]) / det;
+    G[7] = (R[1] * R[6] - R[0] * R[7]) / det;
+    G[8] = (R[0] * R[4] - R[1] * R[3]) / det;
+    double GtG[9];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++)
+        GtG[i * 3 + j] = G[0 * 3 + i] * G[0 * 3 + j] + G[1 * 3 + i] * G[1 * 3 + j] + G[2 * 3 + i] * G[2 * 3 + j];
+    std::copy(G, G + 9, sys.G);
+    const double symGG[6] = {GtG[0], GtG[1] + GtG[3], GtG[2] + GtG[6], GtG[4], GtG[5] + GtG[7], GtG[8]};
+    std::copy(symGG, symGG + 6, sys.symGG);
+    if (diag)
+      for (int i = 0; i < 3; i++)
+      {
+        sys.periodic[i] = 1;
+        sys.L[i]        = R[i * 4];
+        sys.Linv[i]     = 1.0 / R[i * 4];
+      }
+    skew_cell = !diag;
+    invalidate_graph();
+  }
+
   void system_create(const qmi_system_desc* d)
   {
     if (!d)
       throw QmiError(QMI_ERR_ARG, "null system description");
+    skew_cell = false;
     SysC s{};
     s.nup  = d->n_up;
     s.ndn  = d->n_dn;
@@ -880,8 +937,8 @@ struct qmi_ctx
     if (!have_system)
       throw QmiError(QMI_ERR_ARG, "system not created");
     const SysC& s = sys;
-    if (!s.periodic[0] || !s.periodic[1] || !s.periodic[2])
-      throw QmiError(QMI_ERR_UNSUPPORTED, "the periodic Coulomb energy needs a periodic cell");
+    if (!s.periodic[0] || !s.periodic[1] || !s.periodic[2] || skew_cell)
+      throw QmiError(QMI_ERR_UNSUPPORTED, "the periodic Coulomb energy needs a periodic orthorhombic cell");
     if (!(dim_cutoff > 0))
       throw QmiError(QMI_ERR_ARG, "dim_cutoff must be positive");
     if (handler != QMI_LR_EWALD && handler != QMI_LR_OPT_BREAKUP)
@@ -1058,6 +1115,8 @@ struct qmi_ctx
   {
     if (!have_system)
       throw QmiError(QMI_ERR_ARG, "system not created");
+    if (skew_cell)
+      throw QmiError(QMI_ERR_UNSUPPORTED, "the pseudopotential energy needs an orthorhombic cell");
     if (nch < 1 || nch > NLPP_MAX_CH || !ls || ng < 2 || !(dr > 0) || !y || !y2 || !(rcut > 0) ||
         rcut > (ng - 1) * dr || nknot < 1 || nknot > NLPP_MAX_KNOTS || !knots || !weights)
       throw QmiError(QMI_ERR_ARG, "invalid pseudopotential tables (1..4 channels, r_cut inside the grid, 1..64 points)");
@@ -3442,6 +3501,11 @@ extern "C"
     return guarded(c, [&] { c->coulomb_setup(ion_charge, dim_cutoff); });
   }
 
+  int qmi_system_set_lattice(qmi_ctx* c, const double* R)
+  {
+    return guarded(c, [&] { c->set_lattice(R); });
+  }
+
   int qmi_coulomb_setup_handler(qmi_ctx* c, double ion_charge, double dim_cutoff, int lr_handler)
   {
     return guarded(c, [&] { c->coulomb_setup(ion_charge, dim_cutoff, lr_handler); });
@@ -3592,6 +3656,8 @@ extern "C"
         throw QmiError(QMI_ERR_ARG, "kind must be 0 (electron pairs) or 1 (electron-ion), out non-null");
       if (kind == DT_AB && c->sys.nion <= 0)
         throw QmiError(QMI_ERR_ARG, "no ions for an electron-ion table");
+      if (c->skew_cell)
+        throw QmiError(QMI_ERR_UNSUPPORTED, "distance tables need an orthorhombic cell");
       auto t    = std::make_unique<qmi_dtable>();
       t->c      = c;
       t->device = c->device;

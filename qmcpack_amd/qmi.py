@@ -362,6 +362,12 @@ class Context:
         self._check(self.L.qmi_mw_get_inverse(self.h, int(g), _p(out)))
         return self._out(out)
 
+    def system_set_lattice(self, R):
+        """Cell vectors (rows of R) of the orbitals' lattice; a skewed cell serves the orbital paths
+        only (no Jastrow factors, Coulomb, pseudopotential or distance tables)."""
+        Rv = np.ascontiguousarray(R, dtype=np.float64).reshape(3, 3)
+        self._check(self.L.qmi_system_set_lattice(self.h, _p(Rv)))
+
     def coulomb_setup(self, ion_charge, dim_cutoff=15.0, handler="ewald"):
         """Periodic Coulomb tables; handler is the lattice input's LR_handler: "ewald"
         (EwaldHandler3D) or "opt_breakup" (the optimized LPQHI breakup, the reference's default)."""
