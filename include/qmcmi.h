@@ -389,8 +389,8 @@ int qmi_tune_get(qmi_ctx* ctx, int* out3);
  * (1..8); the fused single-move path then runs the moves of P walker ranges on P streams forked from
  * the context's stream (at least 64 walkers per range), so that launches of different ranges overlap
  * on the compute units.  Results are unchanged.  *current (nullable) receives P.  Partitioned
- * sub-steps are launched eagerly, not from a captured graph (the graph's parallel branches replay
- * in sequence). */
+ * sub-steps are launched eagerly, not from a captured graph (replayed, the partitions' chains did
+ * not overlap). */
 int qmi_tune_partitions(qmi_ctx* ctx, int partitions, int* current);
 /* on = 1: qmi_vmc_run replays each sub-step from a captured graph (runs without trace, with one
  * walker partition) */

@@ -2575,8 +2575,8 @@ struct qmi_ctx
       ltrace_on = hltrace != nullptr;
     }
     std::vector<double> ek(nw);
-    // a partitioned sub-step runs eagerly: the forked branches of a captured graph replay one after
-    // the other (measured at 1024 x 512: 56 ms per sweep with 2 partitions from a graph, 43 ms eager)
+    // a partitioned sub-step runs eagerly: replayed from a captured graph its two chains took as long
+    // as one after the other (measured at 1024 x 512: 56 ms per sweep from a graph, 43 ms eager)
     const bool use_graph = graph_on && !htrace && !learned && partitions() == 1;
     // one walker per crowd: split each sub-step after its first N/8 electrons, so that the Gaussians
     // of the rest are drawn while the device runs the first moves (the first part's share hides
