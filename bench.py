@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 METRIC = "electron moves/sec/GPU + effective samples/sec, learned+mixed vs plain FP64"
 HBM_PEAK_GBS = 8000.0
 GRAPH = True  # replay each sweep from a captured HIP graph (--eager: launch kernel by kernel)
+LAUNCH = "auto"  # plain sweep's launch form: auto (timed trial), single (one launch of all walkers), partitioned
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 
@@ -198,6 +199,8 @@ def pick_launch(ctx, tau, seed):
     partitions on two streams (their launches overlap, which hides the tail of each move's launch:
     the workgroups dispatched last on a compute unit finish last).  Same arithmetic either way."""
     modes = ([("graph", 1)] if GRAPH else [("eager", 1)]) + [("eager", 2)]
+    if LAUNCH != "auto":  # one fixed form
+        modes = [m for m in modes if (m[1] == 2) == (LAUNCH == "partitioned")]
     best = None
     for mode, P in modes:
         ctx.tune_partitions(P)
@@ -404,12 +407,16 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="uneven workload: keep the default knobs")
     ap.add_argument("--eager", action="store_true",
                     help="launch the sweeps kernel by kernel instead of replaying a captured graph")
+    ap.add_argument("--launch", choices=["auto", "single", "partitioned"], default="auto",
+                    help="plain sweeps: all walkers in one launch per move (graph unless --eager), two "
+                         "walker partitions on two streams (eager), or the faster of the two in a timed trial")
     ap.add_argument("--workload", choices=["plain", "uneven"], default="plain",
                     help="plain: the 1024 x 512-electron FP64 workload with variants; uneven: 1024 walkers over "
                          "four systems of 128-1024 electrons, a quarter complex, in one captured graph")
     args = ap.parse_args()
-    global GRAPH
+    global GRAPH, LAUNCH
     GRAPH = not args.eager
+    LAUNCH = args.launch
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs (walker sharding, one "
                                    "rank per GPU) are not part of it", "n_gpus": args.gpus}))
