@@ -232,6 +232,11 @@ int qmi_mw_evaluate_ratios(qmi_ctx* ctx, int iat, int nvp, const double* pos, do
  * (the reference's LR_dim_cutoff, 15 by default).  Periodic orthorhombic cells; tables built once
  * per system. */
 int qmi_coulomb_setup(qmi_ctx* ctx, double ion_charge, double dim_cutoff);
+/* StructFact::mw_updateAllPart (Particle/LongRange/StructFact.cpp:63-135, computeRhok 138-205) for
+ * the electrons of every walker: rhok[nw][2][nk][2] = for spin group g (up, down: the electron
+ * species) the sums over its electrons of cos(k . r) and sin(k . r) at the wave vectors
+ * kvecs[nk][3] (Cartesian; e.g. a KContainer's list).  At most 1024 electrons. */
+int qmi_mw_structure_factor(qmi_ctx* ctx, int nk, const double* kvecs, double* rhok);
 /* Cell vectors of the orbitals' lattice, R[3][3] row-major (rows are the vectors; the
  * reference's CrystalLattice R, Lattice/CrystalLattice.h, with G = R^-1 used by SplineR2R /
  * SplineC2C to reduce positions and to map orbital gradients and Hessians,

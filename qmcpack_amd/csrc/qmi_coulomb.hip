@@ -114,6 +114,54 @@ __global__ __launch_bounds__(WG) void k_coulomb(SysC s, State st, CoulombC c, do
   }
 }
 
+// Structure factor of each spin group for nk wave vectors (StructFact::computeRhok,
+// Particle/LongRange/StructFact.cpp:138-205): rhok[w][g][k] = (sum over the electrons i of group g,
+// in order, of cos(k . r_i), and of sin(k . r_i)).  Grid (ceil(nk / WG), nw): one thread per k,
+// the walker's positions staged in LDS.
+__global__ __launch_bounds__(WG) void k_rhok(SysC s, State st, int nk, const double* __restrict__ kv,
+                                             double* __restrict__ out)
+{
+  __shared__ double sx[COUL_MAX_N], sy[COUL_MAX_N], sz[COUL_MAX_N];
+  const int w = blockIdx.y, t = threadIdx.x;
+  const int N = s.N, Np = s.Npad;
+  const double* R = st.R + (size_t)w * 3 * Np;
+  for (int i = t; i < N; i += WG)
+  {
+    sx[i] = R[i];
+    sy[i] = R[Np + i];
+    sz[i] = R[2 * Np + i];
+  }
+  __syncthreads();
+  const int k = blockIdx.x * WG + t;
+  if (k >= nk)
+    return;
+  const double kx = kv[3 * (size_t)k], ky = kv[3 * (size_t)k + 1], kz = kv[3 * (size_t)k + 2];
+  for (int g = 0; g < 2; g++)
+  {
+    double re = 0, im = 0;
+    const int i0 = s.first[g], i1 = s.first[g] + s.n[g];
+    for (int i = i0; i < i1; i++)
+    {
+      double sn, cs;
+      sincos(kx * sx[i] + ky * sy[i] + kz * sz[i], &sn, &cs);
+      re += cs;
+      im += sn;
+    }
+    double* o = out + (((size_t)w * 2 + g) * nk + k) * 2;
+    o[0]      = re;
+    o[1]      = im;
+  }
+}
+
+hipError_t launch_rhok(const SysC& s, const State& st, int nw, int nk, const double* kv, double* out, hipStream_t stream)
+{
+  if (s.N > COUL_MAX_N || nk <= 0 || nw <= 0 || nw > 65535)
+    return hipErrorInvalidValue;
+  const dim3 grid((nk + WG - 1) / WG, nw);
+  k_rhok<<<grid, WG, 0, stream>>>(s, st, nk, kv, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_coulomb(const SysC& s, const State& st, const CoulombC& c, int nw, double* out, hipStream_t stream)
 {
   if (s.N > 1024 || c.ng > COUL_MAX_NG || c.ng < 2)

@@ -3501,6 +3501,27 @@ extern "C"
     return guarded(c, [&] { c->coulomb_setup(ion_charge, dim_cutoff); });
   }
 
+  int qmi_mw_structure_factor(qmi_ctx* c, int nk, const double* kvecs, double* rhok)
+  {
+    return guarded(c, [&] {
+      c->require_walkers();
+      if (nk <= 0 || !kvecs || !rhok)
+        throw QmiError(QMI_ERR_ARG, "need nk > 0 wave vectors and an output array");
+      if (c->sys.N > COUL_MAX_N || c->nw > 65535)
+        throw QmiError(QMI_ERR_UNSUPPORTED, "more than 1024 electrons or 65535 walkers");
+      DevBuf<double> dk, dout;
+      dk.alloc((size_t)nk * 3);
+      dout.alloc((size_t)c->nw * 2 * nk * 2);
+      HIPCHECK(hipMemcpyAsync(dk.p, kvecs, dk.n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      {
+        qmi_ctx::Scoped t(c, "rhok");
+        HIPCHECK(launch_rhok(c->sys, c->st, c->nw, nk, dk.p, dout.p, c->stream));
+      }
+      HIPCHECK(hipMemcpyAsync(rhok, dout.p, dout.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      c->sync();
+    });
+  }
+
   int qmi_system_set_lattice(qmi_ctx* c, const double* R)
   {
     return guarded(c, [&] { c->set_lattice(R); });

@@ -210,6 +210,7 @@ hipError_t launch_copy_walkers(const WalkerArrays& t, const int* pairs, int npai
 
 // Ewald tables of the periodic Coulomb energy (built on the host, device pointers)
 constexpr int COUL_MAX_NG = 1024;
+constexpr int COUL_MAX_N  = 1024; // electrons staged in LDS by the structure-factor kernel
 struct CoulombC
 {
   const double* kvec; // [3][nk] one k of every pair (k, -k) with |k| <= k_c, k != 0
@@ -222,6 +223,8 @@ struct CoulombC
 };
 // out[nw][3] = {V_ee, V_eI, V_II}
 hipError_t launch_coulomb(const SysC& s, const State& st, const CoulombC& c, int nw, double* out, hipStream_t stream);
+// rhok[nw][2][nk][2]: structure factor of each spin group at wave vectors kv[nk][3]
+hipError_t launch_rhok(const SysC& s, const State& st, int nw, int nk, const double* kv, double* out, hipStream_t stream);
 
 // TrialWaveFunction::mw_evaluateRatios for virtual moves of electron iat: pos [nw][nvp][3] ->
 // out [nw][nvp] (complex pairs for complex orbitals); no delayed update pending for iat's group

@@ -362,6 +362,13 @@ class Context:
         self._check(self.L.qmi_mw_get_inverse(self.h, int(g), _p(out)))
         return self._out(out)
 
+    def mw_structure_factor(self, kvecs):
+        """StructFact rho_k of the electrons per spin group: [nw][2][nk] complex at kvecs [nk][3]."""
+        kv = np.ascontiguousarray(kvecs, dtype=np.float64).reshape(-1, 3)
+        out = np.empty((self.nw, 2, kv.shape[0], 2))
+        self._check(self.L.qmi_mw_structure_factor(self.h, C.c_int(kv.shape[0]), _p(kv), _p(out)))
+        return out[..., 0] + 1j * out[..., 1]
+
     def system_set_lattice(self, R):
         """Cell vectors (rows of R) of the orbitals' lattice; a skewed cell serves the orbital paths
         only (no Jastrow factors, Coulomb, pseudopotential or distance tables)."""

@@ -147,3 +147,28 @@ def test_opt_breakup_dmc_and_handler_validation():
     vo = oracle_opt_breakup(s, c.walkers_get(), zion)
     assert worst(v, vo) < REL
     assert c.L.qmi_coulomb_setup_handler(c.h, C.c_double(1.0), C.c_double(15.0), C.c_int(7)) == 1
+
+
+def test_structure_factor_matches_known_answer():
+    """qmi_mw_structure_factor against StructFact's known answer (test_StructFact.cpp:24-70: cube
+    of side 5, 3 + 1 particles, the 263786 k vectors up to k_c = 50, the sum over k of each
+    species' rho_k within 5e-5) and against the direct sum at every k (1e-12)."""
+    from qmcpack_amd.system import System
+    K, _, _ = bk.lattice_kpoints([5.0, 5.0, 5.0], 50.0)
+    R0 = np.array([[0.0, 1.0, 2.0], [1.0, 0.2, 3.0], [0.3, 4.0, 1.4], [3.2, 4.7, 0.7]])
+    rng = np.random.default_rng(8)
+    R = np.stack([R0, rng.uniform(0, 5, size=(4, 3))])
+    s = System(L=np.array([5.0, 5.0, 5.0]), ions=np.zeros((0, 3)), n_up=3, n_dn=1, M=(0, 0, 0), norb=0, coefs=None,
+               j1_params=None, j1_rcut=0.0, j1_cusp=0.0, j2_uu=None, j2_ud=None, j2_rcut=0.0, j2_cusp_uu=0.0,
+               j2_cusp_ud=0.0, delay_rank=1)
+    c = qmi.Context(0)
+    c.system_create(s)
+    c.walkers_load(R, 1)
+    rho = c.mw_structure_factor(K)
+    assert rho.shape == (2, 2, len(K))
+    for g, want in enumerate([-125.80618630936, 68.199075127271]):
+        assert abs(np.sum(rho[0, g]) - want) < 5e-5 * abs(want)
+    for w in range(2):
+        direct = [np.exp(1j * (K @ R[w, :3].T)).sum(1), np.exp(1j * (K @ R[w, 3:].T)).sum(1)]
+        for g in range(2):
+            assert np.max(np.abs(rho[w, g] - direct[g])) < 1e-12 * 3

@@ -142,3 +142,18 @@ def test_spline_energy_matches_exact_short_range():
     ve, _ = bk.coulomb_aa(L, r, q, brk=brk)
     vs, _ = bk.coulomb_aa(L, r, q, brk=brk, spline=True)
     assert abs(ve - vs) < 1e-6 * max(1.0, abs(ve))
+
+
+def test_kcontainer_and_structure_factor_known_answers():
+    """The lattice k list and the structure factor: 78 shells at k_c = 12 in a cube of side 5
+    (Particle/LongRange/tests/test_lrhandler.cpp:30-52), 263786 vectors at k_c = 50 and the sum
+    over them of each species' rho_k for 3 + 1 particles (test_StructFact.cpp:24-70, checked
+    there at 5e-5)."""
+    _, shell_k2, _ = bk.lattice_kpoints([5.0, 5.0, 5.0], 12.0)
+    assert len(shell_k2) == 78
+    K, _, _ = bk.lattice_kpoints([5.0, 5.0, 5.0], 50.0)
+    assert len(K) == 263786
+    R = np.array([[0.0, 1.0, 2.0], [1.0, 0.2, 3.0], [0.3, 4.0, 1.4], [3.2, 4.7, 0.7]])
+    sums = [np.sum(np.exp(1j * (K @ R[:3].T))), np.sum(np.exp(1j * (K @ R[3:].T)))]
+    for got, want in zip(sums, [-125.80618630936, 68.199075127271]):
+        assert abs(got - want) < 1e-9 * abs(want)
