@@ -242,9 +242,10 @@ int qmi_mw_structure_factor(qmi_ctx* ctx, int nk, const double* kvecs, double* r
  * SplineC2C to reduce positions and to map orbital gradients and Hessians,
  * SplineR2R.cpp:338-374).  Replaces the diagonal cell of the system description.  A cell with
  * off-diagonal vectors serves the orbital paths only (qmi_mw_spo_*, determinant ratios and
- * moves without Jastrow factors): with Jastrow functors in the system, or after
- * qmi_coulomb_setup / qmi_nlpp_setup / qmi_dt_create, it returns QMI_ERR_UNSUPPORTED, and those
- * calls refuse it afterwards (their minimum images are orthorhombic). */
+ * moves without Jastrow factors): with Jastrow functors in the system it returns
+ * QMI_ERR_UNSUPPORTED, and qmi_coulomb_setup / qmi_nlpp_setup / qmi_dt_create refuse it afterwards
+ * (their minimum images are orthorhombic).  Any lattice is set before those set-ups (after them:
+ * QMI_ERR_ARG for a diagonal cell, QMI_ERR_UNSUPPORTED for a skewed one). */
 int qmi_system_set_lattice(qmi_ctx* ctx, const double* R);
 /* Long-range handlers of the periodic Coulomb sum (the lattice input's LR_handler,
  * Particle/ParticleIO/LatticeIO.cpp:123-143; LRCoulombSingleton::getHandler,

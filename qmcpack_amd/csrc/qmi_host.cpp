@@ -431,8 +431,9 @@ struct qmi_ctx
           diag = false;
     if (!diag && (sys.has_j1 || sys.has_j2))
       throw QmiError(QMI_ERR_UNSUPPORTED, "Jastrow distances are built for orthorhombic cells");
-    if (!diag && (coul_ready || nlpp_ready || !dtables.empty()))
-      throw QmiError(QMI_ERR_UNSUPPORTED, "Coulomb, pseudopotential and distance tables need an orthorhombic cell");
+    if (coul_ready || nlpp_ready || !dtables.empty())
+      throw QmiError(diag ? QMI_ERR_ARG : QMI_ERR_UNSUPPORTED,
+                     "set the lattice before the Coulomb, pseudopotential and distance-table set-up");
     if (diag && !(R[0] > 0 && R[4] > 0 && R[8] > 0))
       throw QmiError(QMI_ERR_ARG, "diagonal lattice needs positive sides");
     // G = R^-1 by cofactors
