@@ -255,10 +255,13 @@ __device__ inline void stage_rows(const DT* __restrict__ X, int k, int n, int d,
 }
 
 // NCT = ceil(d / 16) column tiles of the delayed block; columns are zero-padded to 16 NCT, so the
-// padded MFMA steps add exact zeros.  NCB = ceil(n / 256) column blocks of 256: the register slab
-// holds one block; step 1 accumulates T over the blocks in order, step 3 updates the last block
-// (still in registers) first and reloads the earlier ones (Ainv read 2 - 1/NCB times, written once).
-template<typename DT, int NCT, int NCB>
+// padded MFMA steps add exact zeros.  NCB = ceil(n / COLS) column blocks of COLS (256, or 128 for
+// FP64 at n <= 256: 160 instead of 234 registers, occupancy 3, 0.358 instead of 0.379 ms at
+// n = 256, k = 32, though Ainv is read 1.5 times; FP32 keeps 256 columns, 0.185 vs 0.207 ms): the
+// register slab holds one block; step 1 accumulates T over the blocks in order, step 3 updates the
+// last block (still in registers) first and reloads the earlier ones (Ainv read 2 - 1/NCB times,
+// written once).
+template<typename DT, int NCT, int NCB, int COLS>
 __global__ __launch_bounds__(WG, 2) void k_flush_mfma(SysC s, State st, int g, int d, int nw)
 {
   using MF    = Mfma<DT>;
@@ -286,14 +289,15 @@ __global__ __launch_bounds__(WG, 2) void k_flush_mfma(SysC s, State st, int g, i
   const DT* Binv = det_ptr<DT>(st.Binv[g]) + (size_t)w * k * k;
   const int* dl  = st.dlist[g] + (size_t)w * k;
   DT* sTw        = sT + wave * 16 * LT;
-  const int ncb  = min(NCB, (n + MF_COLS - 1) / MF_COLS);
+  const int ncb  = min(NCB, (n + COLS - 1) / COLS);
 
-  DT a[64];
+  constexpr int NJT = COLS / 16; // 16-column tiles of the register slab
+  DT a[4 * NJT];
   auto load_block = [&](int cb) {
 #pragma unroll
-    for (int jt = 0; jt < 16; jt++)
+    for (int jt = 0; jt < NJT; jt++)
     {
-      const int j0 = cb * MF_COLS + jt * 16 + 4 * q;
+      const int j0 = cb * COLS + jt * 16 + 4 * q;
       const DT* ap = A + (size_t)row * ld + j0;
       if (rok && j0 + 3 < n)
       {
@@ -311,9 +315,9 @@ __global__ __launch_bounds__(WG, 2) void k_flush_mfma(SysC s, State st, int g, i
   };
   auto store_block = [&](int cb) {
 #pragma unroll
-    for (int jt = 0; jt < 16; jt++)
+    for (int jt = 0; jt < NJT; jt++)
     {
-      const int j0 = cb * MF_COLS + jt * 16 + 4 * q;
+      const int j0 = cb * COLS + jt * 16 + 4 * q;
       DT* ap       = A + (size_t)row * ld + j0;
       if (rok && j0 + 3 < n)
       {
@@ -357,9 +361,9 @@ __global__ __launch_bounds__(WG, 2) void k_flush_mfma(SysC s, State st, int g, i
   {
     load_block(cb);
 #pragma unroll
-    for (int ch = 0; ch < MF_COLS / 64; ch++)
+    for (int ch = 0; ch < COLS / 64; ch++)
     {
-      const int j0 = cb * MF_COLS + ch * 64;
+      const int j0 = cb * COLS + ch * 64;
       if (j0 < n)
       {
         __syncthreads();
@@ -413,9 +417,9 @@ __global__ __launch_bounds__(WG, 2) void k_flush_mfma(SysC s, State st, int g, i
     if (cb != ncb - 1)
       load_block(cb);
 #pragma unroll
-    for (int ch = 0; ch < MF_COLS / 64; ch++)
+    for (int ch = 0; ch < COLS / 64; ch++)
     {
-      const int j0 = cb * MF_COLS + ch * 64;
+      const int j0 = cb * COLS + ch * 64;
       if (j0 < n)
       {
         __syncthreads();
@@ -456,20 +460,26 @@ hipError_t launch_flush(const SysC& s, const State& st, int g, int d, int nw, do
       const int nblk = (n + 63) / 64, ngrp = (nw + 7) / 8;
       const dim3 grid1(nblk * ngrp * 8);
       const size_t lds = flush_lds_bytes(s.k, sizeof(DT));
-      auto go          = [&](auto cbt) {
-        constexpr int CB = decltype(cbt)::value;
+      auto go          = [&](auto cbt, auto colt) {
+        constexpr int CB = decltype(cbt)::value, COLS = decltype(colt)::value;
         switch ((d + 15) / 16)
         {
-        case 1: k_flush_mfma<DT, 1, CB><<<grid1, WG, lds, stream>>>(s, st, g, d, nw); break;
-        case 2: k_flush_mfma<DT, 2, CB><<<grid1, WG, lds, stream>>>(s, st, g, d, nw); break;
-        case 3: k_flush_mfma<DT, 3, CB><<<grid1, WG, lds, stream>>>(s, st, g, d, nw); break;
-        default: k_flush_mfma<DT, 4, CB><<<grid1, WG, lds, stream>>>(s, st, g, d, nw); break;
+        case 1: k_flush_mfma<DT, 1, CB, COLS><<<grid1, WG, lds, stream>>>(s, st, g, d, nw); break;
+        case 2: k_flush_mfma<DT, 2, CB, COLS><<<grid1, WG, lds, stream>>>(s, st, g, d, nw); break;
+        case 3: k_flush_mfma<DT, 3, CB, COLS><<<grid1, WG, lds, stream>>>(s, st, g, d, nw); break;
+        default: k_flush_mfma<DT, 4, CB, COLS><<<grid1, WG, lds, stream>>>(s, st, g, d, nw); break;
         }
       };
-      if (n <= MF_COLS)
-        go(std::integral_constant<int, 1>{});
+      using I1 = std::integral_constant<int, 1>;
+      using I2 = std::integral_constant<int, 2>;
+      using C128 = std::integral_constant<int, MF_COLS / 2>;
+      using C256 = std::integral_constant<int, MF_COLS>;
+      if (std::is_same<DT, double>::value && n > MF_COLS / 2 && n <= MF_COLS)
+        go(I2{}, C128{});
+      else if (n <= MF_COLS)
+        go(I1{}, C256{});
       else
-        go(std::integral_constant<int, 2>{});
+        go(I2{}, C256{});
       return hipGetLastError();
     }
     // three passes; scratch holds T (n x k per walker)
