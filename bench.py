@@ -29,7 +29,7 @@ sys.path.insert(0, ROOT)
 METRIC = "electron moves/sec/GPU + effective samples/sec, learned+mixed vs plain FP64"
 HBM_PEAK_GBS = 8000.0
 GRAPH = True  # replay each sweep from a captured HIP graph (--eager: launch kernel by kernel)
-LAUNCH = "auto"  # plain sweep's launch form: auto (timed trial), single (one launch of all walkers), partitioned
+LAUNCH = "auto"  # plain sweep's launch form: auto (timed trial), single, partitioned or window
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 
@@ -195,14 +195,19 @@ def chain_summary(E, dt, nw, N, steps):
 
 def pick_launch(ctx, tau, seed):
     """Launch form of the plain sweep, timed over 3 untimed sweeps each: one captured graph per
-    sub-step with all walkers in one launch per move, or eager launches with the walkers in two
+    sub-step with all walkers in one launch per move; eager launches with the walkers in two
     partitions on two streams (their launches overlap, which hides the tail of each move's launch:
-    the workgroups dispatched last on a compute unit finish last).  Same arithmetic either way."""
-    modes = ([("graph", 1)] if GRAPH else [("eager", 1)]) + [("eager", 2)]
-    if LAUNCH != "auto":  # one fixed form
-        modes = [m for m in modes if (m[1] == 2) == (LAUNCH == "partitioned")]
+    the workgroups dispatched last on a compute unit finish last); or one launch per delay window
+    (k_sweep_segment: each workgroup runs its walker's moves of the window back to back, so only
+    the window's end waits for the slowest).  Same arithmetic in every form."""
+    forms = {"single": ("graph" if GRAPH else "eager", 1, 0),
+             "partitioned": ("eager", 2, 0),
+             "window": ("graph" if GRAPH else "eager", 1, 1)}
+    names = list(forms) if LAUNCH == "auto" else [LAUNCH]
     best = None
-    for mode, P in modes:
+    for name in names:
+        mode, P, mk = forms[name]
+        ctx.tune_set(move_kernel=mk)
         ctx.tune_partitions(P)
         ctx.graph_enable(mode == "graph")
         ctx.vmc_run(1, 1, tau, 0, seed + 3)
@@ -210,11 +215,15 @@ def pick_launch(ctx, tau, seed):
         ctx.vmc_run(3, 1, tau, 0, seed + 4)
         dt = time.perf_counter() - t1
         if best is None or dt < best[0]:
-            best = (dt, mode, P)
-    _, mode, P = best
+            best = (dt, name)
+    name = best[1]
+    mode, P, mk = forms[name]
+    ctx.tune_set(move_kernel=mk)
     ctx.tune_partitions(P)
     ctx.graph_enable(mode == "graph")
-    return mode if P == 1 else f"{mode}, {P} walker partitions on {P} streams"
+    return {"single": f"{mode}, one launch of all walkers per move",
+            "partitioned": "eager, 2 walker partitions on 2 streams, one launch per move each",
+            "window": f"{mode}, one launch per delay window (k_sweep_segment)"}[name]
 
 
 def run_variant(qmi, system, R, cfg, precision, learned, steps, warmup, train_sweeps, lr, log):
@@ -407,9 +416,10 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="uneven workload: keep the default knobs")
     ap.add_argument("--eager", action="store_true",
                     help="launch the sweeps kernel by kernel instead of replaying a captured graph")
-    ap.add_argument("--launch", choices=["auto", "single", "partitioned"], default="auto",
+    ap.add_argument("--launch", choices=["auto", "single", "partitioned", "window"], default="auto",
                     help="plain sweeps: all walkers in one launch per move (graph unless --eager), two "
-                         "walker partitions on two streams (eager), or the faster of the two in a timed trial")
+                         "walker partitions on two streams (eager), one launch per delay window, or the "
+                         "fastest in a timed trial")
     ap.add_argument("--workload", choices=["plain", "uneven"], default="plain",
                     help="plain: the 1024 x 512-electron FP64 workload with variants; uneven: 1024 walkers over "
                          "four systems of 128-1024 electrons, a quarter complex, in one captured graph")

@@ -9,6 +9,17 @@
 
 namespace qmi
 {
+// threadIdx.x passed through an empty asm, so that the compiler cannot treat values computed from it
+// (LDS and global offsets, masks) as invariant across the moves of a looped kernel: hoisted ahead of
+// k_sweep_segment's move loop they kept ~30 registers live through it, and the loop kernel spilled
+// under its 128-register bound (552 bytes per lane; 20 with this and no machine LICM)
+__device__ __forceinline__ int thread_index()
+{
+  int t = __builtin_amdgcn_workitem_id_x();
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
 constexpr int WG         = 256;
 constexpr int NWAVE      = WG / 64;
 constexpr int MAX_JCOEF  = 68; // functor coefficients (NumParams + 4)
@@ -472,7 +483,7 @@ template<int K>
 __device__ inline void block_sum(double (&v)[K], double* scratch, double* out)
 {
   static_assert(K * 16 <= WG, "block_sum handles at most 16 values");
-  const int t = threadIdx.x;
+  const int t = thread_index();
 #pragma unroll
   for (int k = 0; k < K; k++)
     scratch[k * WG + t] = v[k];

@@ -28,7 +28,7 @@ template<typename DT>
 __device__ inline void delayed_gemv(const DT* __restrict__ X, const double* vec, int n, int k, int d,
                                     double* red, double* out, double scale)
 {
-  const int t = threadIdx.x;
+  const int t = thread_index();
   int dd      = 1;
   while (dd < d)
     dd <<= 1;
@@ -99,7 +99,7 @@ __device__ inline void phaseA(const SysC& s, int n, int ld, int k, int d, int r,
                               const DT* __restrict__ U, const DT* __restrict__ V, DT* __restrict__ Binv, MoveLDS& sh,
                               DT* sB, double (&ainv_r)[MAXM], double (&inv)[MAXM])
 {
-  const int t    = threadIdx.x;
+  const int t    = thread_index();
   const DT* arow = Ainv + (size_t)r * ld;
 #pragma unroll
   for (int m = 0; m < MAXM; m++)
@@ -149,7 +149,7 @@ __device__ inline void phaseA(const SysC& s, int n, int ld, int k, int d, int r,
 template<typename DT, int MAXM>
 __device__ inline void load_dpsi_row(int n, int ld, int r, const DT* __restrict__ dpsi, double (&dg)[MAXM][3])
 {
-  const int t    = threadIdx.x;
+  const int t    = thread_index();
   const DT* drow = dpsi + (size_t)r * 4 * ld;
 #pragma unroll
   for (int m = 0; m < MAXM; m++)
@@ -165,7 +165,7 @@ __device__ inline void load_dpsi_row(int n, int ld, int r, const DT* __restrict_
 template<int MAXM>
 __device__ inline void det_grad_pre(int n, const double (&dg)[MAXM][3], const double (&inv)[MAXM], MoveLDS& sh)
 {
-  const int t  = threadIdx.x;
+  const int t  = thread_index();
   double gv[3] = {0, 0, 0};
 #pragma unroll
   for (int m = 0; m < MAXM; m++)
@@ -186,7 +186,7 @@ template<typename DT, int MAXM>
 __device__ inline void det_grad(int n, int ld, int r, const DT* __restrict__ dpsi, const double (&inv)[MAXM],
                                 MoveLDS& sh)
 {
-  const int t    = threadIdx.x;
+  const int t    = thread_index();
   double gv[3]   = {0, 0, 0};
   const DT* drow = dpsi + (size_t)r * 4 * ld;
 #pragma unroll
@@ -210,7 +210,7 @@ __device__ inline void phaseC(const SysC& s, int g, int iat, int n, const double
                               const double* __restrict__ ions, const ST* __restrict__ coefs,
                               const double (&inv)[MAXM], double (&phv)[MAXM][5], MoveLDS& sh)
 {
-  const int t = threadIdx.x;
+  const int t = thread_index();
   const int N = s.N, Np = s.Npad;
   const double pos[3] = {sh.bc[BC_POS], sh.bc[BC_POS + 1], sh.bc[BC_POS + 2]};
   if (s.has_det && t == 64) // locate once; the Jastrow reduction's barriers publish it
@@ -294,7 +294,7 @@ __device__ inline void phaseE(const SysC& s, int g, int iat, int n, int ld, int 
                               double* __restrict__ logdet, const double (&ainv_r)[MAXM], const double (&phv)[MAXM][5],
                               MoveLDS& sh, DT* sB, bool stage)
 {
-  const int t = threadIdx.x;
+  const int t = thread_index();
   const int N = s.N, Np = s.Npad;
   const bool accepted = sh.bc[BC_ACC] != 0.0;
   const double* C     = sh.bc + BC_C;
@@ -517,11 +517,11 @@ __device__ inline void fused_move(const SysC& s, const State& st, const ST* __re
   }
   else
     __syncthreads(); // publishes the prefetched scalars
-  if (threadIdx.x == 0)
+  if (thread_index() == 0)
     propose_pf(s, a, sh);
   __syncthreads();
   phaseC<ST, MAXM>(s, g, iat, n, v.R, st.ions, coefs, inv, phv, sh);
-  if (threadIdx.x == 0)
+  if (thread_index() == 0)
   {
     double ratio;
     bool flag;
@@ -581,7 +581,7 @@ __global__ __launch_bounds__(WG, 4) void k_sweep_segment(SysC s, State st, const
     const int d = s.k > 1 ? a.d0 + (iat - a.iat0) : 0;
     fused_move<ST, DT, MAXM>(s, st, coefs, a, w, iat, d, sh, sB, ucount, nacc, nrej);
   }
-  if (threadIdx.x == 0)
+  if (thread_index() == 0)
   {
     a.ubase[(size_t)w * (N + 1) + a.iat1] = ucount;
     st.stats[(size_t)w * 2 + 0] += nacc;
@@ -605,7 +605,7 @@ __global__ __launch_bounds__(WG) void k_move_one(SysC s, State st, const ST* __r
   int ucount  = a.ubase[(size_t)w * (N + 1) + a.iat0];
   double nacc = 0, nrej = 0;
   fused_move<ST, DT, MAXM>(s, st, coefs, a, w, a.iat0, a.d0, sh, sB, ucount, nacc, nrej);
-  if (threadIdx.x == 0)
+  if (thread_index() == 0)
   {
     a.ubase[(size_t)w * (N + 1) + a.iat0 + 1] = ucount;
     st.stats[(size_t)w * 2 + 0] += nacc;
@@ -634,12 +634,12 @@ __global__ __launch_bounds__(WG) void k_propose(SysC s, State st, const ST* __re
     phaseA<DT, MAXM>(s, n, ld, k, d, r, v.Ainv, v.U, v.V, v.Binv, sh, sB, ainv_r, inv);
     det_grad<DT, MAXM>(n, ld, r, v.dpsi, inv, sh);
   }
-  if (threadIdx.x == 0)
+  if (thread_index() == 0)
     propose(s, iat, w, a, v.R, v.jas, sh);
   __syncthreads();
   phaseC<ST, MAXM>(s, g, iat, n, v.R, st.ions, coefs, inv, phv, sh);
   double* mv = st.mv + (size_t)w * MV_SIZE;
-  if (threadIdx.x == 0)
+  if (thread_index() == 0)
   {
     double ratio;
     bool flag;
@@ -663,7 +663,7 @@ __global__ __launch_bounds__(WG) void k_propose(SysC s, State st, const ST* __re
 #pragma unroll
     for (int m = 0; m < MAXM; m++)
     {
-      const int o = threadIdx.x + WG * m;
+      const int o = thread_index() + WG * m;
       if (o < n)
         for (int q = 0; q < 5; q++)
           phi[q * ld + o] = phv[m][q];
@@ -690,7 +690,7 @@ __global__ __launch_bounds__(WG) void k_accept_crowd(SysC s, State st, MoveArgs 
   const double* mv = st.mv + (size_t)w * MV_SIZE;
   // prefix count of drawing walkers before w in the crowd, and the crowd total
   double cnt[2] = {0, 0};
-  for (int j = w0 + threadIdx.x; j < w1; j += WG)
+  for (int j = w0 + thread_index(); j < w1; j += WG)
   {
     const double f = st.mv[(size_t)j * MV_SIZE + MV_FLAG];
     cnt[1] += f;
@@ -698,7 +698,7 @@ __global__ __launch_bounds__(WG) void k_accept_crowd(SysC s, State st, MoveArgs 
       cnt[0] += f;
   }
   block_sum<2>(cnt, sh.red, sh.bc + BC_OLD);
-  if (threadIdx.x == 0)
+  if (thread_index() == 0)
   {
     const int base = a.ubase[(size_t)crowd * (N + 1) + iat];
     bool acc       = false;
@@ -727,8 +727,8 @@ __global__ __launch_bounds__(WG) void k_accept_crowd(SysC s, State st, MoveArgs 
     for (int q = 0; q < 13; q++)
       sh.bc[BC_C + q] = mv[MV_C + q];
   }
-  if (s.has_det && k > 1 && threadIdx.x < d)
-    sh.y[threadIdx.x] = v.Binv[d * k + threadIdx.x];
+  if (s.has_det && k > 1 && thread_index() < d)
+    sh.y[thread_index()] = v.Binv[d * k + thread_index()];
   __syncthreads();
   double ainv_r[MAXM], phv[MAXM][5];
   const double* phi = st.phi + (size_t)w * 5 * ld;
@@ -736,7 +736,7 @@ __global__ __launch_bounds__(WG) void k_accept_crowd(SysC s, State st, MoveArgs 
 #pragma unroll
   for (int m = 0; m < MAXM; m++)
   {
-    const int o = threadIdx.x + WG * m;
+    const int o = thread_index() + WG * m;
     ainv_r[m]   = o < nd ? v.Ainv[(size_t)r * ld + o] : 0.0;
     for (int q = 0; q < 5; q++)
       phv[m][q] = o < nd ? phi[q * ld + o] : 0.0;
@@ -772,12 +772,12 @@ __global__ __launch_bounds__(WG) void k_api_eval_grad(SysC s, State st, MoveArgs
 #pragma unroll
     for (int m = 0; m < MAXM; m++)
     {
-      const int o = threadIdx.x + WG * m;
+      const int o = thread_index() + WG * m;
       if (o < n)
         ir[o] = inv[m];
     }
   }
-  if (threadIdx.x == 0)
+  if (thread_index() == 0)
   {
     double gnow[3] = {0, 0, 0};
     for (int q = 0; q < 3; q++)
@@ -819,7 +819,7 @@ __global__ __launch_bounds__(WG) void k_api_ratio_grad(SysC s, State st, const S
 #pragma unroll
       for (int m = 0; m < MAXM; m++)
       {
-        const int o = threadIdx.x + WG * m;
+        const int o = thread_index() + WG * m;
         if (o < n)
           ir[o] = inv[m];
       }
@@ -830,16 +830,16 @@ __global__ __launch_bounds__(WG) void k_api_ratio_grad(SysC s, State st, const S
 #pragma unroll
       for (int m = 0; m < MAXM; m++)
       {
-        const int o = threadIdx.x + WG * m;
+        const int o = thread_index() + WG * m;
         inv[m]      = o < n ? ir[o] : 0.0;
       }
     }
   }
-  if (threadIdx.x < 3)
-    sh.bc[BC_POS + threadIdx.x] = mv[MV_POS + threadIdx.x];
+  if (thread_index() < 3)
+    sh.bc[BC_POS + thread_index()] = mv[MV_POS + thread_index()];
   __syncthreads();
   phaseC<ST, MAXM>(s, g, iat, n, v.R, st.ions, coefs, inv, phv, sh);
-  if (threadIdx.x == 0)
+  if (thread_index() == 0)
   {
     double ratio, gnew[3];
     combine_ratio(s, iat, v.jas, sh, ratio, gnew);
@@ -856,7 +856,7 @@ __global__ __launch_bounds__(WG) void k_api_ratio_grad(SysC s, State st, const S
 #pragma unroll
     for (int m = 0; m < MAXM; m++)
     {
-      const int o = threadIdx.x + WG * m;
+      const int o = thread_index() + WG * m;
       if (o < n)
         for (int q = 0; q < 5; q++)
           phi[q * ld + o] = phv[m][q];
@@ -879,7 +879,7 @@ __global__ __launch_bounds__(WG) void k_api_accept(SysC s, State st, MoveArgs a)
   const int iat = a.iat0, r = iat - s.first[g], d = a.d0;
   WalkerView<DT> v = walker_view<DT>(s, st, g, w);
   const double* mv = st.mv + (size_t)w * MV_SIZE;
-  if (threadIdx.x == 0)
+  if (thread_index() == 0)
   {
     const bool acc = a.host_acc[w] != 0;
     sh.bc[BC_ACC]  = acc ? 1.0 : 0.0;
@@ -889,8 +889,8 @@ __global__ __launch_bounds__(WG) void k_api_accept(SysC s, State st, MoveArgs a)
     for (int q = 0; q < 13; q++)
       sh.bc[BC_C + q] = mv[MV_C + q];
   }
-  if (s.has_det && k > 1 && threadIdx.x < d)
-    sh.y[threadIdx.x] = v.Binv[d * k + threadIdx.x];
+  if (s.has_det && k > 1 && thread_index() < d)
+    sh.y[thread_index()] = v.Binv[d * k + thread_index()];
   __syncthreads();
   double ainv_r[MAXM], phv[MAXM][5];
   const double* phi = st.phi + (size_t)w * 5 * ld;
@@ -898,7 +898,7 @@ __global__ __launch_bounds__(WG) void k_api_accept(SysC s, State st, MoveArgs a)
 #pragma unroll
   for (int m = 0; m < MAXM; m++)
   {
-    const int o = threadIdx.x + WG * m;
+    const int o = thread_index() + WG * m;
     ainv_r[m]   = o < nd ? v.Ainv[(size_t)r * ld + o] : 0.0;
     for (int q = 0; q < 5; q++)
       phv[m][q] = o < nd ? phi[q * ld + o] : 0.0;
@@ -928,7 +928,7 @@ __global__ __launch_bounds__(WG) void k_api_vp_ratios(SysC s, State st, const ST
 {
   __shared__ VpLDS sh;
   stage_jastrow(s, sh);
-  const int v = blockIdx.x, w = blockIdx.y, t = threadIdx.x;
+  const int v = blockIdx.x, w = blockIdx.y, t = thread_index();
   const int g = iat < s.nup ? 0 : 1, n = s.n[g], ld = s.ld, Np = s.Npad;
   const int r       = iat - s.first[g];
   const double* R   = st.R + (size_t)w * 3 * Np;
@@ -981,7 +981,7 @@ hipError_t launch_api_vp_ratios(const SysC& s, const State& st, const void* coef
 // mw_makeMove: proposed position R[iat] + displ
 __global__ void k_api_make_move(SysC s, State st, int iat, int nw, const double* __restrict__ displ)
 {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = blockIdx.x * blockDim.x + thread_index();
   if (w >= nw)
     return;
   const double* R = st.R + (size_t)w * 3 * s.Npad;
@@ -1001,7 +1001,7 @@ __global__ void k_api_make_move(SysC s, State st, int iat, int nw, const double*
 constexpr int SEL_BINS = 64, SEL_CAP = 256;
 __device__ inline void select_nearest(const double* r, int count, int K, int* sel, int* scratch)
 {
-  const int t   = threadIdx.x;
+  const int t   = thread_index();
   int* hist     = scratch;              // [SEL_BINS]
   int* cnt      = scratch + SEL_BINS;   // candidate count
   int* rmaxbits = scratch + SEL_BINS + 1;
@@ -1095,7 +1095,7 @@ __device__ inline void proposal_features(const SysC& s, const double* __restrict
                                          int iat, const double pos[3], const double vd[3], double tau, MoveLDS& sh,
                                          float* __restrict__ x)
 {
-  const int t = threadIdx.x, N = s.N, Np = s.Npad, ni = s.nion;
+  const int t = thread_index(), N = s.N, Np = s.Npad, ni = s.nion;
   double* de = sh.red;
   double* di = sh.red + 1024;
   __syncthreads();
@@ -1186,13 +1186,13 @@ __global__ __launch_bounds__(WG) void k_learn_pre(SysC s, State st, const ST* __
 #pragma unroll
     for (int m = 0; m < MAXM; m++)
     {
-      const int o = threadIdx.x + WG * m;
+      const int o = thread_index() + WG * m;
       if (o < n)
         ir[o] = inv[m];
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0)
+  if (thread_index() == 0)
   {
     double* ls = la.ls + (size_t)w * LS_SIZE;
     double gn[3], dv[3];
@@ -1229,7 +1229,7 @@ __global__ __launch_bounds__(WG) void k_learn_prop(SysC s, State st, const ST* _
   WalkerView<DT> v = walker_view<DT>(s, st, g, w);
   double* ls       = la.ls + (size_t)w * LS_SIZE;
   double* mv       = st.mv + (size_t)w * MV_SIZE;
-  if (threadIdx.x == 0)
+  if (thread_index() == 0)
   {
     const float* of  = la.out + (size_t)w * MLP_OUT;
     const double* gz = a.gauss + ((size_t)iat * a.nw + w) * 3;
@@ -1253,11 +1253,11 @@ __global__ __launch_bounds__(WG) void k_learn_prop(SysC s, State st, const ST* _
 #pragma unroll
   for (int m = 0; m < MAXM; m++)
   {
-    const int o = threadIdx.x + WG * m;
+    const int o = thread_index() + WG * m;
     inv[m]      = (s.has_det && o < n) ? ir[o] : 0.0;
   }
   phaseC<ST, MAXM>(s, g, iat, n, v.R, st.ions, coefs, inv, phv, sh);
-  if (threadIdx.x == 0)
+  if (thread_index() == 0)
   {
     double ratio, gnew[3], vr[3];
     combine_ratio(s, iat, v.jas, sh, ratio, gnew);
@@ -1280,7 +1280,7 @@ __global__ __launch_bounds__(WG) void k_learn_prop(SysC s, State st, const ST* _
 #pragma unroll
   for (int m = 0; m < MAXM; m++)
   {
-    const int o = threadIdx.x + WG * m;
+    const int o = thread_index() + WG * m;
     if (o < n)
       for (int q = 0; q < 5; q++)
         phi[q * ld + o] = phv[m][q];
@@ -1309,7 +1309,7 @@ __global__ __launch_bounds__(WG) void k_learn_accept(SysC s, State st, MoveArgs 
   WalkerView<DT> v = walker_view<DT>(s, st, g, w);
   const double* mv = st.mv + (size_t)w * MV_SIZE;
   const double* ls = la.ls + (size_t)w * LS_SIZE;
-  if (threadIdx.x == 0)
+  if (thread_index() == 0)
   {
     const float* of = la.out + (size_t)w * MLP_OUT;
     const float* orr = la.out + ((size_t)a.nw + w) * MLP_OUT;
@@ -1397,15 +1397,15 @@ __global__ __launch_bounds__(WG) void k_learn_accept(SysC s, State st, MoveArgs 
     for (int q = 0; q < 13; q++)
       sh.bc[BC_C + q] = mv[MV_C + q];
   }
-  if (k > 1 && threadIdx.x < d)
-    sh.y[threadIdx.x] = v.Binv[d * k + threadIdx.x];
+  if (k > 1 && thread_index() < d)
+    sh.y[thread_index()] = v.Binv[d * k + thread_index()];
   __syncthreads();
   double ainv_r[MAXM], phv[MAXM][5];
   const double* phi = st.phi + (size_t)w * 5 * ld;
 #pragma unroll
   for (int m = 0; m < MAXM; m++)
   {
-    const int o = threadIdx.x + WG * m;
+    const int o = thread_index() + WG * m;
     ainv_r[m]   = o < n ? (double)v.Ainv[(size_t)r * ld + o] : 0.0;
     for (int q = 0; q < 5; q++)
       phv[m][q] = o < n ? phi[q * ld + o] : 0.0;
@@ -1423,7 +1423,7 @@ __device__ inline void mlp_single(const MlpParams& P, const float* x, MoveLDS& s
 {
   uint16_t* sA = reinterpret_cast<uint16_t*>(sh.red);
   uint16_t* sB = sA + 16 * MLP_LDA;
-  const int t  = threadIdx.x;
+  const int t  = thread_index();
   for (int e = t; e < 16 * MLP_LDA; e += WG)
     sA[e] = (e < MLP_IN) ? bf16_bits(x[e]) : (uint16_t)0;
   __syncthreads();
@@ -1474,7 +1474,7 @@ __global__ __launch_bounds__(WG, 4) void k_learn_move(SysC s, State st, const ST
     det_grad<DT, MAXM>(n, ld, r, v.dpsi, inv, sh);
   }
   __syncthreads();
-  if (threadIdx.x == 0)
+  if (thread_index() == 0)
   {
     double gn[3] = {0, 0, 0}, dv[3];
     if (s.has_det) // grad_now from the prefetched Jastrow gradients
@@ -1503,7 +1503,7 @@ __global__ __launch_bounds__(WG, 4) void k_learn_move(SysC s, State st, const ST
     proposal_features(s, v.R, st.ions, iat, pos, vd, a.tau, sh, xf);
   }
   mlp_single(P, xf, sh, mo);
-  if (threadIdx.x == 0)
+  if (thread_index() == 0)
   {
     const double* gz = pf + PF_GAUSS;
     const double sig = sqrt(a.tau) * exp((double)mo[3]);
@@ -1520,7 +1520,7 @@ __global__ __launch_bounds__(WG, 4) void k_learn_move(SysC s, State st, const ST
   }
   __syncthreads();
   phaseC<ST, MAXM>(s, g, iat, n, v.R, st.ions, coefs, inv, phv, sh);
-  if (threadIdx.x == 0)
+  if (thread_index() == 0)
   {
     double ratio, gnew[3], vr[3];
     combine_ratio_v(s, pf[PF_VAT], pf[PF_UAT], sh, ratio, gnew);
@@ -1542,7 +1542,7 @@ __global__ __launch_bounds__(WG, 4) void k_learn_move(SysC s, State st, const ST
     proposal_features(s, v.R, st.ions, iat, pos, vd, a.tau, sh, xf);
   }
   mlp_single(P, xf, sh, mo + MLP_OUT);
-  if (threadIdx.x == 0)
+  if (thread_index() == 0)
   {
     const float* of  = mo;
     const float* orr = mo + MLP_OUT;

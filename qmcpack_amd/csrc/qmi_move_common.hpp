@@ -24,7 +24,7 @@ struct MoveLDS
 template<typename SH>
 __device__ inline void stage_jastrow(const SysC& s, SH& sh)
 {
-  for (int e = threadIdx.x; e < 3 * MAX_JCOEF; e += WG)
+  for (int e = thread_index(); e < 3 * MAX_JCOEF; e += WG)
   {
     const int f = e / MAX_JCOEF, i = e - f * MAX_JCOEF;
     sh.jc[f][i] = f == 0 ? s.j1.c[i] : s.j2[f - 1].c[i];
@@ -51,7 +51,7 @@ __host__ __device__ constexpr size_t binv_lds_bytes(int k, size_t esize) { retur
 template<typename DT>
 __device__ inline void stage_binv(const DT* __restrict__ Binv, int k, int d, DT* sB)
 {
-  for (int e = threadIdx.x; e < d * d; e += WG)
+  for (int e = thread_index(); e < d * d; e += WG)
   {
     const int j = e / d, i = e - j * d;
     sB[j * (k + 1) + i] = Binv[j * k + i];
@@ -86,7 +86,7 @@ template<int NJ>
 __device__ inline void load_electrons(const SysC& s, const double* __restrict__ R, double (&px)[NJ], double (&py)[NJ],
                                       double (&pz)[NJ])
 {
-  const int t = threadIdx.x, N = s.N, Np = s.Npad;
+  const int t = thread_index(), N = s.N, Np = s.Npad;
 #pragma unroll
   for (int u = 0; u < NJ; u++)
   {
@@ -119,7 +119,7 @@ __device__ inline void jastrow_ratio_sums_p(const SysC& s, int g, int iat, const
                                             const double (&py)[NJ], const double (&pz)[NJ],
                                             const double* __restrict__ ions, const double pos[3], SH& sh)
 {
-  const int t = threadIdx.x;
+  const int t = thread_index();
   const int N = s.N;
   double acc[9];
 #pragma unroll
@@ -174,7 +174,7 @@ template<int NJ, typename SH>
 __device__ inline void jastrow_accept(const SysC& s, int g, int iat, double* __restrict__ R,
                                       double* __restrict__ jas, SH& sh)
 {
-  const int t = threadIdx.x;
+  const int t = thread_index();
   const int N = s.N, Np = s.Npad;
   const double* C = sh.bc + BC_C;
   if (s.has_j1 && t == 0)
@@ -298,7 +298,7 @@ __device__ inline void prefetch_move(const SysC& s, int iat, int w, int ucount, 
                                      const double* __restrict__ R, const double* __restrict__ jas, SH& sh,
                                      int base = BC_PF)
 {
-  const int t = threadIdx.x, Np = s.Npad;
+  const int t = thread_index(), Np = s.Npad;
   if (t >= 15)
     return;
   double v = 0.0;

@@ -41,10 +41,11 @@ def production():
     return s, make_walkers(s, 3, cfg["seed"]), cfg
 
 
-def device_run(s, R, prec, steps, recompute_period, seed, learned=None):
+def device_run(s, R, prec, steps, recompute_period, seed, learned=None, move_kernel=0):
     c = qmi.Context(0, precision=prec)
     c.system_create(s)
     c.walkers_load(R, 1)
+    c.tune_set(move_kernel=move_kernel)
     c.mw_evaluateLog()
     if learned is not None:
         c.mlp_params(learned)
@@ -79,6 +80,19 @@ def test_production_table_fp64_replay(production):
     assert err["ratio_exact"] <= max(REL, 2 * err["oracle_exact"])
     assert err["ratio"] < 2e-9 and err["grad"] < 1e-9 and err["energy"] < REL
     assert np.array_equal(c.walkers_get(), o.get()["R"])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_production_window_kernel_matches_move_one(production, prec):
+    """One launch per delay window (k_sweep_segment) against one launch per move (k_move_one) on
+    the production system: the same arithmetic, so identical traces, energies and positions."""
+    s, R, cfg = production
+    c1, E1, tr1, _, g1 = device_run(s, R, prec, 2, 0, 88, move_kernel=0)
+    c2, E2, tr2, _, g2 = device_run(s, R, prec, 2, 0, 88, move_kernel=1)
+    assert c2.tune_get()["move_kernel"] == 1
+    assert np.array_equal(tr1, tr2) and np.array_equal(E1, E2)
+    assert np.array_equal(c1.walkers_get(), c2.walkers_get())
+    assert all(np.array_equal(a, b) for a, b in zip(g1, g2))
 
 
 @pytest.mark.parametrize("prec,period", [("fp32", 32), ("bf16", 64), ("t32", 128)])
