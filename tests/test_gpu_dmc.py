@@ -25,12 +25,13 @@ def rel(a, b, floor=1.0):
     return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor))) if a.size else 0.0
 
 
-def dmc_pair(system, nw, steps, seed=21, **kw):
+def dmc_pair(system, nw, steps, seed=21, move_kernel=0, **kw):
     R = make_walkers(system, nw, 5)
     zion = system.N / len(system.ions)
     c = qmi.Context(0)
     c.system_create(system)
     c.walkers_load(R, 1)
+    c.tune_set(move_kernel=move_kernel)
     c.coulomb_setup(zion)
     c.dmc_init(seed=seed, **kw)
     st = c.dmc_run(steps)
@@ -214,3 +215,13 @@ def test_dmc_tmoves_mixed_precision_runs():
     print("T-moves per step", out[1][:, 10].astype(int).tolist())
     assert np.all(np.isfinite(out[1])) and np.all(out[1][:, 4] > 0) and out[1][:, 10].sum() >= 2
     assert abs(out[1][0, 0] - out[0][0, 0]) < 1e-3 * max(1.0, abs(out[0][0, 0]))
+
+
+@pytest.mark.parametrize("k", [1, 4])
+def test_dmc_window_kernel_matches_oracle(k):
+    """DMC with one launch per delay window (k_sweep_segment, the benchmark's FP64 form): node
+    rejection, prob_eff and the tau-weighted sums inside the move loop, against the oracle."""
+    s = make_system(L=7.0, n_ion=3, n_up=7, n_dn=6, norb=8, M=12, delay_rank=k, seed=13)
+    c, o, st, sto = dmc_pair(s, 24, 10, move_kernel=1, tau=0.02, capacity=96, warmup_steps=3, recompute_period=5)
+    assert c.tune_get()["move_kernel"] == 1
+    check_dmc(c, o, st, sto)
