@@ -183,6 +183,38 @@ def cpu_baseline(system, cfg, nwalk=16, nthreads=None):
                        f"({dt:.1f} s), delay rank {system.delay_rank}, oracle in C++ FP64")
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_small_config(nthreads):
+    """The small CPU-runnable configuration (16 walkers x 128 electrons, 48^3 table, delay rank 1,
+    tau 0.3) on the oracle, as specified (one crowd, 1 thread) and with crowds = threads: electron
+    moves/s over 2 sweeps each."""
+    from oracle import oracle as orc
+    from qmcpack_amd.system import make_walkers, workload_system
+    s, cfg = workload_system("small")
+    R = make_walkers(s, cfg["nw"], cfg["seed"])
+    out = {}
+    for label, threads, wpc in (("one_crowd_1_thread", 1, cfg["nw"]), ("crowds_eq_threads", nthreads,
+                                                                       max(1, cfg["nw"] // nthreads))):
+        o = orc.VMC(s, R, wpc, cfg["seed"])
+        o.evaluate_log()
+        t0 = time.perf_counter()
+        o.run(2, 1, cfg["tau"], 0, nthreads=threads)
+        dt = time.perf_counter() - t0
+        out[label] = dict(value=cfg["nw"] * s.N * 2 / dt, unit="electron moves/s", threads=threads,
+                          crowds=(cfg["nw"] + wpc - 1) // wpc)
+    return out
+
+
 def chain_summary(E, dt, nw, N, steps):
     """moves/s, kappa (mean over walkers), effective samples/s and the mean local energy with its
     error bar sqrt(var kappa / (nw steps))."""
@@ -584,8 +616,10 @@ def main():
     if not args.no_cpu:
         t1 = time.perf_counter()
         out["cpu_baseline"] = cpu_baseline(system, cfg, nwalk=args.cpu_walkers)
+        out["cpu_baseline"]["cpu_model"] = cpu_model()
+        out["cpu_baseline"]["small_config"] = cpu_small_config(out["cpu_baseline"]["cores"])
         log(f"[bench] cpu baseline {out['cpu_baseline']['value']:.3e} moves/s "
-            f"({time.perf_counter() - t1:.1f}s)")
+            f"({time.perf_counter() - t1:.1f}s); small config " + json.dumps(out["cpu_baseline"]["small_config"]))
     print(json.dumps(out))
 
 
