@@ -140,3 +140,66 @@ def test_batch_graph_rebuilt_after_context_changes():
         assert np.array_equal(a, b)
     for a, b in zip(Rb, Re):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.timeout(400)
+def test_batch_at_workload_sizes_matches_oracle():
+    """The batched run itself against the oracle at the uneven workload's sizes: 128 / 256 / 512 /
+    1024 electrons on 48 / 64 / 64 / 80^3 tables, a real and a complex system each, delay rank 32,
+    one walker per system, two sweeps with a from-scratch recompute after the second.
+
+    The oracle replays the trajectory move by move (the chain is chaotic, so two FP64
+    implementations cannot be compared free-running).  The per-move record it follows comes from
+    each context run again alone, traced, from the same start and seed; that run must reproduce the
+    batch's energies and positions bit for bit.  Against the oracle: identical accept/reject
+    decisions, final positions and acceptance counts; energies within 1e-10 (relative to
+    max(|E|, 1)); ratios held to the FP64 oracle's own distance from the long-double determinant
+    shadow (within twice it, or 1e-10 if larger; relative to max(|ratio|, 1e-2)), and from the
+    oracle by no more than the two distances from that shadow (plus 10 % and 1e-10): at 256
+    electrons both FP64 implementations are 1e-9 from it, the device the closer one."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from test_gpu_parity import REL, relerr
+    pops = uneven_populations(nw=8, sizes=(128, 256, 512, 1024), grids=(48, 64, 64, 80), complex_fraction=0.5,
+                              seed=31, delay_rank=32)
+    assert [n for _, n, _ in pops] == [1] * 8
+    Rs = [make_walkers(s, n, 60 + i) for i, (s, n, _) in enumerate(pops)]
+    seeds = [300 + 11 * i for i in range(len(pops))]
+    steps, sub, tau, period = 2, 1, 0.3, 2
+    ctxs = [ctx_for(s, R) for (s, n, _), R in zip(pops, Rs)]
+    b = qmi.Batch(ctxs)
+    Eb = b.vmc_run(steps, sub, tau, period, seeds)
+    Rb = [c.walkers_get() for c in ctxs]
+    Sb = [c.vmc_stats() for c in ctxs]
+    b.close()
+    traces = []
+    for i, ((s, n, label), R, sd, c) in enumerate(zip(pops, Rs, seeds, ctxs)):
+        c.walkers_load(R, 1)
+        c.mw_evaluateLog()
+        E, tr = c.vmc_run(steps, sub, tau, period, sd, trace=True)
+        assert np.array_equal(E, Eb[i]) and np.array_equal(c.walkers_get(), Rb[i]), label
+        traces.append(tr)
+        c.close()
+
+    def replay(i):
+        s, R, sd = pops[i][0], Rs[i], seeds[i]
+        o = orc.VMC(s, R, 1, sd)
+        o.evaluate_log()
+        Eo, tro, ex = o.replay(traces[i], sub, tau, period, exact=True)
+        return o.get(), Eo, tro, ex
+
+    with ThreadPoolExecutor(len(pops)) as pool:
+        results = list(pool.map(replay, range(len(pops))))
+    for i, ((s, n, label), (og, Eo, tro, ex)) in enumerate(zip(pops, results)):
+        tr = traces[i]
+        err = dict(decisions=int(np.sum(tr[..., 1] != tro[..., 1])), ratio=relerr(tr[..., 0], tro[..., 0], 1e-2),
+                   ratio_exact=relerr(tr[..., 0], ex[..., 0], 1e-2),
+                   oracle_exact=relerr(tro[..., 0], ex[..., 0], 1e-2),
+                   grad=relerr(tr[..., 5:8], tro[..., 5:8], 1.0), energy=relerr(Eb[i], Eo, 1.0))
+        print(label, err)
+        assert err["decisions"] == 0, label
+        assert err["ratio_exact"] <= max(REL, 2 * err["oracle_exact"]), label
+        assert err["ratio"] <= 1.1 * (err["ratio_exact"] + err["oracle_exact"]) + REL, label
+        assert err["grad"] < 1e-9 and err["energy"] < REL, label
+        assert np.array_equal(Rb[i], og["R"]), label
+        assert Sb[i]["accepted"] == og["accepted"] and Sb[i]["rejected"] == og["rejected"], label
