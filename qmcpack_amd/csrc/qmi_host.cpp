@@ -1106,6 +1106,13 @@ struct qmi_ctx
   bool nlpp_ready = false;
   NlppC nlpp{};
   DevBuf<double> nlpp_poly, nlpp_knots, nlpp_rot, nlpp_part, nlpp_out;
+  // pair-list form of the energy (QMI_NLPP_FORM=electron at set-up selects one workgroup per
+  // electron instead); ke from the fixed ion positions
+  bool nlpp_pairs_on = true;
+  int nlpp_ke        = 1;
+  int nlpp_grid      = 8;
+  DevBuf<int> np_icnt, np_total, np_ecnt, np_pion, np_list;
+  DevBuf<double> np_wres, np_wpts;
   // non-local moves: per-electron scan weights of the last evaluation, selection in / out
   DevBuf<double> tm_tsum, tm_base, tm_target, tm_out, tm_disp;
   DevBuf<int> tm_iel, tm_chosen, tm_acc, tm_count;
@@ -1172,6 +1179,48 @@ struct qmi_ctx
     c.vpoly    = nlpp_poly.p;
     c.knots    = nlpp_knots.p;
     nlpp       = c;
+    {
+      const char* e = std::getenv("QMI_NLPP_FORM");
+      nlpp_pairs_on = !(e && std::string(e) == "electron");
+    }
+    // Ions within r_cut of one point lie within 2 r_cut of each other (triangle inequality of the
+    // minimum-image distance), so 1 + the most ions within 2 r_cut of any one ion bounds the pairs of
+    // an electron.  Same distance formula as the device; a relative margin of 1e-9 covers rounding.
+    {
+      const int ni = sys.nion;
+      std::vector<double> ih(3 * (size_t)ni);
+      if (ni)
+        HIPCHECK(hipMemcpy(ih.data(), ions.p, ih.size() * sizeof(double), hipMemcpyDeviceToHost));
+      const double lim = 2.0 * rcut * (1.0 + 1e-9);
+      int most         = 0;
+      for (int a = 0; a < ni; a++)
+      {
+        int cnt = 0;
+        for (int b = 0; b < ni; b++)
+        {
+          if (b == a)
+            continue;
+          double d2 = 0.0;
+          for (int q = 0; q < 3; q++)
+          {
+            double d = ih[(size_t)q * ni + b] - ih[(size_t)q * ni + a];
+            if (sys.periodic[q])
+            {
+              const double u = d * sys.Linv[q];
+              d              = sys.L[q] * (u - std::round(u));
+            }
+            d2 += d * d;
+          }
+          cnt += std::sqrt(d2) < lim ? 1 : 0;
+        }
+        most = std::max(most, cnt);
+      }
+      nlpp_ke = std::max(1, std::min(ni, 1 + most));
+      int ncu = 0;
+      HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+      // all workgroups resident: 3 per compute unit at 6 points per pass (the register bound), 1 at 12
+      nlpp_grid = std::max(8, (ncu * (c.kc == 6 ? 3 : 1) + 7) / 8 * 8);
+    }
     nlpp_ready = true;
   }
 
@@ -1204,9 +1253,34 @@ struct qmi_ctx
       tm.minus = tmove_minus();
       tm.tsum  = tm_tsum.p;
     }
+    NlppPairs pl{};
+    const bool pairs = nlpp_pairs_on;
+    if (pairs)
+    {
+      const size_t nslot = (size_t)cap * sys.N * nlpp_ke;
+      if (nslot > (size_t)INT32_MAX)
+        throw QmiError(QMI_ERR_UNSUPPORTED, "too many electron-ion pair slots for the pair list");
+      if (np_icnt.n < (size_t)std::max(1, sys.nion))
+        np_icnt.alloc(std::max(1, sys.nion));
+      if (!np_total.n)
+        np_total.alloc(1);
+      if (np_ecnt.n < (size_t)cap * sys.N)
+        np_ecnt.alloc((size_t)cap * sys.N);
+      if (np_pion.n < nslot)
+      {
+        np_pion.alloc(nslot);
+        np_list.alloc(nslot);
+        np_wres.alloc(nslot);
+      }
+      if (with_txy && np_wpts.n < nslot * nlpp.nknot)
+        np_wpts.alloc(nslot * nlpp.nknot);
+      pl = NlppPairs{nlpp_ke,   nlpp_grid, np_icnt.p, np_total.p, np_ecnt.p,
+                     np_pion.p, np_list.p, np_wres.p, with_txy ? np_wpts.p : nullptr};
+    }
     {
       Scoped t(this, "nlpp");
-      HIPCHECK(launch_nlpp(sys, st, coefs.p, nlpp, nw, nlpp_rot.p, nlpp_part.p, nlpp_out.p, tm, stream));
+      HIPCHECK(launch_nlpp(sys, st, coefs.p, nlpp, nw, nlpp_rot.p, nlpp_part.p, nlpp_out.p, tm, stream,
+                           pairs ? &pl : nullptr));
     }
     HIPCHECK(hipMemcpyAsync(v, nlpp_out.p, (size_t)nw * sizeof(double), hipMemcpyDeviceToHost, stream));
     if (with_txy)

@@ -192,9 +192,30 @@ struct TmoveArgs
   const double* sel_target;
   double* sel_out;
 };
+// Pair-list form of modes 0 and 1: the electron-ion pairs of all walkers within the cutoff,
+// counted and sorted by ion on the device, then evaluated by a persistent grid in which the
+// workgroups of one XCD take consecutive pairs, so that the pairs of one ion run together there and
+// their quadrature points gather the table around that ion from the XCD's L2 and the MALL.  Slot
+// (w N + e) ke + k holds the k-th ion (in ion order) within the cutoff of electron e of walker w;
+// ke bounds the ions within the cutoff of any one point (1 + the most ions within 2 r_cut of one
+// ion, from the fixed ion positions).
+struct NlppPairs
+{
+  int ke;
+  int grid;      // workgroups of the persistent pair kernel (a multiple of 8)
+  int* icnt;     // [nion] pairs of each ion, then the scatter cursors
+  int* total;    // [1] pairs of all walkers
+  int* ecnt;     // [nw N] ions within the cutoff of each electron (> ke: reported as a NaN energy)
+  int* pion;     // [nw N ke] ion of each slot
+  int* list;     // [nw N ke] slots sorted by ion
+  double* wres;  // [nw N ke] pair sums (mode 0)
+  double* wpts;  // [nw N ke][nknot] point weights (mode 1)
+};
 // out[nw] = non-local energy of each walker; rot[nw][9] row-major rotation of the grid; part[nw][N]
+// the per-electron sums.  pairs: the pair-list form for mode 0 (nullptr: one workgroup per electron)
 hipError_t launch_nlpp(const SysC& s, const State& st, const void* coefs, const NlppC& c, int nw, const double* rot,
-                       double* part, double* out, const TmoveArgs& tm, hipStream_t stream);
+                       double* part, double* out, const TmoveArgs& tm, hipStream_t stream,
+                       const NlppPairs* pairs = nullptr);
 
 // per-walker slices of the walker state (base of slot 0 and bytes per slot, a multiple of 16)
 constexpr int WA_MAX = 8;

@@ -225,3 +225,36 @@ def test_dmc_window_kernel_matches_oracle(k):
     c, o, st, sto = dmc_pair(s, 24, 10, move_kernel=1, tau=0.02, capacity=96, warmup_steps=3, recompute_period=5)
     assert c.tune_get()["move_kernel"] == 1
     check_dmc(c, o, st, sto)
+
+
+def test_dmc_tmoves_pair_list_matches_per_electron_form(monkeypatch):
+    """DMC with T-moves on 128 electrons and 32 ions (about 40 electron-ion pairs per walker): the
+    pair-list evaluation of the pseudopotential energy and of each electron's scan weight (point
+    weights stored per pair, then summed per electron in the per-electron form's order) against one
+    workgroup per electron.  The per-point arithmetic is the same code in both kernels (the compiler
+    may fuse multiply-adds differently: ~1e-15), so populations, branching counts and T-move counts
+    agree exactly and energies and positions to 1e-10."""
+    from test_oracle_nlpp import channels, icosahedron
+    s = make_system(L=10.0, n_ion=32, n_up=64, n_dn=64, norb=64, M=16, delay_rank=8, seed=3)
+    R = make_walkers(s, 16, 5)
+    ls = [0, 1]
+    dr, v, v2, _ = channels(ls, seed=4)
+    knots, w = icosahedron()
+    kw = dict(tau=0.02, capacity=48, warmup_steps=2, recompute_period=3, nonlocal_moves=1)
+    out = []
+    for form in ("pairs", "electron"):
+        monkeypatch.setenv("QMI_NLPP_FORM", form)
+        c = qmi.Context(0)
+        c.system_create(s)
+        c.walkers_load(R, 1)
+        c.coulomb_setup(s.N / len(s.ions))
+        c.nlpp_setup(ls, dr, v, v2, 1.5, knots, w)
+        c.dmc_init(seed=61, **kw)
+        out.append((c.dmc_run(5), c.walkers_get(), c))
+    (st, Rp, cp), (ste, Re, ce) = out
+    print("T-moves per step", st[:, 10].astype(int).tolist(), "walkers", st[:, 4].astype(int).tolist())
+    assert st[:, 10].sum() >= 5
+    assert np.array_equal(st[:, [3, 4, 8, 9, 10]], ste[:, [3, 4, 8, 9, 10]])
+    for col in (0, 2, 5, 6, 7):
+        assert rel(st[:, col], ste[:, col]) < REL, col
+    assert np.allclose(Rp, Re, rtol=0, atol=1e-10)

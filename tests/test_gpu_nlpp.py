@@ -106,3 +106,62 @@ def test_nlpp_mixed_precision(prec):
     Eo = o.nlpp_energy([0, 1], dr, v, v2, 2.2, knots, w, rot)
     print(prec, worst(E, Eo))
     assert worst(E, Eo) < 1e-4
+
+
+def energies_by_form(monkeypatch, s, R, prec, ls, rcut, seed, form):
+    """mw_evaluateNLPP on fixed positions with the evaluation form chosen at set-up."""
+    if form == "electron":
+        monkeypatch.setenv("QMI_NLPP_FORM", "electron")
+    else:
+        monkeypatch.delenv("QMI_NLPP_FORM", raising=False)
+    dr, v, v2, _ = channels(ls, seed=2)
+    knots, w = icosahedron()
+    c = qmi.Context(0, precision=prec)
+    c.system_create(s)
+    c.walkers_load(R, 1)
+    c.mw_evaluateLog()
+    c.nlpp_setup(ls, dr, v, v2, rcut, knots, w)
+    rot = random_rotations(R.shape[0], seed)
+    E = c.mw_evaluateNLPP(rot)
+    E2 = c.mw_evaluateNLPP(rot)  # the pair list is rebuilt each call
+    assert np.array_equal(E, E2)
+    return E, (dr, v, v2, knots, w, rot)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_pair_list_form_matches_per_electron_form(monkeypatch, prec):
+    """The pair-list form (the electron-ion pairs of all walkers sorted by ion and evaluated by a
+    persistent grid, the pairs of one ion together on one XCD) against one workgroup per electron:
+    the same per-pair code and the same summation order (points, then ions in ion order per
+    electron, then electrons); the two kernels differ only where the compiler fuses multiply-adds
+    (measured 2e-15 relative), so within 1e-13 relative to max(|V|, 1).  Shaped like the benchmark
+    system: 512 electrons at its density, 128 ions, s and p channels, r_cut 1.5, 12 points."""
+    s = make_system(L=18.0, n_ion=128, n_up=256, n_dn=256, norb=256, M=16, delay_rank=32, seed=31)
+    R = make_walkers(s, 24, 5)
+    Ep, _ = energies_by_form(monkeypatch, s, R, prec, [0, 1], 1.5, 3, "pairs")
+    Ee, _ = energies_by_form(monkeypatch, s, R, prec, [0, 1], 1.5, 3, "electron")
+    print(prec, "pair-list vs per-electron:", float(np.max(np.abs(Ep - Ee))), "energies", Ep[:3])
+    assert np.all(Ep != 0.0)
+    assert worst(Ep, Ee) < 1e-13
+
+
+def test_pair_list_with_clustered_ions(monkeypatch):
+    """Seven ions within 0.5 bohr (per coordinate) of one point and five electrons of every walker
+    placed among them, each within r_cut of all seven: the pair-list form against the
+    per-electron form (1e-13) and the oracle (1e-10)."""
+    rng = np.random.default_rng(8)
+    s = make_system(L=8.0, n_ion=10, n_up=9, n_dn=8, norb=9, M=12, delay_rank=4, seed=17)
+    ions = s.ions.copy()
+    ions[:7] = 4.0 + rng.uniform(-0.5, 0.5, size=(7, 3))
+    import dataclasses
+    s = dataclasses.replace(s, ions=ions)
+    R = make_walkers(s, 6, 9)
+    R[:, :5] = 4.0 + rng.uniform(-0.6, 0.6, size=(6, 5, 3))
+    Ep, (dr, v, v2, knots, w, rot) = energies_by_form(monkeypatch, s, R, "fp64", [0, 1, 2], 2.0, 4, "pairs")
+    Ee, _ = energies_by_form(monkeypatch, s, R, "fp64", [0, 1, 2], 2.0, 4, "electron")
+    o = orc.VMC(s, R, 1, 1)
+    o.evaluate_log()
+    Eo = o.nlpp_energy([0, 1, 2], dr, v, v2, 2.0, knots, w, rot)
+    print("clustered ions:", Ep, Eo, worst(Ep, Eo))
+    assert worst(Ep, Ee) < 1e-13
+    assert worst(Ep, Eo) < REL
