@@ -1219,7 +1219,7 @@ struct qmi_ctx
       int ncu = 0;
       HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
       // all workgroups resident: 3 per compute unit at 6 points per pass (the register bound), 1 at 12
-      nlpp_grid = std::max(8, (ncu * (c.kc == 6 ? 3 : 1) + 7) / 8 * 8);
+      nlpp_grid = std::max(8, (ncu * (c.kc == 6 ? 4 : 1) + 7) / 8 * 8);
     }
     nlpp_ready = true;
   }
