@@ -234,7 +234,8 @@ def pick_launch(ctx, tau, seed):
     the window's end waits for the slowest).  Same arithmetic in every form."""
     forms = {"single": ("graph" if GRAPH else "eager", 1, 0),
              "partitioned": ("eager", 2, 0),
-             "window": ("graph" if GRAPH else "eager", 1, 1)}
+             "window": ("graph" if GRAPH else "eager", 1, 1),
+             "window-partitioned": ("eager", 2, 1)}
     names = list(forms) if LAUNCH == "auto" else [LAUNCH]
     best = None
     for name in names:
@@ -255,7 +256,9 @@ def pick_launch(ctx, tau, seed):
     ctx.graph_enable(mode == "graph")
     return {"single": f"{mode}, one launch of all walkers per move",
             "partitioned": "eager, 2 walker partitions on 2 streams, one launch per move each",
-            "window": f"{mode}, one launch per delay window (k_sweep_segment)"}[name]
+            "window": f"{mode}, one launch per delay window (k_sweep_segment)",
+            "window-partitioned": "eager, 2 walker partitions on 2 streams, one launch per delay window each "
+                                  "(k_sweep_segment)"}[name]
 
 
 def run_variant(qmi, system, R, cfg, precision, learned, steps, warmup, train_sweeps, lr, log):
@@ -448,7 +451,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="uneven workload: keep the default knobs")
     ap.add_argument("--eager", action="store_true",
                     help="launch the sweeps kernel by kernel instead of replaying a captured graph")
-    ap.add_argument("--launch", choices=["auto", "single", "partitioned", "window"], default="auto",
+    ap.add_argument("--launch", choices=["auto", "single", "partitioned", "window", "window-partitioned"], default="auto",
                     help="plain sweeps: all walkers in one launch per move (graph unless --eager), two "
                          "walker partitions on two streams (eager), one launch per delay window, or the "
                          "fastest in a timed trial")

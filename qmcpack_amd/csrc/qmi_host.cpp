@@ -2517,15 +2517,19 @@ struct qmi_ctx
       for (int g = 0; g < 2; g++)
       {
         const int last = std::min(s.first[g] + s.n[g], e1);
-        for (int iat = std::max(s.first[g], e0); iat < last; iat++)
+        for (int iat = std::max(s.first[g], e0); iat < last;)
         {
-          b.g    = g;
-          b.d0   = s.k > 1 ? dc[g] : 0;
-          b.iat0 = iat;
-          b.iat1 = iat + 1;
+          // one launch per move, or with the per-window kernel (real orbitals) one per delay window
+          const bool window = !move_one && !cplx;
+          const int seg     = !window ? 1 : s.k > 1 ? std::min(s.k - dc[g], last - iat) : last - iat;
+          b.g               = g;
+          b.d0              = s.k > 1 ? dc[g] : 0;
+          b.iat0            = iat;
+          b.iat1            = iat + seg;
           HIPCHECK(cplx ? launch_move_c(MK_MOVE_ONE, s, st, coefs.p, b, ps)
-                        : launch_move(MK_MOVE_ONE, s, st, coefs.p, b, ps));
-          if (s.k > 1 && ++dc[g] == s.k)
+                        : launch_move(window ? MK_SEGMENT : MK_MOVE_ONE, s, st, coefs.p, b, ps));
+          iat += seg;
+          if (s.k > 1 && (dc[g] += seg) == s.k)
           {
             flush_part(g, dc[g], w0, w1 - w0, ps);
             dc[g] = 0;

@@ -13,11 +13,13 @@ from qmcpack_amd.system import make_complex_system, make_system, make_walkers
 pytestmark = pytest.mark.gpu
 
 
-def run(s, R, P, graph, steps=3, sub=2, rp=2, trace=False):
+def run(s, R, P, graph, steps=3, sub=2, rp=2, trace=False, mk=0):
     c = qmi.Context(0)
     c.system_create(s)
     c.walkers_load(R, 1)
     assert c.tune_partitions(P) == P
+    if mk:
+        c.tune_set(move_kernel=mk)
     c.mw_evaluateLog()
     c.graph_enable(graph)
     E, tr = c.vmc_run(steps, sub, 0.35, rp, 91, trace=trace)
@@ -37,6 +39,18 @@ def test_partitioned_sweep_is_bit_identical(k, P, graph):
         assert np.array_equal(tr1, trP)
 
 
+@pytest.mark.parametrize("k,P", [(1, 2), (4, 3), (8, 2)])
+def test_partitioned_window_kernel_is_bit_identical(k, P):
+    """Partitions running the per-window kernel (one launch per delay window and partition, the
+    flushes of each partition on its stream) against the unpartitioned per-window sweep."""
+    s = make_system(L=8.0, n_ion=4, n_up=11, n_dn=10, norb=12, M=12, delay_rank=k, seed=61)
+    R = make_walkers(s, 64 * P + 5, 62)
+    E1, tr1, R1, st1 = run(s, R, 1, False, trace=True, mk=1)
+    EP, trP, RP, stP = run(s, R, P, False, trace=True, mk=1)
+    assert np.array_equal(E1, EP) and np.array_equal(R1, RP) and np.array_equal(tr1, trP)
+    assert st1 == stP
+
+
 def test_partitioned_complex_sweep_is_bit_identical():
     s = make_complex_system(L=7.0, n_ion=3, n_up=6, n_dn=5, norb=8, M=12, delay_rank=2, seed=63)
     R = make_walkers(s, 130, 64)
@@ -45,7 +59,8 @@ def test_partitioned_complex_sweep_is_bit_identical():
     assert np.array_equal(E1, E2) and np.array_equal(R1, R2) and np.array_equal(tr1, tr2)
 
 
-def test_partitioned_dmc_is_bit_identical():
+@pytest.mark.parametrize("mk", [0, 1])
+def test_partitioned_dmc_is_bit_identical(mk):
     s = make_system(L=7.0, n_ion=3, n_up=7, n_dn=6, norb=8, M=12, delay_rank=4, seed=65)
     R = make_walkers(s, 160, 66)
     out = []
@@ -54,6 +69,7 @@ def test_partitioned_dmc_is_bit_identical():
         c.system_create(s)
         c.walkers_load(R, 1)
         c.tune_partitions(P)
+        c.tune_set(move_kernel=mk)
         c.coulomb_setup(s.N / len(s.ions))
         c.dmc_init(tau=0.02, capacity=480, seed=17, warmup_steps=2)
         st = c.dmc_run(6)
