@@ -370,9 +370,8 @@ __global__ void k_nlpp_pairs_zero(int ni, int* __restrict__ icnt)
     icnt[i] = 0;
 }
 
-// one thread per (walker, electron): the ions within the cutoff in ion order (the distance test of
-// min_image, its square root taken only for squared distances within 1e-12 of the cutoff's), their
-// count per ion
+// one thread per (walker, electron): the ions within the cutoff in ion order (the same distance test
+// as the per-electron form), their count per ion
 __global__ void k_nlpp_pairs_find(SysC s, State st, NlppC c, NlppPairs pl, int nN)
 {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -381,25 +380,11 @@ __global__ void k_nlpp_pairs_find(SysC s, State st, NlppC c, NlppPairs pl, int n
   const int w = idx / s.N, e = idx - w * s.N, ni = s.nion, Np = s.Npad;
   const double* R    = st.R + (size_t)w * 3 * Np;
   const double pe[3] = {R[e], R[Np + e], R[2 * Np + e]};
-  const double rc2   = c.rcut * c.rcut * (1.0 + 1e-12);
   int k              = 0;
   for (int ion = 0; ion < ni; ion++)
   {
-    const double src[3] = {st.ions[ion], st.ions[ni + ion], st.ions[2 * ni + ion]};
-    double dr[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++)
-    {
-      if (s.periodic[d])
-      {
-        const double u = (src[d] - pe[d]) * s.Linv[d];
-        dr[d]          = s.L[d] * (u - round(u));
-      }
-      else
-        dr[d] = src[d] - pe[d];
-    }
-    const double d2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
-    if (d2 < rc2 && sqrt(d2) < c.rcut)
+    double d[3];
+    if (min_image(s, st.ions[ion], st.ions[ni + ion], st.ions[2 * ni + ion], pe, d) < c.rcut)
     {
       if (k < pl.ke)
       {
