@@ -570,7 +570,7 @@ hipError_t launch_nlpp(const SysC& s, const State& st, const void* coefs, const 
         (pts && (!pl.wpts || !tm.tsum)))
       return hipErrorInvalidValue;
     const int tb = 256, nb = (nN + tb - 1) / tb;
-    k_nlpp_pairs_zero<<<(s.nion + tb - 1) / tb, tb, 0, stream>>>(s.nion, pl.icnt);
+    k_nlpp_pairs_zero<<<std::max(1, (s.nion + tb - 1) / tb), tb, 0, stream>>>(s.nion, pl.icnt); // no ions: one idle block
     k_nlpp_pairs_find<<<nb, tb, 0, stream>>>(s, st, c, pl, nN);
     k_nlpp_pairs_scan<<<1, WG, 0, stream>>>(s.nion, pl);
     k_nlpp_pairs_scatter<<<nb, tb, 0, stream>>>(pl, nN);

@@ -165,3 +165,15 @@ def test_pair_list_with_clustered_ions(monkeypatch):
     print("clustered ions:", Ep, Eo, worst(Ep, Eo))
     assert worst(Ep, Ee) < 1e-13
     assert worst(Ep, Eo) < REL
+
+
+@pytest.mark.parametrize("form", ["pairs", "electron"])
+def test_nlpp_without_ions(monkeypatch, form):
+    """A system with no ions has no electron-ion pairs: zero energy in both forms (the pair list's
+    per-ion arrays are empty)."""
+    from test_oracle_nlpp import bare_system
+    L = np.array([6.0, 6.5, 7.0])
+    s = bare_system(4, 3, np.zeros((0, 3)), L)
+    R = np.random.default_rng(2).uniform(0, 1, size=(3, s.N, 3)) * L
+    E, _ = energies_by_form(monkeypatch, s, R, "fp64", [0, 1], 2.0, 5, form)
+    assert np.array_equal(E, np.zeros(3))
